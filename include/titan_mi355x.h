@@ -1,0 +1,162 @@
+/*
+ * titan_mi355x.h — C ABI of the MI355X FP64 mass-spring step engine.
+ *
+ * The step that replaces the reference's computeSpringForces +
+ * massForcesAndUpdate pair (src/sim.cu:1046-1169) driven by
+ * Simulation::execute (src/sim.cu:1503-1617).  Plain pointers and sizes
+ * only; the library owns all device memory, the caller owns host buffers.
+ * Every function returns 0 on success or a negative TITAN_E* code; the
+ * message of the last failure is available from titan_last_error().  No
+ * C++ exception crosses this boundary.  One context may be used from one
+ * host thread at a time; there is no background thread in the library.
+ *
+ * Host vectors are interleaved x,y,z triples (the reference's Vec layout,
+ * include/Titan/vec.h:162); on the device the state is kept as FP64 SoA.
+ */
+#ifndef TITAN_MI355X_H
+#define TITAN_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TITAN_OK 0
+#define TITAN_E_INVALID (-1)     /* bad argument or call order */
+#define TITAN_E_HIP (-2)         /* HIP runtime failure */
+#define TITAN_E_NOMEM (-3)       /* device memory or HBM budget exhausted */
+#define TITAN_E_NOTFOUND (-4)    /* unknown body / ticket */
+#define TITAN_E_INJECTED (-5)    /* failure forced by titan_debug_inject_failure */
+
+#define TITAN_ALL_BODIES (-1)
+
+/* Spring kinds. 0-3 are the reference's SpringType values
+ * (include/Titan/spring.h:17); TITAN_SPRING_GENERAL uses the actuation law
+ * rest * (1 + a * sin(omega * t + phi)). */
+#define TITAN_PASSIVE_SOFT 0
+#define TITAN_PASSIVE_STIFF 1
+#define TITAN_ACTIVE_CONTRACT_THEN_EXPAND 2
+#define TITAN_ACTIVE_EXPAND_THEN_CONTRACT 3
+#define TITAN_SPRING_GENERAL 4
+
+/* Local constraint kinds, the reference's CONSTRAINT_TYPE values
+ * (include/Titan/object.h:231-233). */
+#define TITAN_CONSTRAINT_PLANE 0
+#define TITAN_CONTACT_PLANE 1
+#define TITAN_BALL 2
+#define TITAN_DIRECTION 3
+
+typedef struct titan_ctx titan_ctx;
+
+/* One body: a set of masses and the springs among them, copied in by
+ * titan_body_create.  Optional arrays may be NULL.  Spring endpoints are
+ * body-local mass indices; a negative endpoint marks a spring with no mass,
+ * which does nothing (src/sim.cu:1052). */
+typedef struct {
+    int64_t n_masses;
+    int64_t n_springs;
+    const double *pos;        /* 3*n_masses */
+    const double *vel;        /* 3*n_masses, NULL = zero */
+    const double *m;          /* n_masses (kg) */
+    const double *T;          /* n_masses local time, NULL = 0 */
+    const double *ext;        /* 3*n_masses extern force, NULL = none */
+    const uint8_t *fixed;     /* n_masses, NULL = none fixed */
+    const int32_t *drag;      /* n_masses integer drag coefficient, NULL = 0 */
+    const int32_t *left;      /* n_springs */
+    const int32_t *right;     /* n_springs */
+    const double *k;          /* n_springs */
+    const double *rest;       /* n_springs */
+    const double *damping;    /* n_springs, NULL = 0 */
+    const int32_t *type;      /* n_springs, NULL = passive */
+    const double *omega;      /* n_springs, read for kinds 2-4 */
+    const double *act_a;      /* n_springs, read for kind 4 */
+    const double *act_phi;    /* n_springs, read for kind 4 */
+    /* local constraints, CSR over masses in insertion order
+     * (src/mass.cu:78-96): lc_ptr has n_masses+1 entries; lc_vec holds the
+     * unit normal / unit tangent / ball centre (3 per entry) and lc_num the
+     * offset / friction / radius.  NULL lc_ptr = none. */
+    const int64_t *lc_ptr;
+    const int32_t *lc_type;
+    const double *lc_vec;
+    const double *lc_num;
+    /* body time at creation: the first step's actuation time */
+    double t0;
+} titan_body_desc;
+
+/* ---- context (replaces Simulation's constructor/stop/freeGPU,
+ *      src/sim.cu:77-221,1319-1347) ---- */
+int titan_ctx_create(int device, uint64_t hbm_budget_bytes, titan_ctx **out);
+int titan_ctx_destroy(titan_ctx *ctx);
+/* Message of the last failure on ctx (or of titan_ctx_create if ctx is
+ * NULL); never NULL. */
+const char *titan_last_error(const titan_ctx *ctx);
+
+/* ---- globals (src/sim.cu:797-807,2081-2087,1998-2040) ---- */
+int titan_set_dt(titan_ctx *ctx, double dt);                       /* setTimeStep */
+int titan_get_dt(const titan_ctx *ctx, double *dt);                /* getTimeStep */
+int titan_set_global_acc(titan_ctx *ctx, double gx, double gy, double gz); /* setGlobalAcceleration */
+/* createPlane(abc, d, FRICTION_K, FRICTION_S); the normal must already be
+ * unit length, as ContactPlane's constructor makes it (object.h:102-104). */
+int titan_add_plane(titan_ctx *ctx, double nx, double ny, double nz, double d,
+                    double mu_k, double mu_s);
+int titan_add_ball(titan_ctx *ctx, double cx, double cy, double cz, double r); /* createBall */
+int titan_clear_planes_and_balls(titan_ctx *ctx);
+
+/* ---- bodies (replace toArray/massToArray/springToArray,
+ *      src/sim.cu:829-930) ---- */
+int titan_body_create(titan_ctx *ctx, const titan_body_desc *desc, int32_t *body_id);
+int titan_body_destroy(titan_ctx *ctx, int32_t body_id);
+int titan_body_count(const titan_ctx *ctx, int32_t *n);
+
+/* ---- stepping (replaces the loop body of Simulation::execute,
+ *      src/sim.cu:1569-1590) ---- */
+/* Advance every body by n_steps; each step runs at the body's time T, then
+ * T += dt (src/sim.cu:1590).  Enqueued on the context's stream; returns
+ * without waiting unless a later call needs the results. */
+int titan_step(titan_ctx *ctx, int64_t n_steps);
+/* Step until the first step boundary with T >= t, the breakpoint rule of
+ * src/sim.cu:1505; *steps_done (may be NULL) receives the count. */
+int titan_step_until(titan_ctx *ctx, double t, int64_t *steps_done);
+int titan_time(const titan_ctx *ctx, double *T);   /* Simulation::time, src/sim.cu:1082-1084 */
+int titan_sync(titan_ctx *ctx);
+
+/* ---- readback / write-back (replace getAll/get/setAll/set,
+ *      src/sim.cu:588-764,940-1005) ---- */
+/* Any output pointer may be NULL.  pos/vel/acc: 3*n_masses; T: n_masses. */
+int titan_body_download(titan_ctx *ctx, int32_t body_id, double *pos, double *vel,
+                        double *acc, double *T);
+/* CUDA_MASS(Mass&) semantics (src/mass.cu:57-74): pos, vel, m, T, ext are
+ * written (NULL leaves the field unchanged); acc is not. */
+int titan_body_upload(titan_ctx *ctx, int32_t body_id, const double *pos, const double *vel,
+                      const double *m, const double *T, const double *ext);
+
+/* ---- timing on the context's stream (hipEvents) ---- */
+int titan_timer_start(titan_ctx *ctx);
+/* Stops the timer started by titan_timer_start; *ms = elapsed GPU time. */
+int titan_timer_stop(titan_ctx *ctx, float *ms);
+
+/* ---- introspection used by tests and the benchmark ---- */
+/* Device bytes currently allocated by the context. */
+int titan_mem_used(const titan_ctx *ctx, uint64_t *bytes);
+/* Name of the step kernel variant the next titan_step will launch. */
+int titan_kernel_name(const titan_ctx *ctx, char *buf, int64_t len);
+
+/* Host-side lattice generator with the reference's mass and spring order
+ * (Lattice::Lattice, src/object.cu:232-293).  pos: 3*nx*ny*nz; left/right/
+ * rest: titan_lattice_spring_count(nx, ny, nz) entries each. */
+int64_t titan_lattice_spring_count(int nx, int ny, int nz);
+int titan_lattice(double cx, double cy, double cz, double dx, double dy, double dz,
+                  int nx, int ny, int nz, double *pos, int32_t *left, int32_t *right,
+                  double *rest);
+
+/* Fault injection for host-side tests of the error paths: the next `after`
+ * successful calls proceed, then one call of titan_step returns
+ * TITAN_E_INJECTED without touching the device.  Negative `after` clears it. */
+int titan_debug_inject_failure(titan_ctx *ctx, int32_t after);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

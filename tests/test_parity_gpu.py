@@ -1,0 +1,230 @@
+"""Parity of the HIP step (through the C ABI) with the CPU oracle.
+
+Bar: passive spring networks must be bit-identical to the oracle (the kernel
+keeps the reference's FP64 operation order, is built with -ffp-contract=off
+and sums forces in ascending spring index, as the oracle does).  Actuated
+springs call the device sin, so they are held to a normwise relative error
+of 1e-10 after 100 steps (the acceptance tolerance of the path).
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from titan_amd import Context, scenes
+from helpers import normwise_rel, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL_ACT = 1e-10
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run_engine(scene, n_steps, ctx=None, chunks=None):
+    own = ctx is None
+    ctx = ctx or Context()
+    ids = scene.load(ctx)
+    for n in (chunks or [n_steps]):
+        ctx.step(n)
+    out = [ctx.download(i, pos=True, vel=True, acc=True, T=True) for i in ids]
+    t = ctx.time()
+    if own:
+        ctx.close()
+    cat = {k: np.concatenate([o[k] for o in out]) for k in out[0]}
+    return cat, t
+
+
+def run_oracle(scene, n_steps):
+    o = to_oracle(scene)
+    o.run(n_steps)
+    return o
+
+
+def assert_bitexact(eng, orc):
+    np.testing.assert_array_equal(eng["pos"], orc.pos)
+    np.testing.assert_array_equal(eng["vel"], orc.vel)
+
+
+def test_readme_lattice_free_fall_bitexact():
+    sc = scenes.readme_lattice(10.0)
+    _say(f"case readme lattice cz=10, {sc.n_masses} masses, {sc.n_springs} springs, 100 steps")
+    eng, t = run_engine(sc, 100)
+    orc = run_oracle(sc, 100)
+    assert_bitexact(eng, orc)
+    np.testing.assert_array_equal(eng["acc"], orc.acc)
+    np.testing.assert_array_equal(eng["T"], orc.T)
+    assert t == orc.t
+
+
+def test_readme_lattice_contact_bitexact():
+    sc = scenes.readme_lattice(2.45)
+    _say("case readme lattice cz=2.45 (contact + friction from step 0), 1000 steps")
+    eng, _ = run_engine(sc, 1000)
+    orc = run_oracle(sc, 1000)
+    assert_bitexact(eng, orc)
+    np.testing.assert_array_equal(eng["acc"], orc.acc)
+    assert np.min(orc.pos[:, 2]) < 0  # contact really happened
+
+
+def test_graph_path_matches_direct_launches():
+    sc = scenes.readme_lattice(2.45)
+    _say("case graph replay vs single launches, 300 steps")
+    a, _ = run_engine(sc, 300)                          # mostly hipGraph replays
+    b, _ = run_engine(sc, 300, chunks=[1] * 300)        # one launch per call
+    np.testing.assert_array_equal(a["pos"], b["pos"])
+    np.testing.assert_array_equal(a["vel"], b["vel"])
+
+
+def test_actuated_reference_kinds_and_general_law():
+    sc = scenes.readme_lattice(2.45)
+    b = sc.bodies[0]
+    s = len(b["left"])
+    rng = np.random.default_rng(1)
+    typ = rng.integers(0, 5, s).astype(np.int32)
+    b["type"] = typ
+    b["omega"] = rng.uniform(1, 20, s)
+    b["act_a"] = rng.uniform(0, 0.2, s)
+    b["act_phi"] = rng.uniform(0, 2 * np.pi, s)
+    _say("case actuated springs (kinds 0-4), 100 steps")
+    eng, _ = run_engine(sc, 100)
+    orc = run_oracle(sc, 100)
+    assert normwise_rel(eng["pos"], orc.pos) <= TOL_ACT
+    assert normwise_rel(eng["vel"], orc.vel) <= TOL_ACT
+
+
+def test_fixed_ext_drag_balls_per_spring_k():
+    sc = scenes.readme_lattice(2.45)
+    b = sc.bodies[0]
+    n, s = len(b["pos"]), len(b["left"])
+    rng = np.random.default_rng(2)
+    b["fixed"] = (np.arange(n) % 17 == 0).astype(np.uint8)
+    b["ext"] = rng.normal(0, 0.5, (n, 3)) * (np.arange(n) % 3 == 0)[:, None]
+    b["drag"] = (np.arange(n) % 5).astype(np.int32)
+    b["k"] = rng.uniform(5e3, 2e4, s)
+    b["damping"] = rng.uniform(0, 0.3, s)
+    b["vel"] = rng.normal(0, 0.5, (n, 3))
+    sc.balls = [(0.0, 0.0, 1.0, 1.2), (1.0, 1.0, 4.0, 0.7)]
+    sc.planes.append((0.0, 0.6, 0.8, -1.0, 0.3, 0.5))
+    _say("case fixed + extern + drag + balls + 2 planes + per-spring k/damping, 500 steps")
+    eng, _ = run_engine(sc, 500)
+    orc = run_oracle(sc, 500)
+    assert_bitexact(eng, orc)
+    np.testing.assert_array_equal(eng["T"], orc.T)
+
+
+def test_local_constraints_shared_and_per_mass():
+    sc = scenes.readme_lattice(2.0)
+    b = sc.bodies[0]
+    n = len(b["pos"])
+    # body 0: every mass gets the same local list (Container::addConstraint)
+    kinds = [1, 2, 0, 3, 1]
+    vecs = [(0, 0, 1), (0.5, 0.5, 2.0), (0.6, 0.0, 0.8), (1.0, 0.0, 0.0), (0.0, 0.6, 0.8)]
+    nums = [0.3, 0.8, 0.2, 0.1, -0.5]
+    b["lc_ptr"] = np.arange(n + 1, dtype=np.int64) * len(kinds)
+    b["lc_type"] = np.tile(kinds, n).astype(np.int32)
+    b["lc_vec"] = np.tile(np.asarray(vecs, float), (n, 1))
+    b["lc_num"] = np.tile(nums, n)
+    # body 1: per-mass lists of varying length
+    b2 = scenes.lattice_body((3.0, 0.0, 1.5), (2.0, 2.0, 2.0), 4, damping=0.05)
+    n2 = len(b2["pos"])
+    rng = np.random.default_rng(3)
+    cnt = rng.integers(0, 4, n2)
+    b2["lc_ptr"] = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
+    tot = int(cnt.sum())
+    b2["lc_type"] = rng.integers(0, 4, tot).astype(np.int32)
+    v = rng.normal(size=(tot, 3))
+    v /= np.sqrt((v * v).sum(1))[:, None]
+    b2["lc_vec"] = v
+    b2["lc_num"] = rng.uniform(-0.2, 0.5, tot)
+    b2["vel"] = rng.normal(0, 0.3, (n2, 3))
+    sc.bodies.append(b2)
+    _say("case local constraints (shared list + per-mass lists), 2 bodies, 400 steps")
+    eng, _ = run_engine(sc, 400)
+    orc = run_oracle(sc, 400)
+    assert_bitexact(eng, orc)
+
+
+def test_bodies_alone_equal_bodies_together():
+    s1 = scenes.readme_lattice(2.45)
+    s2 = scenes.readme_lattice(3.0)
+    s2.bodies[0]["k"] = np.full(len(s2.bodies[0]["left"]), 7000.0)
+    both = scenes.Scene(bodies=s1.bodies + s2.bodies, planes=s1.planes, dt=s1.dt)
+    _say("case two bodies alone vs together, 200 steps")
+    a, _ = run_engine(s1, 200)
+    b, _ = run_engine(s2, 200)
+    ab, _ = run_engine(both, 200)
+    np.testing.assert_array_equal(ab["pos"], np.concatenate([a["pos"], b["pos"]]))
+    np.testing.assert_array_equal(ab["vel"], np.concatenate([a["vel"], b["vel"]]))
+
+
+def test_step_until_breakpoint_rule_and_time():
+    sc = scenes.readme_lattice(2.45)
+    with Context() as ctx:
+        ids = sc.load(ctx)
+        k = ctx.step_until(0.0105)
+        t = ctx.time()
+        # first step boundary with T >= 0.0105 under T += 1e-4 accumulation
+        T, n = 0.0, 0
+        while not (0.0105 <= T):
+            T += 1e-4
+            n += 1
+        assert k == n and t == T
+        got = ctx.download(ids[0])
+    orc = run_oracle(sc, n)
+    np.testing.assert_array_equal(got["pos"], orc.pos)
+
+
+def test_upload_download_roundtrip_and_resume():
+    sc = scenes.readme_lattice(2.45)
+    o = to_oracle(sc)
+    with Context() as ctx:
+        (bid,) = sc.load(ctx)
+        ctx.step(50)
+        st = ctx.download(bid, acc=True, T=True)
+        # teleport the body as rotate_unittest does via get/set
+        newpos = st["pos"] + np.array([0.1, -0.2, 0.05])
+        ctx.upload(bid, pos=newpos, vel=st["vel"] * 0.5)
+        ctx.step(50)
+        got = ctx.download(bid)
+    o.run(50)
+    for c in range(3):
+        o.p[c][:] = o.p[c] + [0.1, -0.2, 0.05][c]
+        o.v[c][:] = o.v[c] * 0.5
+    o.run(50)
+    np.testing.assert_array_equal(got["pos"], o.pos)
+    np.testing.assert_array_equal(got["vel"], o.vel)
+
+
+def test_million_mass_lattice_bitexact_few_steps():
+    sc = scenes.big_lattice(100, seed=0)
+    _say(f"case 100^3 lattice {sc.n_masses} masses {sc.n_springs} springs, 5 steps")
+    eng, _ = run_engine(sc, 5)
+    o = to_oracle(sc)
+    o.run(5)
+    assert_bitexact(eng, o)
+
+
+def test_destroy_and_reuse_slots():
+    sc = scenes.readme_lattice(2.45)
+    with Context() as ctx:
+        ids = sc.load(ctx)
+        extra = scenes.lattice_body((5, 5, 3), (1, 1, 1), 3, damping=0.1)
+        e = ctx.add_body(**extra)
+        ctx.step(10)
+        ctx.destroy_body(e)
+        e2 = ctx.add_body(**extra)
+        ctx.step(20)
+        got = ctx.download(ids[0])
+        got2 = ctx.download(e2)
+    o = to_oracle(sc)
+    o.run(30)
+    np.testing.assert_array_equal(got["pos"], o.pos)
+    o2 = to_oracle(scenes.Scene(bodies=[extra], planes=sc.planes, dt=sc.dt))
+    o2.run(20)
+    np.testing.assert_array_equal(got2["pos"], o2.pos)

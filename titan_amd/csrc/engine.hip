@@ -1,0 +1,1127 @@
+// Host runtime of the MI355X step engine and its C ABI (include/titan_mi355x.h).
+//
+// Replaces the reference's per-object cudaMalloc staging (massToArray /
+// springToArray, src/sim.cu:829-914), its pointer-table device vectors
+// (include/Titan/sim.h:150-164) and its asynchronous driver thread
+// (src/sim.cu:1389,1503-1617) with:
+//   * arenas of 64-mass chunks and SELL-64 incidence slots (layout.h), grown
+//     by doubling within an optional HBM budget;
+//   * bodies uploaded in bulk, one H2D copy per array;
+//   * synchronous, stream-ordered stepping: n steps are n launches of the
+//     fused step kernel on the context's stream, the bulk of them replayed
+//     from a captured hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "layout.h"
+#include "titan_mi355x.h"
+
+using namespace titan_mi355x;
+
+namespace {
+
+struct Fail {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string &m) { throw Fail{code, m}; }
+
+#define HIPCHK(expr)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess) fail(TITAN_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// First-fit allocator over [0, cap) used for chunks, incidence slots,
+// actuators, local constraints and body ids.
+class Ranges {
+   public:
+    int64_t cap() const { return cap_; }
+    int64_t alloc(int64_t len) {
+        if (len <= 0) return 0;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= len) {
+                const int64_t b = it->first, rem = it->second - len;
+                free_.erase(it);
+                if (rem) free_[b + len] = rem;
+                return b;
+            }
+        }
+        return -1;
+    }
+    void release(int64_t b, int64_t len) {
+        if (len <= 0) return;
+        auto it = free_.emplace(b, len).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    void grow(int64_t newcap) {
+        if (newcap <= cap_) return;
+        release(cap_, newcap - cap_);
+        cap_ = newcap;
+    }
+    // end of the highest allocated range
+    int64_t high_water() const {
+        if (free_.empty()) return cap_;
+        auto last = std::prev(free_.end());
+        return (last->first + last->second == cap_) ? last->first : cap_;
+    }
+
+   private:
+    std::map<int64_t, int64_t> free_;
+    int64_t cap_ = 0;
+};
+
+struct BodyH {
+    bool live = false;
+    int64_t n_masses = 0, chunk0 = 0, nchunks = 0;
+    int64_t inc0 = 0, inc_len = 0, act0 = 0, act_len = 0, lc0 = 0, lc_len = 0;
+    double T = 0;   // host mirror of the body clock
+    int64_t n = 0;  // steps taken (parity selects the state buffer)
+    std::vector<uint32_t> flags;
+};
+
+}  // namespace
+
+struct titan_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timing = false;
+    std::string err;
+    uint64_t budget = 0, used = 0;
+
+    double dt = 0.0001;                  // src/sim.cu:78
+    double g[3] = {0.0, 0.0, -9.81};     // src/sim.cu:86
+    double T = 0.0;                      // Simulation::T
+    std::vector<double> planes_h, balls_h;
+
+    Ranges chunks, incs, acts, lcs, bids;
+    std::vector<BodyH> bodies;
+
+    // mass arena (ncap = chunks.cap() * 64)
+    int64_t ncap = 0;
+    double *st = nullptr, *m = nullptr, *Tm = nullptr, *acc = nullptr, *ext = nullptr;
+    uint32_t *mflags = nullptr;
+    int32_t *deg = nullptr, *drag = nullptr;
+    int64_t *mlc = nullptr;
+    int32_t *chunk_body = nullptr;
+    int64_t *chunk_ptr = nullptr;
+    // incidence arena
+    uint32_t *inc_nbr = nullptr;
+    double *inc_rest = nullptr, *inc_k = nullptr, *inc_damp = nullptr;
+    int32_t *inc_act = nullptr;
+    double *act = nullptr;
+    LocalC *lc = nullptr;
+    // body table
+    int64_t bcap = 0;
+    double *bT[2] = {nullptr, nullptr};
+    int64_t *bn[2] = {nullptr, nullptr}, *bleft[2] = {nullptr, nullptr}, *bnsync = nullptr;
+    int32_t *b_chunk0 = nullptr, *b_lc = nullptr, *b_lcn = nullptr;
+    double *planes = nullptr, *balls = nullptr;
+    int64_t planes_cap = 0, balls_cap = 0;
+
+    bool kd_known = false, uniform_kd = true;
+    double k_uni = 0, damp_uni = 0;
+    int slot = 0;
+
+    // captured multi-step graphs, valid while `version` is unchanged
+    uint64_t version = 1;
+    struct Graph {
+        hipGraphExec_t exec;
+        uint64_t version;
+        int slot;
+    };
+    std::vector<Graph> graphs;
+    int32_t inject = -1;
+
+    std::map<void *, size_t> allocs;
+};
+
+namespace {
+
+constexpr int kGraphSteps = 64;  // even: a replay leaves the launch slot unchanged
+thread_local std::string g_create_err;
+
+template <class T>
+T *dalloc(titan_ctx *c, int64_t n) {
+    if (n <= 0) return nullptr;
+    const size_t bytes = sizeof(T) * (size_t)n;
+    if (c->budget && c->used + bytes > c->budget)
+        fail(TITAN_E_NOMEM, "HBM budget exhausted: need " + std::to_string(bytes) + " more bytes, " +
+                                std::to_string(c->budget - c->used) + " left");
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) fail(TITAN_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->used += bytes;
+    c->allocs[p] = bytes;
+    return (T *)p;
+}
+
+void dfree(titan_ctx *c, void *p) {
+    if (!p) return;
+    auto it = c->allocs.find(p);
+    if (it != c->allocs.end()) {
+        c->used -= it->second;
+        c->allocs.erase(it);
+    }
+    (void)hipFree(p);
+}
+
+// Reallocate a K-component SoA array from cap_old to cap_new entries per
+// component, keeping the first cap_old entries of each component.
+template <class T>
+void grow_soa(titan_ctx *c, T *&p, int K, int64_t cap_old, int64_t cap_new, bool keep = true) {
+    T *q = dalloc<T>(c, (int64_t)K * cap_new);
+    HIPCHK(hipMemsetAsync(q, 0, sizeof(T) * (size_t)K * cap_new, c->stream));
+    if (p && keep && cap_old > 0) {
+        for (int k = 0; k < K; ++k)
+            HIPCHK(hipMemcpyAsync(q + (int64_t)k * cap_new, p + (int64_t)k * cap_old,
+                                  sizeof(T) * cap_old, hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dfree(c, p);
+    p = q;
+}
+
+template <class T>
+void h2d(titan_ctx *c, T *dst, const T *src, int64_t n) {
+    if (n > 0) HIPCHK(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyHostToDevice, c->stream));
+}
+template <class T>
+void d2h(titan_ctx *c, T *dst, const T *src, int64_t n) {
+    if (n > 0) HIPCHK(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
+}
+
+void drop_graphs(titan_ctx *c) {
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+    c->version++;
+}
+
+void ensure_chunks(titan_ctx *c, int64_t need) {
+    if (c->chunks.cap() >= need) return;
+    const int64_t old = c->chunks.cap();
+    int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 16));
+    const int64_t on = old * kChunk, nn = nc * kChunk;
+    grow_soa(c, c->st, 12, on, nn);
+    grow_soa(c, c->m, 1, on, nn);
+    grow_soa(c, c->Tm, 1, on, nn);
+    grow_soa(c, c->acc, 3, on, nn);
+    grow_soa(c, c->mflags, 1, on, nn);
+    grow_soa(c, c->deg, 1, on, nn);
+    if (c->ext) grow_soa(c, c->ext, 3, on, nn);
+    if (c->drag) grow_soa(c, c->drag, 1, on, nn);
+    if (c->mlc) grow_soa(c, c->mlc, 1, on, nn);
+    // chunk_body of free chunks is -1
+    int32_t *cb = dalloc<int32_t>(c, nc);
+    HIPCHK(hipMemsetAsync(cb, 0xff, sizeof(int32_t) * nc, c->stream));
+    if (c->chunk_body) HIPCHK(hipMemcpyAsync(cb, c->chunk_body, sizeof(int32_t) * old, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dfree(c, c->chunk_body);
+    c->chunk_body = cb;
+    grow_soa(c, c->chunk_ptr, 1, old, nc);
+    c->chunks.grow(nc);
+    c->ncap = nn;
+    drop_graphs(c);
+}
+
+void ensure_incs(titan_ctx *c, int64_t need) {
+    if (c->incs.cap() >= need) return;
+    const int64_t old = c->incs.cap();
+    const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 4096));
+    grow_soa(c, c->inc_nbr, 1, old, nc);
+    grow_soa(c, c->inc_rest, 1, old, nc);
+    if (c->inc_k) grow_soa(c, c->inc_k, 1, old, nc);
+    if (c->inc_damp) grow_soa(c, c->inc_damp, 1, old, nc);
+    if (c->inc_act) {
+        int32_t *q = dalloc<int32_t>(c, nc);
+        HIPCHK(hipMemsetAsync(q, 0xff, sizeof(int32_t) * nc, c->stream));
+        HIPCHK(hipMemcpyAsync(q, c->inc_act, sizeof(int32_t) * old, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        dfree(c, c->inc_act);
+        c->inc_act = q;
+    }
+    c->incs.grow(nc);
+    drop_graphs(c);
+}
+
+void ensure_acts(titan_ctx *c, int64_t need) {
+    if (c->acts.cap() >= need) return;
+    const int64_t old = c->acts.cap();
+    const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 1024));
+    grow_soa(c, c->act, 1, 4 * old, 4 * nc);
+    c->acts.grow(nc);
+    drop_graphs(c);
+}
+
+void ensure_lcs(titan_ctx *c, int64_t need) {
+    if (c->lcs.cap() >= need) return;
+    const int64_t old = c->lcs.cap();
+    const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 64));
+    grow_soa(c, c->lc, 1, old, nc);
+    c->lcs.grow(nc);
+    drop_graphs(c);
+}
+
+void ensure_bodies(titan_ctx *c, int64_t need) {
+    if (c->bcap >= need) return;
+    const int64_t old = c->bcap;
+    const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 64));
+    for (int s = 0; s < 2; ++s) {
+        grow_soa(c, c->bT[s], 1, old, nc);
+        grow_soa(c, c->bn[s], 1, old, nc);
+        grow_soa(c, c->bleft[s], 1, old, nc);
+    }
+    grow_soa(c, c->bnsync, 1, old, nc);
+    grow_soa(c, c->b_lc, 1, old, nc);
+    grow_soa(c, c->b_lcn, 1, old, nc);
+    int32_t *q = dalloc<int32_t>(c, nc);
+    HIPCHK(hipMemsetAsync(q, 0xff, sizeof(int32_t) * nc, c->stream));
+    if (c->b_chunk0) HIPCHK(hipMemcpyAsync(q, c->b_chunk0, sizeof(int32_t) * old, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dfree(c, c->b_chunk0);
+    c->b_chunk0 = q;
+    c->bids.grow(nc);
+    c->bcap = nc;
+    c->bodies.resize(nc);
+    drop_graphs(c);
+}
+
+// Switch from by-value spring constants to per-incidence arrays.
+void make_per_incidence_kd(titan_ctx *c) {
+    if (!c->uniform_kd) return;
+    const int64_t n = c->incs.cap();
+    c->inc_k = dalloc<double>(c, n);
+    c->inc_damp = dalloc<double>(c, n);
+    HIPCHK(launch_fill_f64(c->inc_k, n, c->k_uni, c->stream));
+    HIPCHK(launch_fill_f64(c->inc_damp, n, c->damp_uni, c->stream));
+    c->uniform_kd = false;
+    drop_graphs(c);
+}
+
+void make_act(titan_ctx *c) {
+    if (c->inc_act) return;
+    const int64_t n = c->incs.cap();
+    c->inc_act = dalloc<int32_t>(c, n);
+    HIPCHK(hipMemsetAsync(c->inc_act, 0xff, sizeof(int32_t) * n, c->stream));
+    drop_graphs(c);
+}
+
+bool same_bits(double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; }
+
+int kind_rank(int kind) {
+    // application order of local constraints (src/sim.cu:1118-1132)
+    switch (kind) {
+        case TITAN_CONTACT_PLANE: return 0;
+        case TITAN_BALL: return 1;
+        case TITAN_CONSTRAINT_PLANE: return 2;
+        default: return 3;
+    }
+}
+
+StepArgs make_args(titan_ctx *c, int slot, int write_acc) {
+    StepArgs a{};
+    a.st = c->st;
+    a.ncap = c->ncap;
+    a.m = c->m;
+    a.mflags = c->mflags;
+    a.deg = c->deg;
+    a.ext = c->ext;
+    a.drag = c->drag;
+    a.mlc = c->mlc;
+    a.acc = c->acc;
+    a.chunk_body = c->chunk_body;
+    a.chunk_ptr = c->chunk_ptr;
+    a.nchunks = c->chunks.high_water();
+    a.inc_nbr = c->inc_nbr;
+    a.inc_rest = c->inc_rest;
+    a.inc_k = c->inc_k;
+    a.inc_damp = c->inc_damp;
+    a.inc_act = c->inc_act;
+    a.act = c->act;
+    a.k_uni = c->k_uni;
+    a.damp_uni = c->damp_uni;
+    for (int s = 0; s < 2; ++s) {
+        a.bT[s] = c->bT[s];
+        a.bn[s] = c->bn[s];
+        a.bleft[s] = c->bleft[s];
+    }
+    a.b_chunk0 = c->b_chunk0;
+    a.b_lc = c->b_lc;
+    a.b_lcn = c->b_lcn;
+    a.lc = c->lc;
+    a.planes = c->planes;
+    a.nplanes = (int32_t)(c->planes_h.size() / 8);
+    a.balls = c->balls;
+    a.nballs = (int32_t)(c->balls_h.size() / 4);
+    a.gx = c->g[0];
+    a.gy = c->g[1];
+    a.gz = c->g[2];
+    a.dt = c->dt;
+    a.slot = slot;
+    a.write_acc = write_acc;
+    return a;
+}
+
+void launch(titan_ctx *c, int slot, int write_acc) {
+    StepArgs a = make_args(c, slot, write_acc);
+    HIPCHK(launch_step(a, c->uniform_kd, c->inc_act != nullptr, c->stream));
+}
+
+hipGraphExec_t graph_for(titan_ctx *c, int slot) {
+    for (auto &g : c->graphs)
+        if (g.version == c->version && g.slot == slot) return g.exec;
+    hipGraph_t graph;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int s = slot;
+    for (int i = 0; i < kGraphSteps; ++i) {
+        StepArgs a = make_args(c, s, 0);
+        hipError_t e = launch_step(a, c->uniform_kd, c->inc_act != nullptr, c->stream);
+        if (e != hipSuccess) {
+            hipGraph_t junk;
+            (void)hipStreamEndCapture(c->stream, &junk);
+            fail(TITAN_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        }
+        s ^= 1;
+    }
+    HIPCHK(hipStreamEndCapture(c->stream, &graph));
+    hipGraphExec_t exec;
+    HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    c->graphs.push_back({exec, c->version, slot});
+    return exec;
+}
+
+int64_t live_body_hw(titan_ctx *c) { return c->bids.high_water(); }
+
+void do_step(titan_ctx *c, int64_t n) {
+    if (n < 0) fail(TITAN_E_INVALID, "step: negative step count");
+    if (c->inject >= 0) {
+        if (c->inject == 0) {
+            c->inject = -1;
+            fail(TITAN_E_INJECTED, "step: injected failure");
+        }
+        c->inject--;
+    }
+    if (n == 0) return;
+    const int64_t nb = live_body_hw(c);
+    if (nb > 0 && c->chunks.high_water() > 0) {
+        HIPCHK(launch_set_left(c->bleft[c->slot], c->b_chunk0, nb, n, c->stream));
+        int64_t rest = n - 1;
+        if (rest >= kGraphSteps) {
+            hipGraphExec_t g = graph_for(c, c->slot);
+            for (int64_t r = 0; r < rest / kGraphSteps; ++r) HIPCHK(hipGraphLaunch(g, c->stream));
+            rest %= kGraphSteps;
+        }
+        int s = c->slot;
+        for (int64_t r = 0; r < rest; ++r) {
+            launch(c, s, 0);
+            s ^= 1;
+        }
+        launch(c, s, 1);
+        s ^= 1;
+        HIPCHK(launch_advance_T(c->Tm, c->mflags, c->chunk_body, c->bn[s], c->bnsync,
+                                c->chunks.high_water(), nb, c->dt, c->stream));
+        c->slot = s;
+    }
+    for (auto &b : c->bodies) {
+        if (!b.live) continue;
+        for (int64_t r = 0; r < n; ++r) b.T += c->dt;
+        b.n += n;
+    }
+    for (int64_t r = 0; r < n; ++r) c->T += c->dt;  // src/sim.cu:1590
+}
+
+BodyH &body_of(titan_ctx *c, int32_t id) {
+    if (id < 0 || id >= (int64_t)c->bodies.size() || !c->bodies[id].live)
+        fail(TITAN_E_NOTFOUND, "unknown body id " + std::to_string(id));
+    return c->bodies[id];
+}
+
+int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
+    if (!d) fail(TITAN_E_INVALID, "body_create: null descriptor");
+    const int64_t N = d->n_masses, S = d->n_springs;
+    if (N <= 0) fail(TITAN_E_INVALID, "body_create: a body needs at least one mass");
+    if (S < 0) fail(TITAN_E_INVALID, "body_create: negative spring count");
+    if (!d->pos || !d->m) fail(TITAN_E_INVALID, "body_create: pos and m are required");
+    if (S > 0 && (!d->left || !d->right || !d->k || !d->rest))
+        fail(TITAN_E_INVALID, "body_create: springs need left, right, k and rest");
+    const int64_t nch = (N + kChunk - 1) / kChunk;
+    const int64_t npad = nch * kChunk;
+
+    // degrees and validation
+    std::vector<int32_t> deg(npad, 0);
+    int64_t n_act = 0;
+    bool kd_uni = true;
+    double k0 = 0, d0 = 0;
+    bool first = true;
+    for (int64_t s = 0; s < S; ++s) {
+        const int64_t L = d->left[s], R = d->right[s];
+        if (L >= N || R >= N) fail(TITAN_E_INVALID, "body_create: spring " + std::to_string(s) + " endpoint out of range");
+        if (L < 0 || R < 0) continue;
+        deg[L]++;
+        deg[R]++;
+        const int ty = d->type ? d->type[s] : 0;
+        if (ty < 0 || ty > TITAN_SPRING_GENERAL) fail(TITAN_E_INVALID, "body_create: bad spring type");
+        if (ty >= TITAN_ACTIVE_CONTRACT_THEN_EXPAND) {
+            if (!d->omega) fail(TITAN_E_INVALID, "body_create: actuated springs need omega");
+            if (ty == TITAN_SPRING_GENERAL && (!d->act_a || !d->act_phi))
+                fail(TITAN_E_INVALID, "body_create: general actuation needs act_a and act_phi");
+            n_act++;
+        }
+        const double ks = d->k[s], ds = d->damping ? d->damping[s] : 0.0;
+        if (first) {
+            k0 = ks;
+            d0 = ds;
+            first = false;
+        } else if (!same_bits(ks, k0) || !same_bits(ds, d0)) {
+            kd_uni = false;
+        }
+    }
+    std::vector<int64_t> coff(nch + 1, 0);
+    for (int64_t ch = 0; ch < nch; ++ch) {
+        int32_t w = 0;
+        for (int l = 0; l < kChunk; ++l) w = std::max(w, deg[ch * kChunk + l]);
+        coff[ch + 1] = coff[ch] + (int64_t)w * kChunk;
+    }
+    const int64_t n_inc = coff[nch];
+
+    // local constraints, sorted stably into application order
+    std::vector<std::vector<LocalC>> lists;
+    bool any_lc = false, shared = false;
+    if (d->lc_ptr) {
+        if (!d->lc_type || !d->lc_vec || !d->lc_num) fail(TITAN_E_INVALID, "body_create: incomplete local constraints");
+        lists.resize(N);
+        for (int64_t i = 0; i < N; ++i) {
+            for (int64_t j = d->lc_ptr[i]; j < d->lc_ptr[i + 1]; ++j) {
+                LocalC e{};
+                e.v[0] = d->lc_vec[3 * j];
+                e.v[1] = d->lc_vec[3 * j + 1];
+                e.v[2] = d->lc_vec[3 * j + 2];
+                e.num = d->lc_num[j];
+                e.kind = d->lc_type[j];
+                if (e.kind < 0 || e.kind > 3) fail(TITAN_E_INVALID, "body_create: bad local constraint kind");
+                lists[i].push_back(e);
+            }
+            std::stable_sort(lists[i].begin(), lists[i].end(),
+                             [](const LocalC &a, const LocalC &b) { return kind_rank(a.kind) < kind_rank(b.kind); });
+            if (lists[i].size() > 0xFFFFF) fail(TITAN_E_INVALID, "body_create: too many local constraints on one mass");
+            if (!lists[i].empty()) any_lc = true;
+        }
+        if (any_lc) {
+            shared = true;
+            for (int64_t i = 1; i < N && shared; ++i) {
+                if (lists[i].size() != lists[0].size() ||
+                    std::memcmp(lists[i].data(), lists[0].data(), sizeof(LocalC) * lists[0].size()) != 0)
+                    shared = false;
+            }
+        }
+    }
+    int64_t n_lc = 0;
+    if (any_lc) {
+        if (shared) n_lc = (int64_t)lists[0].size();
+        else for (auto &l : lists) n_lc += (int64_t)l.size();
+    }
+
+    // reserve space; on failure release whatever was taken
+    int64_t ch0 = -1, inc0 = -1, act0 = -1, lc0 = -1, bid = -1;
+    try {
+        ch0 = c->chunks.alloc(nch);
+        if (ch0 < 0) {
+            ensure_chunks(c, c->chunks.high_water() + nch);
+            ch0 = c->chunks.alloc(nch);
+        }
+        if ((ch0 + nch) * kChunk > (int64_t)0x7fffffff)
+            fail(TITAN_E_NOMEM, "body_create: mass index space (2^31) exhausted");
+        inc0 = c->incs.alloc(n_inc);
+        if (inc0 < 0) {
+            ensure_incs(c, c->incs.high_water() + n_inc);
+            inc0 = c->incs.alloc(n_inc);
+        }
+        act0 = c->acts.alloc(n_act);
+        if (act0 < 0) {
+            ensure_acts(c, c->acts.high_water() + n_act);
+            act0 = c->acts.alloc(n_act);
+        }
+        lc0 = c->lcs.alloc(n_lc);
+        if (lc0 < 0) {
+            ensure_lcs(c, c->lcs.high_water() + n_lc);
+            lc0 = c->lcs.alloc(n_lc);
+        }
+        bid = c->bids.alloc(1);
+        if (bid < 0) {
+            ensure_bodies(c, c->bids.high_water() + 1);
+            bid = c->bids.alloc(1);
+        }
+        if (S > 0) {
+            if (!c->kd_known) {
+                c->kd_known = true;
+                c->k_uni = k0;
+                c->damp_uni = d0;
+                c->uniform_kd = true;
+                if (!kd_uni) make_per_incidence_kd(c);
+            } else if (c->uniform_kd && (!kd_uni || !same_bits(k0, c->k_uni) || !same_bits(d0, c->damp_uni))) {
+                make_per_incidence_kd(c);
+            }
+        }
+        if (n_act > 0) make_act(c);
+    } catch (...) {
+        if (ch0 >= 0) c->chunks.release(ch0, nch);
+        if (inc0 >= 0) c->incs.release(inc0, n_inc);
+        if (act0 >= 0) c->acts.release(act0, n_act);
+        if (lc0 >= 0) c->lcs.release(lc0, n_lc);
+        if (bid >= 0) c->bids.release(bid, 1);
+        throw;
+    }
+
+    // stage mass arrays
+    const int64_t g0 = ch0 * kChunk;
+    std::vector<double> st(6 * npad, 0.0), mm(npad, 1.0), Tm(npad, 0.0);
+    std::vector<uint32_t> fl(npad, kPad);
+    std::vector<double> ext;
+    std::vector<int32_t> drag;
+    std::vector<int64_t> mlc;
+    bool any_ext = false, any_drag = false;
+    for (int64_t i = 0; i < N; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            st[k * npad + i] = d->pos[3 * i + k];
+            st[(3 + k) * npad + i] = d->vel ? d->vel[3 * i + k] : 0.0;
+        }
+        mm[i] = d->m[i];
+        Tm[i] = d->T ? d->T[i] : 0.0;
+        uint32_t f = 0;
+        if (d->fixed && d->fixed[i]) f |= kFixed;
+        if (d->ext && (d->ext[3 * i] != 0 || d->ext[3 * i + 1] != 0 || d->ext[3 * i + 2] != 0)) {
+            f |= kExt;
+            any_ext = true;
+        }
+        if (d->drag && d->drag[i] != 0) {
+            f |= kDrag;
+            any_drag = true;
+        }
+        fl[i] = f;
+    }
+    if (any_ext) {
+        if (!c->ext) {
+            c->ext = dalloc<double>(c, 3 * c->ncap);
+            HIPCHK(hipMemsetAsync(c->ext, 0, sizeof(double) * 3 * c->ncap, c->stream));
+        }
+        ext.assign(3 * npad, 0.0);
+        for (int64_t i = 0; i < N; ++i)
+            for (int k = 0; k < 3; ++k) ext[k * npad + i] = d->ext[3 * i + k];
+    }
+    if (any_drag) {
+        if (!c->drag) {
+            c->drag = dalloc<int32_t>(c, c->ncap);
+            HIPCHK(hipMemsetAsync(c->drag, 0, sizeof(int32_t) * c->ncap, c->stream));
+        }
+        drag.assign(npad, 0);
+        for (int64_t i = 0; i < N; ++i) drag[i] = d->drag[i];
+    }
+    std::vector<LocalC> lch;
+    if (any_lc) {
+        if (shared) {
+            lch = lists[0];
+        } else {
+            if (!c->mlc) {
+                c->mlc = dalloc<int64_t>(c, c->ncap);
+                HIPCHK(hipMemsetAsync(c->mlc, 0, sizeof(int64_t) * c->ncap, c->stream));
+            }
+            mlc.assign(npad, 0);
+            for (int64_t i = 0; i < N; ++i) {
+                if (lists[i].empty()) continue;
+                mlc[i] = ((lc0 + (int64_t)lch.size()) << 20) | (int64_t)lists[i].size();
+                fl[i] |= kLocalLC;
+                lch.insert(lch.end(), lists[i].begin(), lists[i].end());
+            }
+        }
+    }
+
+    // SELL-64 incidences, ascending spring index per mass
+    std::vector<uint32_t> nbr(n_inc, 0);
+    std::vector<double> rest(n_inc, 0.0), kk, dd;
+    std::vector<int32_t> ia;
+    std::vector<double> acth;
+    if (!c->uniform_kd) {
+        kk.assign(n_inc, 0.0);
+        dd.assign(n_inc, 0.0);
+    }
+    if (c->inc_act) ia.assign(n_inc, -1);
+    std::vector<int32_t> fillc(npad, 0);
+    for (int64_t s = 0; s < S; ++s) {
+        const int64_t L = d->left[s], R = d->right[s];
+        if (L < 0 || R < 0) continue;
+        int32_t ai = -1;
+        const int ty = d->type ? d->type[s] : 0;
+        if (ty >= TITAN_ACTIVE_CONTRACT_THEN_EXPAND) {
+            // the reference's two kinds are the general law with a = -/+0.2, phi = 0
+            // (src/sim.cu:1058-1062): 1 + (-0.2) s == 1 - 0.2 s and x + 0 == x exactly
+            const double aa = ty == TITAN_ACTIVE_CONTRACT_THEN_EXPAND ? -0.2
+                              : ty == TITAN_ACTIVE_EXPAND_THEN_CONTRACT ? 0.2 : d->act_a[s];
+            const double ph = ty == TITAN_SPRING_GENERAL ? d->act_phi[s] : 0.0;
+            ai = (int32_t)(act0 + (int64_t)acth.size() / 4);
+            acth.insert(acth.end(), {aa, d->omega[s], ph, 0.0});
+        }
+        auto place = [&](int64_t me, int64_t other, bool is_left) {
+            const int64_t e = coff[me / kChunk] + (int64_t)fillc[me] * kChunk + (me % kChunk);
+            fillc[me]++;
+            nbr[e] = (uint32_t)(g0 + other) | (is_left ? kLeftBit : 0u);
+            rest[e] = d->rest[s];
+            if (!kk.empty()) {
+                kk[e] = d->k[s];
+                dd[e] = d->damping ? d->damping[s] : 0.0;
+            }
+            if (!ia.empty()) ia[e] = ai;
+        };
+        place(R, L, false);  // the scatter adds +F to the right endpoint first
+        place(L, R, true);
+    }
+    std::vector<int32_t> cbody(nch, (int32_t)bid);
+    std::vector<int64_t> cptr(nch);
+    for (int64_t ch = 0; ch < nch; ++ch) cptr[ch] = inc0 + coff[ch];
+
+    // upload (buffer parity 0: a new body has taken no steps)
+    for (int k = 0; k < 6; ++k) h2d(c, c->st + k * c->ncap + g0, st.data() + k * npad, npad);
+    h2d(c, c->m + g0, mm.data(), npad);
+    h2d(c, c->Tm + g0, Tm.data(), npad);
+    h2d(c, c->mflags + g0, fl.data(), npad);
+    h2d(c, c->deg + g0, deg.data(), npad);
+    std::vector<double> zeros(3 * npad, 0.0);
+    for (int k = 0; k < 3; ++k) h2d(c, c->acc + k * c->ncap + g0, zeros.data(), npad);
+    if (c->ext) {
+        const double *src = any_ext ? ext.data() : zeros.data();
+        for (int k = 0; k < 3; ++k) h2d(c, c->ext + k * c->ncap + g0, src + (any_ext ? k * npad : 0), npad);
+    }
+    if (c->drag) {
+        std::vector<int32_t> z(npad, 0);
+        h2d(c, c->drag + g0, any_drag ? drag.data() : z.data(), npad);
+    }
+    if (c->mlc) {
+        std::vector<int64_t> z(npad, 0);
+        h2d(c, c->mlc + g0, mlc.empty() ? z.data() : mlc.data(), npad);
+    }
+    h2d(c, c->chunk_body + ch0, cbody.data(), nch);
+    h2d(c, c->chunk_ptr + ch0, cptr.data(), nch);
+    h2d(c, c->inc_nbr + inc0, nbr.data(), n_inc);
+    h2d(c, c->inc_rest + inc0, rest.data(), n_inc);
+    if (!kk.empty()) {
+        h2d(c, c->inc_k + inc0, kk.data(), n_inc);
+        h2d(c, c->inc_damp + inc0, dd.data(), n_inc);
+    }
+    if (!ia.empty()) h2d(c, c->inc_act + inc0, ia.data(), n_inc);
+    if (!acth.empty()) h2d(c, c->act + 4 * act0, acth.data(), (int64_t)acth.size());
+    if (!lch.empty()) h2d(c, c->lc + lc0, lch.data(), (int64_t)lch.size());
+    const double t0 = d->t0;
+    const int64_t zero = 0;
+    const int32_t c0 = (int32_t)ch0, lcb = (int32_t)(shared ? lc0 : 0), lcn = (int32_t)(shared ? n_lc : 0);
+    for (int s = 0; s < 2; ++s) {
+        h2d(c, c->bT[s] + bid, &t0, 1);
+        h2d(c, c->bn[s] + bid, &zero, 1);
+        h2d(c, c->bleft[s] + bid, &zero, 1);
+    }
+    h2d(c, c->bnsync + bid, &zero, 1);
+    h2d(c, c->b_chunk0 + bid, &c0, 1);
+    h2d(c, c->b_lc + bid, &lcb, 1);
+    h2d(c, c->b_lcn + bid, &lcn, 1);
+    HIPCHK(hipStreamSynchronize(c->stream));
+
+    BodyH &b = c->bodies[bid];
+    b.live = true;
+    b.n_masses = N;
+    b.chunk0 = ch0;
+    b.nchunks = nch;
+    b.inc0 = inc0;
+    b.inc_len = n_inc;
+    b.act0 = act0;
+    b.act_len = n_act;
+    b.lc0 = lc0;
+    b.lc_len = n_lc;
+    b.T = t0;
+    b.n = 0;
+    b.flags.assign(fl.begin(), fl.begin() + N);
+    drop_graphs(c);  // the launch grid covers the new chunks
+    return (int32_t)bid;
+}
+
+void body_destroy(titan_ctx *c, int32_t id) {
+    BodyH &b = body_of(c, id);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<int32_t> neg(b.nchunks, -1);
+    h2d(c, c->chunk_body + b.chunk0, neg.data(), b.nchunks);
+    const int32_t m1 = -1;
+    h2d(c, c->b_chunk0 + id, &m1, 1);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->chunks.release(b.chunk0, b.nchunks);
+    c->incs.release(b.inc0, b.inc_len);
+    c->acts.release(b.act0, b.act_len);
+    c->lcs.release(b.lc0, b.lc_len);
+    c->bids.release(id, 1);
+    b = BodyH{};
+    drop_graphs(c);
+}
+
+void body_download(titan_ctx *c, int32_t id, double *pos, double *vel, double *acc, double *T) {
+    BodyH &b = body_of(c, id);
+    const int64_t N = b.n_masses, g0 = b.chunk0 * kChunk, n6 = c->ncap;
+    const int p = (int)(b.n & 1);
+    std::vector<double> tmp(3 * N);
+    auto fetch3 = [&](const double *base, double *out) {
+        for (int k = 0; k < 3; ++k) d2h(c, tmp.data() + k * N, base + k * n6 + g0, N);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int64_t i = 0; i < N; ++i)
+            for (int k = 0; k < 3; ++k) out[3 * i + k] = tmp[k * N + i];
+    };
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (pos) fetch3(c->st + (int64_t)p * 6 * n6, pos);
+    if (vel) fetch3(c->st + (int64_t)p * 6 * n6 + 3 * n6, vel);
+    if (acc) fetch3(c->acc, acc);
+    if (T) {
+        d2h(c, T, c->Tm + g0, N);
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+}
+
+void body_upload(titan_ctx *c, int32_t id, const double *pos, const double *vel, const double *m,
+                 const double *T, const double *ext) {
+    BodyH &b = body_of(c, id);
+    const int64_t N = b.n_masses, g0 = b.chunk0 * kChunk, n6 = c->ncap;
+    const int p = (int)(b.n & 1);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<double> tmp(3 * N);
+    auto put3 = [&](double *base, const double *in) {
+        for (int64_t i = 0; i < N; ++i)
+            for (int k = 0; k < 3; ++k) tmp[k * N + i] = in[3 * i + k];
+        for (int k = 0; k < 3; ++k) h2d(c, base + k * n6 + g0, tmp.data() + k * N, N);
+        HIPCHK(hipStreamSynchronize(c->stream));
+    };
+    if (pos) put3(c->st + (int64_t)p * 6 * n6, pos);
+    if (vel) put3(c->st + (int64_t)p * 6 * n6 + 3 * n6, vel);
+    if (m) {
+        h2d(c, c->m + g0, m, N);
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    if (T) {
+        h2d(c, c->Tm + g0, T, N);
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    if (ext) {
+        bool changed = false;
+        for (int64_t i = 0; i < N; ++i) {
+            const bool nz = ext[3 * i] != 0 || ext[3 * i + 1] != 0 || ext[3 * i + 2] != 0;
+            const uint32_t f = nz ? (b.flags[i] | kExt) : (b.flags[i] & ~kExt);
+            if (f != b.flags[i]) {
+                b.flags[i] = f;
+                changed = true;
+            }
+        }
+        if (!c->ext) {
+            c->ext = dalloc<double>(c, 3 * c->ncap);
+            HIPCHK(hipMemsetAsync(c->ext, 0, sizeof(double) * 3 * c->ncap, c->stream));
+        }
+        put3(c->ext, ext);
+        if (changed) {
+            h2d(c, c->mflags + g0, b.flags.data(), N);
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+    }
+}
+
+void set_err(const titan_ctx *c, const std::string &m) {
+    if (c) const_cast<titan_ctx *>(c)->err = m;
+    else g_create_err = m;
+}
+
+}  // namespace
+
+#define API_BEGIN try {
+#define API_END(C)                                                  \
+    }                                                               \
+    catch (const Fail &f) {                                         \
+        set_err(C, f.msg);                                          \
+        return f.code;                                              \
+    }                                                               \
+    catch (const std::bad_alloc &) {                                \
+        set_err(C, "host allocation failed");                       \
+        return TITAN_E_NOMEM;                                       \
+    }                                                               \
+    catch (const std::exception &e) {                               \
+        set_err(C, e.what());                                       \
+        return TITAN_E_INVALID;                                     \
+    }                                                               \
+    return TITAN_OK;
+
+#define NEED_CTX(C) \
+    if (!(C)) {     \
+        g_create_err = "null context"; \
+        return TITAN_E_INVALID; \
+    }
+
+extern "C" {
+
+int titan_ctx_create(int device, uint64_t hbm_budget_bytes, titan_ctx **out) {
+    if (!out) {
+        g_create_err = "ctx_create: null output pointer";
+        return TITAN_E_INVALID;
+    }
+    *out = nullptr;
+    titan_ctx *c = nullptr;
+    API_BEGIN
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) fail(TITAN_E_INVALID, "ctx_create: no HIP device " + std::to_string(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        fail(TITAN_E_INVALID, std::string("ctx_create: this library is built for gfx950, device is ") + prop.gcnArchName);
+    HIPCHK(hipSetDevice(device));
+    c = new titan_ctx();
+    c->device = device;
+    c->budget = hbm_budget_bytes;
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&c->ev0));
+    HIPCHK(hipEventCreate(&c->ev1));
+    ensure_bodies(c, 64);
+    c->planes = dalloc<double>(c, 8 * 16);
+    c->planes_cap = 16;
+    c->balls = dalloc<double>(c, 4 * 16);
+    c->balls_cap = 16;
+    *out = c;
+    c = nullptr;
+    }
+    catch (const Fail &f) {
+        g_create_err = f.msg;
+        if (c) titan_ctx_destroy(c);
+        return f.code;
+    }
+    catch (const std::exception &e) {
+        g_create_err = e.what();
+        if (c) titan_ctx_destroy(c);
+        return TITAN_E_INVALID;
+    }
+    return TITAN_OK;
+}
+
+int titan_ctx_destroy(titan_ctx *c) {
+    if (!c) return TITAN_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    for (auto &a : c->allocs) (void)hipFree(a.first);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return TITAN_OK;
+}
+
+const char *titan_last_error(const titan_ctx *c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+int titan_set_dt(titan_ctx *c, double dt) {
+    NEED_CTX(c);
+    API_BEGIN
+    if (!(dt > 0)) fail(TITAN_E_INVALID, "Cannot set time step to negative or zero value.");  // src/sim.cu:802-804
+    c->dt = dt;
+    drop_graphs(c);
+    API_END(c)
+}
+
+int titan_get_dt(const titan_ctx *c, double *dt) {
+    NEED_CTX(c);
+    if (!dt) return TITAN_E_INVALID;
+    *dt = c->dt;
+    return TITAN_OK;
+}
+
+int titan_set_global_acc(titan_ctx *c, double gx, double gy, double gz) {
+    NEED_CTX(c);
+    API_BEGIN
+    c->g[0] = gx;
+    c->g[1] = gy;
+    c->g[2] = gz;
+    drop_graphs(c);
+    API_END(c)
+}
+
+int titan_add_plane(titan_ctx *c, double nx, double ny, double nz, double d, double mu_k, double mu_s) {
+    NEED_CTX(c);
+    API_BEGIN
+    c->planes_h.insert(c->planes_h.end(), {nx, ny, nz, d, mu_k, mu_s, 0.0, 0.0});
+    const int64_t n = (int64_t)c->planes_h.size() / 8;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (n > c->planes_cap) {
+        dfree(c, c->planes);
+        c->planes_cap = 2 * n;
+        c->planes = dalloc<double>(c, 8 * c->planes_cap);
+    }
+    h2d(c, c->planes, c->planes_h.data(), (int64_t)c->planes_h.size());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    API_END(c)
+}
+
+int titan_add_ball(titan_ctx *c, double cx, double cy, double cz, double r) {
+    NEED_CTX(c);
+    API_BEGIN
+    c->balls_h.insert(c->balls_h.end(), {cx, cy, cz, r});
+    const int64_t n = (int64_t)c->balls_h.size() / 4;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (n > c->balls_cap) {
+        dfree(c, c->balls);
+        c->balls_cap = 2 * n;
+        c->balls = dalloc<double>(c, 4 * c->balls_cap);
+    }
+    h2d(c, c->balls, c->balls_h.data(), (int64_t)c->balls_h.size());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    API_END(c)
+}
+
+int titan_clear_planes_and_balls(titan_ctx *c) {
+    NEED_CTX(c);
+    API_BEGIN
+    c->planes_h.clear();
+    c->balls_h.clear();
+    drop_graphs(c);
+    API_END(c)
+}
+
+int titan_body_create(titan_ctx *c, const titan_body_desc *desc, int32_t *body_id) {
+    NEED_CTX(c);
+    API_BEGIN
+    int32_t id = body_create(c, desc);
+    if (body_id) *body_id = id;
+    API_END(c)
+}
+
+int titan_body_destroy(titan_ctx *c, int32_t body_id) {
+    NEED_CTX(c);
+    API_BEGIN
+    body_destroy(c, body_id);
+    API_END(c)
+}
+
+int titan_body_count(const titan_ctx *c, int32_t *n) {
+    NEED_CTX(c);
+    if (!n) return TITAN_E_INVALID;
+    int32_t k = 0;
+    for (auto &b : c->bodies) k += b.live ? 1 : 0;
+    *n = k;
+    return TITAN_OK;
+}
+
+int titan_step(titan_ctx *c, int64_t n_steps) {
+    NEED_CTX(c);
+    API_BEGIN
+    do_step(c, n_steps);
+    API_END(c)
+}
+
+int titan_step_until(titan_ctx *c, double t, int64_t *steps_done) {
+    NEED_CTX(c);
+    API_BEGIN
+    if (std::isnan(t)) fail(TITAN_E_INVALID, "step_until: time is NaN");
+    // the driver stops at the first loop iteration with breakpoint <= T
+    // (src/sim.cu:1505); count the steps with the same T += dt recurrence
+    double T = c->T;
+    int64_t k = 0;
+    while (!(t <= T)) {
+        T += c->dt;
+        if (++k > (int64_t)1 << 40) fail(TITAN_E_INVALID, "step_until: target time too far ahead");
+    }
+    do_step(c, k);
+    if (steps_done) *steps_done = k;
+    API_END(c)
+}
+
+int titan_time(const titan_ctx *c, double *T) {
+    NEED_CTX(c);
+    if (!T) return TITAN_E_INVALID;
+    *T = c->T;
+    return TITAN_OK;
+}
+
+int titan_sync(titan_ctx *c) {
+    NEED_CTX(c);
+    API_BEGIN
+    HIPCHK(hipStreamSynchronize(c->stream));
+    API_END(c)
+}
+
+int titan_body_download(titan_ctx *c, int32_t body_id, double *pos, double *vel, double *acc, double *T) {
+    NEED_CTX(c);
+    API_BEGIN
+    body_download(c, body_id, pos, vel, acc, T);
+    API_END(c)
+}
+
+int titan_body_upload(titan_ctx *c, int32_t body_id, const double *pos, const double *vel,
+                      const double *m, const double *T, const double *ext) {
+    NEED_CTX(c);
+    API_BEGIN
+    body_upload(c, body_id, pos, vel, m, T, ext);
+    API_END(c)
+}
+
+int titan_timer_start(titan_ctx *c) {
+    NEED_CTX(c);
+    API_BEGIN
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    c->timing = true;
+    API_END(c)
+}
+
+int titan_timer_stop(titan_ctx *c, float *ms) {
+    NEED_CTX(c);
+    API_BEGIN
+    if (!c->timing) fail(TITAN_E_INVALID, "timer_stop: timer not started");
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float v = 0;
+    HIPCHK(hipEventElapsedTime(&v, c->ev0, c->ev1));
+    c->timing = false;
+    if (ms) *ms = v;
+    API_END(c)
+}
+
+int titan_mem_used(const titan_ctx *c, uint64_t *bytes) {
+    NEED_CTX(c);
+    if (!bytes) return TITAN_E_INVALID;
+    *bytes = c->used;
+    return TITAN_OK;
+}
+
+int titan_kernel_name(const titan_ctx *c, char *buf, int64_t len) {
+    NEED_CTX(c);
+    if (!buf || len <= 0) return TITAN_E_INVALID;
+    std::snprintf(buf, (size_t)len, "%s", step_kernel_name(c->uniform_kd, c->inc_act != nullptr));
+    return TITAN_OK;
+}
+
+int titan_debug_inject_failure(titan_ctx *c, int32_t after) {
+    NEED_CTX(c);
+    c->inject = after < 0 ? -1 : after;
+    return TITAN_OK;
+}
+
+}  // extern "C"

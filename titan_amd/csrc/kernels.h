@@ -1,0 +1,19 @@
+// Launchers for the step kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "layout.h"
+
+namespace titan_mi355x {
+
+hipError_t launch_step(const StepArgs &a, bool uniform_kd, bool actuated, hipStream_t s);
+const char *step_kernel_name(bool uniform_kd, bool actuated);
+hipError_t launch_advance_T(double *Tm, const uint32_t *mflags, const int32_t *chunk_body,
+                            const int64_t *bn, int64_t *bnsync, int64_t nchunks, int64_t nb,
+                            double dt, hipStream_t s);
+hipError_t launch_set_left(int64_t *bleft, const int32_t *b_chunk0, int64_t nb, int64_t n,
+                           hipStream_t s);
+hipError_t launch_fill_f64(double *p, int64_t n, double v, hipStream_t s);
+
+}  // namespace titan_mi355x

@@ -1,0 +1,385 @@
+// Step kernels for gfx950 (MI355X): the FP64 mass-spring step as one fused
+// pass per step.  Replaces the reference's computeSpringForces (one thread
+// per spring, FP64 CAS-loop scatter, src/sim.cu:1046-1080) and
+// massForcesAndUpdate (one thread per mass over 200-byte AoS records,
+// src/sim.cu:1090-1169) with a mass-centric gather: one lane per mass walks
+// its incident springs in ascending spring index (SELL-64 incidence order),
+// recomputes each spring force exactly as the reference writes it, sums the
+// forces in that fixed order, then applies gravity, extern force, global
+// planes and balls, local constraints, drag and the symplectic Euler update
+// in registers and writes the other state buffer.
+//
+// Every FP64 expression keeps the reference's association and operand order
+// and the library is built with -ffp-contract=off, so a passive spring
+// network gives bit-identical states to a serial run of the reference's
+// arithmetic summed in ascending spring index.  sqrt and division are the
+// correctly rounded IEEE operations; sin (actuated springs only) is the
+// device math library's.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "layout.h"
+
+namespace titan_mi355x {
+
+namespace {
+
+constexpr double kNormal = 20000.0;  // src/object.cu:26
+
+// Blocks are dispatched round-robin over the 8 XCDs (block w runs on XCD
+// w % 8).  Map them so that each XCD walks one contiguous range of chunks:
+// lattice neighbours a plane apart are then read through the same L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t w, int64_t G) {
+    const int64_t q = G >> 3, r = G & 7;
+    const int64_t x = w & 7, k = w >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+struct MassRegs {
+    double px, py, pz, vx, vy, vz, fx, fy, fz;
+};
+
+// CudaContactPlane::applyForce, src/object.cu:73-106.  For disp >= 0 the
+// reference adds 0 * normal, which changes nothing but the sign of an
+// exactly zero force component; that addition is skipped.
+__device__ __forceinline__ void contact_plane(MassRegs &r, double nx, double ny, double nz,
+                                              double off, double mu_k, double mu_s) {
+    const double disp = (r.px * nx + r.py * ny + r.pz * nz) - off;
+    if (disp < 0) {
+        const double dF = r.fx * nx + r.fy * ny + r.fz * nz;
+        const double fnx = nx * dF, fny = ny * dF, fnz = nz * dF;
+        if (mu_s > 0 || mu_k > 0) {
+            const double dv = r.vx * nx + r.vy * ny + r.vz * nz;
+            const double vpx = r.vx - nx * dv, vpy = r.vy - ny * dv, vpz = r.vz - nz * dv;
+            const double vn = sqrt(vpx * vpx + vpy * vpy + vpz * vpz);
+            if (vn > 1e-16) {
+                const double fm = mu_k * sqrt(fnx * fnx + fny * fny + fnz * fnz);
+                r.fx = r.fx - (vpx * fm) / vn;
+                r.fy = r.fy - (vpy * fm) / vn;
+                r.fz = r.fz - (vpz * fm) / vn;
+            } else {
+                const double fpx = r.fx - fnx, fpy = r.fy - fny, fpz = r.fz - fnz;
+                if (mu_s * sqrt(fnx * fnx + fny * fny + fnz * fnz) >
+                    sqrt(fpx * fpx + fpy * fpy + fpz * fpz)) {
+                    r.fx = r.fx - fpx;
+                    r.fy = r.fy - fpy;
+                    r.fz = r.fz - fpz;
+                }
+            }
+        }
+        const double c = -disp * kNormal;
+        r.fx = r.fx + nx * c;
+        r.fy = r.fy + ny * c;
+        r.fz = r.fz + nz * c;
+    }
+}
+
+// CudaBall::applyForce, src/object.cu:53-56 (constant magnitude NORMAL).
+__device__ __forceinline__ void ball(MassRegs &r, double cx, double cy, double cz, double rad) {
+    const double dx = r.px - cx, dy = r.py - cy, dz = r.pz - cz;
+    const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+    if (dist <= rad) {
+        r.fx = r.fx + (dx * kNormal) / dist;
+        r.fy = r.fy + (dy * kNormal) / dist;
+        r.fz = r.fz + (dz * kNormal) / dist;
+    }
+}
+
+// CudaConstraintPlane::applyForce, src/object.cu:115-124.
+__device__ __forceinline__ void constraint_plane(MassRegs &r, double nx, double ny, double nz,
+                                                 double fr) {
+    const double nf = r.fx * nx + r.fy * ny + r.fz * nz;
+    r.fx = r.fx + (-nx) * nf;
+    r.fy = r.fy + (-ny) * nf;
+    r.fz = r.fz + (-nz) * nf;
+    const double vn = sqrt(r.vx * r.vx + r.vy * r.vy + r.vz * r.vz);
+    if (vn >= 1e-16) {
+        const double vd = r.vx * nx + r.vy * ny + r.vz * nz;
+        r.vx = r.vx + (-nx) * vd;
+        r.vy = r.vy + (-ny) * vd;
+        r.vz = r.vz + (-nz) * vd;
+        const double c = -fr * nf;
+        r.fx = r.fx + (r.vx * c) / vn;
+        r.fy = r.fy + (r.vy * c) / vn;
+        r.fz = r.fz + (r.vz * c) / vn;
+    }
+}
+
+// CudaDirection::applyForce, src/object.cu:133-141.
+__device__ __forceinline__ void direction(MassRegs &r, double tx, double ty, double tz,
+                                          double fr) {
+    const double dft = r.fx * tx + r.fy * ty + r.fz * tz;
+    const double nfx = r.fx - tx * dft, nfy = r.fy - ty * dft, nfz = r.fz - tz * dft;
+    r.fx = r.fx + (-nfx);
+    r.fy = r.fy + (-nfy);
+    r.fz = r.fz + (-nfz);
+    if (sqrt(r.vx * r.vx + r.vy * r.vy + r.vz * r.vz) >= 1e-16) {
+        const double dvt = r.vx * tx + r.vy * ty + r.vz * tz;
+        r.vx = tx * dvt;
+        r.vy = ty * dvt;
+        r.vz = tz * dvt;
+        const double c = -sqrt(nfx * nfx + nfy * nfy + nfz * nfz) * fr;
+        r.fx = r.fx + tx * c;
+        r.fy = r.fy + ty * c;
+        r.fz = r.fz + tz * c;
+    }
+}
+
+__device__ __forceinline__ void apply_local(MassRegs &r, const LocalC &c) {
+    switch (c.kind) {
+        case 1:  // CONTACT_PLANE: local planes carry no friction (src/object.cu:58-63)
+            contact_plane(r, c.v[0], c.v[1], c.v[2], c.num, 0.0, 0.0);
+            break;
+        case 2:
+            ball(r, c.v[0], c.v[1], c.v[2], c.num);
+            break;
+        case 0:
+            constraint_plane(r, c.v[0], c.v[1], c.v[2], c.num);
+            break;
+        default:
+            direction(r, c.v[0], c.v[1], c.v[2], c.num);
+            break;
+    }
+}
+
+// Everything of massForcesAndUpdate after the spring sum (src/sim.cu:1104-1167,
+// Euler branch), applied to registers.  Returns the acceleration.
+__device__ __forceinline__ void mass_pass(const StepArgs &a, MassRegs &r, int64_t i, uint32_t fl,
+                                          int b, double m, double &ax, double &ay, double &az) {
+    r.fx = r.fx + a.gx * m;                                    // :1104
+    r.fy = r.fy + a.gy * m;
+    r.fz = r.fz + a.gz * m;
+    if (fl & kExt) {                                           // :1105
+        r.fx = r.fx + a.ext[i];
+        r.fy = r.fy + a.ext[a.ncap + i];
+        r.fz = r.fz + a.ext[2 * a.ncap + i];
+    }
+    for (int q = 0; q < a.nplanes; ++q) {                      // :1109-1111
+        const double *P = a.planes + 8 * q;
+        contact_plane(r, P[0], P[1], P[2], P[3], P[4], P[5]);
+    }
+    for (int q = 0; q < a.nballs; ++q) {                       // :1113-1115
+        const double *B = a.balls + 4 * q;
+        ball(r, B[0], B[1], B[2], B[3]);
+    }
+    int64_t lb, ln;                                            // :1118-1132
+    if (fl & kLocalLC) {
+        const int64_t v = a.mlc[i];
+        lb = v >> 20;
+        ln = v & 0xFFFFF;
+    } else {
+        lb = a.b_lc[b];
+        ln = a.b_lcn[b];
+    }
+    for (int64_t q = 0; q < ln; ++q) apply_local(r, a.lc[lb + q]);
+    if (fl & kDrag) {                                          // :1135-1138
+        const double vn = sqrt(r.vx * r.vx + r.vy * r.vy + r.vz * r.vz);
+        if (vn != 0.0) {
+            const double c = (double)(-a.drag[i]) * (vn * vn);
+            r.fx = r.fx + (r.vx * c) / vn;
+            r.fy = r.fy + (r.vy * c) / vn;
+            r.fz = r.fz + (r.vz * c) / vn;
+        }
+    }
+    ax = r.fx / m;                                             // :1162-1164
+    ay = r.fy / m;
+    az = r.fz / m;
+    r.vx = r.vx + ax * a.dt;
+    r.vy = r.vy + ay * a.dt;
+    r.vz = r.vz + az * a.dt;
+    r.px = r.px + r.vx * a.dt;
+    r.py = r.py + r.vy * a.dt;
+    r.pz = r.pz + r.vz * a.dt;
+}
+
+}  // namespace
+
+// One lane per mass slot, one wavefront per 64-mass chunk, four chunks per
+// workgroup.  UNI: spring constant and damping are the same for every
+// spring (passed by value).  ACT: some springs are actuated.
+template <bool UNI, bool ACT>
+__global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
+    const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t chunk = blk * kWavesPerBlock + uniform(threadIdx.x >> 6);
+    if (chunk >= a.nchunks) return;
+    const int lane = threadIdx.x & (kChunk - 1);
+    const int b = uniform(a.chunk_body[chunk]);
+    if (b < 0) return;
+    const int s = a.slot;
+    const int64_t left = a.bleft[s][b];
+    const int64_t nst = a.bn[s][b];
+    const double T = a.bT[s][b];
+    if (lane == 0 && a.b_chunk0[b] == chunk) {  // the body's first lane advances its clock
+        const bool go = left > 0;
+        a.bT[s ^ 1][b] = go ? T + a.dt : T;     // src/sim.cu:1590
+        a.bn[s ^ 1][b] = go ? nst + 1 : nst;
+        a.bleft[s ^ 1][b] = go ? left - 1 : 0;
+    }
+    if (left <= 0) return;
+
+    const int64_t n6 = a.ncap;
+    const int64_t i = chunk * kChunk + lane;
+    const int p = (int)(nst & 1);
+    const double *__restrict__ cur = a.st + (int64_t)p * 6 * n6;
+    double *__restrict__ nxt = a.st + (int64_t)(p ^ 1) * 6 * n6;
+    const uint32_t fl = a.mflags[i];
+    if (fl & kPad) return;
+    MassRegs r;
+    r.px = cur[i];
+    r.py = cur[n6 + i];
+    r.pz = cur[2 * n6 + i];
+    r.vx = cur[3 * n6 + i];
+    r.vy = cur[4 * n6 + i];
+    r.vz = cur[5 * n6 + i];
+    if (fl & kFixed) {  // src/sim.cu:1100-1101: a fixed mass keeps its state
+        nxt[i] = r.px;
+        nxt[n6 + i] = r.py;
+        nxt[2 * n6 + i] = r.pz;
+        nxt[3 * n6 + i] = r.vx;
+        nxt[4 * n6 + i] = r.vy;
+        nxt[5 * n6 + i] = r.vz;
+        return;
+    }
+
+    // Spring forces, src/sim.cu:1055-1073, summed in ascending spring index.
+    r.fx = 0.0;
+    r.fy = 0.0;
+    r.fz = 0.0;
+    const int d = a.deg[i];
+    const int64_t e0 = a.chunk_ptr[chunk] + lane;
+    for (int j = 0; j < d; ++j) {
+        const int64_t e = e0 + (int64_t)j * kChunk;
+        const uint32_t code = a.inc_nbr[e];
+        const int64_t o = code & ~kLeftBit;
+        const bool is_left = (code & kLeftBit) != 0;
+        const double opx = cur[o], opy = cur[n6 + o], opz = cur[2 * n6 + o];
+        const double ovx = cur[3 * n6 + o], ovy = cur[4 * n6 + o], ovz = cur[5 * n6 + o];
+        // temp = right.pos - left.pos; dv = left.vel - right.vel
+        const double tx = (is_left ? opx : r.px) - (is_left ? r.px : opx);
+        const double ty = (is_left ? opy : r.py) - (is_left ? r.py : opy);
+        const double tz = (is_left ? opz : r.pz) - (is_left ? r.pz : opz);
+        const double dvx = (is_left ? r.vx : ovx) - (is_left ? ovx : r.vx);
+        const double dvy = (is_left ? r.vy : ovy) - (is_left ? ovy : r.vy);
+        const double dvz = (is_left ? r.vz : ovz) - (is_left ? ovz : r.vz);
+        const double nrm = sqrt(tx * tx + ty * ty + tz * tz);
+        const double ux = tx / nrm, uy = ty / nrm, uz = tz / nrm;
+        double rl = a.inc_rest[e];
+        if (ACT) {
+            const int32_t ai = a.inc_act[e];
+            if (ai >= 0) {
+                const double *A = a.act + 4 * (int64_t)ai;
+                rl = rl * (1 + A[0] * sin(A[1] * T + A[2]));
+            }
+        }
+        const double k = UNI ? a.k_uni : a.inc_k[e];
+        const double dmp = UNI ? a.damp_uni : a.inc_damp[e];
+        const double mag = k * (rl - nrm);
+        double Fx = ux * mag, Fy = uy * mag, Fz = uz * mag;
+        const double dd = (dvx * ux + dvy * uy + dvz * uz) * dmp;
+        Fx = Fx + ux * dd;
+        Fy = Fy + uy * dd;
+        Fz = Fz + uz * dd;
+        if (is_left) {
+            r.fx = r.fx - Fx;
+            r.fy = r.fy - Fy;
+            r.fz = r.fz - Fz;
+        } else {
+            r.fx = r.fx + Fx;
+            r.fy = r.fy + Fy;
+            r.fz = r.fz + Fz;
+        }
+    }
+
+    const double m = a.m[i];
+    double ax, ay, az;
+    mass_pass(a, r, i, fl, b, m, ax, ay, az);
+    nxt[i] = r.px;
+    nxt[n6 + i] = r.py;
+    nxt[2 * n6 + i] = r.pz;
+    nxt[3 * n6 + i] = r.vx;
+    nxt[4 * n6 + i] = r.vy;
+    nxt[5 * n6 + i] = r.vz;
+    if (a.write_acc) {
+        a.acc[i] = ax;
+        a.acc[n6 + i] = ay;
+        a.acc[2 * n6 + i] = az;
+    }
+}
+
+// Per-mass local time: T += dt once per step the body took since the last
+// sync (src/sim.cu:1165), for every non-fixed mass.
+__global__ void k_advance_T(double *Tm, const uint32_t *mflags, const int32_t *chunk_body,
+                            const int64_t *bn, const int64_t *bnsync, int64_t nchunks,
+                            double dt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t chunk = i / kChunk;
+    if (chunk >= nchunks) return;
+    const int b = chunk_body[chunk];
+    if (b < 0) return;
+    if (mflags[i] & (kFixed | kPad)) return;
+    const int64_t cnt = bn[b] - bnsync[b];
+    double t = Tm[i];
+    for (int64_t q = 0; q < cnt; ++q) t += dt;
+    Tm[i] = t;
+}
+
+__global__ void k_sync_bodies(const int64_t *bn, int64_t *bnsync, int64_t nb) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nb) bnsync[b] = bn[b];
+}
+
+__global__ void k_set_left(int64_t *bleft, const int32_t *b_chunk0, int64_t nb, int64_t n) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nb) bleft[b] = b_chunk0[b] >= 0 ? n : 0;
+}
+
+__global__ void k_fill_f64(double *p, int64_t n, double v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---- launchers ----
+
+hipError_t launch_step(const StepArgs &a, bool uni, bool act, hipStream_t s) {
+    const int64_t blocks = (a.nchunks + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (blocks <= 0) return hipSuccess;
+    if (uni && !act) hipLaunchKernelGGL((k_step_gather<true, false>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else if (uni && act) hipLaunchKernelGGL((k_step_gather<true, true>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else if (!uni && !act) hipLaunchKernelGGL((k_step_gather<false, false>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL((k_step_gather<false, true>), dim3(blocks), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+const char *step_kernel_name(bool uni, bool act) {
+    if (uni && !act) return "k_step_gather<uniform_kd,passive>";
+    if (uni && act) return "k_step_gather<uniform_kd,actuated>";
+    if (!uni && !act) return "k_step_gather<per_spring_kd,passive>";
+    return "k_step_gather<per_spring_kd,actuated>";
+}
+
+hipError_t launch_advance_T(double *Tm, const uint32_t *mflags, const int32_t *chunk_body,
+                            const int64_t *bn, int64_t *bnsync, int64_t nchunks, int64_t nb,
+                            double dt, hipStream_t s) {
+    const int64_t n = nchunks * kChunk;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_advance_T, dim3((n + 255) / 256), dim3(256), 0, s, Tm, mflags,
+                           chunk_body, bn, (const int64_t *)bnsync, nchunks, dt);
+    }
+    if (nb > 0) hipLaunchKernelGGL(k_sync_bodies, dim3((nb + 255) / 256), dim3(256), 0, s, bn, bnsync, nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_left(int64_t *bleft, const int32_t *b_chunk0, int64_t nb, int64_t n,
+                           hipStream_t s) {
+    if (nb > 0) hipLaunchKernelGGL(k_set_left, dim3((nb + 255) / 256), dim3(256), 0, s, bleft, b_chunk0, nb, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(double *p, int64_t n, double v, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_f64, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace titan_mi355x

@@ -1,0 +1,85 @@
+// Device data layout of the MI355X step engine (shared by host and kernels).
+//
+// Masses live in a chunked arena: a chunk is 64 consecutive mass slots, one
+// wavefront's worth, and every body starts on a chunk boundary so that a
+// wave never straddles two bodies.  State is FP64 SoA, double-buffered:
+//   st[(p * 6 + c) * ncap + i], p = buffer parity, c = px py pz vx vy vz.
+// A body's current buffer is the parity of its own step count, so bodies
+// that are not stepped need no copy.
+//
+// Springs are stored per incidence in SELL-64 order (sliced ELLPACK with a
+// slice of one wavefront): incidence j of mass slot `lane` in chunk c sits at
+// chunk_ptr[c] + j * 64 + lane, so the loads of one incidence slot across a
+// wave are consecutive.  Each mass lists its incident springs in ascending
+// spring index, which fixes the order of the FP64 force sum.
+#pragma once
+
+#include <stdint.h>
+
+namespace titan_mi355x {
+
+constexpr int kChunk = 64;          // masses per chunk = wavefront size
+constexpr int kWavesPerBlock = 4;   // chunks per workgroup of the gather kernel
+constexpr int kBlock = kChunk * kWavesPerBlock;
+
+// per-mass flag bits (mflags)
+constexpr uint32_t kFixed = 1u;     // constraints.fixed (src/sim.cu:1100)
+constexpr uint32_t kLocalLC = 2u;   // own local-constraint list
+constexpr uint32_t kExt = 4u;       // extern_force may be non-zero
+constexpr uint32_t kDrag = 8u;      // non-zero integer drag coefficient
+constexpr uint32_t kPad = 16u;      // padding slot of a chunk (never read)
+
+// bit 31 of inc_nbr: this mass is the spring's left endpoint (receives -F)
+constexpr uint32_t kLeftBit = 0x80000000u;
+
+// local constraint entry (a reference CudaContactPlane / CudaBall /
+// CudaConstraintPlane / CudaDirection, src/object.cu:53-141)
+struct LocalC {
+    double v[3];   // unit normal, ball centre or unit tangent
+    double num;    // offset, radius or friction
+    int32_t kind;  // TITAN_CONSTRAINT_PLANE .. TITAN_DIRECTION
+    int32_t pad;
+};
+
+struct StepArgs {
+    // mass arena
+    double *st;               // 2 x 6 x ncap
+    int64_t ncap;
+    const double *m;
+    const uint32_t *mflags;
+    const int32_t *deg;       // incident springs per mass
+    const double *ext;        // 3 x ncap or null
+    const int32_t *drag;      // ncap or null
+    const int64_t *mlc;       // per-mass local list: begin << 20 | count
+    double *acc;              // 3 x ncap
+    // chunk arrays
+    const int32_t *chunk_body;
+    const int64_t *chunk_ptr;
+    int64_t nchunks;
+    // incidences
+    const uint32_t *inc_nbr;
+    const double *inc_rest;
+    const double *inc_k;      // null when uniform
+    const double *inc_damp;   // null when uniform
+    const int32_t *inc_act;   // null when no actuators
+    const double *act;        // 4 per actuator: a, omega, phi, 0
+    double k_uni, damp_uni;
+    // bodies: dynamic fields double-buffered by launch slot
+    double *bT[2];
+    int64_t *bn[2];
+    int64_t *bleft[2];
+    const int32_t *b_chunk0;
+    const int32_t *b_lc;      // shared local list: begin
+    const int32_t *b_lcn;     // shared local list: count
+    const LocalC *lc;
+    // globals
+    const double *planes;     // 8 per plane: nx ny nz d mu_k mu_s 0 0
+    int32_t nplanes;
+    const double *balls;      // 4 per ball: cx cy cz r
+    int32_t nballs;
+    double gx, gy, gz, dt;
+    int32_t slot;
+    int32_t write_acc;
+};
+
+}  // namespace titan_mi355x
