@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -144,6 +145,7 @@ struct titan_ctx {
     bool kd_known = false, uniform_kd = true;
     double k_uni = 0, damp_uni = 0;
     int slot = 0;
+    int unroll = 2;  // incident springs per gather iteration (TITAN_GATHER_UNROLL)
 
     // captured multi-step graphs, valid while `version` is unchanged
     uint64_t version = 1;
@@ -223,6 +225,10 @@ void ensure_chunks(titan_ctx *c, int64_t need) {
     if (c->chunks.cap() >= need) return;
     const int64_t old = c->chunks.cap();
     int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 16));
+    // the kernels address one state component with a 32-bit byte offset
+    const int64_t max_chunks = ((int64_t)1 << 29) / kChunk;
+    if (need > max_chunks) fail(TITAN_E_NOMEM, "mass arena limit: at most 2^29 mass slots");
+    nc = std::min(nc, max_chunks);
     const int64_t on = old * kChunk, nn = nc * kChunk;
     grow_soa(c, c->st, 12, on, nn);
     grow_soa(c, c->m, 1, on, nn);
@@ -386,7 +392,7 @@ StepArgs make_args(titan_ctx *c, int slot, int write_acc) {
 
 void launch(titan_ctx *c, int slot, int write_acc) {
     StepArgs a = make_args(c, slot, write_acc);
-    HIPCHK(launch_step(a, c->uniform_kd, c->inc_act != nullptr, c->stream));
+    HIPCHK(launch_step(a, c->uniform_kd, c->inc_act != nullptr, c->unroll, c->stream));
 }
 
 hipGraphExec_t graph_for(titan_ctx *c, int slot) {
@@ -397,7 +403,7 @@ hipGraphExec_t graph_for(titan_ctx *c, int slot) {
     int s = slot;
     for (int i = 0; i < kGraphSteps; ++i) {
         StepArgs a = make_args(c, s, 0);
-        hipError_t e = launch_step(a, c->uniform_kd, c->inc_act != nullptr, c->stream);
+        hipError_t e = launch_step(a, c->uniform_kd, c->inc_act != nullptr, c->unroll, c->stream);
         if (e != hipSuccess) {
             hipGraph_t junk;
             (void)hipStreamEndCapture(c->stream, &junk);
@@ -897,6 +903,7 @@ int titan_ctx_create(int device, uint64_t hbm_budget_bytes, titan_ctx **out) {
     HIPCHK(hipSetDevice(device));
     c = new titan_ctx();
     c->device = device;
+    if (const char *u = std::getenv("TITAN_GATHER_UNROLL")) c->unroll = std::atoi(u);
     c->budget = hbm_budget_bytes;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&c->ev0));

@@ -37,6 +37,15 @@ __device__ __forceinline__ int64_t xcd_block(int64_t w, int64_t G) {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Load base[byte_off / sizeof(T)] with a 32-bit byte offset from a
+// wave-uniform base, so the access compiles to a scalar-base global load.
+// Every per-component array is below 4 GiB (the arena caps a component at
+// 2^29 doubles).
+template <class T>
+__device__ __forceinline__ T ldo(const T *__restrict__ base, uint32_t byte_off) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
 struct MassRegs {
     double px, py, pz, vx, vy, vz, fx, fy, fz;
 };
@@ -194,12 +203,49 @@ __device__ __forceinline__ void mass_pass(const StepArgs &a, MassRegs &r, int64_
     r.pz = r.pz + r.vz * a.dt;
 }
 
+// The incidence arrays of one chunk and the state buffer being read.
+template <bool UNI, bool ACT>
+struct IncView {
+    const double *__restrict__ cpx, *__restrict__ cpy, *__restrict__ cpz;
+    const double *__restrict__ cvx, *__restrict__ cvy, *__restrict__ cvz;
+    const uint32_t *__restrict__ inbr;
+    const double *__restrict__ irest, *__restrict__ ik, *__restrict__ idamp;
+    const int32_t *__restrict__ iact;
+    const double *__restrict__ act;
+    double k_uni, damp_uni, T;
+
+    // G for incidence slot e (see the kernel comment): the force this mass
+    // receives from the spring is -G.
+    __device__ __forceinline__ void force(uint32_t e, const MassRegs &r, double G[3]) const {
+        const uint32_t ob = (ldo(inbr, 4u * e) & ~kLeftBit) * 8u;
+        const double tx = ldo(cpx, ob) - r.px, ty = ldo(cpy, ob) - r.py, tz = ldo(cpz, ob) - r.pz;
+        const double dvx = r.vx - ldo(cvx, ob), dvy = r.vy - ldo(cvy, ob), dvz = r.vz - ldo(cvz, ob);
+        const double nrm = sqrt(tx * tx + ty * ty + tz * tz);
+        const double ux = tx / nrm, uy = ty / nrm, uz = tz / nrm;
+        double rl = ldo(irest, 8u * e);
+        if (ACT) {
+            const int32_t ai = ldo(iact, 4u * e);
+            if (ai >= 0) {
+                const double *A = act + 4 * (int64_t)ai;
+                rl = rl * (1 + A[0] * sin(A[1] * T + A[2]));
+            }
+        }
+        const double k = UNI ? k_uni : ldo(ik, 8u * e);
+        const double dmp = UNI ? damp_uni : ldo(idamp, 8u * e);
+        const double mag = k * (rl - nrm);
+        const double dd = (dvx * ux + dvy * uy + dvz * uz) * dmp;
+        G[0] = ux * mag + ux * dd;
+        G[1] = uy * mag + uy * dd;
+        G[2] = uz * mag + uz * dd;
+    }
+};
+
 }  // namespace
 
 // One lane per mass slot, one wavefront per 64-mass chunk, four chunks per
 // workgroup.  UNI: spring constant and damping are the same for every
 // spring (passed by value).  ACT: some springs are actuated.
-template <bool UNI, bool ACT>
+template <bool UNI, bool ACT, int U>
 __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int64_t chunk = blk * kWavesPerBlock + uniform(threadIdx.x >> 6);
@@ -244,52 +290,57 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     }
 
     // Spring forces, src/sim.cu:1055-1073, summed in ascending spring index.
+    //
+    // Each incident spring is evaluated from this mass's side: tt = p_other -
+    // p_self and dv = v_self - v_other.  If this mass is the left endpoint,
+    // tt and dv are the reference's temp and dv and G below is its F; this
+    // mass receives -F = -G.  If it is the right endpoint, the reference's
+    // temp and dv are -tt and -dv exactly (x - y == -(y - x) under
+    // round-to-nearest), so its F is -G exactly and this mass receives
+    // +F = -G.  Either way the contribution is -G, bit for bit (an exact zero
+    // may differ in sign), with no select on the spring's orientation.
     r.fx = 0.0;
     r.fy = 0.0;
     r.fz = 0.0;
     const int d = a.deg[i];
-    const int64_t e0 = a.chunk_ptr[chunk] + lane;
-    for (int j = 0; j < d; ++j) {
-        const int64_t e = e0 + (int64_t)j * kChunk;
-        const uint32_t code = a.inc_nbr[e];
-        const int64_t o = code & ~kLeftBit;
-        const bool is_left = (code & kLeftBit) != 0;
-        const double opx = cur[o], opy = cur[n6 + o], opz = cur[2 * n6 + o];
-        const double ovx = cur[3 * n6 + o], ovy = cur[4 * n6 + o], ovz = cur[5 * n6 + o];
-        // temp = right.pos - left.pos; dv = left.vel - right.vel
-        const double tx = (is_left ? opx : r.px) - (is_left ? r.px : opx);
-        const double ty = (is_left ? opy : r.py) - (is_left ? r.py : opy);
-        const double tz = (is_left ? opz : r.pz) - (is_left ? r.pz : opz);
-        const double dvx = (is_left ? r.vx : ovx) - (is_left ? ovx : r.vx);
-        const double dvy = (is_left ? r.vy : ovy) - (is_left ? ovy : r.vy);
-        const double dvz = (is_left ? r.vz : ovz) - (is_left ? ovz : r.vz);
-        const double nrm = sqrt(tx * tx + ty * ty + tz * tz);
-        const double ux = tx / nrm, uy = ty / nrm, uz = tz / nrm;
-        double rl = a.inc_rest[e];
-        if (ACT) {
-            const int32_t ai = a.inc_act[e];
-            if (ai >= 0) {
-                const double *A = a.act + 4 * (int64_t)ai;
-                rl = rl * (1 + A[0] * sin(A[1] * T + A[2]));
-            }
+    // wave-uniform bases + 32-bit lane offsets (scalar-base global loads)
+    IncView<UNI, ACT> v;
+    v.cpx = cur;
+    v.cpy = cur + n6;
+    v.cpz = cur + 2 * n6;
+    v.cvx = cur + 3 * n6;
+    v.cvy = cur + 4 * n6;
+    v.cvz = cur + 5 * n6;
+    const int64_t ebase = a.chunk_ptr[chunk];
+    v.inbr = a.inc_nbr + ebase;
+    v.irest = a.inc_rest + ebase;
+    v.ik = UNI ? nullptr : a.inc_k + ebase;
+    v.idamp = UNI ? nullptr : a.inc_damp + ebase;
+    v.iact = ACT ? a.inc_act + ebase : nullptr;
+    v.act = a.act;
+    v.k_uni = a.k_uni;
+    v.damp_uni = a.damp_uni;
+    v.T = T;
+    int j = 0;
+    // U independent incident springs per iteration expose instruction-level
+    // parallelism; their forces are still added in ascending order.
+    for (; j + U <= d; j += U) {
+        double G[U][3];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v.force((uint32_t)((j + u) * kChunk + lane), r, G[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            r.fx = r.fx - G[u][0];
+            r.fy = r.fy - G[u][1];
+            r.fz = r.fz - G[u][2];
         }
-        const double k = UNI ? a.k_uni : a.inc_k[e];
-        const double dmp = UNI ? a.damp_uni : a.inc_damp[e];
-        const double mag = k * (rl - nrm);
-        double Fx = ux * mag, Fy = uy * mag, Fz = uz * mag;
-        const double dd = (dvx * ux + dvy * uy + dvz * uz) * dmp;
-        Fx = Fx + ux * dd;
-        Fy = Fy + uy * dd;
-        Fz = Fz + uz * dd;
-        if (is_left) {
-            r.fx = r.fx - Fx;
-            r.fy = r.fy - Fy;
-            r.fz = r.fz - Fz;
-        } else {
-            r.fx = r.fx + Fx;
-            r.fy = r.fy + Fy;
-            r.fz = r.fz + Fz;
-        }
+    }
+    for (; j < d; ++j) {
+        double G[3];
+        v.force((uint32_t)(j * kChunk + lane), r, G);
+        r.fx = r.fx - G[0];
+        r.fy = r.fy - G[1];
+        r.fz = r.fz - G[2];
     }
 
     const double m = a.m[i];
@@ -342,13 +393,20 @@ __global__ void k_fill_f64(double *p, int64_t n, double v) {
 
 // ---- launchers ----
 
-hipError_t launch_step(const StepArgs &a, bool uni, bool act, hipStream_t s) {
+template <int U>
+static void launch_u(const StepArgs &a, bool uni, bool act, int64_t blocks, hipStream_t s) {
+    if (uni && !act) hipLaunchKernelGGL((k_step_gather<true, false, U>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else if (uni && act) hipLaunchKernelGGL((k_step_gather<true, true, U>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else if (!uni && !act) hipLaunchKernelGGL((k_step_gather<false, false, U>), dim3(blocks), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL((k_step_gather<false, true, U>), dim3(blocks), dim3(kBlock), 0, s, a);
+}
+
+hipError_t launch_step(const StepArgs &a, bool uni, bool act, int unroll, hipStream_t s) {
     const int64_t blocks = (a.nchunks + kWavesPerBlock - 1) / kWavesPerBlock;
     if (blocks <= 0) return hipSuccess;
-    if (uni && !act) hipLaunchKernelGGL((k_step_gather<true, false>), dim3(blocks), dim3(kBlock), 0, s, a);
-    else if (uni && act) hipLaunchKernelGGL((k_step_gather<true, true>), dim3(blocks), dim3(kBlock), 0, s, a);
-    else if (!uni && !act) hipLaunchKernelGGL((k_step_gather<false, false>), dim3(blocks), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL((k_step_gather<false, true>), dim3(blocks), dim3(kBlock), 0, s, a);
+    if (unroll >= 4) launch_u<4>(a, uni, act, blocks, s);
+    else if (unroll == 2) launch_u<2>(a, uni, act, blocks, s);
+    else launch_u<1>(a, uni, act, blocks, s);
     return hipGetLastError();
 }
 
