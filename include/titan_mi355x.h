@@ -47,6 +47,10 @@ extern "C" {
 #define TITAN_BALL 2
 #define TITAN_DIRECTION 3
 
+#define TITAN_LAYOUT_AUTO 0
+#define TITAN_LAYOUT_GATHER 1
+#define TITAN_LAYOUT_TILED 2
+
 typedef struct titan_ctx titan_ctx;
 
 /* One body: a set of masses and the springs among them, copied in by
@@ -82,6 +86,10 @@ typedef struct {
     const double *lc_num;
     /* body time at creation: the first step's actuation time */
     double t0;
+    /* device layout: TITAN_LAYOUT_AUTO (tiled when the body's tiles fit the
+     * kernel's limits, else gather), TITAN_LAYOUT_GATHER or TITAN_LAYOUT_TILED
+     * (fails if a tile does not fit).  Results are identical either way. */
+    int32_t layout;
 } titan_body_desc;
 
 /* ---- context (replaces Simulation's constructor/stop/freeGPU,
@@ -139,8 +147,11 @@ int titan_timer_stop(titan_ctx *ctx, float *ms);
 /* ---- introspection used by tests and the benchmark ---- */
 /* Device bytes currently allocated by the context. */
 int titan_mem_used(const titan_ctx *ctx, uint64_t *bytes);
-/* Name of the step kernel variant the next titan_step will launch. */
+/* Names of the step kernel variants the next titan_step will launch
+ * ("+"-separated when both layouts are in use). */
 int titan_kernel_name(const titan_ctx *ctx, char *buf, int64_t len);
+/* Layout a body was given: TITAN_LAYOUT_GATHER or TITAN_LAYOUT_TILED. */
+int titan_body_layout(const titan_ctx *ctx, int32_t body_id, int32_t *layout);
 
 /* Host-side lattice generator with the reference's mass and spring order
  * (Lattice::Lattice, src/object.cu:232-293).  pos: 3*nx*ny*nz; left/right/

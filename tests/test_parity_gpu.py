@@ -13,10 +13,11 @@ import sys
 import numpy as np
 import pytest
 
-from titan_amd import Context, scenes
+from titan_amd import LAYOUT_GATHER, LAYOUT_TILED, Context, scenes
 from helpers import normwise_rel, to_oracle
 
 pytestmark = pytest.mark.gpu
+LAYOUTS = pytest.mark.parametrize("layout", [LAYOUT_GATHER, LAYOUT_TILED], ids=["gather", "tiled"])
 
 TOL_ACT = 1e-10
 
@@ -26,10 +27,12 @@ def _say(msg):
     sys.stdout.flush()
 
 
-def run_engine(scene, n_steps, ctx=None, chunks=None):
+def run_engine(scene, n_steps, ctx=None, chunks=None, layout=0):
     own = ctx is None
     ctx = ctx or Context()
-    ids = scene.load(ctx)
+    ids = scene.load(ctx, layout=layout)
+    if layout:
+        assert all(ctx.body_layout(i) == layout for i in ids)
     for n in (chunks or [n_steps]):
         ctx.step(n)
     out = [ctx.download(i, pos=True, vel=True, acc=True, T=True) for i in ids]
@@ -51,10 +54,11 @@ def assert_bitexact(eng, orc):
     np.testing.assert_array_equal(eng["vel"], orc.vel)
 
 
-def test_readme_lattice_free_fall_bitexact():
+@LAYOUTS
+def test_readme_lattice_free_fall_bitexact(layout):
     sc = scenes.readme_lattice(10.0)
     _say(f"case readme lattice cz=10, {sc.n_masses} masses, {sc.n_springs} springs, 100 steps")
-    eng, t = run_engine(sc, 100)
+    eng, t = run_engine(sc, 100, layout=layout)
     orc = run_oracle(sc, 100)
     assert_bitexact(eng, orc)
     np.testing.assert_array_equal(eng["acc"], orc.acc)
@@ -62,26 +66,29 @@ def test_readme_lattice_free_fall_bitexact():
     assert t == orc.t
 
 
-def test_readme_lattice_contact_bitexact():
+@LAYOUTS
+def test_readme_lattice_contact_bitexact(layout):
     sc = scenes.readme_lattice(2.45)
     _say("case readme lattice cz=2.45 (contact + friction from step 0), 1000 steps")
-    eng, _ = run_engine(sc, 1000)
+    eng, _ = run_engine(sc, 1000, layout=layout)
     orc = run_oracle(sc, 1000)
     assert_bitexact(eng, orc)
     np.testing.assert_array_equal(eng["acc"], orc.acc)
     assert np.min(sc.bodies[0]["pos"][:, 2]) < 0  # in contact from the first step
 
 
-def test_graph_path_matches_direct_launches():
+@LAYOUTS
+def test_graph_path_matches_direct_launches(layout):
     sc = scenes.readme_lattice(2.45)
     _say("case graph replay vs single launches, 300 steps")
-    a, _ = run_engine(sc, 300)                          # mostly hipGraph replays
-    b, _ = run_engine(sc, 300, chunks=[1] * 300)        # one launch per call
+    a, _ = run_engine(sc, 300, layout=layout)                      # mostly hipGraph replays
+    b, _ = run_engine(sc, 300, chunks=[1] * 300, layout=layout)    # one launch per call
     np.testing.assert_array_equal(a["pos"], b["pos"])
     np.testing.assert_array_equal(a["vel"], b["vel"])
 
 
-def test_actuated_reference_kinds_and_general_law():
+@LAYOUTS
+def test_actuated_reference_kinds_and_general_law(layout):
     sc = scenes.readme_lattice(2.45)
     b = sc.bodies[0]
     s = len(b["left"])
@@ -92,13 +99,14 @@ def test_actuated_reference_kinds_and_general_law():
     b["act_a"] = rng.uniform(0, 0.2, s)
     b["act_phi"] = rng.uniform(0, 2 * np.pi, s)
     _say("case actuated springs (kinds 0-4), 100 steps")
-    eng, _ = run_engine(sc, 100)
+    eng, _ = run_engine(sc, 100, layout=layout)
     orc = run_oracle(sc, 100)
     assert normwise_rel(eng["pos"], orc.pos) <= TOL_ACT
     assert normwise_rel(eng["vel"], orc.vel) <= TOL_ACT
 
 
-def test_fixed_ext_drag_balls_per_spring_k():
+@LAYOUTS
+def test_fixed_ext_drag_balls_per_spring_k(layout):
     sc = scenes.readme_lattice(2.45)
     b = sc.bodies[0]
     n, s = len(b["pos"]), len(b["left"])
@@ -112,13 +120,14 @@ def test_fixed_ext_drag_balls_per_spring_k():
     sc.balls = [(0.0, 0.0, 1.0, 1.2), (1.0, 1.0, 4.0, 0.7)]
     sc.planes.append((0.0, 0.6, 0.8, -1.0, 0.3, 0.5))
     _say("case fixed + extern + drag + balls + 2 planes + per-spring k/damping, 500 steps")
-    eng, _ = run_engine(sc, 500)
+    eng, _ = run_engine(sc, 500, layout=layout)
     orc = run_oracle(sc, 500)
     assert_bitexact(eng, orc)
     np.testing.assert_array_equal(eng["T"], orc.T)
 
 
-def test_local_constraints_shared_and_per_mass():
+@LAYOUTS
+def test_local_constraints_shared_and_per_mass(layout):
     sc = scenes.readme_lattice(2.0)
     b = sc.bodies[0]
     n = len(b["pos"])
@@ -145,20 +154,21 @@ def test_local_constraints_shared_and_per_mass():
     b2["vel"] = rng.normal(0, 0.3, (n2, 3))
     sc.bodies.append(b2)
     _say("case local constraints (shared list + per-mass lists), 2 bodies, 400 steps")
-    eng, _ = run_engine(sc, 400)
+    eng, _ = run_engine(sc, 400, layout=layout)
     orc = run_oracle(sc, 400)
     assert_bitexact(eng, orc)
 
 
-def test_bodies_alone_equal_bodies_together():
+@LAYOUTS
+def test_bodies_alone_equal_bodies_together(layout):
     s1 = scenes.readme_lattice(2.45)
     s2 = scenes.readme_lattice(3.0)
     s2.bodies[0]["k"] = np.full(len(s2.bodies[0]["left"]), 7000.0)
     both = scenes.Scene(bodies=s1.bodies + s2.bodies, planes=s1.planes, dt=s1.dt)
     _say("case two bodies alone vs together, 200 steps")
-    a, _ = run_engine(s1, 200)
-    b, _ = run_engine(s2, 200)
-    ab, _ = run_engine(both, 200)
+    a, _ = run_engine(s1, 200, layout=layout)
+    b, _ = run_engine(s2, 200, layout=layout)
+    ab, _ = run_engine(both, 200, layout=layout)
     np.testing.assert_array_equal(ab["pos"], np.concatenate([a["pos"], b["pos"]]))
     np.testing.assert_array_equal(ab["vel"], np.concatenate([a["vel"], b["vel"]]))
 
@@ -201,10 +211,11 @@ def test_upload_download_roundtrip_and_resume():
     np.testing.assert_array_equal(got["vel"], o.vel)
 
 
-def test_million_mass_lattice_bitexact_few_steps():
+@LAYOUTS
+def test_million_mass_lattice_bitexact_few_steps(layout):
     sc = scenes.big_lattice(100, seed=0)
     _say(f"case 100^3 lattice {sc.n_masses} masses {sc.n_springs} springs, 5 steps")
-    eng, _ = run_engine(sc, 5)
+    eng, _ = run_engine(sc, 5, layout=layout)
     o = to_oracle(sc)
     o.run(5)
     assert_bitexact(eng, o)
@@ -228,3 +239,44 @@ def test_destroy_and_reuse_slots():
     o2 = to_oracle(scenes.Scene(bodies=[extra], planes=sc.planes, dt=sc.dt))
     o2.run(20)
     np.testing.assert_array_equal(got2["pos"], o2.pos)
+
+
+@LAYOUTS
+def test_multi_tile_lattice_with_contact(layout):
+    # 12^3 masses: several 256-mass tiles along a Morton curve, with halos
+    sc = scenes.Scene(bodies=[scenes.lattice_body((0.0, 0.0, 0.55), (1.2, 1.2, 1.2), 12, damping=0.1)],
+                      planes=[(0.0, 0.0, 1.0, 0.0, 0.8, 1.0)], dt=1e-4)
+    b = sc.bodies[0]
+    rng = np.random.default_rng(9)
+    b["pos"] = b["pos"] + rng.normal(0, 1e-3, b["pos"].shape)
+    b["vel"] = rng.normal(0, 0.2, b["pos"].shape)
+    _say(f"case 12^3 lattice layout {layout}, 400 steps")
+    eng, _ = run_engine(sc, 400, layout=layout)
+    orc = run_oracle(sc, 400)
+    assert_bitexact(eng, orc)
+    np.testing.assert_array_equal(eng["acc"], orc.acc)
+    np.testing.assert_array_equal(eng["T"], orc.T)
+
+
+def test_mixed_layouts_in_one_context():
+    s1 = scenes.readme_lattice(2.45)
+    s2 = scenes.Scene(bodies=[scenes.lattice_body((3.0, 0.0, 0.5), (1.0, 1.0, 1.0), 9, damping=0.1)],
+                      planes=s1.planes, dt=s1.dt)
+    _say("case gather body + tiled body in one context, 300 steps")
+    with Context() as ctx:
+        a = s1.load(ctx, layout=LAYOUT_GATHER)[0]
+        ctx.clear_planes_and_balls()
+        b = s2.load(ctx, layout=LAYOUT_TILED)[0]
+        ctx.step(300)
+        ga, gb = ctx.download(a), ctx.download(b)
+        assert "k_step_gather" in ctx.kernel_name() and "k_step_tiles" in ctx.kernel_name()
+    oa, ob = run_oracle(s1, 300), run_oracle(s2, 300)
+    np.testing.assert_array_equal(ga["pos"], oa.pos)
+    np.testing.assert_array_equal(gb["pos"], ob.pos)
+
+
+def test_auto_layout_prefers_tiles():
+    sc = scenes.readme_lattice(2.45)
+    with Context() as ctx:
+        (bid,) = sc.load(ctx)
+        assert ctx.body_layout(bid) == LAYOUT_TILED

@@ -33,8 +33,10 @@ EXPORTS = (
     "titan_body_create", "titan_body_destroy", "titan_body_count", "titan_step", "titan_step_until",
     "titan_time", "titan_sync", "titan_body_download", "titan_body_upload", "titan_timer_start",
     "titan_timer_stop", "titan_mem_used", "titan_kernel_name", "titan_lattice_spring_count",
-    "titan_lattice", "titan_debug_inject_failure",
+    "titan_lattice", "titan_debug_inject_failure", "titan_body_layout",
 )
+
+LAYOUT_AUTO, LAYOUT_GATHER, LAYOUT_TILED = 0, 1, 2
 
 
 class TitanError(RuntimeError):
@@ -53,6 +55,7 @@ class BodyDesc(ctypes.Structure):
         ("type", POINTER(c_int32)), ("omega", POINTER(c_double)), ("act_a", POINTER(c_double)),
         ("act_phi", POINTER(c_double)), ("lc_ptr", POINTER(c_int64)), ("lc_type", POINTER(c_int32)),
         ("lc_vec", POINTER(c_double)), ("lc_num", POINTER(c_double)), ("t0", c_double),
+        ("layout", c_int32),
     ]
 
 
@@ -95,6 +98,7 @@ def lib() -> ctypes.CDLL:
         "titan_lattice": (c_int, [c_double] * 6 + [c_int] * 3 + [POINTER(c_double), POINTER(c_int32),
                                                                    POINTER(c_int32), POINTER(c_double)]),
         "titan_debug_inject_failure": (c_int, [vp, c_int32]),
+        "titan_body_layout": (c_int, [vp, c_int32, POINTER(c_int32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -198,7 +202,7 @@ class Context:
     def add_body(self, pos, m, *, vel=None, T=None, ext=None, fixed=None, drag=None,
                  left=None, right=None, k=None, rest=None, damping=None, type=None, omega=None,
                  act_a=None, act_phi=None, lc_ptr=None, lc_type=None, lc_vec=None, lc_num=None,
-                 t0: float = 0.0) -> int:
+                 t0: float = 0.0, layout: int = LAYOUT_AUTO) -> int:
         pos = _f64(pos, (-1, 3))
         n = pos.shape[0]
         m = _f64(np.broadcast_to(np.asarray(m, dtype=np.float64), (n,)))
@@ -218,6 +222,7 @@ class Context:
         d.n_masses = n
         d.n_springs = s
         d.t0 = float(t0)
+        d.layout = int(layout)
         ctypes_of = dict(fixed=c_uint8, drag=c_int32, left=c_int32, right=c_int32, type=c_int32,
                          lc_ptr=c_int64, lc_type=c_int32)
         for name, arr in keep.items():
@@ -284,9 +289,14 @@ class Context:
         self._chk(self._L.titan_mem_used(self._h, ctypes.byref(v)))
         return v.value
 
+    def body_layout(self, bid: int) -> int:
+        v = c_int32()
+        self._chk(self._L.titan_body_layout(self._h, bid, ctypes.byref(v)))
+        return v.value
+
     def kernel_name(self) -> str:
-        buf = ctypes.create_string_buffer(128)
-        self._chk(self._L.titan_kernel_name(self._h, buf, 128))
+        buf = ctypes.create_string_buffer(256)
+        self._chk(self._L.titan_kernel_name(self._h, buf, 256))
         return buf.value.decode()
 
     def inject_failure(self, after: int):

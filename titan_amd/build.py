@@ -34,7 +34,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-command-line-argument",
 ]
 
-ENGINE_SOURCES = [CSRC / "kernels.hip", CSRC / "engine.hip", CSRC / "lattice.cpp"]
+ENGINE_SOURCES = [CSRC / "kernels.hip", CSRC / "engine.hip", CSRC / "tiles.cpp", CSRC / "lattice.cpp"]
 FACADE_SOURCES = [CPP / "sim.cpp"]
 
 LIB = PKG / "libtitan_mi355x.so"
@@ -79,10 +79,25 @@ def build_facade(verbose: bool = False) -> Path:
     return FACADE_LIB
 
 
+FACADE_TESTS_SRC = ROOT / "tests" / "cpp" / "facade_tests.cpp"
+FACADE_TESTS_BIN = ROOT / "tests" / "cpp" / "facade_tests"
+
+
+def build_facade_tests(verbose: bool = False, out: Path | None = None) -> Path:
+    """The reference's physics tests restated against the C++ facade."""
+    out = out or FACADE_TESTS_BIN
+    tmp = out.with_name(out.name + ".tmp")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", f"-I{CPP}", f"-I{INCLUDE}", FACADE_TESTS_SRC,
+          "-o", tmp, f"-L{PKG}", "-ltitan", "-ltitan_mi355x", f"-Wl,-rpath,{PKG}"], verbose)
+    os.replace(tmp, out)
+    return out
+
+
 def build_all(verbose: bool = False) -> None:
     build_engine(verbose)
-    if all(p.exists() for p in FACADE_SOURCES):
-        build_facade(verbose)
+    build_facade(verbose)
+    if FACADE_TESTS_SRC.exists():
+        build_facade_tests(verbose)
     shutil.rmtree(PKG / "_build", ignore_errors=True)
 
 

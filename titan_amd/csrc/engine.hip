@@ -24,6 +24,7 @@
 
 #include "kernels.h"
 #include "layout.h"
+#include "tiles.h"
 #include "titan_mi355x.h"
 
 using namespace titan_mi355x;
@@ -95,11 +96,15 @@ class Ranges {
 
 struct BodyH {
     bool live = false;
+    bool tiled = false;
     int64_t n_masses = 0, chunk0 = 0, nchunks = 0;
     int64_t inc0 = 0, inc_len = 0, act0 = 0, act_len = 0, lc0 = 0, lc_len = 0;
+    int64_t tile0 = 0, ntiles = 0, halo0 = 0, nhalo = 0, spr0 = 0, nspr = 0, grp0 = 0, ngrp = 0;
+    int32_t hmax = 0;
     double T = 0;   // host mirror of the body clock
     int64_t n = 0;  // steps taken (parity selects the state buffer)
-    std::vector<uint32_t> flags;
+    std::vector<uint32_t> flags;   // device order
+    std::vector<int32_t> perm;     // device order -> caller's order (empty = identity)
 };
 
 }  // namespace
@@ -117,7 +122,7 @@ struct titan_ctx {
     double T = 0.0;                      // Simulation::T
     std::vector<double> planes_h, balls_h;
 
-    Ranges chunks, incs, acts, lcs, bids;
+    Ranges chunks, incs, acts, lcs, bids, tiles, halos, sprs, grps;
     std::vector<BodyH> bodies;
 
     // mass arena (ncap = chunks.cap() * 64)
@@ -134,6 +139,14 @@ struct titan_ctx {
     int32_t *inc_act = nullptr;
     double *act = nullptr;
     LocalC *lc = nullptr;
+    // tiled layout arenas
+    TileDesc *tile = nullptr;
+    int32_t *halo = nullptr;
+    uint32_t *t_ends = nullptr;
+    double *t_rest = nullptr, *t_k = nullptr, *t_damp = nullptr;
+    int32_t *t_act = nullptr;
+    uint64_t *t_inc = nullptr;
+    uint8_t *b_tiled = nullptr;
     // body table
     int64_t bcap = 0;
     double *bT[2] = {nullptr, nullptr};
@@ -302,6 +315,7 @@ void ensure_bodies(titan_ctx *c, int64_t need) {
     grow_soa(c, c->bnsync, 1, old, nc);
     grow_soa(c, c->b_lc, 1, old, nc);
     grow_soa(c, c->b_lcn, 1, old, nc);
+    grow_soa(c, c->b_tiled, 1, old, nc);
     int32_t *q = dalloc<int32_t>(c, nc);
     HIPCHK(hipMemsetAsync(q, 0xff, sizeof(int32_t) * nc, c->stream));
     if (c->b_chunk0) HIPCHK(hipMemcpyAsync(q, c->b_chunk0, sizeof(int32_t) * old, hipMemcpyDeviceToDevice, c->stream));
@@ -314,24 +328,90 @@ void ensure_bodies(titan_ctx *c, int64_t need) {
     drop_graphs(c);
 }
 
-// Switch from by-value spring constants to per-incidence arrays.
+void ensure_tiles(titan_ctx *c, int64_t need) {
+    if (c->tiles.cap() >= need) return;
+    const int64_t old = c->tiles.cap();
+    const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 64));
+    grow_soa(c, c->tile, 1, old, nc);
+    c->tiles.grow(nc);
+    drop_graphs(c);
+}
+
+void ensure_halos(titan_ctx *c, int64_t need) {
+    if (c->halos.cap() >= need) return;
+    const int64_t old = c->halos.cap();
+    const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 1024));
+    grow_soa(c, c->halo, 1, old, nc);
+    c->halos.grow(nc);
+    drop_graphs(c);
+}
+
+void ensure_sprs(titan_ctx *c, int64_t need) {
+    if (c->sprs.cap() >= need) return;
+    const int64_t old = c->sprs.cap();
+    const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 4096));
+    grow_soa(c, c->t_ends, 1, old, nc);
+    grow_soa(c, c->t_rest, 1, old, nc);
+    if (c->t_k) grow_soa(c, c->t_k, 1, old, nc);
+    if (c->t_damp) grow_soa(c, c->t_damp, 1, old, nc);
+    if (c->t_act) {
+        int32_t *q = dalloc<int32_t>(c, nc);
+        HIPCHK(hipMemsetAsync(q, 0xff, sizeof(int32_t) * nc, c->stream));
+        HIPCHK(hipMemcpyAsync(q, c->t_act, sizeof(int32_t) * old, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        dfree(c, c->t_act);
+        c->t_act = q;
+    }
+    c->sprs.grow(nc);
+    drop_graphs(c);
+}
+
+void ensure_grps(titan_ctx *c, int64_t need) {
+    if (c->grps.cap() >= need) return;
+    const int64_t old = c->grps.cap();
+    const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 4096));
+    grow_soa(c, c->t_inc, 1, old * kChunk, nc * kChunk);
+    c->grps.grow(nc);
+    drop_graphs(c);
+}
+
+// Switch from by-value spring constants to per-spring arrays (both layouts).
 void make_per_incidence_kd(titan_ctx *c) {
     if (!c->uniform_kd) return;
-    const int64_t n = c->incs.cap();
+    const int64_t n = c->incs.cap(), m = c->sprs.cap();
     c->inc_k = dalloc<double>(c, n);
     c->inc_damp = dalloc<double>(c, n);
     HIPCHK(launch_fill_f64(c->inc_k, n, c->k_uni, c->stream));
     HIPCHK(launch_fill_f64(c->inc_damp, n, c->damp_uni, c->stream));
+    c->t_k = dalloc<double>(c, m);
+    c->t_damp = dalloc<double>(c, m);
+    HIPCHK(launch_fill_f64(c->t_k, m, c->k_uni, c->stream));
+    HIPCHK(launch_fill_f64(c->t_damp, m, c->damp_uni, c->stream));
     c->uniform_kd = false;
     drop_graphs(c);
 }
 
 void make_act(titan_ctx *c) {
     if (c->inc_act) return;
-    const int64_t n = c->incs.cap();
+    const int64_t n = c->incs.cap(), m = c->sprs.cap();
     c->inc_act = dalloc<int32_t>(c, n);
     HIPCHK(hipMemsetAsync(c->inc_act, 0xff, sizeof(int32_t) * n, c->stream));
+    c->t_act = dalloc<int32_t>(c, m);
+    HIPCHK(hipMemsetAsync(c->t_act, 0xff, sizeof(int32_t) * m, c->stream));
     drop_graphs(c);
+}
+
+int32_t tiled_hmax(const titan_ctx *c) {
+    int32_t h = 0;
+    for (auto &b : c->bodies)
+        if (b.live && b.tiled) h = std::max(h, b.hmax);
+    return h;
+}
+
+bool any_body(const titan_ctx *c, bool tiled) {
+    for (auto &b : c->bodies)
+        if (b.live && b.tiled == tiled) return true;
+    return false;
 }
 
 bool same_bits(double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; }
@@ -387,13 +467,35 @@ StepArgs make_args(titan_ctx *c, int slot, int write_acc) {
     a.dt = c->dt;
     a.slot = slot;
     a.write_acc = write_acc;
+    a.tiles = c->tile;
+    a.ntiles = c->tiles.high_water();
+    a.halo = c->halo;
+    a.t_ends = c->t_ends;
+    a.t_rest = c->t_rest;
+    a.t_k = c->t_k;
+    a.t_damp = c->t_damp;
+    a.t_act = c->t_act;
+    a.t_inc = c->t_inc;
+    a.ldim = kTileMasses + tiled_hmax(c);
+    a.b_tiled = c->b_tiled;
     return a;
 }
 
-void launch(titan_ctx *c, int slot, int write_acc) {
+// One step of every body: the gather kernel over gather-layout bodies and
+// the tiled kernel over tiled bodies (disjoint, both read launch slot
+// `slot` and write slot ^ 1).
+hipError_t launch_both(titan_ctx *c, int slot, int write_acc) {
     StepArgs a = make_args(c, slot, write_acc);
-    HIPCHK(launch_step(a, c->uniform_kd, c->inc_act != nullptr, c->unroll, c->stream));
+    const bool act = c->inc_act != nullptr;
+    if (any_body(c, false)) {
+        hipError_t e = launch_step(a, c->uniform_kd, act, c->unroll, c->stream);
+        if (e != hipSuccess) return e;
+    }
+    if (any_body(c, true)) return launch_step_tiles(a, c->uniform_kd, act, c->stream);
+    return hipSuccess;
 }
+
+void launch(titan_ctx *c, int slot, int write_acc) { HIPCHK(launch_both(c, slot, write_acc)); }
 
 hipGraphExec_t graph_for(titan_ctx *c, int slot) {
     for (auto &g : c->graphs)
@@ -402,8 +504,7 @@ hipGraphExec_t graph_for(titan_ctx *c, int slot) {
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int s = slot;
     for (int i = 0; i < kGraphSteps; ++i) {
-        StepArgs a = make_args(c, s, 0);
-        hipError_t e = launch_step(a, c->uniform_kd, c->inc_act != nullptr, c->unroll, c->stream);
+        hipError_t e = launch_both(c, s, 0);
         if (e != hipSuccess) {
             hipGraph_t junk;
             (void)hipStreamEndCapture(c->stream, &junk);
@@ -469,32 +570,40 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     if (!d) fail(TITAN_E_INVALID, "body_create: null descriptor");
     const int64_t N = d->n_masses, S = d->n_springs;
     if (N <= 0) fail(TITAN_E_INVALID, "body_create: a body needs at least one mass");
-    if (S < 0) fail(TITAN_E_INVALID, "body_create: negative spring count");
+    if (N > ((int64_t)1 << 29)) fail(TITAN_E_INVALID, "body_create: too many masses in one body");
+    if (S < 0 || S > (int64_t)0x7fffffff) fail(TITAN_E_INVALID, "body_create: bad spring count");
     if (!d->pos || !d->m) fail(TITAN_E_INVALID, "body_create: pos and m are required");
     if (S > 0 && (!d->left || !d->right || !d->k || !d->rest))
         fail(TITAN_E_INVALID, "body_create: springs need left, right, k and rest");
+    if (d->layout < TITAN_LAYOUT_AUTO || d->layout > TITAN_LAYOUT_TILED)
+        fail(TITAN_E_INVALID, "body_create: bad layout");
     const int64_t nch = (N + kChunk - 1) / kChunk;
     const int64_t npad = nch * kChunk;
 
-    // degrees and validation
-    std::vector<int32_t> deg(npad, 0);
+    // ---- springs: validation, uniform constants, actuators (caller's order)
     int64_t n_act = 0;
-    bool kd_uni = true;
+    bool kd_uni = true, first = true;
     double k0 = 0, d0 = 0;
-    bool first = true;
+    std::vector<int32_t> act_local(S > 0 ? S : 0, -1);
+    std::vector<double> acth;
     for (int64_t s = 0; s < S; ++s) {
         const int64_t L = d->left[s], R = d->right[s];
         if (L >= N || R >= N) fail(TITAN_E_INVALID, "body_create: spring " + std::to_string(s) + " endpoint out of range");
         if (L < 0 || R < 0) continue;
-        deg[L]++;
-        deg[R]++;
         const int ty = d->type ? d->type[s] : 0;
         if (ty < 0 || ty > TITAN_SPRING_GENERAL) fail(TITAN_E_INVALID, "body_create: bad spring type");
         if (ty >= TITAN_ACTIVE_CONTRACT_THEN_EXPAND) {
             if (!d->omega) fail(TITAN_E_INVALID, "body_create: actuated springs need omega");
             if (ty == TITAN_SPRING_GENERAL && (!d->act_a || !d->act_phi))
                 fail(TITAN_E_INVALID, "body_create: general actuation needs act_a and act_phi");
-            n_act++;
+            // the reference's two kinds are the general law with a = -/+0.2 and
+            // phi = 0 (src/sim.cu:1058-1062): 1 + (-0.2) s == 1 - 0.2 s and
+            // x + 0 == x exactly
+            const double aa = ty == TITAN_ACTIVE_CONTRACT_THEN_EXPAND ? -0.2
+                              : ty == TITAN_ACTIVE_EXPAND_THEN_CONTRACT ? 0.2 : d->act_a[s];
+            const double ph = ty == TITAN_SPRING_GENERAL ? d->act_phi[s] : 0.0;
+            act_local[s] = (int32_t)n_act++;
+            acth.insert(acth.end(), {aa, d->omega[s], ph, 0.0});
         }
         const double ks = d->k[s], ds = d->damping ? d->damping[s] : 0.0;
         if (first) {
@@ -505,15 +614,14 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
             kd_uni = false;
         }
     }
-    std::vector<int64_t> coff(nch + 1, 0);
-    for (int64_t ch = 0; ch < nch; ++ch) {
-        int32_t w = 0;
-        for (int l = 0; l < kChunk; ++l) w = std::max(w, deg[ch * kChunk + l]);
-        coff[ch + 1] = coff[ch] + (int64_t)w * kChunk;
-    }
-    const int64_t n_inc = coff[nch];
+    const bool has_springs = !first;
+    // will the context still pass k and damping by value after this body?
+    const bool uniform_after =
+        !has_springs ? c->uniform_kd
+                     : (kd_uni && (!c->kd_known || (c->uniform_kd && same_bits(k0, c->k_uni) &&
+                                                    same_bits(d0, c->damp_uni))));
 
-    // local constraints, sorted stably into application order
+    // ---- local constraints, sorted stably into application order
     std::vector<std::vector<LocalC>> lists;
     bool any_lc = false, shared = false;
     if (d->lc_ptr) {
@@ -550,44 +658,91 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
         else for (auto &l : lists) n_lc += (int64_t)l.size();
     }
 
-    // reserve space; on failure release whatever was taken
-    int64_t ch0 = -1, inc0 = -1, act0 = -1, lc0 = -1, bid = -1;
+    // ---- layout: tiled (masses along a Morton curve) or gather (caller's order)
+    std::vector<int32_t> perm;  // device order -> caller's order
+    std::vector<int32_t> deg(npad, 0);
+    TileBuild tb;
+    bool tiled = false;
+    if (d->layout != TITAN_LAYOUT_GATHER) {
+        if (N > kTileMasses) perm = morton_order(d->pos, N);
+        std::vector<int32_t> inv(N);
+        for (int64_t q = 0; q < N; ++q) inv[perm.empty() ? q : perm[q]] = (int32_t)q;
+        std::vector<int32_t> nl(S), nr(S);
+        for (int64_t s = 0; s < S; ++s) {
+            nl[s] = d->left[s] >= 0 && d->right[s] >= 0 ? inv[d->left[s]] : -1;
+            nr[s] = d->left[s] >= 0 && d->right[s] >= 0 ? inv[d->right[s]] : -1;
+        }
+        SpringsIn sp;
+        sp.n_springs = S;
+        sp.left = nl.data();
+        sp.right = nr.data();
+        sp.k = d->k;
+        sp.rest = d->rest;
+        sp.damping = d->damping;
+        sp.act = n_act > 0 ? act_local.data() : nullptr;
+        std::vector<int32_t> tdeg;
+        tiled = build_tiles(N, sp, !uniform_after, tb, tdeg);
+        if (!tiled && d->layout == TITAN_LAYOUT_TILED)
+            fail(TITAN_E_INVALID, "body_create: tiled layout does not fit: " + tb.why);
+        if (tiled) std::copy(tdeg.begin(), tdeg.end(), deg.begin());
+        else perm.clear();
+    }
+    auto orig = [&](int64_t q) -> int64_t { return perm.empty() ? q : perm[q]; };
+
+    // gather layout: SELL-64 incidences, ascending spring index per mass
+    std::vector<int64_t> coff(nch + 1, 0);
+    std::vector<uint32_t> nbr;
+    std::vector<double> grest, gk, gd;
+    std::vector<int32_t> gact;
+    int64_t n_inc = 0;
+    if (!tiled) {
+        for (int64_t s = 0; s < S; ++s) {
+            if (d->left[s] < 0 || d->right[s] < 0) continue;
+            deg[d->left[s]]++;
+            deg[d->right[s]]++;
+        }
+        for (int64_t ch = 0; ch < nch; ++ch) {
+            int32_t w = 0;
+            for (int l = 0; l < kChunk; ++l) w = std::max(w, deg[ch * kChunk + l]);
+            coff[ch + 1] = coff[ch] + (int64_t)w * kChunk;
+        }
+        n_inc = coff[nch];
+    }
+
+    // ---- reserve space; on failure release whatever was taken
+    const int64_t n_tiles = tiled ? (int64_t)tb.tiles.size() : 0;
+    const int64_t n_halo = tiled ? (int64_t)tb.halo.size() : 0;
+    const int64_t n_spr = tiled ? (int64_t)tb.ends.size() : 0;
+    const int64_t n_grp = tiled ? (int64_t)tb.inc.size() / kChunk : 0;
+    int64_t ch0 = -1, inc0 = -1, act0 = -1, lc0 = -1, bid = -1, tl0 = -1, hl0 = -1, sp0 = -1, gp0 = -1;
+    auto take = [&](Ranges &r, int64_t len, void (*grow)(titan_ctx *, int64_t)) -> int64_t {
+        int64_t at = r.alloc(len);
+        if (at < 0) {
+            grow(c, r.high_water() + len);
+            at = r.alloc(len);
+        }
+        return at;
+    };
     try {
-        ch0 = c->chunks.alloc(nch);
-        if (ch0 < 0) {
-            ensure_chunks(c, c->chunks.high_water() + nch);
-            ch0 = c->chunks.alloc(nch);
-        }
-        if ((ch0 + nch) * kChunk > (int64_t)0x7fffffff)
-            fail(TITAN_E_NOMEM, "body_create: mass index space (2^31) exhausted");
-        inc0 = c->incs.alloc(n_inc);
-        if (inc0 < 0) {
-            ensure_incs(c, c->incs.high_water() + n_inc);
-            inc0 = c->incs.alloc(n_inc);
-        }
-        act0 = c->acts.alloc(n_act);
-        if (act0 < 0) {
-            ensure_acts(c, c->acts.high_water() + n_act);
-            act0 = c->acts.alloc(n_act);
-        }
-        lc0 = c->lcs.alloc(n_lc);
-        if (lc0 < 0) {
-            ensure_lcs(c, c->lcs.high_water() + n_lc);
-            lc0 = c->lcs.alloc(n_lc);
-        }
-        bid = c->bids.alloc(1);
-        if (bid < 0) {
-            ensure_bodies(c, c->bids.high_water() + 1);
-            bid = c->bids.alloc(1);
-        }
-        if (S > 0) {
+        ch0 = take(c->chunks, nch, ensure_chunks);
+        if ((ch0 + nch) * kChunk > ((int64_t)1 << 29))
+            fail(TITAN_E_NOMEM, "body_create: mass arena limit (2^29 slots) reached");
+        inc0 = take(c->incs, n_inc, ensure_incs);
+        act0 = take(c->acts, n_act, ensure_acts);
+        lc0 = take(c->lcs, n_lc, ensure_lcs);
+        tl0 = take(c->tiles, n_tiles, ensure_tiles);
+        hl0 = take(c->halos, n_halo, ensure_halos);
+        sp0 = take(c->sprs, n_spr, ensure_sprs);
+        gp0 = take(c->grps, n_grp, ensure_grps);
+        bid = take(c->bids, 1, ensure_bodies);
+        if (has_springs) {
             if (!c->kd_known) {
                 c->kd_known = true;
                 c->k_uni = k0;
                 c->damp_uni = d0;
                 c->uniform_kd = true;
                 if (!kd_uni) make_per_incidence_kd(c);
-            } else if (c->uniform_kd && (!kd_uni || !same_bits(k0, c->k_uni) || !same_bits(d0, c->damp_uni))) {
+            } else if (!uniform_after) {
                 make_per_incidence_kd(c);
             }
         }
@@ -597,25 +752,30 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
         if (inc0 >= 0) c->incs.release(inc0, n_inc);
         if (act0 >= 0) c->acts.release(act0, n_act);
         if (lc0 >= 0) c->lcs.release(lc0, n_lc);
+        if (tl0 >= 0) c->tiles.release(tl0, n_tiles);
+        if (hl0 >= 0) c->halos.release(hl0, n_halo);
+        if (sp0 >= 0) c->sprs.release(sp0, n_spr);
+        if (gp0 >= 0) c->grps.release(gp0, n_grp);
         if (bid >= 0) c->bids.release(bid, 1);
         throw;
     }
-
-    // stage mass arrays
     const int64_t g0 = ch0 * kChunk;
+
+    // ---- mass arrays in device order
     std::vector<double> st(6 * npad, 0.0), mm(npad, 1.0), Tm(npad, 0.0);
     std::vector<uint32_t> fl(npad, kPad);
     std::vector<double> ext;
     std::vector<int32_t> drag;
     std::vector<int64_t> mlc;
     bool any_ext = false, any_drag = false;
-    for (int64_t i = 0; i < N; ++i) {
+    for (int64_t q = 0; q < N; ++q) {
+        const int64_t i = orig(q);
         for (int k = 0; k < 3; ++k) {
-            st[k * npad + i] = d->pos[3 * i + k];
-            st[(3 + k) * npad + i] = d->vel ? d->vel[3 * i + k] : 0.0;
+            st[k * npad + q] = d->pos[3 * i + k];
+            st[(3 + k) * npad + q] = d->vel ? d->vel[3 * i + k] : 0.0;
         }
-        mm[i] = d->m[i];
-        Tm[i] = d->T ? d->T[i] : 0.0;
+        mm[q] = d->m[i];
+        Tm[q] = d->T ? d->T[i] : 0.0;
         uint32_t f = 0;
         if (d->fixed && d->fixed[i]) f |= kFixed;
         if (d->ext && (d->ext[3 * i] != 0 || d->ext[3 * i + 1] != 0 || d->ext[3 * i + 2] != 0)) {
@@ -626,7 +786,7 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
             f |= kDrag;
             any_drag = true;
         }
-        fl[i] = f;
+        fl[q] = f;
     }
     if (any_ext) {
         if (!c->ext) {
@@ -634,8 +794,8 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
             HIPCHK(hipMemsetAsync(c->ext, 0, sizeof(double) * 3 * c->ncap, c->stream));
         }
         ext.assign(3 * npad, 0.0);
-        for (int64_t i = 0; i < N; ++i)
-            for (int k = 0; k < 3; ++k) ext[k * npad + i] = d->ext[3 * i + k];
+        for (int64_t q = 0; q < N; ++q)
+            for (int k = 0; k < 3; ++k) ext[k * npad + q] = d->ext[3 * orig(q) + k];
     }
     if (any_drag) {
         if (!c->drag) {
@@ -643,7 +803,7 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
             HIPCHK(hipMemsetAsync(c->drag, 0, sizeof(int32_t) * c->ncap, c->stream));
         }
         drag.assign(npad, 0);
-        for (int64_t i = 0; i < N; ++i) drag[i] = d->drag[i];
+        for (int64_t q = 0; q < N; ++q) drag[q] = d->drag[orig(q)];
     }
     std::vector<LocalC> lch;
     if (any_lc) {
@@ -655,59 +815,63 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
                 HIPCHK(hipMemsetAsync(c->mlc, 0, sizeof(int64_t) * c->ncap, c->stream));
             }
             mlc.assign(npad, 0);
-            for (int64_t i = 0; i < N; ++i) {
-                if (lists[i].empty()) continue;
-                mlc[i] = ((lc0 + (int64_t)lch.size()) << 20) | (int64_t)lists[i].size();
-                fl[i] |= kLocalLC;
-                lch.insert(lch.end(), lists[i].begin(), lists[i].end());
+            for (int64_t q = 0; q < N; ++q) {
+                const auto &l = lists[orig(q)];
+                if (l.empty()) continue;
+                mlc[q] = ((lc0 + (int64_t)lch.size()) << 20) | (int64_t)l.size();
+                fl[q] |= kLocalLC;
+                lch.insert(lch.end(), l.begin(), l.end());
             }
         }
     }
 
-    // SELL-64 incidences, ascending spring index per mass
-    std::vector<uint32_t> nbr(n_inc, 0);
-    std::vector<double> rest(n_inc, 0.0), kk, dd;
-    std::vector<int32_t> ia;
-    std::vector<double> acth;
-    if (!c->uniform_kd) {
-        kk.assign(n_inc, 0.0);
-        dd.assign(n_inc, 0.0);
-    }
-    if (c->inc_act) ia.assign(n_inc, -1);
-    std::vector<int32_t> fillc(npad, 0);
-    for (int64_t s = 0; s < S; ++s) {
-        const int64_t L = d->left[s], R = d->right[s];
-        if (L < 0 || R < 0) continue;
-        int32_t ai = -1;
-        const int ty = d->type ? d->type[s] : 0;
-        if (ty >= TITAN_ACTIVE_CONTRACT_THEN_EXPAND) {
-            // the reference's two kinds are the general law with a = -/+0.2, phi = 0
-            // (src/sim.cu:1058-1062): 1 + (-0.2) s == 1 - 0.2 s and x + 0 == x exactly
-            const double aa = ty == TITAN_ACTIVE_CONTRACT_THEN_EXPAND ? -0.2
-                              : ty == TITAN_ACTIVE_EXPAND_THEN_CONTRACT ? 0.2 : d->act_a[s];
-            const double ph = ty == TITAN_SPRING_GENERAL ? d->act_phi[s] : 0.0;
-            ai = (int32_t)(act0 + (int64_t)acth.size() / 4);
-            acth.insert(acth.end(), {aa, d->omega[s], ph, 0.0});
+    // ---- gather layout incidences
+    if (!tiled && n_inc > 0) {
+        nbr.assign(n_inc, 0);
+        grest.assign(n_inc, 0.0);
+        if (!c->uniform_kd) {
+            gk.assign(n_inc, 0.0);
+            gd.assign(n_inc, 0.0);
         }
-        auto place = [&](int64_t me, int64_t other, bool is_left) {
-            const int64_t e = coff[me / kChunk] + (int64_t)fillc[me] * kChunk + (me % kChunk);
-            fillc[me]++;
-            nbr[e] = (uint32_t)(g0 + other) | (is_left ? kLeftBit : 0u);
-            rest[e] = d->rest[s];
-            if (!kk.empty()) {
-                kk[e] = d->k[s];
-                dd[e] = d->damping ? d->damping[s] : 0.0;
-            }
-            if (!ia.empty()) ia[e] = ai;
-        };
-        place(R, L, false);  // the scatter adds +F to the right endpoint first
-        place(L, R, true);
+        if (c->inc_act) gact.assign(n_inc, -1);
+        std::vector<int32_t> fillc(npad, 0);
+        for (int64_t s = 0; s < S; ++s) {
+            const int64_t L = d->left[s], R = d->right[s];
+            if (L < 0 || R < 0) continue;
+            const int32_t ai = act_local[s] >= 0 ? (int32_t)(act0 + act_local[s]) : -1;
+            auto place = [&](int64_t me, int64_t other, bool is_left) {
+                const int64_t e = coff[me / kChunk] + (int64_t)fillc[me] * kChunk + (me % kChunk);
+                fillc[me]++;
+                nbr[e] = (uint32_t)(g0 + other) | (is_left ? kLeftBit : 0u);
+                grest[e] = d->rest[s];
+                if (!gk.empty()) {
+                    gk[e] = d->k[s];
+                    gd[e] = d->damping ? d->damping[s] : 0.0;
+                }
+                if (!gact.empty()) gact[e] = ai;
+            };
+            place(R, L, false);  // the scatter adds +F to the right endpoint first
+            place(L, R, true);
+        }
     }
+    // ---- tiled layout: rebase to the arenas
+    if (tiled) {
+        for (auto &t : tb.tiles) {
+            t.body = (int32_t)bid;
+            t.chunk0 += (int32_t)ch0;
+            t.halo_off += hl0;
+            t.spr_off += sp0;
+            t.inc_off += gp0 * kChunk;
+        }
+        for (auto &h : tb.halo) h += (int32_t)g0;
+        for (auto &x : tb.act)
+            if (x >= 0) x += (int32_t)act0;
+    }
+
+    // ---- upload (buffer parity 0: a new body has taken no steps)
     std::vector<int32_t> cbody(nch, (int32_t)bid);
     std::vector<int64_t> cptr(nch);
     for (int64_t ch = 0; ch < nch; ++ch) cptr[ch] = inc0 + coff[ch];
-
-    // upload (buffer parity 0: a new body has taken no steps)
     for (int k = 0; k < 6; ++k) h2d(c, c->st + k * c->ncap + g0, st.data() + k * npad, npad);
     h2d(c, c->m + g0, mm.data(), npad);
     h2d(c, c->Tm + g0, Tm.data(), npad);
@@ -716,31 +880,50 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     std::vector<double> zeros(3 * npad, 0.0);
     for (int k = 0; k < 3; ++k) h2d(c, c->acc + k * c->ncap + g0, zeros.data(), npad);
     if (c->ext) {
-        const double *src = any_ext ? ext.data() : zeros.data();
-        for (int k = 0; k < 3; ++k) h2d(c, c->ext + k * c->ncap + g0, src + (any_ext ? k * npad : 0), npad);
+        for (int k = 0; k < 3; ++k)
+            h2d(c, c->ext + k * c->ncap + g0, any_ext ? ext.data() + k * npad : zeros.data(), npad);
     }
-    if (c->drag) {
-        std::vector<int32_t> z(npad, 0);
-        h2d(c, c->drag + g0, any_drag ? drag.data() : z.data(), npad);
-    }
-    if (c->mlc) {
-        std::vector<int64_t> z(npad, 0);
-        h2d(c, c->mlc + g0, mlc.empty() ? z.data() : mlc.data(), npad);
-    }
+    std::vector<int32_t> zi(npad, 0);
+    std::vector<int64_t> zl(npad, 0);
+    if (c->drag) h2d(c, c->drag + g0, any_drag ? drag.data() : zi.data(), npad);
+    if (c->mlc) h2d(c, c->mlc + g0, mlc.empty() ? zl.data() : mlc.data(), npad);
     h2d(c, c->chunk_body + ch0, cbody.data(), nch);
     h2d(c, c->chunk_ptr + ch0, cptr.data(), nch);
-    h2d(c, c->inc_nbr + inc0, nbr.data(), n_inc);
-    h2d(c, c->inc_rest + inc0, rest.data(), n_inc);
-    if (!kk.empty()) {
-        h2d(c, c->inc_k + inc0, kk.data(), n_inc);
-        h2d(c, c->inc_damp + inc0, dd.data(), n_inc);
+    if (n_inc > 0) {
+        h2d(c, c->inc_nbr + inc0, nbr.data(), n_inc);
+        h2d(c, c->inc_rest + inc0, grest.data(), n_inc);
+        if (!gk.empty()) {
+            h2d(c, c->inc_k + inc0, gk.data(), n_inc);
+            h2d(c, c->inc_damp + inc0, gd.data(), n_inc);
+        }
+        if (!gact.empty()) h2d(c, c->inc_act + inc0, gact.data(), n_inc);
     }
-    if (!ia.empty()) h2d(c, c->inc_act + inc0, ia.data(), n_inc);
+    if (tiled) {
+        h2d(c, c->tile + tl0, tb.tiles.data(), n_tiles);
+        h2d(c, c->halo + hl0, tb.halo.data(), n_halo);
+        h2d(c, c->t_ends + sp0, tb.ends.data(), n_spr);
+        h2d(c, c->t_rest + sp0, tb.rest.data(), n_spr);
+        if (!c->uniform_kd) {
+            if ((int64_t)tb.k.size() == n_spr) {
+                h2d(c, c->t_k + sp0, tb.k.data(), n_spr);
+                h2d(c, c->t_damp + sp0, tb.damp.data(), n_spr);
+            } else {  // built for by-value constants: this body's are uniform
+                HIPCHK(launch_fill_f64(c->t_k + sp0, n_spr, k0, c->stream));
+                HIPCHK(launch_fill_f64(c->t_damp + sp0, n_spr, d0, c->stream));
+            }
+        }
+        if (c->t_act) {
+            if (!tb.act.empty()) h2d(c, c->t_act + sp0, tb.act.data(), n_spr);
+            else if (n_spr > 0) HIPCHK(hipMemsetAsync(c->t_act + sp0, 0xff, sizeof(int32_t) * n_spr, c->stream));
+        }
+        h2d(c, c->t_inc + gp0 * kChunk, tb.inc.data(), n_grp * kChunk);
+    }
     if (!acth.empty()) h2d(c, c->act + 4 * act0, acth.data(), (int64_t)acth.size());
     if (!lch.empty()) h2d(c, c->lc + lc0, lch.data(), (int64_t)lch.size());
     const double t0 = d->t0;
     const int64_t zero = 0;
     const int32_t c0 = (int32_t)ch0, lcb = (int32_t)(shared ? lc0 : 0), lcn = (int32_t)(shared ? n_lc : 0);
+    const uint8_t tflag = tiled ? 1 : 0;
     for (int s = 0; s < 2; ++s) {
         h2d(c, c->bT[s] + bid, &t0, 1);
         h2d(c, c->bn[s] + bid, &zero, 1);
@@ -750,10 +933,13 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     h2d(c, c->b_chunk0 + bid, &c0, 1);
     h2d(c, c->b_lc + bid, &lcb, 1);
     h2d(c, c->b_lcn + bid, &lcn, 1);
+    h2d(c, c->b_tiled + bid, &tflag, 1);
     HIPCHK(hipStreamSynchronize(c->stream));
 
     BodyH &b = c->bodies[bid];
+    b = BodyH{};
     b.live = true;
+    b.tiled = tiled;
     b.n_masses = N;
     b.chunk0 = ch0;
     b.nchunks = nch;
@@ -763,10 +949,20 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     b.act_len = n_act;
     b.lc0 = lc0;
     b.lc_len = n_lc;
+    b.tile0 = tl0;
+    b.ntiles = n_tiles;
+    b.halo0 = hl0;
+    b.nhalo = n_halo;
+    b.spr0 = sp0;
+    b.nspr = n_spr;
+    b.grp0 = gp0;
+    b.ngrp = n_grp;
+    b.hmax = tb.hmax;
     b.T = t0;
     b.n = 0;
     b.flags.assign(fl.begin(), fl.begin() + N);
-    drop_graphs(c);  // the launch grid covers the new chunks
+    b.perm = std::move(perm);
+    drop_graphs(c);  // the launch grids cover the new chunks and tiles
     return (int32_t)bid;
 }
 
@@ -775,16 +971,39 @@ void body_destroy(titan_ctx *c, int32_t id) {
     HIPCHK(hipStreamSynchronize(c->stream));
     std::vector<int32_t> neg(b.nchunks, -1);
     h2d(c, c->chunk_body + b.chunk0, neg.data(), b.nchunks);
+    if (b.ntiles > 0) {
+        std::vector<TileDesc> dead(b.ntiles);
+        for (auto &t : dead) {
+            t = TileDesc{};
+            t.body = -1;
+        }
+        h2d(c, c->tile + b.tile0, dead.data(), b.ntiles);
+    }
     const int32_t m1 = -1;
+    const uint8_t z = 0;
     h2d(c, c->b_chunk0 + id, &m1, 1);
+    h2d(c, c->b_tiled + id, &z, 1);
     HIPCHK(hipStreamSynchronize(c->stream));
     c->chunks.release(b.chunk0, b.nchunks);
     c->incs.release(b.inc0, b.inc_len);
     c->acts.release(b.act0, b.act_len);
     c->lcs.release(b.lc0, b.lc_len);
+    c->tiles.release(b.tile0, b.ntiles);
+    c->halos.release(b.halo0, b.nhalo);
+    c->sprs.release(b.spr0, b.nspr);
+    c->grps.release(b.grp0, b.ngrp);
     c->bids.release(id, 1);
     b = BodyH{};
     drop_graphs(c);
+}
+
+// device order -> caller's order for a (N, K) host array read from K SoA rows
+void unpermute(const BodyH &b, const std::vector<double> &soa, int K, double *out) {
+    const int64_t N = b.n_masses;
+    for (int64_t q = 0; q < N; ++q) {
+        const int64_t i = b.perm.empty() ? q : b.perm[q];
+        for (int k = 0; k < K; ++k) out[K * i + k] = soa[k * N + q];
+    }
 }
 
 void body_download(titan_ctx *c, int32_t id, double *pos, double *vel, double *acc, double *T) {
@@ -792,20 +1011,16 @@ void body_download(titan_ctx *c, int32_t id, double *pos, double *vel, double *a
     const int64_t N = b.n_masses, g0 = b.chunk0 * kChunk, n6 = c->ncap;
     const int p = (int)(b.n & 1);
     std::vector<double> tmp(3 * N);
-    auto fetch3 = [&](const double *base, double *out) {
-        for (int k = 0; k < 3; ++k) d2h(c, tmp.data() + k * N, base + k * n6 + g0, N);
+    auto fetch = [&](const double *base, int K, double *out) {
+        for (int k = 0; k < K; ++k) d2h(c, tmp.data() + k * N, base + k * n6 + g0, N);
         HIPCHK(hipStreamSynchronize(c->stream));
-        for (int64_t i = 0; i < N; ++i)
-            for (int k = 0; k < 3; ++k) out[3 * i + k] = tmp[k * N + i];
+        unpermute(b, tmp, K, out);
     };
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (pos) fetch3(c->st + (int64_t)p * 6 * n6, pos);
-    if (vel) fetch3(c->st + (int64_t)p * 6 * n6 + 3 * n6, vel);
-    if (acc) fetch3(c->acc, acc);
-    if (T) {
-        d2h(c, T, c->Tm + g0, N);
-        HIPCHK(hipStreamSynchronize(c->stream));
-    }
+    if (pos) fetch(c->st + (int64_t)p * 6 * n6, 3, pos);
+    if (vel) fetch(c->st + (int64_t)p * 6 * n6 + 3 * n6, 3, vel);
+    if (acc) fetch(c->acc, 3, acc);
+    if (T) fetch(c->Tm, 1, T);
 }
 
 void body_upload(titan_ctx *c, int32_t id, const double *pos, const double *vel, const double *m,
@@ -815,29 +1030,26 @@ void body_upload(titan_ctx *c, int32_t id, const double *pos, const double *vel,
     const int p = (int)(b.n & 1);
     HIPCHK(hipStreamSynchronize(c->stream));
     std::vector<double> tmp(3 * N);
-    auto put3 = [&](double *base, const double *in) {
-        for (int64_t i = 0; i < N; ++i)
-            for (int k = 0; k < 3; ++k) tmp[k * N + i] = in[3 * i + k];
-        for (int k = 0; k < 3; ++k) h2d(c, base + k * n6 + g0, tmp.data() + k * N, N);
+    auto put = [&](double *base, int K, const double *in) {
+        for (int64_t q = 0; q < N; ++q) {
+            const int64_t i = b.perm.empty() ? q : b.perm[q];
+            for (int k = 0; k < K; ++k) tmp[k * N + q] = in[K * i + k];
+        }
+        for (int k = 0; k < K; ++k) h2d(c, base + k * n6 + g0, tmp.data() + k * N, N);
         HIPCHK(hipStreamSynchronize(c->stream));
     };
-    if (pos) put3(c->st + (int64_t)p * 6 * n6, pos);
-    if (vel) put3(c->st + (int64_t)p * 6 * n6 + 3 * n6, vel);
-    if (m) {
-        h2d(c, c->m + g0, m, N);
-        HIPCHK(hipStreamSynchronize(c->stream));
-    }
-    if (T) {
-        h2d(c, c->Tm + g0, T, N);
-        HIPCHK(hipStreamSynchronize(c->stream));
-    }
+    if (pos) put(c->st + (int64_t)p * 6 * n6, 3, pos);
+    if (vel) put(c->st + (int64_t)p * 6 * n6 + 3 * n6, 3, vel);
+    if (m) put(c->m, 1, m);
+    if (T) put(c->Tm, 1, T);
     if (ext) {
         bool changed = false;
-        for (int64_t i = 0; i < N; ++i) {
+        for (int64_t q = 0; q < N; ++q) {
+            const int64_t i = b.perm.empty() ? q : b.perm[q];
             const bool nz = ext[3 * i] != 0 || ext[3 * i + 1] != 0 || ext[3 * i + 2] != 0;
-            const uint32_t f = nz ? (b.flags[i] | kExt) : (b.flags[i] & ~kExt);
-            if (f != b.flags[i]) {
-                b.flags[i] = f;
+            const uint32_t f = nz ? (b.flags[q] | kExt) : (b.flags[q] & ~kExt);
+            if (f != b.flags[q]) {
+                b.flags[q] = f;
                 changed = true;
             }
         }
@@ -845,7 +1057,7 @@ void body_upload(titan_ctx *c, int32_t id, const double *pos, const double *vel,
             c->ext = dalloc<double>(c, 3 * c->ncap);
             HIPCHK(hipMemsetAsync(c->ext, 0, sizeof(double) * 3 * c->ncap, c->stream));
         }
-        put3(c->ext, ext);
+        put(c->ext, 3, ext);
         if (changed) {
             h2d(c, c->mflags + g0, b.flags.data(), N);
             HIPCHK(hipStreamSynchronize(c->stream));
@@ -905,6 +1117,7 @@ int titan_ctx_create(int device, uint64_t hbm_budget_bytes, titan_ctx **out) {
     c->device = device;
     if (const char *u = std::getenv("TITAN_GATHER_UNROLL")) c->unroll = std::atoi(u);
     c->budget = hbm_budget_bytes;
+    HIPCHK(set_tiles_lds_limit());
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
@@ -1121,7 +1334,19 @@ int titan_mem_used(const titan_ctx *c, uint64_t *bytes) {
 int titan_kernel_name(const titan_ctx *c, char *buf, int64_t len) {
     NEED_CTX(c);
     if (!buf || len <= 0) return TITAN_E_INVALID;
-    std::snprintf(buf, (size_t)len, "%s", step_kernel_name(c->uniform_kd, c->inc_act != nullptr));
+    const bool act = c->inc_act != nullptr;
+    std::string n;
+    if (any_body(c, false)) n = step_kernel_name(c->uniform_kd, act);
+    if (any_body(c, true)) n += std::string(n.empty() ? "" : "+") + step_tiles_kernel_name(c->uniform_kd, act);
+    std::snprintf(buf, (size_t)len, "%s", n.c_str());
+    return TITAN_OK;
+}
+
+int titan_body_layout(const titan_ctx *c, int32_t body_id, int32_t *layout) {
+    NEED_CTX(c);
+    if (!layout) return TITAN_E_INVALID;
+    if (body_id < 0 || body_id >= (int64_t)c->bodies.size() || !c->bodies[body_id].live) return TITAN_E_NOTFOUND;
+    *layout = c->bodies[body_id].tiled ? TITAN_LAYOUT_TILED : TITAN_LAYOUT_GATHER;
     return TITAN_OK;
 }
 

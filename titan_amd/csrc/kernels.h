@@ -9,6 +9,10 @@ namespace titan_mi355x {
 
 hipError_t launch_step(const StepArgs &a, bool uniform_kd, bool actuated, int unroll, hipStream_t s);
 const char *step_kernel_name(bool uniform_kd, bool actuated);
+hipError_t launch_step_tiles(const StepArgs &a, bool uniform_kd, bool actuated, hipStream_t s);
+const char *step_tiles_kernel_name(bool uniform_kd, bool actuated);
+size_t tiles_lds_bytes(int ldim);
+hipError_t set_tiles_lds_limit();
 hipError_t launch_advance_T(double *Tm, const uint32_t *mflags, const int32_t *chunk_body,
                             const int64_t *bn, int64_t *bnsync, int64_t nchunks, int64_t nb,
                             double dt, hipStream_t s);

@@ -252,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     if (chunk >= a.nchunks) return;
     const int lane = threadIdx.x & (kChunk - 1);
     const int b = uniform(a.chunk_body[chunk]);
-    if (b < 0) return;
+    if (b < 0 || a.b_tiled[b]) return;
     const int s = a.slot;
     const int64_t left = a.bleft[s][b];
     const int64_t nst = a.bn[s][b];
@@ -359,6 +359,173 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     }
 }
 
+// Tiled step: one workgroup per tile of up to 256 masses of one body.
+//   stage:   tile and halo positions/velocities -> LDS (SoA rows of ldim);
+//   batches: springs of the tile in ascending index, kTileBatch at a time:
+//            each spring's force F (the reference's orientation, on the
+//            right endpoint) -> LDS slot; then every tile mass adds the
+//            slots of its incident springs that fall in the batch, +F as the
+//            right endpoint and -F as the left one, in ascending spring
+//            index (so the sum is the oracle's, bit for bit);
+//   update:  gravity, extern force, planes, balls, local constraints, drag,
+//            Euler, written to the other state buffer.
+// A spring with both endpoints in the tile is evaluated once; one that
+// crosses the tile boundary is evaluated by both tiles.
+template <bool UNI, bool ACT>
+__global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
+    extern __shared__ double lds[];
+    const int64_t tile = xcd_block(blockIdx.x, gridDim.x);
+    if (tile >= a.ntiles) return;
+    const int t = threadIdx.x;
+    const TileDesc *__restrict__ td = a.tiles + tile;
+    const int b = td->body;
+    if (b < 0) return;
+    const int s = a.slot;
+    const int64_t left = a.bleft[s][b];
+    const int64_t nst = a.bn[s][b];
+    const double T = a.bT[s][b];
+    if (t == 0 && td->first) {  // the body's first tile advances its clock
+        const bool go = left > 0;
+        a.bT[s ^ 1][b] = go ? T + a.dt : T;  // src/sim.cu:1590
+        a.bn[s ^ 1][b] = go ? nst + 1 : nst;
+        a.bleft[s ^ 1][b] = go ? left - 1 : 0;
+    }
+    if (left <= 0) return;  // uniform over the workgroup
+
+    const int64_t n6 = a.ncap;
+    const int p = (int)(nst & 1);
+    const double *__restrict__ cur = a.st + (int64_t)p * 6 * n6;
+    double *__restrict__ nxt = a.st + (int64_t)(p ^ 1) * 6 * n6;
+    const int L = a.ldim;
+    double *sx = lds, *sy = lds + L, *sz = lds + 2 * L, *svx = lds + 3 * L, *svy = lds + 4 * L,
+           *svz = lds + 5 * L;
+    double *Fx = lds + 6 * L, *Fy = Fx + kTileBatch, *Fz = Fy + kTileBatch;
+
+    const int nmass = td->nmass, nhalo = td->nhalo, nspr = td->nspr;
+    const int64_t i = (int64_t)td->chunk0 * kChunk + t;
+    const bool mine = t < nmass;
+    MassRegs r{};
+    uint32_t fl = kPad;
+    if (mine) {
+        fl = a.mflags[i];
+        r.px = cur[i];
+        r.py = cur[n6 + i];
+        r.pz = cur[2 * n6 + i];
+        r.vx = cur[3 * n6 + i];
+        r.vy = cur[4 * n6 + i];
+        r.vz = cur[5 * n6 + i];
+    }
+    sx[t] = r.px;
+    sy[t] = r.py;
+    sz[t] = r.pz;
+    svx[t] = r.vx;
+    svy[t] = r.vy;
+    svz[t] = r.vz;
+    const int32_t *__restrict__ hl = a.halo + td->halo_off;
+    for (int h = t; h < nhalo; h += kTileMasses) {
+        const uint32_t ob = (uint32_t)hl[h] * 8u;
+        const int q = kTileMasses + h;
+        sx[q] = ldo(cur, ob);
+        sy[q] = ldo(cur + n6, ob);
+        sz[q] = ldo(cur + 2 * n6, ob);
+        svx[q] = ldo(cur + 3 * n6, ob);
+        svy[q] = ldo(cur + 4 * n6, ob);
+        svz[q] = ldo(cur + 5 * n6, ob);
+    }
+
+    // this lane's incidence list: groups of four uint16 entries
+    const bool accumulate = mine && !(fl & (kFixed | kPad));
+    const int deg = accumulate ? a.deg[i] : 0;
+    const int chunk_in_tile = t >> 6, lane = t & (kChunk - 1);
+    const uint64_t *__restrict__ inc =
+        a.t_inc + td->inc_off + (int64_t)td->gstart[chunk_in_tile] * kChunk + lane;
+    uint64_t g0 = deg > 0 ? inc[0] : 0, g1 = deg > 4 ? inc[kChunk] : 0;
+    int j = 0;
+    r.fx = 0.0;
+    r.fy = 0.0;
+    r.fz = 0.0;
+
+    const uint32_t *__restrict__ ends = a.t_ends + td->spr_off;
+    const double *__restrict__ rest = a.t_rest + td->spr_off;
+    __syncthreads();
+    for (int bs = 0; bs < nspr; bs += kTileBatch) {
+        const int bl = min(kTileBatch, nspr - bs);
+        for (int q = t; q < bl; q += kTileMasses) {
+            const int e = bs + q;
+            const uint32_t lr = ends[e];
+            const int li = (int)(lr >> 16), ri = (int)(lr & 0xffff);
+            // src/sim.cu:1055-1065
+            const double tx = sx[ri] - sx[li], ty = sy[ri] - sy[li], tz = sz[ri] - sz[li];
+            const double nrm = sqrt(tx * tx + ty * ty + tz * tz);
+            const double ux = tx / nrm, uy = ty / nrm, uz = tz / nrm;
+            double rl = rest[e];
+            if (ACT) {
+                const int32_t ai = a.t_act[td->spr_off + e];
+                if (ai >= 0) {
+                    const double *A = a.act + 4 * (int64_t)ai;
+                    rl = rl * (1 + A[0] * sin(A[1] * T + A[2]));
+                }
+            }
+            const double k = UNI ? a.k_uni : a.t_k[td->spr_off + e];
+            const double dmp = UNI ? a.damp_uni : a.t_damp[td->spr_off + e];
+            const double mag = k * (rl - nrm);
+            const double dvx = svx[li] - svx[ri], dvy = svy[li] - svy[ri], dvz = svz[li] - svz[ri];
+            const double dd = (dvx * ux + dvy * uy + dvz * uz) * dmp;
+            Fx[q] = ux * mag + ux * dd;
+            Fy[q] = uy * mag + uy * dd;
+            Fz[q] = uz * mag + uz * dd;
+        }
+        __syncthreads();
+        const int hi = bs + bl;
+        while (j < deg) {
+            const uint32_t ent = (uint32_t)(g0 >> (16 * (j & 3))) & 0xffffu;
+            const int slot = (int)(ent & 0x7fffu);
+            if (slot >= hi) break;
+            const int q = slot - bs;
+            if (ent & kIncLeft) {  // src/sim.cu:1071-1073: left endpoint gets -F
+                r.fx = r.fx - Fx[q];
+                r.fy = r.fy - Fy[q];
+                r.fz = r.fz - Fz[q];
+            } else {               // src/sim.cu:1068-1070: right endpoint gets +F
+                r.fx = r.fx + Fx[q];
+                r.fy = r.fy + Fy[q];
+                r.fz = r.fz + Fz[q];
+            }
+            ++j;
+            if ((j & 3) == 0) {
+                g0 = g1;
+                g1 = (j + 4 < deg) ? inc[(int64_t)(j / 4 + 1) * kChunk] : 0;
+            }
+        }
+        __syncthreads();
+    }
+
+    if (!mine) return;
+    if (fl & kFixed) {  // src/sim.cu:1100-1101
+        nxt[i] = r.px;
+        nxt[n6 + i] = r.py;
+        nxt[2 * n6 + i] = r.pz;
+        nxt[3 * n6 + i] = r.vx;
+        nxt[4 * n6 + i] = r.vy;
+        nxt[5 * n6 + i] = r.vz;
+        return;
+    }
+    const double m = a.m[i];
+    double ax, ay, az;
+    mass_pass(a, r, i, fl, b, m, ax, ay, az);
+    nxt[i] = r.px;
+    nxt[n6 + i] = r.py;
+    nxt[2 * n6 + i] = r.pz;
+    nxt[3 * n6 + i] = r.vx;
+    nxt[4 * n6 + i] = r.vy;
+    nxt[5 * n6 + i] = r.vz;
+    if (a.write_acc) {
+        a.acc[i] = ax;
+        a.acc[n6 + i] = ay;
+        a.acc[2 * n6 + i] = az;
+    }
+}
+
 // Per-mass local time: T += dt once per step the body took since the last
 // sync (src/sim.cu:1165), for every non-fixed mass.
 __global__ void k_advance_T(double *Tm, const uint32_t *mflags, const int32_t *chunk_body,
@@ -408,6 +575,40 @@ hipError_t launch_step(const StepArgs &a, bool uni, bool act, int unroll, hipStr
     else if (unroll == 2) launch_u<2>(a, uni, act, blocks, s);
     else launch_u<1>(a, uni, act, blocks, s);
     return hipGetLastError();
+}
+
+size_t tiles_lds_bytes(int ldim) {
+    return sizeof(double) * ((size_t)6 * ldim + (size_t)3 * kTileBatch);
+}
+
+hipError_t launch_step_tiles(const StepArgs &a, bool uni, bool act, hipStream_t s) {
+    if (a.ntiles <= 0) return hipSuccess;
+    const size_t lds = tiles_lds_bytes(a.ldim);
+    const dim3 g((unsigned)a.ntiles), blk(kTileMasses);
+    if (uni && !act) hipLaunchKernelGGL((k_step_tiles<true, false>), g, blk, lds, s, a);
+    else if (uni && act) hipLaunchKernelGGL((k_step_tiles<true, true>), g, blk, lds, s, a);
+    else if (!uni && !act) hipLaunchKernelGGL((k_step_tiles<false, false>), g, blk, lds, s, a);
+    else hipLaunchKernelGGL((k_step_tiles<false, true>), g, blk, lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t set_tiles_lds_limit() {
+    const int bytes = (int)tiles_lds_bytes(kTileMasses + kMaxHalo);
+    hipError_t e = hipSuccess;
+    e = hipFuncSetAttribute((const void *)k_step_tiles<true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void *)k_step_tiles<true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void *)k_step_tiles<false, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void *)k_step_tiles<false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+const char *step_tiles_kernel_name(bool uni, bool act) {
+    if (uni && !act) return "k_step_tiles<uniform_kd,passive>";
+    if (uni && act) return "k_step_tiles<uniform_kd,actuated>";
+    if (!uni && !act) return "k_step_tiles<per_spring_kd,passive>";
+    return "k_step_tiles<per_spring_kd,actuated>";
 }
 
 const char *step_kernel_name(bool uni, bool act) {

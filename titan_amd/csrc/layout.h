@@ -32,6 +32,32 @@ constexpr uint32_t kPad = 16u;      // padding slot of a chunk (never read)
 // bit 31 of inc_nbr: this mass is the spring's left endpoint (receives -F)
 constexpr uint32_t kLeftBit = 0x80000000u;
 
+// Tiled layout (k_step_tiles): a tile is up to 4 chunks (256 masses) of one
+// body.  Its spring list holds every spring with an endpoint in the tile,
+// in ascending spring index; the endpoints are tile-local indices (0..255 =
+// tile masses, 256 + h = halo mass h).  Each tile mass lists its incidences
+// as uint16 (slot | 0x8000 if it is the left endpoint), four per lane in a
+// uint64 group, groups of one chunk stored lane-major.
+constexpr int kTileMasses = 256;
+constexpr int kTileBatch = 1024;        // force slots per LDS batch
+constexpr int kMaxTileSprings = 32767;  // slot must fit in 15 bits
+constexpr int kMaxHalo = 1536;          // LDS budget for staged halo masses
+constexpr uint16_t kIncLeft = 0x8000;
+
+struct TileDesc {
+    int32_t body;
+    int32_t chunk0;     // first chunk (arena index once placed)
+    int32_t nmass;      // masses in the tile (<= 256)
+    int32_t nhalo;
+    int32_t nspr;
+    int32_t first;      // 1 for the body's first tile (advances the body clock)
+    int64_t halo_off;   // into the halo arena
+    int64_t spr_off;    // into the tile-spring arenas
+    int64_t inc_off;    // into the incidence-group arena (uint64 units)
+    int32_t gstart[4];  // per chunk: first group, relative to inc_off / 64
+    int32_t pad[4];
+};
+
 // local constraint entry (a reference CudaContactPlane / CudaBall /
 // CudaConstraintPlane / CudaDirection, src/object.cu:53-141)
 struct LocalC {
@@ -80,6 +106,18 @@ struct StepArgs {
     double gx, gy, gz, dt;
     int32_t slot;
     int32_t write_acc;
+    // tiled path
+    const TileDesc *tiles;
+    int64_t ntiles;
+    const int32_t *halo;
+    const uint32_t *t_ends;
+    const double *t_rest;
+    const double *t_k;        // null when uniform
+    const double *t_damp;     // null when uniform
+    const int32_t *t_act;     // null when no actuators
+    const uint64_t *t_inc;
+    int32_t ldim;             // LDS row length: kTileMasses + largest halo
+    const uint8_t *b_tiled;   // per body: 1 = stepped by k_step_tiles
 };
 
 }  // namespace titan_mi355x

@@ -47,14 +47,14 @@ class Scene:
                 n += int(np.count_nonzero(np.asarray(t) >= 2))
         return n
 
-    def load(self, ctx: Context) -> list[int]:
+    def load(self, ctx: Context, layout: int = 0) -> list[int]:
         ctx.set_dt(self.dt)
         ctx.set_global_acc(self.g)
         for p in self.planes:
             ctx.add_plane(p[:3], p[3], p[4], p[5])
         for b in self.balls:
             ctx.add_ball(b[:3], b[3])
-        return [ctx.add_body(**b) for b in self.bodies]
+        return [ctx.add_body(**b, layout=layout) for b in self.bodies]
 
     def algorithmic_bytes_per_step(self) -> int:
         """104 B per mass + 32 B per spring + 24 B per actuated spring (+24 B per
