@@ -91,6 +91,8 @@ def main() -> int:
     ap.add_argument("--side", type=int, default=100)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--layout", choices=["auto", "gather", "tiled"], default="auto",
+                    help="device layout of the body (results are identical; speed differs)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times (the "
                          "reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -103,10 +105,16 @@ def main() -> int:
         a = torch.ones(n, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
         torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        b.copy_(a)
+        e0.record()
         for _ in range(3):
             b.copy_(a)
+        e1.record()
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "copies": 3}))
+        ms = e0.elapsed_time(e1) / 3
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "copies": 4,
+                          "copy_ms": ms, "copy_GBps": 2 * n * 8 / ms / 1e6}))
         return 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -125,7 +133,7 @@ def main() -> int:
 
     ctx = Context()
     t0 = time.perf_counter()
-    (bid,) = scene.load(ctx)
+    (bid,) = scene.load(ctx, layout={"auto": 0, "gather": 1, "tiled": 2}[args.layout])
     ctx.sync()
     log(f"uploaded in {time.perf_counter() - t0:.1f} s; device bytes {ctx.mem_used()}; kernel {ctx.kernel_name()}")
 

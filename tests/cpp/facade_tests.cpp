@@ -224,9 +224,12 @@ static void control_test() {
     sim.resume();
     sim.waitForEvent();
     EXPECT_TRUE(sim.time() >= 0.003);
-    // a mass added while paused joins the body
+    // a mass added while paused joins the body; like the reference's
+    // createMass (src/sim.cu:236-252) it is copied at creation, so a force set
+    // afterwards reaches the device through set()
     titan::Mass *c = sim.createMass(Vec(1, 0, 1));
     c->setExternalForce(Vec(0.1, 0, 0));
+    sim.set(c);
     sim.resume();
     sim.wait(0.01);
     sim.get(c);

@@ -156,6 +156,7 @@ struct titan_ctx {
     int64_t planes_cap = 0, balls_cap = 0;
 
     bool kd_known = false, uniform_kd = true;
+    bool has_act = false;  // some spring is actuated: launch the actuated variants
     double k_uni = 0, damp_uni = 0;
     int slot = 0;
     int unroll = 2;  // incident springs per gather iteration (TITAN_GATHER_UNROLL)
@@ -238,12 +239,12 @@ void ensure_chunks(titan_ctx *c, int64_t need) {
     if (c->chunks.cap() >= need) return;
     const int64_t old = c->chunks.cap();
     int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 16));
-    // the kernels address one state component with a 32-bit byte offset
-    const int64_t max_chunks = ((int64_t)1 << 29) / kChunk;
-    if (need > max_chunks) fail(TITAN_E_NOMEM, "mass arena limit: at most 2^29 mass slots");
+    // the kernels address a state buffer with a 32-bit byte offset
+    const int64_t max_chunks = kMaxMassSlots / kChunk;
+    if (need > max_chunks) fail(TITAN_E_NOMEM, "mass arena limit: at most 2^26 mass slots");
     nc = std::min(nc, max_chunks);
     const int64_t on = old * kChunk, nn = nc * kChunk;
-    grow_soa(c, c->st, 12, on, nn);
+    grow_soa(c, c->st, 2, 6 * on, 6 * nn);
     grow_soa(c, c->m, 1, on, nn);
     grow_soa(c, c->Tm, 1, on, nn);
     grow_soa(c, c->acc, 3, on, nn);
@@ -378,7 +379,9 @@ void ensure_grps(titan_ctx *c, int64_t need) {
 // Switch from by-value spring constants to per-spring arrays (both layouts).
 void make_per_incidence_kd(titan_ctx *c) {
     if (!c->uniform_kd) return;
-    const int64_t n = c->incs.cap(), m = c->sprs.cap();
+    // at least one entry each, so that the arrays exist (and grow with their
+    // arenas) even while one of the two layouts holds no springs
+    const int64_t n = std::max<int64_t>(c->incs.cap(), 1), m = std::max<int64_t>(c->sprs.cap(), 1);
     c->inc_k = dalloc<double>(c, n);
     c->inc_damp = dalloc<double>(c, n);
     HIPCHK(launch_fill_f64(c->inc_k, n, c->k_uni, c->stream));
@@ -392,8 +395,9 @@ void make_per_incidence_kd(titan_ctx *c) {
 }
 
 void make_act(titan_ctx *c) {
-    if (c->inc_act) return;
-    const int64_t n = c->incs.cap(), m = c->sprs.cap();
+    if (c->has_act) return;
+    c->has_act = true;
+    const int64_t n = std::max<int64_t>(c->incs.cap(), 1), m = std::max<int64_t>(c->sprs.cap(), 1);
     c->inc_act = dalloc<int32_t>(c, n);
     HIPCHK(hipMemsetAsync(c->inc_act, 0xff, sizeof(int32_t) * n, c->stream));
     c->t_act = dalloc<int32_t>(c, m);
@@ -486,7 +490,7 @@ StepArgs make_args(titan_ctx *c, int slot, int write_acc) {
 // `slot` and write slot ^ 1).
 hipError_t launch_both(titan_ctx *c, int slot, int write_acc) {
     StepArgs a = make_args(c, slot, write_acc);
-    const bool act = c->inc_act != nullptr;
+    const bool act = c->has_act;
     if (any_body(c, false)) {
         hipError_t e = launch_step(a, c->uniform_kd, act, c->unroll, c->stream);
         if (e != hipSuccess) return e;
@@ -570,7 +574,7 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     if (!d) fail(TITAN_E_INVALID, "body_create: null descriptor");
     const int64_t N = d->n_masses, S = d->n_springs;
     if (N <= 0) fail(TITAN_E_INVALID, "body_create: a body needs at least one mass");
-    if (N > ((int64_t)1 << 29)) fail(TITAN_E_INVALID, "body_create: too many masses in one body");
+    if (N > kMaxMassSlots) fail(TITAN_E_INVALID, "body_create: too many masses in one body");
     if (S < 0 || S > (int64_t)0x7fffffff) fail(TITAN_E_INVALID, "body_create: bad spring count");
     if (!d->pos || !d->m) fail(TITAN_E_INVALID, "body_create: pos and m are required");
     if (S > 0 && (!d->left || !d->right || !d->k || !d->rest))
@@ -704,7 +708,7 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
         for (int64_t ch = 0; ch < nch; ++ch) {
             int32_t w = 0;
             for (int l = 0; l < kChunk; ++l) w = std::max(w, deg[ch * kChunk + l]);
-            coff[ch + 1] = coff[ch] + (int64_t)w * kChunk;
+            coff[ch + 1] = coff[ch] + (int64_t)((w + 3) / 4) * 4 * kChunk;
         }
         n_inc = coff[nch];
     }
@@ -725,8 +729,8 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     };
     try {
         ch0 = take(c->chunks, nch, ensure_chunks);
-        if ((ch0 + nch) * kChunk > ((int64_t)1 << 29))
-            fail(TITAN_E_NOMEM, "body_create: mass arena limit (2^29 slots) reached");
+        if ((ch0 + nch) * kChunk > kMaxMassSlots)
+            fail(TITAN_E_NOMEM, "body_create: mass arena limit (2^26 slots) reached");
         inc0 = take(c->incs, n_inc, ensure_incs);
         act0 = take(c->acts, n_act, ensure_acts);
         lc0 = take(c->lcs, n_lc, ensure_lcs);
@@ -762,7 +766,7 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     const int64_t g0 = ch0 * kChunk;
 
     // ---- mass arrays in device order
-    std::vector<double> st(6 * npad, 0.0), mm(npad, 1.0), Tm(npad, 0.0);
+    std::vector<double> st(6 * npad, 0.0), mm(npad, 1.0), Tm(npad, 0.0);  // st: 6 per mass
     std::vector<uint32_t> fl(npad, kPad);
     std::vector<double> ext;
     std::vector<int32_t> drag;
@@ -771,8 +775,8 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     for (int64_t q = 0; q < N; ++q) {
         const int64_t i = orig(q);
         for (int k = 0; k < 3; ++k) {
-            st[k * npad + q] = d->pos[3 * i + k];
-            st[(3 + k) * npad + q] = d->vel ? d->vel[3 * i + k] : 0.0;
+            st[6 * q + k] = d->pos[3 * i + k];
+            st[6 * q + 3 + k] = d->vel ? d->vel[3 * i + k] : 0.0;
         }
         mm[q] = d->m[i];
         Tm[q] = d->T ? d->T[i] : 0.0;
@@ -827,8 +831,13 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
 
     // ---- gather layout incidences
     if (!tiled && n_inc > 0) {
+        // padding entries point at the mass itself (an in-bounds load whose
+        // result is not used)
         nbr.assign(n_inc, 0);
-        grest.assign(n_inc, 0.0);
+        for (int64_t ch = 0; ch < nch; ++ch)
+            for (int64_t e = coff[ch]; e < coff[ch + 1]; ++e)
+                nbr[e] = (uint32_t)(g0 + ch * kChunk + ((e - coff[ch]) / 4) % kChunk);
+        grest.assign(n_inc, 1.0);
         if (!c->uniform_kd) {
             gk.assign(n_inc, 0.0);
             gd.assign(n_inc, 0.0);
@@ -840,8 +849,8 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
             if (L < 0 || R < 0) continue;
             const int32_t ai = act_local[s] >= 0 ? (int32_t)(act0 + act_local[s]) : -1;
             auto place = [&](int64_t me, int64_t other, bool is_left) {
-                const int64_t e = coff[me / kChunk] + (int64_t)fillc[me] * kChunk + (me % kChunk);
-                fillc[me]++;
+                const int64_t j = fillc[me]++;
+                const int64_t e = coff[me / kChunk] + ((j / 4) * kChunk + (me % kChunk)) * 4 + j % 4;
                 nbr[e] = (uint32_t)(g0 + other) | (is_left ? kLeftBit : 0u);
                 grest[e] = d->rest[s];
                 if (!gk.empty()) {
@@ -872,7 +881,7 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     std::vector<int32_t> cbody(nch, (int32_t)bid);
     std::vector<int64_t> cptr(nch);
     for (int64_t ch = 0; ch < nch; ++ch) cptr[ch] = inc0 + coff[ch];
-    for (int k = 0; k < 6; ++k) h2d(c, c->st + k * c->ncap + g0, st.data() + k * npad, npad);
+    h2d(c, c->st + 6 * g0, st.data(), 6 * npad);
     h2d(c, c->m + g0, mm.data(), npad);
     h2d(c, c->Tm + g0, Tm.data(), npad);
     h2d(c, c->mflags + g0, fl.data(), npad);
@@ -1010,15 +1019,28 @@ void body_download(titan_ctx *c, int32_t id, double *pos, double *vel, double *a
     BodyH &b = body_of(c, id);
     const int64_t N = b.n_masses, g0 = b.chunk0 * kChunk, n6 = c->ncap;
     const int p = (int)(b.n & 1);
+    auto dev = [&](int64_t q) -> int64_t { return q; };
+    auto host = [&](int64_t q) -> int64_t { return b.perm.empty() ? q : b.perm[q]; };
+    (void)dev;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (pos || vel) {
+        std::vector<double> st(6 * N);
+        d2h(c, st.data(), c->st + ((int64_t)p * n6 + g0) * 6, 6 * N);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int64_t q = 0; q < N; ++q) {
+            const int64_t i = host(q);
+            for (int k = 0; k < 3; ++k) {
+                if (pos) pos[3 * i + k] = st[6 * q + k];
+                if (vel) vel[3 * i + k] = st[6 * q + 3 + k];
+            }
+        }
+    }
     std::vector<double> tmp(3 * N);
     auto fetch = [&](const double *base, int K, double *out) {
         for (int k = 0; k < K; ++k) d2h(c, tmp.data() + k * N, base + k * n6 + g0, N);
         HIPCHK(hipStreamSynchronize(c->stream));
         unpermute(b, tmp, K, out);
     };
-    HIPCHK(hipStreamSynchronize(c->stream));
-    if (pos) fetch(c->st + (int64_t)p * 6 * n6, 3, pos);
-    if (vel) fetch(c->st + (int64_t)p * 6 * n6 + 3 * n6, 3, vel);
     if (acc) fetch(c->acc, 3, acc);
     if (T) fetch(c->Tm, 1, T);
 }
@@ -1029,23 +1051,37 @@ void body_upload(titan_ctx *c, int32_t id, const double *pos, const double *vel,
     const int64_t N = b.n_masses, g0 = b.chunk0 * kChunk, n6 = c->ncap;
     const int p = (int)(b.n & 1);
     HIPCHK(hipStreamSynchronize(c->stream));
+    auto host = [&](int64_t q) -> int64_t { return b.perm.empty() ? q : b.perm[q]; };
+    if (pos || vel) {  // read-modify-write of the 48-byte records
+        std::vector<double> st(6 * N);
+        double *base = c->st + ((int64_t)p * n6 + g0) * 6;
+        d2h(c, st.data(), base, 6 * N);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int64_t q = 0; q < N; ++q) {
+            const int64_t i = host(q);
+            for (int k = 0; k < 3; ++k) {
+                if (pos) st[6 * q + k] = pos[3 * i + k];
+                if (vel) st[6 * q + 3 + k] = vel[3 * i + k];
+            }
+        }
+        h2d(c, base, st.data(), 6 * N);
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
     std::vector<double> tmp(3 * N);
     auto put = [&](double *base, int K, const double *in) {
         for (int64_t q = 0; q < N; ++q) {
-            const int64_t i = b.perm.empty() ? q : b.perm[q];
+            const int64_t i = host(q);
             for (int k = 0; k < K; ++k) tmp[k * N + q] = in[K * i + k];
         }
         for (int k = 0; k < K; ++k) h2d(c, base + k * n6 + g0, tmp.data() + k * N, N);
         HIPCHK(hipStreamSynchronize(c->stream));
     };
-    if (pos) put(c->st + (int64_t)p * 6 * n6, 3, pos);
-    if (vel) put(c->st + (int64_t)p * 6 * n6 + 3 * n6, 3, vel);
     if (m) put(c->m, 1, m);
     if (T) put(c->Tm, 1, T);
     if (ext) {
         bool changed = false;
         for (int64_t q = 0; q < N; ++q) {
-            const int64_t i = b.perm.empty() ? q : b.perm[q];
+            const int64_t i = host(q);
             const bool nz = ext[3 * i] != 0 || ext[3 * i + 1] != 0 || ext[3 * i + 2] != 0;
             const uint32_t f = nz ? (b.flags[q] | kExt) : (b.flags[q] & ~kExt);
             if (f != b.flags[q]) {
@@ -1334,7 +1370,7 @@ int titan_mem_used(const titan_ctx *c, uint64_t *bytes) {
 int titan_kernel_name(const titan_ctx *c, char *buf, int64_t len) {
     NEED_CTX(c);
     if (!buf || len <= 0) return TITAN_E_INVALID;
-    const bool act = c->inc_act != nullptr;
+    const bool act = c->has_act;
     std::string n;
     if (any_body(c, false)) n = step_kernel_name(c->uniform_kd, act);
     if (any_body(c, true)) n += std::string(n.empty() ? "" : "+") + step_tiles_kernel_name(c->uniform_kd, act);

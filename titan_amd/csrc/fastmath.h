@@ -1,0 +1,69 @@
+// Correctly rounded FP64 division and square root for gfx950, cheaper than
+// the general `/` and `sqrt` inside a guarded exponent range.
+//
+// The compiler lowers an FP64 division a / b to: v_div_scale on both
+// operands, v_rcp_f64, two Newton-Raphson steps on the reciprocal
+// (e = 1 - b y; y = y + y e), q = a y, r = a - b q (fma), a final
+// fma(r, y, q) (v_div_fmas) and v_div_fixup for special operands; and sqrt(x)
+// to: a scaling of tiny x, v_rsq_f64, a Goldschmidt/Newton refinement of
+// s ~ sqrt(x) and h ~ 1/(2 sqrt(x)), the inverse scaling and a select for
+// +-0 / +inf.  When b is in [2^-500, 2^500] and a is 0 or has |a| in
+// [2^-500, 2^501], v_div_scale returns its operand unchanged and sets no
+// scale flag, v_div_fmas is a plain fma and v_div_fixup returns its first
+// operand; when x is in [2^-767, 2^1000] the sqrt scaling is by 2^0 and the
+// special-case select keeps s.  In those ranges the operations below are the
+// same operations on the same values, so the results are the correctly
+// rounded IEEE ones, bit for bit -- with one difference: a quotient whose
+// numerator is -0 comes out +0.  The reciprocal refinement, which depends
+// only on b, is computed once and shared by the three components of a unit
+// vector.
+//
+// Callers test fast_range() and fall back to `/` and `sqrt` outside it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace titan_mi355x {
+
+// sqrt(x) for x in [2^-767, 2^1000]
+__device__ __forceinline__ double fast_sqrt(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double s = x * y;
+    double h = y * 0.5;
+    const double e = fma(-h, s, 0.5);
+    s = fma(s, e, s);
+    h = fma(h, e, h);
+    double d = fma(-s, s, x);
+    s = fma(d, h, s);
+    d = fma(-s, s, x);
+    s = fma(d, h, s);
+    return s;
+}
+
+// The refined reciprocal of b in [2^-500, 2^500].
+__device__ __forceinline__ double fast_recip(double b) {
+    double y = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    return y;
+}
+
+// a / b given y = fast_recip(b), for a == 0 or |a| in [2^-500, 2^501].
+__device__ __forceinline__ double fast_div(double a, double b, double y) {
+    const double q = a * y;
+    const double r = fma(-b, q, a);
+    return fma(r, y, q);
+}
+
+__device__ __forceinline__ bool num_ok(double a) { return a == 0.0 || fabs(a) >= 0x1p-500; }
+
+// The guard for a spring axis t with squared length n2 = |t|^2: sqrt(n2) and
+// t / sqrt(n2) may take the fast path.  (n2 in [2^-767, 2^1000] puts
+// sqrt(n2) in [2^-384, 2^500], and every |t_c| <= sqrt(n2) (1 + 2^-52).)
+__device__ __forceinline__ bool fast_range(double n2, double tx, double ty, double tz) {
+    return n2 >= 0x1p-767 && n2 <= 0x1p1000 && num_ok(tx) && num_ok(ty) && num_ok(tz);
+}
+
+}  // namespace titan_mi355x

@@ -203,48 +203,24 @@ __device__ __forceinline__ void mass_pass(const StepArgs &a, MassRegs &r, int64_
     r.pz = r.pz + r.vz * a.dt;
 }
 
-// The incidence arrays of one chunk and the state buffer being read.
-template <bool UNI, bool ACT>
-struct IncView {
-    const double *__restrict__ cpx, *__restrict__ cpy, *__restrict__ cpz;
-    const double *__restrict__ cvx, *__restrict__ cvy, *__restrict__ cvz;
-    const uint32_t *__restrict__ inbr;
-    const double *__restrict__ irest, *__restrict__ ik, *__restrict__ idamp;
-    const int32_t *__restrict__ iact;
-    const double *__restrict__ act;
-    double k_uni, damp_uni, T;
-
-    // G for incidence slot e (see the kernel comment): the force this mass
-    // receives from the spring is -G.
-    __device__ __forceinline__ void force(uint32_t e, const MassRegs &r, double G[3]) const {
-        const uint32_t ob = (ldo(inbr, 4u * e) & ~kLeftBit) * 8u;
-        const double tx = ldo(cpx, ob) - r.px, ty = ldo(cpy, ob) - r.py, tz = ldo(cpz, ob) - r.pz;
-        const double dvx = r.vx - ldo(cvx, ob), dvy = r.vy - ldo(cvy, ob), dvz = r.vz - ldo(cvz, ob);
-        const double nrm = sqrt(tx * tx + ty * ty + tz * tz);
-        const double ux = tx / nrm, uy = ty / nrm, uz = tz / nrm;
-        double rl = ldo(irest, 8u * e);
-        if (ACT) {
-            const int32_t ai = ldo(iact, 4u * e);
-            if (ai >= 0) {
-                const double *A = act + 4 * (int64_t)ai;
-                rl = rl * (1 + A[0] * sin(A[1] * T + A[2]));
-            }
-        }
-        const double k = UNI ? k_uni : ldo(ik, 8u * e);
-        const double dmp = UNI ? damp_uni : ldo(idamp, 8u * e);
-        const double mag = k * (rl - nrm);
-        const double dd = (dvx * ux + dvy * uy + dvz * uz) * dmp;
-        G[0] = ux * mag + ux * dd;
-        G[1] = uy * mag + uy * dd;
-        G[2] = uz * mag + uz * dd;
-    }
-};
-
 }  // namespace
 
 // One lane per mass slot, one wavefront per 64-mass chunk, four chunks per
 // workgroup.  UNI: spring constant and damping are the same for every
 // spring (passed by value).  ACT: some springs are actuated.
+//
+// Incidences are stored four per lane per group ([group][lane][4]), so one
+// 16-byte load gives a lane the other endpoints of four springs; the next
+// group's indices are prefetched while the current group's twelve 16-byte
+// neighbour loads are in flight.
+//
+// Each incident spring is evaluated from this mass's side: tt = p_other -
+// p_self and dv = v_self - v_other.  If this mass is the left endpoint, tt
+// and dv are the reference's temp and dv and G below is its F; this mass
+// receives -F = -G.  If it is the right endpoint, the reference's temp and
+// dv are -tt and -dv exactly (x - y == -(y - x) under round-to-nearest), so
+// its F is -G exactly and this mass receives +F = -G.  Either way the
+// contribution is -G, bit for bit (an exact zero may differ in sign).
 template <bool UNI, bool ACT, int U>
 __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -268,90 +244,95 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     const int64_t n6 = a.ncap;
     const int64_t i = chunk * kChunk + lane;
     const int p = (int)(nst & 1);
-    const double *__restrict__ cur = a.st + (int64_t)p * 6 * n6;
-    double *__restrict__ nxt = a.st + (int64_t)(p ^ 1) * 6 * n6;
+    const double2 *__restrict__ cur = reinterpret_cast<const double2 *>(a.st + (int64_t)p * 6 * n6);
+    double2 *__restrict__ nxt = reinterpret_cast<double2 *>(a.st + (int64_t)(p ^ 1) * 6 * n6);
     const uint32_t fl = a.mflags[i];
     if (fl & kPad) return;
-    MassRegs r;
-    r.px = cur[i];
-    r.py = cur[n6 + i];
-    r.pz = cur[2 * n6 + i];
-    r.vx = cur[3 * n6 + i];
-    r.vy = cur[4 * n6 + i];
-    r.vz = cur[5 * n6 + i];
+    const double2 o0 = cur[3 * i], o1 = cur[3 * i + 1], o2 = cur[3 * i + 2];
     if (fl & kFixed) {  // src/sim.cu:1100-1101: a fixed mass keeps its state
-        nxt[i] = r.px;
-        nxt[n6 + i] = r.py;
-        nxt[2 * n6 + i] = r.pz;
-        nxt[3 * n6 + i] = r.vx;
-        nxt[4 * n6 + i] = r.vy;
-        nxt[5 * n6 + i] = r.vz;
+        nxt[3 * i] = o0;
+        nxt[3 * i + 1] = o1;
+        nxt[3 * i + 2] = o2;
         return;
     }
+    MassRegs r;
+    r.px = o0.x;
+    r.py = o0.y;
+    r.pz = o1.x;
+    r.vx = o1.y;
+    r.vy = o2.x;
+    r.vz = o2.y;
 
     // Spring forces, src/sim.cu:1055-1073, summed in ascending spring index.
-    //
-    // Each incident spring is evaluated from this mass's side: tt = p_other -
-    // p_self and dv = v_self - v_other.  If this mass is the left endpoint,
-    // tt and dv are the reference's temp and dv and G below is its F; this
-    // mass receives -F = -G.  If it is the right endpoint, the reference's
-    // temp and dv are -tt and -dv exactly (x - y == -(y - x) under
-    // round-to-nearest), so its F is -G exactly and this mass receives
-    // +F = -G.  Either way the contribution is -G, bit for bit (an exact zero
-    // may differ in sign), with no select on the spring's orientation.
     r.fx = 0.0;
     r.fy = 0.0;
     r.fz = 0.0;
     const int d = a.deg[i];
-    // wave-uniform bases + 32-bit lane offsets (scalar-base global loads)
-    IncView<UNI, ACT> v;
-    v.cpx = cur;
-    v.cpy = cur + n6;
-    v.cpz = cur + 2 * n6;
-    v.cvx = cur + 3 * n6;
-    v.cvy = cur + 4 * n6;
-    v.cvz = cur + 5 * n6;
+    const int ng = (d + 3) >> 2;
     const int64_t ebase = a.chunk_ptr[chunk];
-    v.inbr = a.inc_nbr + ebase;
-    v.irest = a.inc_rest + ebase;
-    v.ik = UNI ? nullptr : a.inc_k + ebase;
-    v.idamp = UNI ? nullptr : a.inc_damp + ebase;
-    v.iact = ACT ? a.inc_act + ebase : nullptr;
-    v.act = a.act;
-    v.k_uni = a.k_uni;
-    v.damp_uni = a.damp_uni;
-    v.T = T;
-    int j = 0;
-    // U independent incident springs per iteration expose instruction-level
-    // parallelism; their forces are still added in ascending order.
-    for (; j + U <= d; j += U) {
-        double G[U][3];
+    const uint4 *__restrict__ gi = reinterpret_cast<const uint4 *>(a.inc_nbr + ebase) + lane;
+    const double2 *__restrict__ gr = reinterpret_cast<const double2 *>(a.inc_rest + ebase) + 2 * lane;
+    const double2 *__restrict__ gk = UNI ? nullptr : reinterpret_cast<const double2 *>(a.inc_k + ebase) + 2 * lane;
+    const double2 *__restrict__ gd = UNI ? nullptr : reinterpret_cast<const double2 *>(a.inc_damp + ebase) + 2 * lane;
+    const int4 *__restrict__ ga = ACT ? reinterpret_cast<const int4 *>(a.inc_act + ebase) + lane : nullptr;
+    uint4 ni = make_uint4(0, 0, 0, 0);
+    if (ng > 0) ni = gi[0];
+    for (int g = 0; g < ng; ++g) {
+        const uint4 ci = ni;
+        if (g + 1 < ng) ni = gi[(g + 1) * kChunk];
+        const uint32_t oq[4] = {ci.x & ~kLeftBit, ci.y & ~kLeftBit, ci.z & ~kLeftBit, ci.w & ~kLeftBit};
+        double2 nb[4][3];
 #pragma unroll
-        for (int u = 0; u < U; ++u) v.force((uint32_t)((j + u) * kChunk + lane), r, G[u]);
+        for (int q = 0; q < 4; ++q) {
+            // 48 bytes per mass; the arena caps a buffer below 4 GiB
+            const uint32_t ob = oq[q] * 48u;
+            nb[q][0] = ldo(cur, ob);
+            nb[q][1] = ldo(cur, ob + 16u);
+            nb[q][2] = ldo(cur, ob + 32u);
+        }
+        const double2 r01 = gr[g * 2 * kChunk], r23 = gr[g * 2 * kChunk + 1];
+        const double rq[4] = {r01.x, r01.y, r23.x, r23.y};
+        double kq[4], dq[4];
+        int32_t aq[4] = {-1, -1, -1, -1};
+        if (!UNI) {
+            const double2 k01 = gk[g * 2 * kChunk], k23 = gk[g * 2 * kChunk + 1];
+            const double2 d01 = gd[g * 2 * kChunk], d23 = gd[g * 2 * kChunk + 1];
+            kq[0] = k01.x; kq[1] = k01.y; kq[2] = k23.x; kq[3] = k23.y;
+            dq[0] = d01.x; dq[1] = d01.y; dq[2] = d23.x; dq[3] = d23.y;
+        }
+        if (ACT) {
+            const int4 a4 = ga[g * kChunk];
+            aq[0] = a4.x; aq[1] = a4.y; aq[2] = a4.z; aq[3] = a4.w;
+        }
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            r.fx = r.fx - G[u][0];
-            r.fy = r.fy - G[u][1];
-            r.fz = r.fz - G[u][2];
+        for (int q = 0; q < 4; ++q) {
+            if (4 * g + q >= d) break;
+            const double tx = nb[q][0].x - r.px, ty = nb[q][0].y - r.py, tz = nb[q][1].x - r.pz;
+            const double dvx = r.vx - nb[q][1].y, dvy = r.vy - nb[q][2].x, dvz = r.vz - nb[q][2].y;
+            const double nrm = sqrt(tx * tx + ty * ty + tz * tz);
+            const double ux = tx / nrm, uy = ty / nrm, uz = tz / nrm;
+            double rl = rq[q];
+            if (ACT && aq[q] >= 0) {
+                const double *A = a.act + 4 * (int64_t)aq[q];
+                rl = rl * (1 + A[0] * sin(A[1] * T + A[2]));
+            }
+            const double k = UNI ? a.k_uni : kq[q];
+            const double dmp = UNI ? a.damp_uni : dq[q];
+            const double mag = k * (rl - nrm);
+            const double dd = (dvx * ux + dvy * uy + dvz * uz) * dmp;
+            r.fx = r.fx - (ux * mag + ux * dd);
+            r.fy = r.fy - (uy * mag + uy * dd);
+            r.fz = r.fz - (uz * mag + uz * dd);
         }
     }
-    for (; j < d; ++j) {
-        double G[3];
-        v.force((uint32_t)(j * kChunk + lane), r, G);
-        r.fx = r.fx - G[0];
-        r.fy = r.fy - G[1];
-        r.fz = r.fz - G[2];
-    }
+    (void)U;
 
     const double m = a.m[i];
     double ax, ay, az;
     mass_pass(a, r, i, fl, b, m, ax, ay, az);
-    nxt[i] = r.px;
-    nxt[n6 + i] = r.py;
-    nxt[2 * n6 + i] = r.pz;
-    nxt[3 * n6 + i] = r.vx;
-    nxt[4 * n6 + i] = r.vy;
-    nxt[5 * n6 + i] = r.vz;
+    nxt[3 * i] = make_double2(r.px, r.py);
+    nxt[3 * i + 1] = make_double2(r.pz, r.vx);
+    nxt[3 * i + 2] = make_double2(r.vy, r.vz);
     if (a.write_acc) {
         a.acc[i] = ax;
         a.acc[n6 + i] = ay;
@@ -360,13 +341,14 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
 }
 
 // Tiled step: one workgroup per tile of up to 256 masses of one body.
-//   stage:   tile and halo positions/velocities -> LDS (SoA rows of ldim);
+//   stage:   tile and halo positions/velocities -> LDS, 48 bytes per mass
+//            (px py pz vx vy vz, read back as three 16-byte pairs);
 //   batches: springs of the tile in ascending index, kTileBatch at a time:
 //            each spring's force F (the reference's orientation, on the
-//            right endpoint) -> LDS slot; then every tile mass adds the
-//            slots of its incident springs that fall in the batch, +F as the
-//            right endpoint and -F as the left one, in ascending spring
-//            index (so the sum is the oracle's, bit for bit);
+//            right endpoint) -> a 32-byte LDS slot; then every tile mass
+//            adds the slots of its incident springs that fall in the batch,
+//            +F as the right endpoint and -F as the left one, in ascending
+//            spring index (so the sum is the oracle's, bit for bit);
 //   update:  gravity, extern force, planes, balls, local constraints, drag,
 //            Euler, written to the other state buffer.
 // A spring with both endpoints in the tile is evaluated once; one that
@@ -394,46 +376,43 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
 
     const int64_t n6 = a.ncap;
     const int p = (int)(nst & 1);
-    const double *__restrict__ cur = a.st + (int64_t)p * 6 * n6;
-    double *__restrict__ nxt = a.st + (int64_t)(p ^ 1) * 6 * n6;
-    const int L = a.ldim;
-    double *sx = lds, *sy = lds + L, *sz = lds + 2 * L, *svx = lds + 3 * L, *svy = lds + 4 * L,
-           *svz = lds + 5 * L;
-    double *Fx = lds + 6 * L, *Fy = Fx + kTileBatch, *Fz = Fy + kTileBatch;
+    const double2 *__restrict__ cur = reinterpret_cast<const double2 *>(a.st + (int64_t)p * 6 * n6);
+    double2 *__restrict__ nxt = reinterpret_cast<double2 *>(a.st + (int64_t)(p ^ 1) * 6 * n6);
+    double2 *S = reinterpret_cast<double2 *>(lds);                 // 3 pairs per mass
+    double2 *F = reinterpret_cast<double2 *>(lds + 6 * a.ldim);    // 2 pairs per slot
 
     const int nmass = td->nmass, nhalo = td->nhalo, nspr = td->nspr;
     const int64_t i = (int64_t)td->chunk0 * kChunk + t;
     const bool mine = t < nmass;
     MassRegs r{};
     uint32_t fl = kPad;
+    double2 o0 = make_double2(0.0, 0.0), o1 = o0, o2 = o0;
     if (mine) {
         fl = a.mflags[i];
-        r.px = cur[i];
-        r.py = cur[n6 + i];
-        r.pz = cur[2 * n6 + i];
-        r.vx = cur[3 * n6 + i];
-        r.vy = cur[4 * n6 + i];
-        r.vz = cur[5 * n6 + i];
+        o0 = cur[3 * i];
+        o1 = cur[3 * i + 1];
+        o2 = cur[3 * i + 2];
+        r.px = o0.x;
+        r.py = o0.y;
+        r.pz = o1.x;
+        r.vx = o1.y;
+        r.vy = o2.x;
+        r.vz = o2.y;
     }
-    sx[t] = r.px;
-    sy[t] = r.py;
-    sz[t] = r.pz;
-    svx[t] = r.vx;
-    svy[t] = r.vy;
-    svz[t] = r.vz;
+    S[3 * t] = o0;
+    S[3 * t + 1] = o1;
+    S[3 * t + 2] = o2;
     const int32_t *__restrict__ hl = a.halo + td->halo_off;
     for (int h = t; h < nhalo; h += kTileMasses) {
-        const uint32_t ob = (uint32_t)hl[h] * 8u;
-        const int q = kTileMasses + h;
-        sx[q] = ldo(cur, ob);
-        sy[q] = ldo(cur + n6, ob);
-        sz[q] = ldo(cur + 2 * n6, ob);
-        svx[q] = ldo(cur + 3 * n6, ob);
-        svy[q] = ldo(cur + 4 * n6, ob);
-        svz[q] = ldo(cur + 5 * n6, ob);
+        const uint32_t ob = (uint32_t)hl[h] * 48u;
+        const int q = 3 * (kTileMasses + h);
+        S[q] = ldo(cur, ob);
+        S[q + 1] = ldo(cur, ob + 16u);
+        S[q + 2] = ldo(cur, ob + 32u);
     }
 
-    // this lane's incidence list: groups of four uint16 entries
+    // this lane's incidence list: uint16 entries, four per group, the next
+    // group prefetched; consumed entries are shifted out of g0
     const bool accumulate = mine && !(fl & (kFixed | kPad));
     const int deg = accumulate ? a.deg[i] : 0;
     const int chunk_in_tile = t >> 6, lane = t & (kChunk - 1);
@@ -445,53 +424,85 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     r.fy = 0.0;
     r.fz = 0.0;
 
+    // Spring records of the next batch are loaded into registers while the
+    // current batch is evaluated.
+    constexpr int kPer = kTileBatch / kTileMasses;
     const uint32_t *__restrict__ ends = a.t_ends + td->spr_off;
     const double *__restrict__ rest = a.t_rest + td->spr_off;
+    const int32_t *__restrict__ tact = ACT ? a.t_act + td->spr_off : nullptr;
+    const double *__restrict__ tk = UNI ? nullptr : a.t_k + td->spr_off;
+    const double *__restrict__ tdm = UNI ? nullptr : a.t_damp + td->spr_off;
+    uint32_t pe[kPer];
+    double pr[kPer], pk[kPer], pd[kPer];
+    int32_t pa[kPer];
+    auto prefetch = [&](int bs) {
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = bs + t + u * kTileMasses;
+            const bool ok = e < nspr;
+            pe[u] = ok ? ends[e] : 0u;
+            pr[u] = ok ? rest[e] : 0.0;
+            if (!UNI) {
+                pk[u] = ok ? tk[e] : 0.0;
+                pd[u] = ok ? tdm[e] : 0.0;
+            }
+            if (ACT) pa[u] = ok ? tact[e] : -1;
+        }
+    };
+    prefetch(0);
     __syncthreads();
     for (int bs = 0; bs < nspr; bs += kTileBatch) {
         const int bl = min(kTileBatch, nspr - bs);
-        for (int q = t; q < bl; q += kTileMasses) {
-            const int e = bs + q;
-            const uint32_t lr = ends[e];
-            const int li = (int)(lr >> 16), ri = (int)(lr & 0xffff);
+        uint32_t ce[kPer];
+        double cr[kPer], ck[kPer], cd[kPer];
+        int32_t ca[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            ce[u] = pe[u];
+            cr[u] = pr[u];
+            ck[u] = UNI ? a.k_uni : pk[u];
+            cd[u] = UNI ? a.damp_uni : pd[u];
+            ca[u] = ACT ? pa[u] : -1;
+        }
+        if (bs + kTileBatch < nspr) prefetch(bs + kTileBatch);
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int q = t + u * kTileMasses;
+            if (q >= bl) continue;
+            const double2 *Ls = S + 3 * (int)(ce[u] >> 16), *Rs = S + 3 * (int)(ce[u] & 0xffff);
+            const double2 l0 = Ls[0], l1 = Ls[1], l2 = Ls[2], r0 = Rs[0], r1 = Rs[1], r2 = Rs[2];
             // src/sim.cu:1055-1065
-            const double tx = sx[ri] - sx[li], ty = sy[ri] - sy[li], tz = sz[ri] - sz[li];
+            const double tx = r0.x - l0.x, ty = r0.y - l0.y, tz = r1.x - l1.x;
             const double nrm = sqrt(tx * tx + ty * ty + tz * tz);
             const double ux = tx / nrm, uy = ty / nrm, uz = tz / nrm;
-            double rl = rest[e];
-            if (ACT) {
-                const int32_t ai = a.t_act[td->spr_off + e];
-                if (ai >= 0) {
-                    const double *A = a.act + 4 * (int64_t)ai;
-                    rl = rl * (1 + A[0] * sin(A[1] * T + A[2]));
-                }
+            double rl = cr[u];
+            if (ACT && ca[u] >= 0) {
+                const double *A = a.act + 4 * (int64_t)ca[u];
+                rl = rl * (1 + A[0] * sin(A[1] * T + A[2]));
             }
-            const double k = UNI ? a.k_uni : a.t_k[td->spr_off + e];
-            const double dmp = UNI ? a.damp_uni : a.t_damp[td->spr_off + e];
-            const double mag = k * (rl - nrm);
-            const double dvx = svx[li] - svx[ri], dvy = svy[li] - svy[ri], dvz = svz[li] - svz[ri];
-            const double dd = (dvx * ux + dvy * uy + dvz * uz) * dmp;
-            Fx[q] = ux * mag + ux * dd;
-            Fy[q] = uy * mag + uy * dd;
-            Fz[q] = uz * mag + uz * dd;
+            const double mag = ck[u] * (rl - nrm);
+            const double dvx = l1.y - r1.y, dvy = l2.x - r2.x, dvz = l2.y - r2.y;
+            const double dd = (dvx * ux + dvy * uy + dvz * uz) * cd[u];
+            F[2 * q] = make_double2(ux * mag + ux * dd, uy * mag + uy * dd);
+            F[2 * q + 1].x = uz * mag + uz * dd;
         }
         __syncthreads();
         const int hi = bs + bl;
         while (j < deg) {
-            const uint32_t ent = (uint32_t)(g0 >> (16 * (j & 3))) & 0xffffu;
+            const uint32_t ent = (uint32_t)g0 & 0xffffu;
             const int slot = (int)(ent & 0x7fffu);
             if (slot >= hi) break;
             const int q = slot - bs;
-            if (ent & kIncLeft) {  // src/sim.cu:1071-1073: left endpoint gets -F
-                r.fx = r.fx - Fx[q];
-                r.fy = r.fy - Fy[q];
-                r.fz = r.fz - Fz[q];
-            } else {               // src/sim.cu:1068-1070: right endpoint gets +F
-                r.fx = r.fx + Fx[q];
-                r.fy = r.fy + Fy[q];
-                r.fz = r.fz + Fz[q];
-            }
+            const double2 fxy = F[2 * q];
+            const double fz = F[2 * q + 1].x;
+            // the left endpoint receives -F (src/sim.cu:1071-1073): flip the
+            // sign bit, an exact negation
+            const uint32_t neg = (ent & (uint32_t)kIncLeft) << 16;
+            r.fx = r.fx + __hiloint2double(__double2hiint(fxy.x) ^ neg, __double2loint(fxy.x));
+            r.fy = r.fy + __hiloint2double(__double2hiint(fxy.y) ^ neg, __double2loint(fxy.y));
+            r.fz = r.fz + __hiloint2double(__double2hiint(fz) ^ neg, __double2loint(fz));
             ++j;
+            g0 >>= 16;
             if ((j & 3) == 0) {
                 g0 = g1;
                 g1 = (j + 4 < deg) ? inc[(int64_t)(j / 4 + 1) * kChunk] : 0;
@@ -502,23 +513,17 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
 
     if (!mine) return;
     if (fl & kFixed) {  // src/sim.cu:1100-1101
-        nxt[i] = r.px;
-        nxt[n6 + i] = r.py;
-        nxt[2 * n6 + i] = r.pz;
-        nxt[3 * n6 + i] = r.vx;
-        nxt[4 * n6 + i] = r.vy;
-        nxt[5 * n6 + i] = r.vz;
+        nxt[3 * i] = o0;
+        nxt[3 * i + 1] = o1;
+        nxt[3 * i + 2] = o2;
         return;
     }
     const double m = a.m[i];
     double ax, ay, az;
     mass_pass(a, r, i, fl, b, m, ax, ay, az);
-    nxt[i] = r.px;
-    nxt[n6 + i] = r.py;
-    nxt[2 * n6 + i] = r.pz;
-    nxt[3 * n6 + i] = r.vx;
-    nxt[4 * n6 + i] = r.vy;
-    nxt[5 * n6 + i] = r.vz;
+    nxt[3 * i] = make_double2(r.px, r.py);
+    nxt[3 * i + 1] = make_double2(r.pz, r.vx);
+    nxt[3 * i + 2] = make_double2(r.vy, r.vz);
     if (a.write_acc) {
         a.acc[i] = ax;
         a.acc[n6 + i] = ay;
@@ -578,7 +583,7 @@ hipError_t launch_step(const StepArgs &a, bool uni, bool act, int unroll, hipStr
 }
 
 size_t tiles_lds_bytes(int ldim) {
-    return sizeof(double) * ((size_t)6 * ldim + (size_t)3 * kTileBatch);
+    return sizeof(double) * ((size_t)6 * ldim + (size_t)4 * kTileBatch);
 }
 
 hipError_t launch_step_tiles(const StepArgs &a, bool uni, bool act, hipStream_t s) {

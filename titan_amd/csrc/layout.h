@@ -2,16 +2,19 @@
 //
 // Masses live in a chunked arena: a chunk is 64 consecutive mass slots, one
 // wavefront's worth, and every body starts on a chunk boundary so that a
-// wave never straddles two bodies.  State is FP64 SoA, double-buffered:
-//   st[(p * 6 + c) * ncap + i], p = buffer parity, c = px py pz vx vy vz.
+// wave never straddles two bodies.  State is FP64, double-buffered, 48 bytes
+// per mass (px py pz vx vy vz, loaded as three 16-byte pairs):
+//   st[(p * ncap + i) * 6 + c], p = buffer parity.
 // A body's current buffer is the parity of its own step count, so bodies
-// that are not stepped need no copy.
+// that are not stepped need no copy.  Other per-mass arrays are SoA.
 //
-// Springs are stored per incidence in SELL-64 order (sliced ELLPACK with a
-// slice of one wavefront): incidence j of mass slot `lane` in chunk c sits at
-// chunk_ptr[c] + j * 64 + lane, so the loads of one incidence slot across a
-// wave are consecutive.  Each mass lists its incident springs in ascending
-// spring index, which fixes the order of the FP64 force sum.
+// Gather layout: springs are stored per incidence in a sliced ELLPACK order
+// with a slice of one wavefront and groups of four: incidence j of mass
+// slot `lane` in chunk c sits at chunk_ptr[c] + ((j / 4) * 64 + lane) * 4 +
+// j % 4, so one 16-byte load gives a lane four consecutive incidences and a
+// wave's loads of one group are contiguous.  Each mass lists its incident
+// springs in ascending spring index, which fixes the order of the FP64
+// force sum.
 #pragma once
 
 #include <stdint.h>
@@ -39,7 +42,7 @@ constexpr uint32_t kLeftBit = 0x80000000u;
 // as uint16 (slot | 0x8000 if it is the left endpoint), four per lane in a
 // uint64 group, groups of one chunk stored lane-major.
 constexpr int kTileMasses = 256;
-constexpr int kTileBatch = 1024;        // force slots per LDS batch
+constexpr int kTileBatch = 512;         // force slots per LDS batch
 constexpr int kMaxTileSprings = 32767;  // slot must fit in 15 bits
 constexpr int kMaxHalo = 1536;          // LDS budget for staged halo masses
 constexpr uint16_t kIncLeft = 0x8000;
@@ -67,9 +70,11 @@ struct LocalC {
     int32_t pad;
 };
 
+constexpr int64_t kMaxMassSlots = (int64_t)1 << 26;  // 48 * slots < 4 GiB per buffer
+
 struct StepArgs {
     // mass arena
-    double *st;               // 2 x 6 x ncap
+    double *st;               // 2 x ncap x 6
     int64_t ncap;
     const double *m;
     const uint32_t *mflags;
