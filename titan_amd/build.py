@@ -93,11 +93,26 @@ def build_facade_tests(verbose: bool = False, out: Path | None = None) -> Path:
     return out
 
 
+FASTMATH_CHECK_SRC = ROOT / "tests" / "cpp" / "fastmath_check.hip"
+FASTMATH_CHECK_BIN = ROOT / "tests" / "cpp" / "fastmath_check"
+
+
+def build_fastmath_check(verbose: bool = False, out: Path | None = None) -> Path:
+    """Bitwise check of the guarded FP64 division/sqrt against IEEE `/` and `sqrt`."""
+    out = out or FASTMATH_CHECK_BIN
+    tmp = out.with_name(out.name + ".tmp")
+    _run([HIPCC, *HIP_FLAGS, "-x", "hip", f"-I{CSRC}", FASTMATH_CHECK_SRC, "-o", tmp], verbose)
+    os.replace(tmp, out)
+    return out
+
+
 def build_all(verbose: bool = False) -> None:
     build_engine(verbose)
     build_facade(verbose)
     if FACADE_TESTS_SRC.exists():
         build_facade_tests(verbose)
+    if FASTMATH_CHECK_SRC.exists():
+        build_fastmath_check(verbose)
     shutil.rmtree(PKG / "_build", ignore_errors=True)
 
 

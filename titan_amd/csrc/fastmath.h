@@ -8,15 +8,15 @@
 // to: a scaling of tiny x, v_rsq_f64, a Goldschmidt/Newton refinement of
 // s ~ sqrt(x) and h ~ 1/(2 sqrt(x)), the inverse scaling and a select for
 // +-0 / +inf.  When b is in [2^-500, 2^500] and a is 0 or has |a| in
-// [2^-500, 2^501], v_div_scale returns its operand unchanged and sets no
+// [2^-500, 2 b], v_div_scale returns its operand unchanged and sets no
 // scale flag, v_div_fmas is a plain fma and v_div_fixup returns its first
-// operand; when x is in [2^-767, 2^1000] the sqrt scaling is by 2^0 and the
-// special-case select keeps s.  In those ranges the operations below are the
-// same operations on the same values, so the results are the correctly
-// rounded IEEE ones, bit for bit -- with one difference: a quotient whose
-// numerator is -0 comes out +0.  The reciprocal refinement, which depends
-// only on b, is computed once and shared by the three components of a unit
-// vector.
+// operand with the sign of a / b; when x is in [2^-767, 2^1000] the sqrt
+// scaling is by 2^0 and the special-case select keeps s.  In those ranges the
+// operations below are the same operations on the same values, so the results
+// are the correctly rounded IEEE ones, bit for bit (a zero numerator is
+// returned as is, which is v_div_fixup's +-0 for b > 0).  The reciprocal
+// refinement, which depends only on b, is computed once and shared by the
+// three components of a unit vector.
 //
 // Callers test fast_range() and fall back to `/` and `sqrt` outside it.
 #pragma once
@@ -50,11 +50,11 @@ __device__ __forceinline__ double fast_recip(double b) {
     return y;
 }
 
-// a / b given y = fast_recip(b), for a == 0 or |a| in [2^-500, 2^501].
+// a / b given y = fast_recip(b), for b > 0 and a == 0 or |a| in [2^-500, 2 b].
 __device__ __forceinline__ double fast_div(double a, double b, double y) {
     const double q = a * y;
     const double r = fma(-b, q, a);
-    return fma(r, y, q);
+    return a == 0.0 ? a : fma(r, y, q);
 }
 
 __device__ __forceinline__ bool num_ok(double a) { return a == 0.0 || fabs(a) >= 0x1p-500; }
@@ -64,6 +64,25 @@ __device__ __forceinline__ bool num_ok(double a) { return a == 0.0 || fabs(a) >=
 // sqrt(n2) in [2^-384, 2^500], and every |t_c| <= sqrt(n2) (1 + 2^-52).)
 __device__ __forceinline__ bool fast_range(double n2, double tx, double ty, double tz) {
     return n2 >= 0x1p-767 && n2 <= 0x1p1000 && num_ok(tx) && num_ok(ty) && num_ok(tz);
+}
+
+// nrm = sqrt(tx tx + ty ty + tz tz) and u = t / nrm, correctly rounded, as
+// the reference computes a spring's length and direction.
+__device__ __forceinline__ void unit_axis(double tx, double ty, double tz, double &nrm, double &ux,
+                                          double &uy, double &uz) {
+    const double n2 = tx * tx + ty * ty + tz * tz;
+    if (fast_range(n2, tx, ty, tz)) {
+        nrm = fast_sqrt(n2);
+        const double y = fast_recip(nrm);
+        ux = fast_div(tx, nrm, y);
+        uy = fast_div(ty, nrm, y);
+        uz = fast_div(tz, nrm, y);
+    } else {
+        nrm = sqrt(n2);
+        ux = tx / nrm;
+        uy = ty / nrm;
+        uz = tz / nrm;
+    }
 }
 
 }  // namespace titan_mi355x

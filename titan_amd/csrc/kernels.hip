@@ -17,6 +17,7 @@
 // device math library's.
 #include <hip/hip_runtime.h>
 
+#include "fastmath.h"
 #include "kernels.h"
 #include "layout.h"
 
@@ -309,8 +310,8 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
             if (4 * g + q >= d) break;
             const double tx = nb[q][0].x - r.px, ty = nb[q][0].y - r.py, tz = nb[q][1].x - r.pz;
             const double dvx = r.vx - nb[q][1].y, dvy = r.vy - nb[q][2].x, dvz = r.vz - nb[q][2].y;
-            const double nrm = sqrt(tx * tx + ty * ty + tz * tz);
-            const double ux = tx / nrm, uy = ty / nrm, uz = tz / nrm;
+            double nrm, ux, uy, uz;
+            unit_axis(tx, ty, tz, nrm, ux, uy, uz);
             double rl = rq[q];
             if (ACT && aq[q] >= 0) {
                 const double *A = a.act + 4 * (int64_t)aq[q];
@@ -473,8 +474,8 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
             const double2 l0 = Ls[0], l1 = Ls[1], l2 = Ls[2], r0 = Rs[0], r1 = Rs[1], r2 = Rs[2];
             // src/sim.cu:1055-1065
             const double tx = r0.x - l0.x, ty = r0.y - l0.y, tz = r1.x - l1.x;
-            const double nrm = sqrt(tx * tx + ty * ty + tz * tz);
-            const double ux = tx / nrm, uy = ty / nrm, uz = tz / nrm;
+            double nrm, ux, uy, uz;
+            unit_axis(tx, ty, tz, nrm, ux, uy, uz);
             double rl = cr[u];
             if (ACT && ca[u] >= 0) {
                 const double *A = a.act + 4 * (int64_t)ca[u];
