@@ -1,6 +1,8 @@
-// Host check of the tiled-layout builder (titan_amd/csrc/tiles.cpp): for
-// random graphs and lattices, every tile's spring list, halo, endpoint
-// encoding and per-mass incidence order are compared with brute force.
+// Host check of the layout builders (titan_amd/csrc/tiles.cpp): for random
+// graphs and a Morton-ordered lattice, every tile's halo and every mass's
+// incidence list (other endpoint, spring parameters, ascending spring
+// order, padding) in both layouts are compared with brute force.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -20,13 +22,71 @@ static int fails = 0;
         }                                                                 \
     } while (0)
 
+// Decode mass m's incidence list from a SELL build and compare it with the
+// brute-force list of (other endpoint, spring) in ascending spring index.
+static void check_sell(int64_t n, const std::vector<int32_t> &L, const std::vector<int32_t> &R,
+                       const std::vector<double> &k, const std::vector<double> &rest,
+                       const std::vector<double> &damp, const std::vector<int32_t> &act, bool kd,
+                       const TileBuild *tb, int64_t g0, const SellBuild &sb) {
+    const int64_t S = (int64_t)L.size();
+    const int64_t nch = (n + kChunk - 1) / kChunk;
+    CHECK((int64_t)sb.coff.size() == nch + 1 && (int64_t)sb.deg.size() == nch * kChunk);
+    CHECK((int64_t)sb.nbr.size() == sb.coff[nch] && (int64_t)sb.rest.size() == sb.coff[nch]);
+    CHECK(kd ? (int64_t)sb.k.size() == sb.coff[nch] : sb.k.empty());
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> want(n);  // (other, spring)
+    for (int64_t s = 0; s < S; ++s) {
+        if (L[s] < 0 || R[s] < 0) continue;
+        want[R[s]].push_back({L[s], (int32_t)s});
+        want[L[s]].push_back({R[s], (int32_t)s});
+    }
+    const int G = tb ? kGroupTiled : kGroupGather;
+    // the name stored for entry e of chunk c
+    auto name = [&](int64_t c, int64_t e) -> uint32_t {
+        if (!tb) return sb.nbr[e];
+        const int64_t rel = e - sb.coff[c];
+        return (sb.nbr[sb.coff[c] + rel / 2] >> (16 * (rel % 2))) & 0xffffu;
+    };
+    auto decode = [&](int64_t me, uint32_t v) -> int64_t {
+        if (!tb) return (int64_t)v - g0;
+        const int64_t t = me / kTileMasses, m0 = t * kTileMasses;
+        const TileDesc &d = tb->tiles[t];
+        if (v < (uint32_t)kTileMasses) return m0 + v;
+        CHECK(v - kTileMasses < (uint32_t)d.nhalo);
+        return tb->halo[d.halo_off + v - kTileMasses];
+    };
+    for (int64_t c = 0; c < nch; ++c) {
+        int32_t w = 0;
+        for (int l = 0; l < kChunk; ++l) w = std::max(w, sb.deg[c * kChunk + l]);
+        CHECK(sb.coff[c + 1] - sb.coff[c] == (int64_t)((w + G - 1) / G) * G * kChunk);
+        for (int l = 0; l < kChunk; ++l) {
+            const int64_t m = c * kChunk + l;
+            const int32_t dm = m < n ? (int32_t)want[m].size() : 0;
+            CHECK(sb.deg[m] == dm);
+            for (int64_t j = 0; j < (sb.coff[c + 1] - sb.coff[c]) / kChunk; ++j) {
+                const int64_t e = sb.coff[c] + ((j / G) * kChunk + l) * G + j % G;
+                if (j >= dm) {  // padding: the mass itself, rest 1
+                    CHECK(decode(m, name(c, e)) == m && sb.rest[e] == 1.0);
+                    continue;
+                }
+                const int32_t s = want[m][j].second;
+                CHECK(decode(m, name(c, e)) == want[m][j].first);
+                CHECK(sb.rest[e] == rest[s]);
+                if (kd) CHECK(sb.k[e] == k[s] && sb.damp[e] == damp[s]);
+                if (!act.empty()) CHECK(sb.act[e] == act[s]);
+            }
+        }
+    }
+}
+
 static void check(int64_t n, const std::vector<int32_t> &L, const std::vector<int32_t> &R, bool kd) {
     const int64_t S = (int64_t)L.size();
     std::vector<double> k(S), rest(S), damp(S);
+    std::vector<int32_t> act(S);
     for (int64_t s = 0; s < S; ++s) {
         k[s] = 100.0 + s;
         rest[s] = 0.5 + 0.001 * s;
         damp[s] = 0.01 * (s % 7);
+        act[s] = s % 3 == 0 ? (int32_t)(s / 3) : -1;
     }
     SpringsIn sp;
     sp.n_springs = S;
@@ -35,58 +95,39 @@ static void check(int64_t n, const std::vector<int32_t> &L, const std::vector<in
     sp.k = k.data();
     sp.rest = rest.data();
     sp.damping = damp.data();
+    sp.act = kd ? act.data() : nullptr;
+    const std::vector<int32_t> noact;
+    // gather layout: arena slots
+    SellBuild sg;
+    build_sell(n, sp, kd, nullptr, 4096, sg);
+    check_sell(n, L, R, k, rest, damp, kd ? act : noact, kd, nullptr, 4096, sg);
+    // tiled layout: tiles, halos and LDS rows
     TileBuild tb;
-    std::vector<int32_t> deg;
-    CHECK(build_tiles(n, sp, kd, tb, deg));
+    CHECK(build_tiles(n, sp, tb));
     const int64_t ntiles = (n + kTileMasses - 1) / kTileMasses;
     CHECK((int64_t)tb.tiles.size() == ntiles);
+    int32_t hmax = 0;
     for (int64_t t = 0; t < ntiles; ++t) {
         const TileDesc &d = tb.tiles[t];
         const int64_t m0 = t * kTileMasses, m1 = std::min<int64_t>(n, m0 + kTileMasses);
         CHECK(d.nmass == m1 - m0 && d.chunk0 == m0 / kChunk && d.first == (t == 0));
-        std::vector<int32_t> want;
         std::set<int32_t> halo;
         for (int64_t s = 0; s < S; ++s) {
             if (L[s] < 0 || R[s] < 0) continue;
             const bool in = (L[s] >= m0 && L[s] < m1) || (R[s] >= m0 && R[s] < m1);
             if (!in) continue;
-            want.push_back((int32_t)s);
             for (int32_t e : {L[s], R[s]})
                 if (e < m0 || e >= m1) halo.insert(e);
         }
-        CHECK(d.nspr == (int32_t)want.size() && d.nhalo == (int32_t)halo.size());
+        CHECK(d.nhalo == (int32_t)halo.size());
+        hmax = std::max<int32_t>(hmax, d.nhalo);
         std::vector<int32_t> hv(halo.begin(), halo.end());
         for (int32_t h = 0; h < d.nhalo; ++h) CHECK(tb.halo[d.halo_off + h] == hv[h]);
-        auto decode = [&](uint32_t l) -> int64_t { return l < (uint32_t)kTileMasses ? m0 + l : tb.halo[d.halo_off + l - kTileMasses]; };
-        for (int32_t q = 0; q < d.nspr; ++q) {
-            const int32_t s = tb.sid[d.spr_off + q];
-            CHECK(s == want[q]);
-            const uint32_t e = tb.ends[d.spr_off + q];
-            CHECK(decode(e >> 16) == L[s] && decode(e & 0xffff) == R[s]);
-            CHECK(tb.rest[d.spr_off + q] == rest[s]);
-            if (kd) CHECK(tb.k[d.spr_off + q] == k[s] && tb.damp[d.spr_off + q] == damp[s]);
-        }
-        for (int64_t m = m0; m < m1; ++m) {
-            std::vector<std::pair<int32_t, int>> inc;  // (spring, +1 right / -1 left)
-            for (int64_t s = 0; s < S; ++s) {
-                if (L[s] < 0 || R[s] < 0) continue;
-                if (R[s] == m) inc.push_back({(int32_t)s, 1});
-                if (L[s] == m) inc.push_back({(int32_t)s, -1});
-            }
-            CHECK(deg[m] == (int32_t)inc.size());
-            const int64_t c = (m - m0) / kChunk, lane = (m - m0) % kChunk;
-            int prev = -1;
-            for (size_t j = 0; j < inc.size(); ++j) {
-                const uint64_t g = tb.inc[d.inc_off + (int64_t)(d.gstart[c] + j / 4) * kChunk + lane];
-                const uint32_t ent = (uint32_t)(g >> (16 * (j % 4))) & 0xffff;
-                const int slot = (int)(ent & 0x7fff);
-                CHECK(slot < d.nspr && slot >= prev);
-                prev = slot;
-                CHECK(tb.sid[d.spr_off + slot] == inc[j].first);
-                CHECK(((ent & kIncLeft) != 0) == (inc[j].second < 0));
-            }
-        }
     }
+    CHECK(tb.hmax == hmax);
+    SellBuild st;
+    build_sell(n, sp, kd, &tb, 0, st);
+    check_sell(n, L, R, k, rest, damp, kd ? act : noact, kd, &tb, 0, st);
 }
 
 int main() {
@@ -142,8 +183,7 @@ int main() {
         sp.k = ones.data();
         sp.rest = ones.data();
         TileBuild tb;
-        std::vector<int32_t> deg;
-        CHECK(build_tiles(n, sp, false, tb, deg));
+        CHECK(build_tiles(n, sp, tb));
         CHECK(tb.hmax < 2 * kTileMasses);
     }
     std::printf(fails ? "FAILED %d\n" : "OK\n", fails);

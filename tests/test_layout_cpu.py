@@ -1,7 +1,8 @@
-"""Host-side checks of the device layout builders (no GPU): the tiled-layout
-builder is compiled with g++ into a small driver that compares every tile's
-spring list, halo, endpoint encoding and per-mass incidence order with brute
-force on random graphs and a Morton-ordered lattice."""
+"""Host-side checks of the device layout builders (no GPU): the builders are
+compiled with g++ into a small driver that compares every tile's halo and
+every mass's incidence list (other endpoint as arena slot or LDS row, spring
+parameters, ascending spring order, padding) with brute force on random
+graphs and a Morton-ordered lattice."""
 import subprocess
 from pathlib import Path
 

@@ -99,7 +99,7 @@ struct BodyH {
     bool tiled = false;
     int64_t n_masses = 0, chunk0 = 0, nchunks = 0;
     int64_t inc0 = 0, inc_len = 0, act0 = 0, act_len = 0, lc0 = 0, lc_len = 0;
-    int64_t tile0 = 0, ntiles = 0, halo0 = 0, nhalo = 0, spr0 = 0, nspr = 0, grp0 = 0, ngrp = 0;
+    int64_t tile0 = 0, ntiles = 0, halo0 = 0, nhalo = 0;
     int32_t hmax = 0;
     double T = 0;   // host mirror of the body clock
     int64_t n = 0;  // steps taken (parity selects the state buffer)
@@ -122,7 +122,7 @@ struct titan_ctx {
     double T = 0.0;                      // Simulation::T
     std::vector<double> planes_h, balls_h;
 
-    Ranges chunks, incs, acts, lcs, bids, tiles, halos, sprs, grps;
+    Ranges chunks, incs, acts, lcs, bids, tiles, halos;
     std::vector<BodyH> bodies;
 
     // mass arena (ncap = chunks.cap() * 64)
@@ -142,10 +142,6 @@ struct titan_ctx {
     // tiled layout arenas
     TileDesc *tile = nullptr;
     int32_t *halo = nullptr;
-    uint32_t *t_ends = nullptr;
-    double *t_rest = nullptr, *t_k = nullptr, *t_damp = nullptr;
-    int32_t *t_act = nullptr;
-    uint64_t *t_inc = nullptr;
     uint8_t *b_tiled = nullptr;
     // body table
     int64_t bcap = 0;
@@ -159,7 +155,6 @@ struct titan_ctx {
     bool has_act = false;  // some spring is actuated: launch the actuated variants
     double k_uni = 0, damp_uni = 0;
     int slot = 0;
-    int unroll = 2;  // incident springs per gather iteration (TITAN_GATHER_UNROLL)
 
     // captured multi-step graphs, valid while `version` is unchanged
     uint64_t version = 1;
@@ -347,49 +342,16 @@ void ensure_halos(titan_ctx *c, int64_t need) {
     drop_graphs(c);
 }
 
-void ensure_sprs(titan_ctx *c, int64_t need) {
-    if (c->sprs.cap() >= need) return;
-    const int64_t old = c->sprs.cap();
-    const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 4096));
-    grow_soa(c, c->t_ends, 1, old, nc);
-    grow_soa(c, c->t_rest, 1, old, nc);
-    if (c->t_k) grow_soa(c, c->t_k, 1, old, nc);
-    if (c->t_damp) grow_soa(c, c->t_damp, 1, old, nc);
-    if (c->t_act) {
-        int32_t *q = dalloc<int32_t>(c, nc);
-        HIPCHK(hipMemsetAsync(q, 0xff, sizeof(int32_t) * nc, c->stream));
-        HIPCHK(hipMemcpyAsync(q, c->t_act, sizeof(int32_t) * old, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        dfree(c, c->t_act);
-        c->t_act = q;
-    }
-    c->sprs.grow(nc);
-    drop_graphs(c);
-}
-
-void ensure_grps(titan_ctx *c, int64_t need) {
-    if (c->grps.cap() >= need) return;
-    const int64_t old = c->grps.cap();
-    const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 4096));
-    grow_soa(c, c->t_inc, 1, old * kChunk, nc * kChunk);
-    c->grps.grow(nc);
-    drop_graphs(c);
-}
-
-// Switch from by-value spring constants to per-spring arrays (both layouts).
+// Switch from by-value spring constants to per-incidence arrays.
 void make_per_incidence_kd(titan_ctx *c) {
     if (!c->uniform_kd) return;
-    // at least one entry each, so that the arrays exist (and grow with their
-    // arenas) even while one of the two layouts holds no springs
-    const int64_t n = std::max<int64_t>(c->incs.cap(), 1), m = std::max<int64_t>(c->sprs.cap(), 1);
+    // at least one entry, so that the arrays exist (and grow with their
+    // arena) even while the arena is empty
+    const int64_t n = std::max<int64_t>(c->incs.cap(), 1);
     c->inc_k = dalloc<double>(c, n);
     c->inc_damp = dalloc<double>(c, n);
     HIPCHK(launch_fill_f64(c->inc_k, n, c->k_uni, c->stream));
     HIPCHK(launch_fill_f64(c->inc_damp, n, c->damp_uni, c->stream));
-    c->t_k = dalloc<double>(c, m);
-    c->t_damp = dalloc<double>(c, m);
-    HIPCHK(launch_fill_f64(c->t_k, m, c->k_uni, c->stream));
-    HIPCHK(launch_fill_f64(c->t_damp, m, c->damp_uni, c->stream));
     c->uniform_kd = false;
     drop_graphs(c);
 }
@@ -397,11 +359,9 @@ void make_per_incidence_kd(titan_ctx *c) {
 void make_act(titan_ctx *c) {
     if (c->has_act) return;
     c->has_act = true;
-    const int64_t n = std::max<int64_t>(c->incs.cap(), 1), m = std::max<int64_t>(c->sprs.cap(), 1);
+    const int64_t n = std::max<int64_t>(c->incs.cap(), 1);
     c->inc_act = dalloc<int32_t>(c, n);
     HIPCHK(hipMemsetAsync(c->inc_act, 0xff, sizeof(int32_t) * n, c->stream));
-    c->t_act = dalloc<int32_t>(c, m);
-    HIPCHK(hipMemsetAsync(c->t_act, 0xff, sizeof(int32_t) * m, c->stream));
     drop_graphs(c);
 }
 
@@ -474,12 +434,6 @@ StepArgs make_args(titan_ctx *c, int slot, int write_acc) {
     a.tiles = c->tile;
     a.ntiles = c->tiles.high_water();
     a.halo = c->halo;
-    a.t_ends = c->t_ends;
-    a.t_rest = c->t_rest;
-    a.t_k = c->t_k;
-    a.t_damp = c->t_damp;
-    a.t_act = c->t_act;
-    a.t_inc = c->t_inc;
     a.ldim = kTileMasses + tiled_hmax(c);
     a.b_tiled = c->b_tiled;
     return a;
@@ -492,7 +446,7 @@ hipError_t launch_both(titan_ctx *c, int slot, int write_acc) {
     StepArgs a = make_args(c, slot, write_acc);
     const bool act = c->has_act;
     if (any_body(c, false)) {
-        hipError_t e = launch_step(a, c->uniform_kd, act, c->unroll, c->stream);
+        hipError_t e = launch_step(a, c->uniform_kd, act, c->stream);
         if (e != hipSuccess) return e;
     }
     if (any_body(c, true)) return launch_step_tiles(a, c->uniform_kd, act, c->stream);
@@ -664,61 +618,55 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
 
     // ---- layout: tiled (masses along a Morton curve) or gather (caller's order)
     std::vector<int32_t> perm;  // device order -> caller's order
-    std::vector<int32_t> deg(npad, 0);
     TileBuild tb;
     bool tiled = false;
-    if (d->layout != TITAN_LAYOUT_GATHER) {
-        if (N > kTileMasses) perm = morton_order(d->pos, N);
-        std::vector<int32_t> inv(N);
-        for (int64_t q = 0; q < N; ++q) inv[perm.empty() ? q : perm[q]] = (int32_t)q;
-        std::vector<int32_t> nl(S), nr(S);
-        for (int64_t s = 0; s < S; ++s) {
-            nl[s] = d->left[s] >= 0 && d->right[s] >= 0 ? inv[d->left[s]] : -1;
-            nr[s] = d->left[s] >= 0 && d->right[s] >= 0 ? inv[d->right[s]] : -1;
+    std::vector<int32_t> nl, nr;  // endpoints in device order
+    if (d->layout != TITAN_LAYOUT_GATHER && N > kTileMasses) perm = morton_order(d->pos, N);
+    {
+        std::vector<int32_t> inv;
+        if (!perm.empty()) {
+            inv.resize(N);
+            for (int64_t q = 0; q < N; ++q) inv[perm[q]] = (int32_t)q;
         }
-        SpringsIn sp;
-        sp.n_springs = S;
-        sp.left = nl.data();
-        sp.right = nr.data();
-        sp.k = d->k;
-        sp.rest = d->rest;
-        sp.damping = d->damping;
-        sp.act = n_act > 0 ? act_local.data() : nullptr;
-        std::vector<int32_t> tdeg;
-        tiled = build_tiles(N, sp, !uniform_after, tb, tdeg);
+        nl.resize(S);
+        nr.resize(S);
+        for (int64_t s = 0; s < S; ++s) {
+            const bool ok = d->left[s] >= 0 && d->right[s] >= 0;
+            nl[s] = !ok ? -1 : perm.empty() ? (int32_t)d->left[s] : inv[d->left[s]];
+            nr[s] = !ok ? -1 : perm.empty() ? (int32_t)d->right[s] : inv[d->right[s]];
+        }
+    }
+    SpringsIn sp;
+    sp.n_springs = S;
+    sp.left = nl.data();
+    sp.right = nr.data();
+    sp.k = d->k;
+    sp.rest = d->rest;
+    sp.damping = d->damping;
+    sp.act = n_act > 0 ? act_local.data() : nullptr;
+    if (d->layout != TITAN_LAYOUT_GATHER) {
+        tiled = build_tiles(N, sp, tb);
         if (!tiled && d->layout == TITAN_LAYOUT_TILED)
             fail(TITAN_E_INVALID, "body_create: tiled layout does not fit: " + tb.why);
-        if (tiled) std::copy(tdeg.begin(), tdeg.end(), deg.begin());
-        else perm.clear();
+        if (!tiled && !perm.empty()) {  // fall back to the caller's order
+            perm.clear();
+            for (int64_t s = 0; s < S; ++s) {
+                const bool ok = d->left[s] >= 0 && d->right[s] >= 0;
+                nl[s] = ok ? (int32_t)d->left[s] : -1;
+                nr[s] = ok ? (int32_t)d->right[s] : -1;
+            }
+        }
     }
     auto orig = [&](int64_t q) -> int64_t { return perm.empty() ? q : perm[q]; };
-
-    // gather layout: SELL-64 incidences, ascending spring index per mass
-    std::vector<int64_t> coff(nch + 1, 0);
-    std::vector<uint32_t> nbr;
-    std::vector<double> grest, gk, gd;
-    std::vector<int32_t> gact;
-    int64_t n_inc = 0;
-    if (!tiled) {
-        for (int64_t s = 0; s < S; ++s) {
-            if (d->left[s] < 0 || d->right[s] < 0) continue;
-            deg[d->left[s]]++;
-            deg[d->right[s]]++;
-        }
-        for (int64_t ch = 0; ch < nch; ++ch) {
-            int32_t w = 0;
-            for (int l = 0; l < kChunk; ++l) w = std::max(w, deg[ch * kChunk + l]);
-            coff[ch + 1] = coff[ch] + (int64_t)((w + 3) / 4) * 4 * kChunk;
-        }
-        n_inc = coff[nch];
-    }
+    // incidences of both layouts; the gather layout's name arena slots,
+    // which are known once the chunks are placed (built below)
+    SellBuild sb;
+    if (tiled) build_sell(N, sp, !uniform_after, &tb, 0, sb);
 
     // ---- reserve space; on failure release whatever was taken
     const int64_t n_tiles = tiled ? (int64_t)tb.tiles.size() : 0;
     const int64_t n_halo = tiled ? (int64_t)tb.halo.size() : 0;
-    const int64_t n_spr = tiled ? (int64_t)tb.ends.size() : 0;
-    const int64_t n_grp = tiled ? (int64_t)tb.inc.size() / kChunk : 0;
-    int64_t ch0 = -1, inc0 = -1, act0 = -1, lc0 = -1, bid = -1, tl0 = -1, hl0 = -1, sp0 = -1, gp0 = -1;
+    int64_t ch0 = -1, inc0 = -1, act0 = -1, lc0 = -1, bid = -1, tl0 = -1, hl0 = -1;
     auto take = [&](Ranges &r, int64_t len, void (*grow)(titan_ctx *, int64_t)) -> int64_t {
         int64_t at = r.alloc(len);
         if (at < 0) {
@@ -731,13 +679,15 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
         ch0 = take(c->chunks, nch, ensure_chunks);
         if ((ch0 + nch) * kChunk > kMaxMassSlots)
             fail(TITAN_E_NOMEM, "body_create: mass arena limit (2^26 slots) reached");
-        inc0 = take(c->incs, n_inc, ensure_incs);
+        if (!tiled) {  // gather incidences name arena slots: build them now
+            build_sell(N, sp, !uniform_after, nullptr, ch0 * kChunk, sb);
+        }
+        const int64_t n_inc_all = sb.coff[nch];
+        inc0 = take(c->incs, n_inc_all, ensure_incs);
         act0 = take(c->acts, n_act, ensure_acts);
         lc0 = take(c->lcs, n_lc, ensure_lcs);
         tl0 = take(c->tiles, n_tiles, ensure_tiles);
         hl0 = take(c->halos, n_halo, ensure_halos);
-        sp0 = take(c->sprs, n_spr, ensure_sprs);
-        gp0 = take(c->grps, n_grp, ensure_grps);
         bid = take(c->bids, 1, ensure_bodies);
         if (has_springs) {
             if (!c->kd_known) {
@@ -753,13 +703,11 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
         if (n_act > 0) make_act(c);
     } catch (...) {
         if (ch0 >= 0) c->chunks.release(ch0, nch);
-        if (inc0 >= 0) c->incs.release(inc0, n_inc);
+        if (inc0 >= 0) c->incs.release(inc0, sb.coff[nch]);
         if (act0 >= 0) c->acts.release(act0, n_act);
         if (lc0 >= 0) c->lcs.release(lc0, n_lc);
         if (tl0 >= 0) c->tiles.release(tl0, n_tiles);
         if (hl0 >= 0) c->halos.release(hl0, n_halo);
-        if (sp0 >= 0) c->sprs.release(sp0, n_spr);
-        if (gp0 >= 0) c->grps.release(gp0, n_grp);
         if (bid >= 0) c->bids.release(bid, 1);
         throw;
     }
@@ -829,63 +777,28 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
         }
     }
 
-    // ---- gather layout incidences
-    if (!tiled && n_inc > 0) {
-        // padding entries point at the mass itself (an in-bounds load whose
-        // result is not used)
-        nbr.assign(n_inc, 0);
-        for (int64_t ch = 0; ch < nch; ++ch)
-            for (int64_t e = coff[ch]; e < coff[ch + 1]; ++e)
-                nbr[e] = (uint32_t)(g0 + ch * kChunk + ((e - coff[ch]) / 4) % kChunk);
-        grest.assign(n_inc, 1.0);
-        if (!c->uniform_kd) {
-            gk.assign(n_inc, 0.0);
-            gd.assign(n_inc, 0.0);
-        }
-        if (c->inc_act) gact.assign(n_inc, -1);
-        std::vector<int32_t> fillc(npad, 0);
-        for (int64_t s = 0; s < S; ++s) {
-            const int64_t L = d->left[s], R = d->right[s];
-            if (L < 0 || R < 0) continue;
-            const int32_t ai = act_local[s] >= 0 ? (int32_t)(act0 + act_local[s]) : -1;
-            auto place = [&](int64_t me, int64_t other, bool is_left) {
-                const int64_t j = fillc[me]++;
-                const int64_t e = coff[me / kChunk] + ((j / 4) * kChunk + (me % kChunk)) * 4 + j % 4;
-                nbr[e] = (uint32_t)(g0 + other) | (is_left ? kLeftBit : 0u);
-                grest[e] = d->rest[s];
-                if (!gk.empty()) {
-                    gk[e] = d->k[s];
-                    gd[e] = d->damping ? d->damping[s] : 0.0;
-                }
-                if (!gact.empty()) gact[e] = ai;
-            };
-            place(R, L, false);  // the scatter adds +F to the right endpoint first
-            place(L, R, true);
-        }
-    }
-    // ---- tiled layout: rebase to the arenas
+    // ---- incidences: actuator indices to the arena; tiles rebased
+    const int64_t n_inc = sb.coff[nch];
+    for (auto &x : sb.act)
+        if (x >= 0) x += (int32_t)act0;
     if (tiled) {
         for (auto &t : tb.tiles) {
             t.body = (int32_t)bid;
             t.chunk0 += (int32_t)ch0;
             t.halo_off += hl0;
-            t.spr_off += sp0;
-            t.inc_off += gp0 * kChunk;
         }
         for (auto &h : tb.halo) h += (int32_t)g0;
-        for (auto &x : tb.act)
-            if (x >= 0) x += (int32_t)act0;
     }
 
     // ---- upload (buffer parity 0: a new body has taken no steps)
     std::vector<int32_t> cbody(nch, (int32_t)bid);
     std::vector<int64_t> cptr(nch);
-    for (int64_t ch = 0; ch < nch; ++ch) cptr[ch] = inc0 + coff[ch];
+    for (int64_t ch = 0; ch < nch; ++ch) cptr[ch] = inc0 + sb.coff[ch];
     h2d(c, c->st + 6 * g0, st.data(), 6 * npad);
     h2d(c, c->m + g0, mm.data(), npad);
     h2d(c, c->Tm + g0, Tm.data(), npad);
     h2d(c, c->mflags + g0, fl.data(), npad);
-    h2d(c, c->deg + g0, deg.data(), npad);
+    h2d(c, c->deg + g0, sb.deg.data(), npad);
     std::vector<double> zeros(3 * npad, 0.0);
     for (int k = 0; k < 3; ++k) h2d(c, c->acc + k * c->ncap + g0, zeros.data(), npad);
     if (c->ext) {
@@ -899,33 +812,20 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     h2d(c, c->chunk_body + ch0, cbody.data(), nch);
     h2d(c, c->chunk_ptr + ch0, cptr.data(), nch);
     if (n_inc > 0) {
-        h2d(c, c->inc_nbr + inc0, nbr.data(), n_inc);
-        h2d(c, c->inc_rest + inc0, grest.data(), n_inc);
-        if (!gk.empty()) {
-            h2d(c, c->inc_k + inc0, gk.data(), n_inc);
-            h2d(c, c->inc_damp + inc0, gd.data(), n_inc);
+        h2d(c, c->inc_nbr + inc0, sb.nbr.data(), n_inc);
+        h2d(c, c->inc_rest + inc0, sb.rest.data(), n_inc);
+        if (!c->uniform_kd) {
+            h2d(c, c->inc_k + inc0, sb.k.data(), n_inc);
+            h2d(c, c->inc_damp + inc0, sb.damp.data(), n_inc);
         }
-        if (!gact.empty()) h2d(c, c->inc_act + inc0, gact.data(), n_inc);
+        if (c->inc_act) {
+            if (!sb.act.empty()) h2d(c, c->inc_act + inc0, sb.act.data(), n_inc);
+            else HIPCHK(hipMemsetAsync(c->inc_act + inc0, 0xff, sizeof(int32_t) * n_inc, c->stream));
+        }
     }
     if (tiled) {
         h2d(c, c->tile + tl0, tb.tiles.data(), n_tiles);
         h2d(c, c->halo + hl0, tb.halo.data(), n_halo);
-        h2d(c, c->t_ends + sp0, tb.ends.data(), n_spr);
-        h2d(c, c->t_rest + sp0, tb.rest.data(), n_spr);
-        if (!c->uniform_kd) {
-            if ((int64_t)tb.k.size() == n_spr) {
-                h2d(c, c->t_k + sp0, tb.k.data(), n_spr);
-                h2d(c, c->t_damp + sp0, tb.damp.data(), n_spr);
-            } else {  // built for by-value constants: this body's are uniform
-                HIPCHK(launch_fill_f64(c->t_k + sp0, n_spr, k0, c->stream));
-                HIPCHK(launch_fill_f64(c->t_damp + sp0, n_spr, d0, c->stream));
-            }
-        }
-        if (c->t_act) {
-            if (!tb.act.empty()) h2d(c, c->t_act + sp0, tb.act.data(), n_spr);
-            else if (n_spr > 0) HIPCHK(hipMemsetAsync(c->t_act + sp0, 0xff, sizeof(int32_t) * n_spr, c->stream));
-        }
-        h2d(c, c->t_inc + gp0 * kChunk, tb.inc.data(), n_grp * kChunk);
     }
     if (!acth.empty()) h2d(c, c->act + 4 * act0, acth.data(), (int64_t)acth.size());
     if (!lch.empty()) h2d(c, c->lc + lc0, lch.data(), (int64_t)lch.size());
@@ -962,10 +862,6 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     b.ntiles = n_tiles;
     b.halo0 = hl0;
     b.nhalo = n_halo;
-    b.spr0 = sp0;
-    b.nspr = n_spr;
-    b.grp0 = gp0;
-    b.ngrp = n_grp;
     b.hmax = tb.hmax;
     b.T = t0;
     b.n = 0;
@@ -999,8 +895,6 @@ void body_destroy(titan_ctx *c, int32_t id) {
     c->lcs.release(b.lc0, b.lc_len);
     c->tiles.release(b.tile0, b.ntiles);
     c->halos.release(b.halo0, b.nhalo);
-    c->sprs.release(b.spr0, b.nspr);
-    c->grps.release(b.grp0, b.ngrp);
     c->bids.release(id, 1);
     b = BodyH{};
     drop_graphs(c);
@@ -1151,7 +1045,6 @@ int titan_ctx_create(int device, uint64_t hbm_budget_bytes, titan_ctx **out) {
     HIPCHK(hipSetDevice(device));
     c = new titan_ctx();
     c->device = device;
-    if (const char *u = std::getenv("TITAN_GATHER_UNROLL")) c->unroll = std::atoi(u);
     c->budget = hbm_budget_bytes;
     HIPCHK(set_tiles_lds_limit());
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));

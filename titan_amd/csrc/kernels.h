@@ -7,7 +7,7 @@
 
 namespace titan_mi355x {
 
-hipError_t launch_step(const StepArgs &a, bool uniform_kd, bool actuated, int unroll, hipStream_t s);
+hipError_t launch_step(const StepArgs &a, bool uniform_kd, bool actuated, hipStream_t s);
 const char *step_kernel_name(bool uniform_kd, bool actuated);
 hipError_t launch_step_tiles(const StepArgs &a, bool uniform_kd, bool actuated, hipStream_t s);
 const char *step_tiles_kernel_name(bool uniform_kd, bool actuated);

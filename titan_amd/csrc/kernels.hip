@@ -206,14 +206,11 @@ __device__ __forceinline__ void mass_pass(const StepArgs &a, MassRegs &r, int64_
 
 }  // namespace
 
-// One lane per mass slot, one wavefront per 64-mass chunk, four chunks per
-// workgroup.  UNI: spring constant and damping are the same for every
-// spring (passed by value).  ACT: some springs are actuated.
-//
-// Incidences are stored four per lane per group ([group][lane][4]), so one
-// 16-byte load gives a lane the other endpoints of four springs; the next
-// group's indices are prefetched while the current group's twelve 16-byte
-// neighbour loads are in flight.
+// Spring forces on one mass, src/sim.cu:1055-1073, summed in ascending
+// spring index: the lane walks its d incidences (SELL-64 groups of G
+// starting at entry `ebase` of its chunk) and fetches the other endpoint's
+// state (px py | pz vx | vy vz) with fetch(name, q0, q1, q2).  G = 4: four
+// uint32 names per 16-byte load; G = 2: two uint16 names per 4-byte load.
 //
 // Each incident spring is evaluated from this mass's side: tt = p_other -
 // p_self and dv = v_self - v_other.  If this mass is the left endpoint, tt
@@ -222,7 +219,129 @@ __device__ __forceinline__ void mass_pass(const StepArgs &a, MassRegs &r, int64_
 // dv are -tt and -dv exactly (x - y == -(y - x) under round-to-nearest), so
 // its F is -G exactly and this mass receives +F = -G.  Either way the
 // contribution is -G, bit for bit (an exact zero may differ in sign).
-template <bool UNI, bool ACT, int U>
+//
+// The next group's names are prefetched while the current group's
+// neighbours are fetched.
+template <int G>
+struct Names;
+template <>
+struct Names<4> {
+    uint4 v;
+    __device__ static Names load(const uint32_t *base, int64_t ebase, int lane, int g) {
+        return {reinterpret_cast<const uint4 *>(base + ebase)[(int64_t)g * kChunk + lane]};
+    }
+    __device__ void get(uint32_t (&o)[4]) const {
+        o[0] = v.x;
+        o[1] = v.y;
+        o[2] = v.z;
+        o[3] = v.w;
+    }
+};
+template <>
+struct Names<2> {
+    uint32_t v;
+    __device__ static Names load(const uint32_t *base, int64_t ebase, int lane, int g) {
+        return {base[ebase + (int64_t)g * kChunk + lane]};
+    }
+    __device__ void get(uint32_t (&o)[2]) const {
+        o[0] = v & 0xffffu;
+        o[1] = v >> 16;
+    }
+};
+
+template <bool UNI, bool ACT, int G, class Fetch>
+__device__ __forceinline__ void spring_forces(const StepArgs &a, MassRegs &r, int d, int64_t ebase,
+                                              int lane, double T, Fetch fetch) {
+    static_assert(G == 2 || G == 4, "groups of 2 or 4");
+    constexpr int P = G / 2;  // 16-byte pairs per group and lane
+    r.fx = 0.0;
+    r.fy = 0.0;
+    r.fz = 0.0;
+    const int ng = (d + G - 1) / G;
+    const double2 *__restrict__ gr = reinterpret_cast<const double2 *>(a.inc_rest + ebase) + P * lane;
+    const double2 *__restrict__ gk = UNI ? nullptr : reinterpret_cast<const double2 *>(a.inc_k + ebase) + P * lane;
+    const double2 *__restrict__ gd = UNI ? nullptr : reinterpret_cast<const double2 *>(a.inc_damp + ebase) + P * lane;
+    const int32_t *__restrict__ ga = ACT ? a.inc_act + ebase + G * lane : nullptr;
+    Names<G> ni{};
+    if (ng > 0) ni = Names<G>::load(a.inc_nbr, ebase, lane, 0);
+    for (int g = 0; g < ng; ++g) {
+        const Names<G> ci = ni;
+        if (g + 1 < ng) ni = Names<G>::load(a.inc_nbr, ebase, lane, g + 1);
+        uint32_t oq[G];
+        ci.get(oq);
+        double2 nb[G][3];
+#pragma unroll
+        for (int q = 0; q < G; ++q) fetch(oq[q], nb[q][0], nb[q][1], nb[q][2]);
+        double rq[G], kq[G], dq[G];
+        int32_t aq[G];
+#pragma unroll
+        for (int h = 0; h < P; ++h) {
+            const double2 rr = gr[(int64_t)g * P * kChunk + h];
+            rq[2 * h] = rr.x;
+            rq[2 * h + 1] = rr.y;
+            if (!UNI) {
+                const double2 kk = gk[(int64_t)g * P * kChunk + h], dd = gd[(int64_t)g * P * kChunk + h];
+                kq[2 * h] = kk.x;
+                kq[2 * h + 1] = kk.y;
+                dq[2 * h] = dd.x;
+                dq[2 * h + 1] = dd.y;
+            }
+        }
+        if (ACT) {
+#pragma unroll
+            for (int q = 0; q < G; ++q) aq[q] = ga[(int64_t)g * G * kChunk + q];
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            if (G * g + q >= d) break;
+            const double tx = nb[q][0].x - r.px, ty = nb[q][0].y - r.py, tz = nb[q][1].x - r.pz;
+            const double dvx = r.vx - nb[q][1].y, dvy = r.vy - nb[q][2].x, dvz = r.vz - nb[q][2].y;
+            double nrm, ux, uy, uz;
+            unit_axis(tx, ty, tz, nrm, ux, uy, uz);
+            double rl = rq[q];
+            if (ACT && aq[q] >= 0) {
+                const double *A = a.act + 4 * (int64_t)aq[q];
+                rl = rl * (1 + A[0] * sin(A[1] * T + A[2]));
+            }
+            const double k = UNI ? a.k_uni : kq[q];
+            const double dmp = UNI ? a.damp_uni : dq[q];
+            const double mag = k * (rl - nrm);
+            const double dd = (dvx * ux + dvy * uy + dvz * uz) * dmp;
+            r.fx = r.fx - (ux * mag + ux * dd);
+            r.fy = r.fy - (uy * mag + uy * dd);
+            r.fz = r.fz - (uz * mag + uz * dd);
+        }
+    }
+}
+
+// The body clock of launch slot s is advanced once per step by the body's
+// first lane: T += dt (src/sim.cu:1590) while steps are left.
+__device__ __forceinline__ void advance_body_clock(const StepArgs &a, int b, int s, int64_t left,
+                                                   int64_t nst, double T) {
+    const bool go = left > 0;
+    a.bT[s ^ 1][b] = go ? T + a.dt : T;
+    a.bn[s ^ 1][b] = go ? nst + 1 : nst;
+    a.bleft[s ^ 1][b] = go ? left - 1 : 0;
+}
+
+__device__ __forceinline__ void store_mass(const StepArgs &a, double2 *__restrict__ nxt, int64_t i,
+                                           const MassRegs &r, double ax, double ay, double az) {
+    nxt[3 * i] = make_double2(r.px, r.py);
+    nxt[3 * i + 1] = make_double2(r.pz, r.vx);
+    nxt[3 * i + 2] = make_double2(r.vy, r.vz);
+    if (a.write_acc) {
+        a.acc[i] = ax;
+        a.acc[a.ncap + i] = ay;
+        a.acc[2 * a.ncap + i] = az;
+    }
+}
+
+// Gather step: one lane per mass slot, one wavefront per 64-mass chunk, four
+// chunks per workgroup; the other endpoints' states are read from the
+// current state buffer (global memory).  UNI: spring constant and damping
+// are the same for every spring (passed by value).  ACT: some springs are
+// actuated.
+template <bool UNI, bool ACT>
 __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int64_t chunk = blk * kWavesPerBlock + uniform(threadIdx.x >> 6);
@@ -234,12 +353,7 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     const int64_t left = a.bleft[s][b];
     const int64_t nst = a.bn[s][b];
     const double T = a.bT[s][b];
-    if (lane == 0 && a.b_chunk0[b] == chunk) {  // the body's first lane advances its clock
-        const bool go = left > 0;
-        a.bT[s ^ 1][b] = go ? T + a.dt : T;     // src/sim.cu:1590
-        a.bn[s ^ 1][b] = go ? nst + 1 : nst;
-        a.bleft[s ^ 1][b] = go ? left - 1 : 0;
-    }
+    if (lane == 0 && a.b_chunk0[b] == chunk) advance_body_clock(a, b, s, left, nst, T);
     if (left <= 0) return;
 
     const int64_t n6 = a.ncap;
@@ -263,100 +377,28 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     r.vx = o1.y;
     r.vy = o2.x;
     r.vz = o2.y;
-
-    // Spring forces, src/sim.cu:1055-1073, summed in ascending spring index.
-    r.fx = 0.0;
-    r.fy = 0.0;
-    r.fz = 0.0;
-    const int d = a.deg[i];
-    const int ng = (d + 3) >> 2;
-    const int64_t ebase = a.chunk_ptr[chunk];
-    const uint4 *__restrict__ gi = reinterpret_cast<const uint4 *>(a.inc_nbr + ebase) + lane;
-    const double2 *__restrict__ gr = reinterpret_cast<const double2 *>(a.inc_rest + ebase) + 2 * lane;
-    const double2 *__restrict__ gk = UNI ? nullptr : reinterpret_cast<const double2 *>(a.inc_k + ebase) + 2 * lane;
-    const double2 *__restrict__ gd = UNI ? nullptr : reinterpret_cast<const double2 *>(a.inc_damp + ebase) + 2 * lane;
-    const int4 *__restrict__ ga = ACT ? reinterpret_cast<const int4 *>(a.inc_act + ebase) + lane : nullptr;
-    uint4 ni = make_uint4(0, 0, 0, 0);
-    if (ng > 0) ni = gi[0];
-    for (int g = 0; g < ng; ++g) {
-        const uint4 ci = ni;
-        if (g + 1 < ng) ni = gi[(g + 1) * kChunk];
-        const uint32_t oq[4] = {ci.x & ~kLeftBit, ci.y & ~kLeftBit, ci.z & ~kLeftBit, ci.w & ~kLeftBit};
-        double2 nb[4][3];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            // 48 bytes per mass; the arena caps a buffer below 4 GiB
-            const uint32_t ob = oq[q] * 48u;
-            nb[q][0] = ldo(cur, ob);
-            nb[q][1] = ldo(cur, ob + 16u);
-            nb[q][2] = ldo(cur, ob + 32u);
-        }
-        const double2 r01 = gr[g * 2 * kChunk], r23 = gr[g * 2 * kChunk + 1];
-        const double rq[4] = {r01.x, r01.y, r23.x, r23.y};
-        double kq[4], dq[4];
-        int32_t aq[4] = {-1, -1, -1, -1};
-        if (!UNI) {
-            const double2 k01 = gk[g * 2 * kChunk], k23 = gk[g * 2 * kChunk + 1];
-            const double2 d01 = gd[g * 2 * kChunk], d23 = gd[g * 2 * kChunk + 1];
-            kq[0] = k01.x; kq[1] = k01.y; kq[2] = k23.x; kq[3] = k23.y;
-            dq[0] = d01.x; dq[1] = d01.y; dq[2] = d23.x; dq[3] = d23.y;
-        }
-        if (ACT) {
-            const int4 a4 = ga[g * kChunk];
-            aq[0] = a4.x; aq[1] = a4.y; aq[2] = a4.z; aq[3] = a4.w;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            if (4 * g + q >= d) break;
-            const double tx = nb[q][0].x - r.px, ty = nb[q][0].y - r.py, tz = nb[q][1].x - r.pz;
-            const double dvx = r.vx - nb[q][1].y, dvy = r.vy - nb[q][2].x, dvz = r.vz - nb[q][2].y;
-            double nrm, ux, uy, uz;
-            unit_axis(tx, ty, tz, nrm, ux, uy, uz);
-            double rl = rq[q];
-            if (ACT && aq[q] >= 0) {
-                const double *A = a.act + 4 * (int64_t)aq[q];
-                rl = rl * (1 + A[0] * sin(A[1] * T + A[2]));
-            }
-            const double k = UNI ? a.k_uni : kq[q];
-            const double dmp = UNI ? a.damp_uni : dq[q];
-            const double mag = k * (rl - nrm);
-            const double dd = (dvx * ux + dvy * uy + dvz * uz) * dmp;
-            r.fx = r.fx - (ux * mag + ux * dd);
-            r.fy = r.fy - (uy * mag + uy * dd);
-            r.fz = r.fz - (uz * mag + uz * dd);
-        }
-    }
-    (void)U;
-
+    // 48 bytes per mass; the arena caps a buffer below 4 GiB
+    spring_forces<UNI, ACT, kGroupGather>(a, r, a.deg[i], a.chunk_ptr[chunk], lane, T,
+                            [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
+                                const uint32_t ob = o * 48u;
+                                q0 = ldo(cur, ob);
+                                q1 = ldo(cur, ob + 16u);
+                                q2 = ldo(cur, ob + 32u);
+                            });
     const double m = a.m[i];
     double ax, ay, az;
     mass_pass(a, r, i, fl, b, m, ax, ay, az);
-    nxt[3 * i] = make_double2(r.px, r.py);
-    nxt[3 * i + 1] = make_double2(r.pz, r.vx);
-    nxt[3 * i + 2] = make_double2(r.vy, r.vz);
-    if (a.write_acc) {
-        a.acc[i] = ax;
-        a.acc[n6 + i] = ay;
-        a.acc[2 * n6 + i] = az;
-    }
+    store_mass(a, nxt, i, r, ax, ay, az);
 }
 
-// Tiled step: one workgroup per tile of up to 256 masses of one body.
-//   stage:   tile and halo positions/velocities -> LDS, 48 bytes per mass
-//            (px py pz vx vy vz, read back as three 16-byte pairs);
-//   batches: springs of the tile in ascending index, kTileBatch at a time:
-//            each spring's force F (the reference's orientation, on the
-//            right endpoint) -> a 32-byte LDS slot; then every tile mass
-//            adds the slots of its incident springs that fall in the batch,
-//            +F as the right endpoint and -F as the left one, in ascending
-//            spring index (so the sum is the oracle's, bit for bit);
-//   update:  gravity, extern force, planes, balls, local constraints, drag,
-//            Euler, written to the other state buffer.
-// A spring with both endpoints in the tile is evaluated once; one that
-// crosses the tile boundary is evaluated by both tiles.
+// Tiled step: one workgroup per tile of up to 256 masses of one body (masses
+// in Morton order).  The tile's masses and its halo (the other endpoints of
+// its springs outside the tile) are staged in LDS, 48 bytes per mass, after
+// which each lane walks its incidences as the gather step does, reading the
+// other endpoints from LDS rows instead of global memory.
 template <bool UNI, bool ACT>
 __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
-    extern __shared__ double lds[];
+    extern __shared__ double2 S[];  // 3 per LDS row
     const int64_t tile = xcd_block(blockIdx.x, gridDim.x);
     if (tile >= a.ntiles) return;
     const int t = threadIdx.x;
@@ -367,25 +409,17 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     const int64_t left = a.bleft[s][b];
     const int64_t nst = a.bn[s][b];
     const double T = a.bT[s][b];
-    if (t == 0 && td->first) {  // the body's first tile advances its clock
-        const bool go = left > 0;
-        a.bT[s ^ 1][b] = go ? T + a.dt : T;  // src/sim.cu:1590
-        a.bn[s ^ 1][b] = go ? nst + 1 : nst;
-        a.bleft[s ^ 1][b] = go ? left - 1 : 0;
-    }
+    if (t == 0 && td->first) advance_body_clock(a, b, s, left, nst, T);
     if (left <= 0) return;  // uniform over the workgroup
 
     const int64_t n6 = a.ncap;
     const int p = (int)(nst & 1);
     const double2 *__restrict__ cur = reinterpret_cast<const double2 *>(a.st + (int64_t)p * 6 * n6);
     double2 *__restrict__ nxt = reinterpret_cast<double2 *>(a.st + (int64_t)(p ^ 1) * 6 * n6);
-    double2 *S = reinterpret_cast<double2 *>(lds);                 // 3 pairs per mass
-    double2 *F = reinterpret_cast<double2 *>(lds + 6 * a.ldim);    // 2 pairs per slot
-
-    const int nmass = td->nmass, nhalo = td->nhalo, nspr = td->nspr;
-    const int64_t i = (int64_t)td->chunk0 * kChunk + t;
+    const int nmass = td->nmass, nhalo = td->nhalo;
+    const int64_t chunk = (int64_t)td->chunk0 + (t >> 6);
+    const int64_t i = chunk * kChunk + (t & (kChunk - 1));
     const bool mine = t < nmass;
-    MassRegs r{};
     uint32_t fl = kPad;
     double2 o0 = make_double2(0.0, 0.0), o1 = o0, o2 = o0;
     if (mine) {
@@ -393,12 +427,6 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
         o0 = cur[3 * i];
         o1 = cur[3 * i + 1];
         o2 = cur[3 * i + 2];
-        r.px = o0.x;
-        r.py = o0.y;
-        r.pz = o1.x;
-        r.vx = o1.y;
-        r.vy = o2.x;
-        r.vz = o2.y;
     }
     S[3 * t] = o0;
     S[3 * t + 1] = o1;
@@ -411,125 +439,31 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
         S[q + 1] = ldo(cur, ob + 16u);
         S[q + 2] = ldo(cur, ob + 32u);
     }
-
-    // this lane's incidence list: uint16 entries, four per group, the next
-    // group prefetched; consumed entries are shifted out of g0
-    const bool accumulate = mine && !(fl & (kFixed | kPad));
-    const int deg = accumulate ? a.deg[i] : 0;
-    const int chunk_in_tile = t >> 6, lane = t & (kChunk - 1);
-    const uint64_t *__restrict__ inc =
-        a.t_inc + td->inc_off + (int64_t)td->gstart[chunk_in_tile] * kChunk + lane;
-    uint64_t g0 = deg > 0 ? inc[0] : 0, g1 = deg > 4 ? inc[kChunk] : 0;
-    int j = 0;
-    r.fx = 0.0;
-    r.fy = 0.0;
-    r.fz = 0.0;
-
-    // Spring records of the next batch are loaded into registers while the
-    // current batch is evaluated.
-    constexpr int kPer = kTileBatch / kTileMasses;
-    const uint32_t *__restrict__ ends = a.t_ends + td->spr_off;
-    const double *__restrict__ rest = a.t_rest + td->spr_off;
-    const int32_t *__restrict__ tact = ACT ? a.t_act + td->spr_off : nullptr;
-    const double *__restrict__ tk = UNI ? nullptr : a.t_k + td->spr_off;
-    const double *__restrict__ tdm = UNI ? nullptr : a.t_damp + td->spr_off;
-    uint32_t pe[kPer];
-    double pr[kPer], pk[kPer], pd[kPer];
-    int32_t pa[kPer];
-    auto prefetch = [&](int bs) {
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int e = bs + t + u * kTileMasses;
-            const bool ok = e < nspr;
-            pe[u] = ok ? ends[e] : 0u;
-            pr[u] = ok ? rest[e] : 0.0;
-            if (!UNI) {
-                pk[u] = ok ? tk[e] : 0.0;
-                pd[u] = ok ? tdm[e] : 0.0;
-            }
-            if (ACT) pa[u] = ok ? tact[e] : -1;
-        }
-    };
-    prefetch(0);
-    __syncthreads();
-    for (int bs = 0; bs < nspr; bs += kTileBatch) {
-        const int bl = min(kTileBatch, nspr - bs);
-        uint32_t ce[kPer];
-        double cr[kPer], ck[kPer], cd[kPer];
-        int32_t ca[kPer];
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            ce[u] = pe[u];
-            cr[u] = pr[u];
-            ck[u] = UNI ? a.k_uni : pk[u];
-            cd[u] = UNI ? a.damp_uni : pd[u];
-            ca[u] = ACT ? pa[u] : -1;
-        }
-        if (bs + kTileBatch < nspr) prefetch(bs + kTileBatch);
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int q = t + u * kTileMasses;
-            if (q >= bl) continue;
-            const double2 *Ls = S + 3 * (int)(ce[u] >> 16), *Rs = S + 3 * (int)(ce[u] & 0xffff);
-            const double2 l0 = Ls[0], l1 = Ls[1], l2 = Ls[2], r0 = Rs[0], r1 = Rs[1], r2 = Rs[2];
-            // src/sim.cu:1055-1065
-            const double tx = r0.x - l0.x, ty = r0.y - l0.y, tz = r1.x - l1.x;
-            double nrm, ux, uy, uz;
-            unit_axis(tx, ty, tz, nrm, ux, uy, uz);
-            double rl = cr[u];
-            if (ACT && ca[u] >= 0) {
-                const double *A = a.act + 4 * (int64_t)ca[u];
-                rl = rl * (1 + A[0] * sin(A[1] * T + A[2]));
-            }
-            const double mag = ck[u] * (rl - nrm);
-            const double dvx = l1.y - r1.y, dvy = l2.x - r2.x, dvz = l2.y - r2.y;
-            const double dd = (dvx * ux + dvy * uy + dvz * uz) * cd[u];
-            F[2 * q] = make_double2(ux * mag + ux * dd, uy * mag + uy * dd);
-            F[2 * q + 1].x = uz * mag + uz * dd;
-        }
-        __syncthreads();
-        const int hi = bs + bl;
-        while (j < deg) {
-            const uint32_t ent = (uint32_t)g0 & 0xffffu;
-            const int slot = (int)(ent & 0x7fffu);
-            if (slot >= hi) break;
-            const int q = slot - bs;
-            const double2 fxy = F[2 * q];
-            const double fz = F[2 * q + 1].x;
-            // the left endpoint receives -F (src/sim.cu:1071-1073): flip the
-            // sign bit, an exact negation
-            const uint32_t neg = (ent & (uint32_t)kIncLeft) << 16;
-            r.fx = r.fx + __hiloint2double(__double2hiint(fxy.x) ^ neg, __double2loint(fxy.x));
-            r.fy = r.fy + __hiloint2double(__double2hiint(fxy.y) ^ neg, __double2loint(fxy.y));
-            r.fz = r.fz + __hiloint2double(__double2hiint(fz) ^ neg, __double2loint(fz));
-            ++j;
-            g0 >>= 16;
-            if ((j & 3) == 0) {
-                g0 = g1;
-                g1 = (j + 4 < deg) ? inc[(int64_t)(j / 4 + 1) * kChunk] : 0;
-            }
-        }
-        __syncthreads();
-    }
-
-    if (!mine) return;
+    __syncthreads();  // the last barrier: lanes may leave from here on
+    if (fl & kPad) return;
     if (fl & kFixed) {  // src/sim.cu:1100-1101
         nxt[3 * i] = o0;
         nxt[3 * i + 1] = o1;
         nxt[3 * i + 2] = o2;
         return;
     }
+    MassRegs r;
+    r.px = o0.x;
+    r.py = o0.y;
+    r.pz = o1.x;
+    r.vx = o1.y;
+    r.vy = o2.x;
+    r.vz = o2.y;
+    spring_forces<UNI, ACT, kGroupTiled>(a, r, a.deg[i], a.chunk_ptr[chunk], t & (kChunk - 1), T,
+                            [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
+                                q0 = S[3 * o];
+                                q1 = S[3 * o + 1];
+                                q2 = S[3 * o + 2];
+                            });
     const double m = a.m[i];
     double ax, ay, az;
     mass_pass(a, r, i, fl, b, m, ax, ay, az);
-    nxt[3 * i] = make_double2(r.px, r.py);
-    nxt[3 * i + 1] = make_double2(r.pz, r.vx);
-    nxt[3 * i + 2] = make_double2(r.vy, r.vz);
-    if (a.write_acc) {
-        a.acc[i] = ax;
-        a.acc[n6 + i] = ay;
-        a.acc[2 * n6 + i] = az;
-    }
+    store_mass(a, nxt, i, r, ax, ay, az);
 }
 
 // Per-mass local time: T += dt once per step the body took since the last
@@ -566,26 +500,18 @@ __global__ void k_fill_f64(double *p, int64_t n, double v) {
 
 // ---- launchers ----
 
-template <int U>
-static void launch_u(const StepArgs &a, bool uni, bool act, int64_t blocks, hipStream_t s) {
-    if (uni && !act) hipLaunchKernelGGL((k_step_gather<true, false, U>), dim3(blocks), dim3(kBlock), 0, s, a);
-    else if (uni && act) hipLaunchKernelGGL((k_step_gather<true, true, U>), dim3(blocks), dim3(kBlock), 0, s, a);
-    else if (!uni && !act) hipLaunchKernelGGL((k_step_gather<false, false, U>), dim3(blocks), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL((k_step_gather<false, true, U>), dim3(blocks), dim3(kBlock), 0, s, a);
-}
-
-hipError_t launch_step(const StepArgs &a, bool uni, bool act, int unroll, hipStream_t s) {
+hipError_t launch_step(const StepArgs &a, bool uni, bool act, hipStream_t s) {
     const int64_t blocks = (a.nchunks + kWavesPerBlock - 1) / kWavesPerBlock;
     if (blocks <= 0) return hipSuccess;
-    if (unroll >= 4) launch_u<4>(a, uni, act, blocks, s);
-    else if (unroll == 2) launch_u<2>(a, uni, act, blocks, s);
-    else launch_u<1>(a, uni, act, blocks, s);
+    const dim3 g((unsigned)blocks), blk(kBlock);
+    if (uni && !act) hipLaunchKernelGGL((k_step_gather<true, false>), g, blk, 0, s, a);
+    else if (uni && act) hipLaunchKernelGGL((k_step_gather<true, true>), g, blk, 0, s, a);
+    else if (!uni && !act) hipLaunchKernelGGL((k_step_gather<false, false>), g, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_step_gather<false, true>), g, blk, 0, s, a);
     return hipGetLastError();
 }
 
-size_t tiles_lds_bytes(int ldim) {
-    return sizeof(double) * ((size_t)6 * ldim + (size_t)4 * kTileBatch);
-}
+size_t tiles_lds_bytes(int ldim) { return (size_t)48 * ldim; }
 
 hipError_t launch_step_tiles(const StepArgs &a, bool uni, bool act, hipStream_t s) {
     if (a.ntiles <= 0) return hipSuccess;

@@ -8,13 +8,15 @@
 // A body's current buffer is the parity of its own step count, so bodies
 // that are not stepped need no copy.  Other per-mass arrays are SoA.
 //
-// Gather layout: springs are stored per incidence in a sliced ELLPACK order
-// with a slice of one wavefront and groups of four: incidence j of mass
-// slot `lane` in chunk c sits at chunk_ptr[c] + ((j / 4) * 64 + lane) * 4 +
-// j % 4, so one 16-byte load gives a lane four consecutive incidences and a
-// wave's loads of one group are contiguous.  Each mass lists its incident
-// springs in ascending spring index, which fixes the order of the FP64
-// force sum.
+// Springs are stored per incidence in a sliced ELLPACK order with a slice of
+// one wavefront and groups of G: incidence j of mass slot `lane` in chunk c
+// sits at chunk_ptr[c] + ((j / G) * 64 + lane) * G + j % G, so a lane loads
+// a group's G rest lengths (and constants) as 16-byte pairs and a wave's
+// loads of one group are contiguous.  The other endpoint is named by its
+// arena slot (gather layout, G = 4: four uint32 per 16-byte load) or by its
+// LDS row in the tile (tiled layout, G = 2: two uint16 per 4-byte load).
+// Each mass lists its incident springs in ascending spring index, which
+// fixes the order of the FP64 force sum.
 #pragma once
 
 #include <stdint.h>
@@ -32,33 +34,28 @@ constexpr uint32_t kExt = 4u;       // extern_force may be non-zero
 constexpr uint32_t kDrag = 8u;      // non-zero integer drag coefficient
 constexpr uint32_t kPad = 16u;      // padding slot of a chunk (never read)
 
-// bit 31 of inc_nbr: this mass is the spring's left endpoint (receives -F)
-constexpr uint32_t kLeftBit = 0x80000000u;
-
-// Tiled layout (k_step_tiles): a tile is up to 4 chunks (256 masses) of one
-// body.  Its spring list holds every spring with an endpoint in the tile,
-// in ascending spring index; the endpoints are tile-local indices (0..255 =
-// tile masses, 256 + h = halo mass h).  Each tile mass lists its incidences
-// as uint16 (slot | 0x8000 if it is the left endpoint), four per lane in a
-// uint64 group, groups of one chunk stored lane-major.
+// Tiled layout (k_step_tiles): a body's masses are ordered along a Morton
+// curve and cut into tiles of up to 4 chunks (256 masses).  A tile's halo is
+// the sorted list of masses outside it that share a spring with one of its
+// masses.  The kernel stages the tile's and the halo's state in LDS, so a
+// tiled body's incidences name the other endpoint by its LDS row: 0..255 for
+// a tile mass, 256 + h for halo mass h.
 constexpr int kTileMasses = 256;
-constexpr int kTileBatch = 512;         // force slots per LDS batch
-constexpr int kMaxTileSprings = 32767;  // slot must fit in 15 bits
-constexpr int kMaxHalo = 1536;          // LDS budget for staged halo masses
-constexpr uint16_t kIncLeft = 0x8000;
+constexpr int kMaxHalo = 2048;  // LDS budget: (256 + 2048) x 48 bytes = 108 KiB
+
+// incidences per SELL group: gather (four uint32 slots in one 16-byte load)
+// and tiled (two uint16 LDS rows in one 4-byte load)
+constexpr int kGroupGather = 4;
+constexpr int kGroupTiled = 2;
 
 struct TileDesc {
     int32_t body;
     int32_t chunk0;     // first chunk (arena index once placed)
     int32_t nmass;      // masses in the tile (<= 256)
     int32_t nhalo;
-    int32_t nspr;
-    int32_t first;      // 1 for the body's first tile (advances the body clock)
     int64_t halo_off;   // into the halo arena
-    int64_t spr_off;    // into the tile-spring arenas
-    int64_t inc_off;    // into the incidence-group arena (uint64 units)
-    int32_t gstart[4];  // per chunk: first group, relative to inc_off / 64
-    int32_t pad[4];
+    int32_t first;      // 1 for the body's first tile (advances the body clock)
+    int32_t pad;
 };
 
 // local constraint entry (a reference CudaContactPlane / CudaBall /
@@ -114,14 +111,8 @@ struct StepArgs {
     // tiled path
     const TileDesc *tiles;
     int64_t ntiles;
-    const int32_t *halo;
-    const uint32_t *t_ends;
-    const double *t_rest;
-    const double *t_k;        // null when uniform
-    const double *t_damp;     // null when uniform
-    const int32_t *t_act;     // null when no actuators
-    const uint64_t *t_inc;
-    int32_t ldim;             // LDS row length: kTileMasses + largest halo
+    const int32_t *halo;      // arena mass slots
+    int32_t ldim;             // LDS rows: kTileMasses + largest halo
     const uint8_t *b_tiled;   // per body: 1 = stepped by k_step_tiles
 };
 

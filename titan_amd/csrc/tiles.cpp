@@ -55,11 +55,9 @@ std::vector<int32_t> morton_order(const double *pos, int64_t n) {
     return perm;
 }
 
-bool build_tiles(int64_t n, const SpringsIn &sp, bool per_spring_kd, TileBuild &tb,
-                 std::vector<int32_t> &deg_out) {
+bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb) {
     const int64_t S = sp.n_springs;
-    // incident springs per mass, ascending spring index; the right endpoint
-    // is listed first for a spring that joins a mass to itself
+    // other endpoints per mass (CSR), to collect each tile's halo
     std::vector<int64_t> row(n + 1, 0);
     for (int64_t s = 0; s < S; ++s) {
         if (sp.left[s] < 0 || sp.right[s] < 0) continue;
@@ -67,58 +65,26 @@ bool build_tiles(int64_t n, const SpringsIn &sp, bool per_spring_kd, TileBuild &
         row[sp.left[s] + 1]++;
     }
     for (int64_t i = 0; i < n; ++i) row[i + 1] += row[i];
-    std::vector<int32_t> inc_s(row[n]);
-    std::vector<int8_t> inc_left(row[n]);
+    std::vector<int32_t> other(row[n]);
     {
         std::vector<int64_t> fill(row.begin(), row.end() - 1);
         for (int64_t s = 0; s < S; ++s) {
             const int32_t L = sp.left[s], R = sp.right[s];
             if (L < 0 || R < 0) continue;
-            inc_s[fill[R]] = (int32_t)s;
-            inc_left[fill[R]++] = 0;
-            inc_s[fill[L]] = (int32_t)s;
-            inc_left[fill[L]++] = 1;
+            other[fill[R]++] = L;
+            other[fill[L]++] = R;
         }
     }
-    deg_out.assign(n, 0);
-    for (int64_t i = 0; i < n; ++i) deg_out[i] = (int32_t)(row[i + 1] - row[i]);
-
     const int64_t ntiles = (n + kTileMasses - 1) / kTileMasses;
-    std::vector<int32_t> mark(S, -1), slot_of(S, 0);
-    std::vector<int32_t> list, halo;
     tb.tiles.clear();
     tb.halo.clear();
-    tb.ends.clear();
-    tb.rest.clear();
-    tb.k.clear();
-    tb.damp.clear();
-    tb.act.clear();
-    tb.inc.clear();
-    tb.sid.clear();
     tb.hmax = 0;
+    std::vector<int32_t> halo;
     for (int64_t t = 0; t < ntiles; ++t) {
         const int64_t m0 = t * kTileMasses, m1 = std::min(n, m0 + kTileMasses);
-        list.clear();
-        for (int64_t m = m0; m < m1; ++m)
-            for (int64_t j = row[m]; j < row[m + 1]; ++j) {
-                const int32_t s = inc_s[j];
-                if (mark[s] != (int32_t)t) {
-                    mark[s] = (int32_t)t;
-                    list.push_back(s);
-                }
-            }
-        std::sort(list.begin(), list.end());
-        if ((int64_t)list.size() > kMaxTileSprings) {
-            tb.why = "a tile touches more than " + std::to_string(kMaxTileSprings) + " springs";
-            return false;
-        }
         halo.clear();
-        for (size_t q = 0; q < list.size(); ++q) {
-            const int32_t s = list[q];
-            slot_of[s] = (int32_t)q;
-            for (int32_t e : {sp.left[s], sp.right[s]})
-                if (e < m0 || e >= m1) halo.push_back(e);
-        }
+        for (int64_t j = row[m0]; j < row[m1]; ++j)
+            if (other[j] < m0 || other[j] >= m1) halo.push_back(other[j]);
         std::sort(halo.begin(), halo.end());
         halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
         if ((int64_t)halo.size() > kMaxHalo) {
@@ -126,60 +92,94 @@ bool build_tiles(int64_t n, const SpringsIn &sp, bool per_spring_kd, TileBuild &
             return false;
         }
         tb.hmax = std::max<int32_t>(tb.hmax, (int32_t)halo.size());
-
         TileDesc d{};
         d.body = -1;
         d.chunk0 = (int32_t)(m0 / kChunk);
         d.nmass = (int32_t)(m1 - m0);
         d.nhalo = (int32_t)halo.size();
-        d.nspr = (int32_t)list.size();
-        d.first = t == 0;
         d.halo_off = (int64_t)tb.halo.size();
-        d.spr_off = (int64_t)tb.ends.size();
-        d.inc_off = (int64_t)tb.inc.size();
-        auto local = [&](int32_t e) -> uint32_t {
-            if (e >= m0 && e < m1) return (uint32_t)(e - m0);
-            return (uint32_t)(kTileMasses + (std::lower_bound(halo.begin(), halo.end(), e) - halo.begin()));
-        };
-        tb.sid.insert(tb.sid.end(), list.begin(), list.end());
-        for (int32_t s : list) {
-            tb.ends.push_back((local(sp.left[s]) << 16) | local(sp.right[s]));
-            tb.rest.push_back(sp.rest[s]);
-            if (per_spring_kd) {
-                tb.k.push_back(sp.k[s]);
-                tb.damp.push_back(sp.damping ? sp.damping[s] : 0.0);
-            }
-            if (sp.act) tb.act.push_back(sp.act[s]);
-        }
+        d.first = t == 0;
         tb.halo.insert(tb.halo.end(), halo.begin(), halo.end());
-        // incidence groups, chunk by chunk
-        int32_t g = 0;
-        const int64_t nch = (m1 - m0 + kChunk - 1) / kChunk;
-        for (int64_t c = 0; c < 4; ++c) {
-            d.gstart[c] = g;
-            if (c >= nch) continue;
-            int32_t w = 0;
-            for (int l = 0; l < kChunk; ++l) {
-                const int64_t m = m0 + c * kChunk + l;
-                if (m < m1) w = std::max(w, deg_out[m]);
-            }
-            const int32_t groups = (w + 3) / 4;
-            const size_t base = tb.inc.size();
-            tb.inc.resize(base + (size_t)groups * kChunk, 0);
-            for (int l = 0; l < kChunk; ++l) {
-                const int64_t m = m0 + c * kChunk + l;
-                if (m >= m1) continue;
-                for (int64_t j = row[m]; j < row[m + 1]; ++j) {
-                    const int64_t q = j - row[m];
-                    const uint64_t ent = (uint64_t)slot_of[inc_s[j]] | (inc_left[j] ? kIncLeft : 0);
-                    tb.inc[base + (size_t)(q / 4) * kChunk + l] |= ent << (16 * (q % 4));
-                }
-            }
-            g += groups;
-        }
         tb.tiles.push_back(d);
     }
     return true;
+}
+
+void build_sell(int64_t n, const SpringsIn &sp, bool per_spring_kd, const TileBuild *tb, int64_t g0,
+                SellBuild &out) {
+    const int64_t S = sp.n_springs;
+    const int64_t nch = (n + kChunk - 1) / kChunk;
+    const int G = tb ? kGroupTiled : kGroupGather;
+    out.deg.assign(nch * kChunk, 0);
+    for (int64_t s = 0; s < S; ++s) {
+        if (sp.left[s] < 0 || sp.right[s] < 0) continue;
+        out.deg[sp.left[s]]++;
+        out.deg[sp.right[s]]++;
+    }
+    out.coff.assign(nch + 1, 0);
+    for (int64_t c = 0; c < nch; ++c) {
+        int32_t w = 0;
+        for (int l = 0; l < kChunk; ++l) w = std::max(w, out.deg[c * kChunk + l]);
+        out.coff[c + 1] = out.coff[c] + (int64_t)((w + G - 1) / G) * G * kChunk;
+    }
+    const int64_t ne = out.coff[nch];
+    // the LDS row of mass `o` in the tile of mass `me`
+    auto row_of = [&](int64_t me, int64_t o) -> uint32_t {
+        const int64_t t = me / kTileMasses, m0 = t * kTileMasses;
+        if (o >= m0 && o < m0 + kTileMasses) return (uint32_t)(o - m0);
+        const TileDesc &d = tb->tiles[t];
+        const int32_t *h = tb->halo.data() + d.halo_off;
+        return (uint32_t)(kTileMasses + (std::lower_bound(h, h + d.nhalo, (int32_t)o) - h));
+    };
+    // entry e of a chunk holds incidence j of lane l: e = ((j / G) * 64 + l) * G + j % G
+    auto entry = [&](int64_t me, int64_t j) -> int64_t {
+        return out.coff[me / kChunk] + ((j / G) * kChunk + me % kChunk) * G + j % G;
+    };
+    // gather: one uint32 arena slot per entry; tiled: the LDS rows of a
+    // group's two entries packed in the uint32 at (entry - coff) / 2
+    auto put_nbr = [&](int64_t me, int64_t e, int64_t o) {
+        if (!tb) {
+            out.nbr[e] = (uint32_t)(g0 + o);
+            return;
+        }
+        const int64_t c0 = out.coff[me / kChunk], rel = e - c0;
+        uint32_t &w = out.nbr[c0 + rel / 2];
+        const int sh = 16 * (int)(rel % 2);
+        w = (w & ~(0xffffu << sh)) | (row_of(me, o) << sh);
+    };
+    out.nbr.assign(ne, 0);
+    out.rest.assign(ne, 1.0);
+    out.k.clear();
+    out.damp.clear();
+    out.act.clear();
+    if (per_spring_kd) {
+        out.k.assign(ne, 0.0);
+        out.damp.assign(ne, 0.0);
+    }
+    if (sp.act) out.act.assign(ne, -1);
+    // padding entries: the mass itself
+    for (int64_t c = 0; c < nch; ++c)
+        for (int64_t e = out.coff[c]; e < out.coff[c + 1]; ++e) {
+            const int64_t me = c * kChunk + ((e - out.coff[c]) / G) % kChunk;
+            put_nbr(me, e, me);
+        }
+    std::vector<int32_t> fill(nch * kChunk, 0);
+    auto place = [&](int64_t me, int64_t o, int64_t s) {
+        const int64_t e = entry(me, fill[me]++);
+        put_nbr(me, e, o);
+        out.rest[e] = sp.rest[s];
+        if (per_spring_kd) {
+            out.k[e] = sp.k[s];
+            out.damp[e] = sp.damping ? sp.damping[s] : 0.0;
+        }
+        if (sp.act) out.act[e] = sp.act[s];
+    };
+    for (int64_t s = 0; s < S; ++s) {
+        const int32_t L = sp.left[s], R = sp.right[s];
+        if (L < 0 || R < 0) continue;
+        place(R, L, s);  // a self-spring is listed for its right end first
+        place(L, R, s);
+    }
 }
 
 }  // namespace titan_mi355x

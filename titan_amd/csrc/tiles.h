@@ -1,4 +1,5 @@
-// Host-side builder of the tiled layout used by k_step_tiles.
+// Host-side builders of the device layouts: the Morton order and tiles of
+// the tiled layout, and the SELL-64 incidence lists shared by both layouts.
 #pragma once
 
 #include <stdint.h>
@@ -20,25 +21,41 @@ struct SpringsIn {
 };
 
 struct TileBuild {
-    std::vector<TileDesc> tiles;   // offsets relative to this body's ranges
-    std::vector<int32_t> halo;     // body-local mass indices (device order)
-    std::vector<uint32_t> ends;    // (left << 16) | right, tile-local
-    std::vector<double> rest, k, damp;
-    std::vector<int32_t> act;
-    std::vector<uint64_t> inc;     // 4 incidences (uint16) per lane per group
-    std::vector<int32_t> sid;      // spring index of each tile-spring slot (host only)
+    std::vector<TileDesc> tiles;   // chunk0 and halo_off relative to this body
+    std::vector<int32_t> halo;     // body-local mass indices (device order), ascending per tile
     int32_t hmax = 0;
     std::string why;               // reason when the layout does not apply
+};
+
+// Incidences of a body in SELL-64 order (see layout.h): for the masses of
+// chunk c, incidence j of lane l is entry coff[c] + ((j / G) * 64 + l) * G +
+// j % G, with groups of G = 4 (gather) or 2 (tiled).  Every mass lists its
+// springs in ascending spring index (a spring joining a mass to itself is
+// listed twice).  Padding entries name the mass itself with rest 1 and are
+// never evaluated.
+struct SellBuild {
+    std::vector<int64_t> coff;     // nch + 1 entry offsets, body-relative
+    std::vector<int32_t> deg;      // per mass slot, nch * 64
+    // gather: nbr[e] = arena slot g0 + other endpoint.  tiled: the uint32 at
+    // coff[c] + (e - coff[c]) / 2 holds the LDS rows of one group's two
+    // entries (low half: the even entry); the chunk's other half is unused
+    std::vector<uint32_t> nbr;
+    std::vector<double> rest, k, damp;  // k and damp only with per-spring constants
+    std::vector<int32_t> act;      // body-local actuator index or -1, only if sp.act
 };
 
 // Order of the masses along a Morton curve of their positions (3 doubles
 // each).  Returns perm with perm[new] = old; ties keep the original order.
 std::vector<int32_t> morton_order(const double *pos, int64_t n);
 
-// Build the tiles of a body with n masses (device order) and the given
-// springs.  deg_out receives each mass's incident-spring count.  Returns
-// false (with tb.why) if a tile exceeds the kernel's limits.
-bool build_tiles(int64_t n, const SpringsIn &sp, bool per_spring_kd, TileBuild &tb,
-                 std::vector<int32_t> &deg_out);
+// Cut a body of n masses (device order) into tiles of kTileMasses and list
+// each tile's halo.  Returns false (with tb.why) if a halo exceeds kMaxHalo.
+bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb);
+
+// Build the incidence lists.  With tb == nullptr (gather layout) an
+// incidence names the other endpoint's arena slot, g0 + its body index;
+// with a tile build it names the endpoint's LDS row in the mass's tile.
+void build_sell(int64_t n, const SpringsIn &sp, bool per_spring_kd, const TileBuild *tb, int64_t g0,
+                SellBuild &out);
 
 }  // namespace titan_mi355x
