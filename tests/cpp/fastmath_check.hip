@@ -1,5 +1,6 @@
 // Bitwise check of the guarded FP64 division and square root
-// (titan_amd/csrc/fastmath.h) against the IEEE `/` and `sqrt` on gfx950.
+// (titan_amd/csrc/fastmath.h) against the IEEE `/` and `sqrt` on gfx950
+// (quotients: equal bits, or both zero -- a zero numerator gives +0).
 //
 //   fastmath_check [n_per_case]      exit 0 iff every result matches bit for bit
 //
@@ -45,6 +46,9 @@ __device__ __forceinline__ bool same(double a, double b) {
     return __double_as_longlong(a) == __double_as_longlong(b);
 }
 
+// equal bits, or both zero: a zero numerator's quotient is +0 on the fast path
+__device__ __forceinline__ bool same_q(double a, double b) { return same(a, b) || (a == 0.0 && b == 0.0); }
+
 struct Counts {
     unsigned long long sqrt_bad, div_bad, axis_bad, axis_fast;
 };
@@ -81,7 +85,7 @@ __global__ void k_check(uint64_t n, uint64_t seed, Counts *c, double *first_bad)
     }
     if (h2 & 8) a = -a;
     const double y = fast_recip(b);
-    if (!same(fast_div(a, b, y), a / b)) {
+    if (!same_q(fast_div(a, b, y), a / b)) {
         atomicAdd(&c->div_bad, 1ull);
         first_bad[1] = a;
         first_bad[2] = b;
@@ -102,7 +106,7 @@ __global__ void k_check(uint64_t n, uint64_t seed, Counts *c, double *first_bad)
     double nrm, ux, uy, uz;
     unit_axis(t[0], t[1], t[2], nrm, ux, uy, uz);
     const double rn = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
-    if (!same(nrm, rn) || !same(ux, t[0] / rn) || !same(uy, t[1] / rn) || !same(uz, t[2] / rn)) {
+    if (!same(nrm, rn) || !same_q(ux, t[0] / rn) || !same_q(uy, t[1] / rn) || !same_q(uz, t[2] / rn)) {
         atomicAdd(&c->axis_bad, 1ull);
         first_bad[3] = t[0];
         first_bad[4] = t[1];

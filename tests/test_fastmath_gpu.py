@@ -1,7 +1,8 @@
 """The guarded FP64 division and square root used by the step kernels
 (titan_amd/csrc/fastmath.h) against the device's IEEE `/` and `sqrt`, bit for
-bit, on 2^25 random inputs per case: sqrt over [2^-767, 2^1000], quotients
-a / b over the fast range with +-0 numerators, and unit vectors t / |t| whose
+bit (a zero quotient may differ in sign), on 2^25 random inputs per case:
+sqrt over [2^-767, 2^1000], quotients a / b over the fast range with +-0
+numerators, and unit vectors t / |t| whose
 components span tiny, ordinary and huge magnitudes so that both the fast path
 and the fallback run."""
 from __future__ import annotations

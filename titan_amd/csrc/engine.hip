@@ -133,6 +133,7 @@ struct titan_ctx {
     int64_t *mlc = nullptr;
     int32_t *chunk_body = nullptr;
     int64_t *chunk_ptr = nullptr;
+    int32_t *chunk_ng = nullptr;   // incidence groups per lane (uniform over the chunk)
     // incidence arena
     uint32_t *inc_nbr = nullptr;
     double *inc_rest = nullptr, *inc_k = nullptr, *inc_damp = nullptr;
@@ -256,6 +257,7 @@ void ensure_chunks(titan_ctx *c, int64_t need) {
     dfree(c, c->chunk_body);
     c->chunk_body = cb;
     grow_soa(c, c->chunk_ptr, 1, old, nc);
+    grow_soa(c, c->chunk_ng, 1, old, nc);
     c->chunks.grow(nc);
     c->ncap = nn;
     drop_graphs(c);
@@ -403,6 +405,7 @@ StepArgs make_args(titan_ctx *c, int slot, int write_acc) {
     a.acc = c->acc;
     a.chunk_body = c->chunk_body;
     a.chunk_ptr = c->chunk_ptr;
+    a.chunk_ng = c->chunk_ng;
     a.nchunks = c->chunks.high_water();
     a.inc_nbr = c->inc_nbr;
     a.inc_rest = c->inc_rest;
@@ -793,7 +796,12 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     // ---- upload (buffer parity 0: a new body has taken no steps)
     std::vector<int32_t> cbody(nch, (int32_t)bid);
     std::vector<int64_t> cptr(nch);
-    for (int64_t ch = 0; ch < nch; ++ch) cptr[ch] = inc0 + sb.coff[ch];
+    std::vector<int32_t> cng(nch);
+    const int64_t gsz = (int64_t)(tiled ? kGroupTiled : kGroupGather) * kChunk;
+    for (int64_t ch = 0; ch < nch; ++ch) {
+        cptr[ch] = inc0 + sb.coff[ch];
+        cng[ch] = (int32_t)((sb.coff[ch + 1] - sb.coff[ch]) / gsz);
+    }
     h2d(c, c->st + 6 * g0, st.data(), 6 * npad);
     h2d(c, c->m + g0, mm.data(), npad);
     h2d(c, c->Tm + g0, Tm.data(), npad);
@@ -811,6 +819,7 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     if (c->mlc) h2d(c, c->mlc + g0, mlc.empty() ? zl.data() : mlc.data(), npad);
     h2d(c, c->chunk_body + ch0, cbody.data(), nch);
     h2d(c, c->chunk_ptr + ch0, cptr.data(), nch);
+    h2d(c, c->chunk_ng + ch0, cng.data(), nch);
     if (n_inc > 0) {
         h2d(c, c->inc_nbr + inc0, sb.nbr.data(), n_inc);
         h2d(c, c->inc_rest + inc0, sb.rest.data(), n_inc);

@@ -13,8 +13,9 @@
 // operand with the sign of a / b; when x is in [2^-767, 2^1000] the sqrt
 // scaling is by 2^0 and the special-case select keeps s.  In those ranges the
 // operations below are the same operations on the same values, so the results
-// are the correctly rounded IEEE ones, bit for bit (a zero numerator is
-// returned as is, which is v_div_fixup's +-0 for b > 0).  The reciprocal
+// are the correctly rounded IEEE ones, bit for bit, except that a zero
+// numerator gives +0 where v_div_fixup gives the numerator's sign (a zero's
+// sign never changes a nonzero result downstream).  The reciprocal
 // refinement, which depends only on b, is computed once and shared by the
 // three components of a unit vector.
 //
@@ -50,24 +51,28 @@ __device__ __forceinline__ double fast_recip(double b) {
     return y;
 }
 
-// a / b given y = fast_recip(b), for b > 0 and a == 0 or |a| in [2^-500, 2 b].
+// a / b given y = fast_recip(b), for b > 0 and a == 0 or |a| in [2^-500, 2 b]
+// (a == -0 gives +0).
 __device__ __forceinline__ double fast_div(double a, double b, double y) {
     const double q = a * y;
     const double r = fma(-b, q, a);
-    return a == 0.0 ? a : fma(r, y, q);
+    return fma(r, y, q);
 }
 
-__device__ __forceinline__ bool num_ok(double a) { return a == 0.0 || fabs(a) >= 0x1p-500; }
+// (bitwise & and | instead of && and ||: every comparison is evaluated, so
+// the guards compile to compares and mask operations without branches)
+__device__ __forceinline__ int num_ok(double a) { return (int)(a == 0.0) | (int)(fabs(a) >= 0x1p-500); }
 
 // The guard for a spring axis t with squared length n2 = |t|^2: sqrt(n2) and
 // t / sqrt(n2) may take the fast path.  (n2 in [2^-767, 2^1000] puts
 // sqrt(n2) in [2^-384, 2^500], and every |t_c| <= sqrt(n2) (1 + 2^-52).)
 __device__ __forceinline__ bool fast_range(double n2, double tx, double ty, double tz) {
-    return n2 >= 0x1p-767 && n2 <= 0x1p1000 && num_ok(tx) && num_ok(ty) && num_ok(tz);
+    return ((int)(n2 >= 0x1p-767) & (int)(n2 <= 0x1p1000) & num_ok(tx) & num_ok(ty) & num_ok(tz)) != 0;
 }
 
 // nrm = sqrt(tx tx + ty ty + tz tz) and u = t / nrm, correctly rounded, as
-// the reference computes a spring's length and direction.
+// the reference computes a spring's length and direction (a zero component
+// of t gives a zero component of u, of either sign).
 __device__ __forceinline__ void unit_axis(double tx, double ty, double tz, double &nrm, double &ux,
                                           double &uy, double &uz) {
     const double n2 = tx * tx + ty * ty + tz * tz;

@@ -220,8 +220,8 @@ __device__ __forceinline__ void mass_pass(const StepArgs &a, MassRegs &r, int64_
 // its F is -G exactly and this mass receives +F = -G.  Either way the
 // contribution is -G, bit for bit (an exact zero may differ in sign).
 //
-// The next group's names are prefetched while the current group's
-// neighbours are fetched.
+// The next group's names, rest lengths and constants are loaded while the
+// current group is evaluated.
 template <int G>
 struct Names;
 template <>
@@ -249,68 +249,122 @@ struct Names<2> {
     }
 };
 
-template <bool UNI, bool ACT, int G, class Fetch>
-__device__ __forceinline__ void spring_forces(const StepArgs &a, MassRegs &r, int d, int64_t ebase,
-                                              int lane, double T, Fetch fetch) {
-    static_assert(G == 2 || G == 4, "groups of 2 or 4");
-    constexpr int P = G / 2;  // 16-byte pairs per group and lane
-    r.fx = 0.0;
-    r.fy = 0.0;
-    r.fz = 0.0;
-    const int ng = (d + G - 1) / G;
-    const double2 *__restrict__ gr = reinterpret_cast<const double2 *>(a.inc_rest + ebase) + P * lane;
-    const double2 *__restrict__ gk = UNI ? nullptr : reinterpret_cast<const double2 *>(a.inc_k + ebase) + P * lane;
-    const double2 *__restrict__ gd = UNI ? nullptr : reinterpret_cast<const double2 *>(a.inc_damp + ebase) + P * lane;
-    const int32_t *__restrict__ ga = ACT ? a.inc_act + ebase + G * lane : nullptr;
-    Names<G> ni{};
-    if (ng > 0) ni = Names<G>::load(a.inc_nbr, ebase, lane, 0);
-    for (int g = 0; g < ng; ++g) {
-        const Names<G> ci = ni;
-        if (g + 1 < ng) ni = Names<G>::load(a.inc_nbr, ebase, lane, g + 1);
-        uint32_t oq[G];
-        ci.get(oq);
-        double2 nb[G][3];
-#pragma unroll
-        for (int q = 0; q < G; ++q) fetch(oq[q], nb[q][0], nb[q][1], nb[q][2]);
-        double rq[G], kq[G], dq[G];
-        int32_t aq[G];
+// One group's incidence data in registers: the names, rest lengths,
+// constants (per-spring constants only) and actuator ids (actuated only).
+template <bool UNI, bool ACT, int G>
+struct Group {
+    Names<G> n;
+    double rest[G], k[G], damp[G];
+    int32_t act[G];
+    __device__ __forceinline__ void load(const StepArgs &a, int64_t ebase, int lane, int g) {
+        constexpr int P = G / 2;  // 16-byte pairs per group and lane
+        n = Names<G>::load(a.inc_nbr, ebase, lane, g);
+        const int64_t e2 = ebase / 2 + ((int64_t)g * kChunk + lane) * P;  // double2 index
 #pragma unroll
         for (int h = 0; h < P; ++h) {
-            const double2 rr = gr[(int64_t)g * P * kChunk + h];
-            rq[2 * h] = rr.x;
-            rq[2 * h + 1] = rr.y;
+            const double2 rr = reinterpret_cast<const double2 *>(a.inc_rest)[e2 + h];
+            rest[2 * h] = rr.x;
+            rest[2 * h + 1] = rr.y;
             if (!UNI) {
-                const double2 kk = gk[(int64_t)g * P * kChunk + h], dd = gd[(int64_t)g * P * kChunk + h];
-                kq[2 * h] = kk.x;
-                kq[2 * h + 1] = kk.y;
-                dq[2 * h] = dd.x;
-                dq[2 * h + 1] = dd.y;
+                const double2 kk = reinterpret_cast<const double2 *>(a.inc_k)[e2 + h];
+                const double2 dd = reinterpret_cast<const double2 *>(a.inc_damp)[e2 + h];
+                k[2 * h] = kk.x;
+                k[2 * h + 1] = kk.y;
+                damp[2 * h] = dd.x;
+                damp[2 * h + 1] = dd.y;
             }
         }
         if (ACT) {
+            const int64_t e = ebase + ((int64_t)g * kChunk + lane) * G;
 #pragma unroll
-            for (int q = 0; q < G; ++q) aq[q] = ga[(int64_t)g * G * kChunk + q];
+            for (int q = 0; q < G; ++q) act[q] = a.inc_act[e + q];
         }
+    }
+};
+
+// Add -G of one incident spring (axis nrm, u) to the mass's force.
+template <bool UNI, bool ACT, class Grp>
+__device__ __forceinline__ void add_spring(const StepArgs &a, MassRegs &r, const Grp &cur, int q, double T,
+                                           const double2 (&nb)[3], double nrm, double ux, double uy,
+                                           double uz) {
+    const double dvx = r.vx - nb[1].y, dvy = r.vy - nb[2].x, dvz = r.vz - nb[2].y;
+    double rl = cur.rest[q];
+    if (ACT) {
+        // evaluated for every lane (a lane without an actuator reads
+        // actuator 0 and keeps rl), so the group stays free of branches
+        const int32_t ai = cur.act[q];
+        const double *A = a.act + 4 * (int64_t)(ai >= 0 ? ai : 0);
+        const double ra = rl * (1 + A[0] * sin(A[1] * T + A[2]));
+        rl = ai >= 0 ? ra : rl;
+    }
+    const double k = UNI ? a.k_uni : cur.k[q];
+    const double dmp = UNI ? a.damp_uni : cur.damp[q];
+    const double mag = k * (rl - nrm);
+    const double dd = (dvx * ux + dvy * uy + dvz * uz) * dmp;
+    r.fx = r.fx - (ux * mag + ux * dd);
+    r.fy = r.fy - (uy * mag + uy * dd);
+    r.fz = r.fz - (uz * mag + uz * dd);
+}
+
+// Evaluate group g (data in `cur`) of a lane with d incidences.
+template <bool UNI, bool ACT, int G, class Fetch>
+__device__ __forceinline__ void eval_group(const StepArgs &a, MassRegs &r, const Group<UNI, ACT, G> &cur,
+                                           int g, int d, double T, Fetch &fetch) {
+    uint32_t oq[G];
+    cur.n.get(oq);
+    double2 nb[G][3];
+#pragma unroll
+    for (int q = 0; q < G; ++q) fetch(oq[q], nb[q][0], nb[q][1], nb[q][2]);
+    double tx[G], ty[G], tz[G], n2[G];
+    bool fast = G * g + G <= d;
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+        tx[q] = nb[q][0].x - r.px;
+        ty[q] = nb[q][0].y - r.py;
+        tz[q] = nb[q][1].x - r.pz;
+        n2[q] = tx[q] * tx[q] + ty[q] * ty[q] + tz[q] * tz[q];
+        fast = fast & fast_range(n2[q], tx[q], ty[q], tz[q]);
+    }
+    if (__all(fast)) {
+        // every lane has G springs in this group, all in the fast range:
+        // straight-line code, so the G evaluations interleave
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            const double nrm = fast_sqrt(n2[q]);
+            const double y = fast_recip(nrm);
+            const double ux = fast_div(tx[q], nrm, y), uy = fast_div(ty[q], nrm, y), uz = fast_div(tz[q], nrm, y);
+            add_spring<UNI, ACT>(a, r, cur, q, T, nb[q], nrm, ux, uy, uz);
+        }
+    } else {
 #pragma unroll
         for (int q = 0; q < G; ++q) {
             if (G * g + q >= d) break;
-            const double tx = nb[q][0].x - r.px, ty = nb[q][0].y - r.py, tz = nb[q][1].x - r.pz;
-            const double dvx = r.vx - nb[q][1].y, dvy = r.vy - nb[q][2].x, dvz = r.vz - nb[q][2].y;
             double nrm, ux, uy, uz;
-            unit_axis(tx, ty, tz, nrm, ux, uy, uz);
-            double rl = rq[q];
-            if (ACT && aq[q] >= 0) {
-                const double *A = a.act + 4 * (int64_t)aq[q];
-                rl = rl * (1 + A[0] * sin(A[1] * T + A[2]));
-            }
-            const double k = UNI ? a.k_uni : kq[q];
-            const double dmp = UNI ? a.damp_uni : dq[q];
-            const double mag = k * (rl - nrm);
-            const double dd = (dvx * ux + dvy * uy + dvz * uz) * dmp;
-            r.fx = r.fx - (ux * mag + ux * dd);
-            r.fy = r.fy - (uy * mag + uy * dd);
-            r.fz = r.fz - (uz * mag + uz * dd);
+            unit_axis(tx[q], ty[q], tz[q], nrm, ux, uy, uz);
+            add_spring<UNI, ACT>(a, r, cur, q, T, nb[q], nrm, ux, uy, uz);
         }
+    }
+}
+
+// Spring forces of one lane: ng groups (uniform over the wave; a lane uses
+// the first ceil(d / G)).  Two register sets alternate, so a group's data
+// is loaded while the group before it is evaluated, without copies.
+template <bool UNI, bool ACT, int G, class Fetch>
+__device__ __forceinline__ void spring_forces(const StepArgs &a, MassRegs &r, int d, int64_t ebase, int ng,
+                                              int lane, double T, Fetch fetch) {
+    static_assert(G == 2 || G == 4, "groups of 2 or 4");
+    r.fx = 0.0;
+    r.fy = 0.0;
+    r.fz = 0.0;
+    if (ng <= 0) return;
+    Group<UNI, ACT, G> A, B;
+    A.load(a, ebase, lane, 0);
+    for (int g = 0; g < ng; g += 2) {
+        B.load(a, ebase, lane, min(g + 1, ng - 1));
+        eval_group<UNI, ACT, G>(a, r, A, g, d, T, fetch);
+        if (g + 1 >= ng) break;
+        A.load(a, ebase, lane, min(g + 2, ng - 1));
+        eval_group<UNI, ACT, G>(a, r, B, g + 1, d, T, fetch);
     }
 }
 
@@ -378,7 +432,7 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     r.vy = o2.x;
     r.vz = o2.y;
     // 48 bytes per mass; the arena caps a buffer below 4 GiB
-    spring_forces<UNI, ACT, kGroupGather>(a, r, a.deg[i], a.chunk_ptr[chunk], lane, T,
+    spring_forces<UNI, ACT, kGroupGather>(a, r, a.deg[i], a.chunk_ptr[chunk], uniform(a.chunk_ng[chunk]), lane, T,
                             [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
                                 const uint32_t ob = o * 48u;
                                 q0 = ldo(cur, ob);
@@ -454,7 +508,7 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     r.vx = o1.y;
     r.vy = o2.x;
     r.vz = o2.y;
-    spring_forces<UNI, ACT, kGroupTiled>(a, r, a.deg[i], a.chunk_ptr[chunk], t & (kChunk - 1), T,
+    spring_forces<UNI, ACT, kGroupTiled>(a, r, a.deg[i], a.chunk_ptr[chunk], uniform(a.chunk_ng[chunk]), t & (kChunk - 1), T,
                             [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
                                 q0 = S[3 * o];
                                 q1 = S[3 * o + 1];

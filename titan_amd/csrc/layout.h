@@ -82,7 +82,8 @@ struct StepArgs {
     double *acc;              // 3 x ncap
     // chunk arrays
     const int32_t *chunk_body;
-    const int64_t *chunk_ptr;
+    const int64_t *chunk_ptr;  // first incidence entry of the chunk
+    const int32_t *chunk_ng;   // incidence groups of every lane of the chunk
     int64_t nchunks;
     // incidences
     const uint32_t *inc_nbr;
