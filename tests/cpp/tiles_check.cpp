@@ -48,11 +48,12 @@ static void check_sell(int64_t n, const std::vector<int32_t> &L, const std::vect
     };
     auto decode = [&](int64_t me, uint32_t v) -> int64_t {
         if (!tb) return (int64_t)v - g0;
-        const int64_t t = me / kTileMasses, m0 = t * kTileMasses;
-        const TileDesc &d = tb->tiles[t];
-        if (v < (uint32_t)kTileMasses) return m0 + v;
-        CHECK(v - kTileMasses < (uint32_t)d.nhalo);
-        return tb->halo[d.halo_off + v - kTileMasses];
+        const TileDesc &d = tb->tiles[tb->tile_of_chunk[me / kChunk]];
+        const int64_t m0 = (int64_t)d.chunk0 * kChunk;
+        CHECK(me >= m0 && me < m0 + d.nmass && me < n);
+        if (v < (uint32_t)d.nmass) return m0 + v;
+        CHECK(v - d.nmass < (uint32_t)d.nhalo);
+        return tb->halo[d.halo_off + v - d.nmass];
     };
     for (int64_t c = 0; c < nch; ++c) {
         int32_t w = 0;
@@ -64,8 +65,10 @@ static void check_sell(int64_t n, const std::vector<int32_t> &L, const std::vect
             CHECK(sb.deg[m] == dm);
             for (int64_t j = 0; j < (sb.coff[c + 1] - sb.coff[c]) / kChunk; ++j) {
                 const int64_t e = sb.coff[c] + ((j / G) * kChunk + l) * G + j % G;
-                if (j >= dm) {  // padding: the mass itself, rest 1
-                    CHECK(decode(m, name(c, e)) == m && sb.rest[e] == 1.0);
+                if (j >= dm) {  // padding: the mass itself (row 0 past the body), rest 1
+                    if (m < n) CHECK(decode(m, name(c, e)) == m);
+                    else if (tb) CHECK(name(c, e) == 0);
+                    CHECK(sb.rest[e] == 1.0);
                     continue;
                 }
                 const int32_t s = want[m][j].second;
@@ -104,13 +107,14 @@ static void check(int64_t n, const std::vector<int32_t> &L, const std::vector<in
     // tiled layout: tiles, halos and LDS rows
     TileBuild tb;
     CHECK(build_tiles(n, sp, tb));
-    const int64_t ntiles = (n + kTileMasses - 1) / kTileMasses;
-    CHECK((int64_t)tb.tiles.size() == ntiles);
-    int32_t hmax = 0;
-    for (int64_t t = 0; t < ntiles; ++t) {
+    int32_t hmax = 0, rmax = 0;
+    int64_t next = 0;  // tiles cover the masses in order, chunk-aligned
+    for (size_t t = 0; t < tb.tiles.size(); ++t) {
         const TileDesc &d = tb.tiles[t];
-        const int64_t m0 = t * kTileMasses, m1 = std::min<int64_t>(n, m0 + kTileMasses);
-        CHECK(d.nmass == m1 - m0 && d.chunk0 == m0 / kChunk && d.first == (t == 0));
+        const int64_t m0 = (int64_t)d.chunk0 * kChunk, m1 = m0 + d.nmass;
+        CHECK(m0 == next && m1 <= n && d.nmass > 0 && d.nmass <= kTileMasses && d.first == (t == 0));
+        CHECK(m1 == n || d.nmass % kChunk == 0);
+        next = m1;
         std::set<int32_t> halo;
         for (int64_t s = 0; s < S; ++s) {
             if (L[s] < 0 || R[s] < 0) continue;
@@ -120,10 +124,17 @@ static void check(int64_t n, const std::vector<int32_t> &L, const std::vector<in
                 if (e < m0 || e >= m1) halo.insert(e);
         }
         CHECK(d.nhalo == (int32_t)halo.size());
+        // within the row budget, unless the tile is a single chunk
+        CHECK(d.nmass + d.nhalo <= kTileRows || d.nmass <= kChunk);
+        CHECK(d.nmass + d.nhalo <= kMaxTileRows);
         hmax = std::max<int32_t>(hmax, d.nhalo);
+        rmax = std::max<int32_t>(rmax, d.nmass + d.nhalo);
         std::vector<int32_t> hv(halo.begin(), halo.end());
         for (int32_t h = 0; h < d.nhalo; ++h) CHECK(tb.halo[d.halo_off + h] == hv[h]);
+        for (int64_t q = m0; q < m1; ++q) CHECK(tb.tile_of_chunk[q / kChunk] == (int32_t)t);
     }
+    CHECK(next == n);
+    CHECK(tb.rmax == rmax);
     CHECK(tb.hmax == hmax);
     SellBuild st;
     build_sell(n, sp, kd, &tb, 0, st);
@@ -184,7 +195,7 @@ int main() {
         sp.rest = ones.data();
         TileBuild tb;
         CHECK(build_tiles(n, sp, tb));
-        CHECK(tb.hmax < 2 * kTileMasses);
+        CHECK(tb.rmax <= kTileRows);
     }
     std::printf(fails ? "FAILED %d\n" : "OK\n", fails);
     return fails ? 1 : 0;

@@ -100,7 +100,7 @@ struct BodyH {
     int64_t n_masses = 0, chunk0 = 0, nchunks = 0;
     int64_t inc0 = 0, inc_len = 0, act0 = 0, act_len = 0, lc0 = 0, lc_len = 0;
     int64_t tile0 = 0, ntiles = 0, halo0 = 0, nhalo = 0;
-    int32_t hmax = 0;
+    int32_t rows = 0;  // largest LDS row count of a tile (tiled layout)
     double T = 0;   // host mirror of the body clock
     int64_t n = 0;  // steps taken (parity selects the state buffer)
     std::vector<uint32_t> flags;   // device order
@@ -367,10 +367,10 @@ void make_act(titan_ctx *c) {
     drop_graphs(c);
 }
 
-int32_t tiled_hmax(const titan_ctx *c) {
+int32_t tiled_rows(const titan_ctx *c) {
     int32_t h = 0;
     for (auto &b : c->bodies)
-        if (b.live && b.tiled) h = std::max(h, b.hmax);
+        if (b.live && b.tiled) h = std::max(h, b.rows);
     return h;
 }
 
@@ -437,7 +437,7 @@ StepArgs make_args(titan_ctx *c, int slot, int write_acc) {
     a.tiles = c->tile;
     a.ntiles = c->tiles.high_water();
     a.halo = c->halo;
-    a.ldim = kTileMasses + tiled_hmax(c);
+    a.ldim = tiled_rows(c);
     a.b_tiled = c->b_tiled;
     return a;
 }
@@ -871,7 +871,7 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     b.ntiles = n_tiles;
     b.halo0 = hl0;
     b.nhalo = n_halo;
-    b.hmax = tb.hmax;
+    b.rows = tb.rmax;
     b.T = t0;
     b.n = 0;
     b.flags.assign(fl.begin(), fl.begin() + N);

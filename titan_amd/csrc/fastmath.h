@@ -70,6 +70,14 @@ __device__ __forceinline__ bool fast_range(double n2, double tx, double ty, doub
     return ((int)(n2 >= 0x1p-767) & (int)(n2 <= 0x1p1000) & num_ok(tx) & num_ok(ty) & num_ok(tz)) != 0;
 }
 
+// A cheaper, stricter form of fast_range for straight-line code: also false
+// when a component is zero (such axes take the general path, which admits
+// zeros).
+__device__ __forceinline__ bool fast_range_nz(double n2, double tx, double ty, double tz) {
+    const double m = fmin(fmin(fabs(tx), fabs(ty)), fabs(tz));
+    return ((int)(n2 >= 0x1p-767) & (int)(n2 <= 0x1p1000) & (int)(m >= 0x1p-500)) != 0;
+}
+
 // nrm = sqrt(tx tx + ty ty + tz tz) and u = t / nrm, correctly rounded, as
 // the reference computes a spring's length and direction (a zero component
 // of t gives a zero component of u, of either sign).

@@ -306,67 +306,94 @@ __device__ __forceinline__ void add_spring(const StepArgs &a, MassRegs &r, const
     r.fz = r.fz - (uz * mag + uz * dd);
 }
 
-// Evaluate group g (data in `cur`) of a lane with d incidences.
-template <bool UNI, bool ACT, int G, class Fetch>
-__device__ __forceinline__ void eval_group(const StepArgs &a, MassRegs &r, const Group<UNI, ACT, G> &cur,
-                                           int g, int d, double T, Fetch &fetch) {
-    uint32_t oq[G];
-    cur.n.get(oq);
-    double2 nb[G][3];
+// Evaluate the K groups of one register set (four incidences, the first
+// being incidence 4 g0 of the lane; the lane has d).  When every lane of the
+// wave has all four and they are in the fast range, the four evaluations
+// are straight-line code that interleaves; otherwise each incidence that
+// exists takes the general path.
+template <bool UNI, bool ACT, int G, int K, class Fetch>
+__device__ __forceinline__ void eval_set(const StepArgs &a, MassRegs &r, const Group<UNI, ACT, G> (&set)[K],
+                                         int g0, int d, double T, Fetch &fetch) {
+    constexpr int N = G * K;
+    double2 nb[N][3];
 #pragma unroll
-    for (int q = 0; q < G; ++q) fetch(oq[q], nb[q][0], nb[q][1], nb[q][2]);
-    double tx[G], ty[G], tz[G], n2[G];
-    bool fast = G * g + G <= d;
+    for (int j = 0; j < K; ++j) {
+        uint32_t oq[G];
+        set[j].n.get(oq);
 #pragma unroll
-    for (int q = 0; q < G; ++q) {
-        tx[q] = nb[q][0].x - r.px;
-        ty[q] = nb[q][0].y - r.py;
-        tz[q] = nb[q][1].x - r.pz;
-        n2[q] = tx[q] * tx[q] + ty[q] * ty[q] + tz[q] * tz[q];
-        fast = fast & fast_range(n2[q], tx[q], ty[q], tz[q]);
+        for (int q = 0; q < G; ++q) fetch(oq[q], nb[j * G + q][0], nb[j * G + q][1], nb[j * G + q][2]);
+    }
+    double tx[N], ty[N], tz[N], n2[N];
+    bool fast = G * g0 + N <= d;
+#pragma unroll
+    for (int e = 0; e < N; ++e) {
+        tx[e] = nb[e][0].x - r.px;
+        ty[e] = nb[e][0].y - r.py;
+        tz[e] = nb[e][1].x - r.pz;
+        n2[e] = tx[e] * tx[e] + ty[e] * ty[e] + tz[e] * tz[e];
+        fast = fast & fast_range_nz(n2[e], tx[e], ty[e], tz[e]);
     }
     if (__all(fast)) {
-        // every lane has G springs in this group, all in the fast range:
-        // straight-line code, so the G evaluations interleave
 #pragma unroll
-        for (int q = 0; q < G; ++q) {
-            const double nrm = fast_sqrt(n2[q]);
+        for (int e = 0; e < N; ++e) {
+            const double nrm = fast_sqrt(n2[e]);
             const double y = fast_recip(nrm);
-            const double ux = fast_div(tx[q], nrm, y), uy = fast_div(ty[q], nrm, y), uz = fast_div(tz[q], nrm, y);
-            add_spring<UNI, ACT>(a, r, cur, q, T, nb[q], nrm, ux, uy, uz);
+            const double ux = fast_div(tx[e], nrm, y), uy = fast_div(ty[e], nrm, y), uz = fast_div(tz[e], nrm, y);
+            add_spring<UNI, ACT>(a, r, set[e / G], e % G, T, nb[e], nrm, ux, uy, uz);
         }
     } else {
 #pragma unroll
-        for (int q = 0; q < G; ++q) {
-            if (G * g + q >= d) break;
+        for (int e = 0; e < N; ++e) {
+            if (G * g0 + e >= d) break;
             double nrm, ux, uy, uz;
-            unit_axis(tx[q], ty[q], tz[q], nrm, ux, uy, uz);
-            add_spring<UNI, ACT>(a, r, cur, q, T, nb[q], nrm, ux, uy, uz);
+            unit_axis(tx[e], ty[e], tz[e], nrm, ux, uy, uz);
+            add_spring<UNI, ACT>(a, r, set[e / G], e % G, T, nb[e], nrm, ux, uy, uz);
         }
     }
 }
 
-// Spring forces of one lane: ng groups (uniform over the wave; a lane uses
-// the first ceil(d / G)).  Two register sets alternate, so a group's data
-// is loaded while the group before it is evaluated, without copies.
-template <bool UNI, bool ACT, int G, class Fetch>
-__device__ __forceinline__ void spring_forces(const StepArgs &a, MassRegs &r, int d, int64_t ebase, int ng,
-                                              int lane, double T, Fetch fetch) {
+// Spring forces of one lane, walking ng groups (uniform over the wave; a
+// lane uses the first ceil(d / G)).  The group data streams from HBM, so two
+// register sets of K consecutive groups alternate: one set's K groups are
+// loaded while the other's are evaluated, keeping K groups of loads in
+// flight per wave without copies (a refill index past the last group is
+// clamped to it and re-reads a group already loaded).  prime() issues the
+// first set's loads, so that a kernel can start them before it waits for
+// anything else.
+template <bool UNI, bool ACT, int G>
+struct SpringWalk {
     static_assert(G == 2 || G == 4, "groups of 2 or 4");
-    r.fx = 0.0;
-    r.fy = 0.0;
-    r.fz = 0.0;
-    if (ng <= 0) return;
-    Group<UNI, ACT, G> A, B;
-    A.load(a, ebase, lane, 0);
-    for (int g = 0; g < ng; g += 2) {
-        B.load(a, ebase, lane, min(g + 1, ng - 1));
-        eval_group<UNI, ACT, G>(a, r, A, g, d, T, fetch);
-        if (g + 1 >= ng) break;
-        A.load(a, ebase, lane, min(g + 2, ng - 1));
-        eval_group<UNI, ACT, G>(a, r, B, g + 1, d, T, fetch);
+    static constexpr int K = 4 / G;  // groups per register set: four incidences
+    Group<UNI, ACT, G> A[K], B[K];
+    int64_t ebase;
+    int ng, lane;
+
+    __device__ __forceinline__ void prime(const StepArgs &a, int64_t eb, int n, int ln) {
+        ebase = eb;
+        ng = n;
+        lane = ln;
+        if (ng <= 0) return;
+#pragma unroll
+        for (int j = 0; j < K; ++j) A[j].load(a, ebase, lane, min(j, ng - 1));
     }
-}
+
+    template <class Fetch>
+    __device__ __forceinline__ void run(const StepArgs &a, MassRegs &r, int d, double T, Fetch fetch) {
+        r.fx = 0.0;
+        r.fy = 0.0;
+        r.fz = 0.0;
+        const int last = ng - 1;
+        for (int g = 0; g < ng; g += 2 * K) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) B[j].load(a, ebase, lane, min(g + K + j, last));
+            eval_set<UNI, ACT, G, K>(a, r, A, g, d, T, fetch);
+            if (g + K >= ng) break;
+#pragma unroll
+            for (int j = 0; j < K; ++j) A[j].load(a, ebase, lane, min(g + 2 * K + j, last));
+            eval_set<UNI, ACT, G, K>(a, r, B, g + K, d, T, fetch);
+        }
+    }
+};
 
 // The body clock of launch slot s is advanced once per step by the body's
 // first lane: T += dt (src/sim.cu:1590) while steps are left.
@@ -378,11 +405,19 @@ __device__ __forceinline__ void advance_body_clock(const StepArgs &a, int b, int
     a.bleft[s ^ 1][b] = go ? left - 1 : 0;
 }
 
+// The new state is stored non-temporally: it is read again only by the next
+// launch, and lines left dirty in the XCDs' L2 caches would have to be
+// written back at the end of this one (measured: about 5% of the step).
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_nt(double2 *p, double x, double y) {
+    __builtin_nontemporal_store((f64x2){x, y}, reinterpret_cast<f64x2 *>(p));
+}
+
 __device__ __forceinline__ void store_mass(const StepArgs &a, double2 *__restrict__ nxt, int64_t i,
                                            const MassRegs &r, double ax, double ay, double az) {
-    nxt[3 * i] = make_double2(r.px, r.py);
-    nxt[3 * i + 1] = make_double2(r.pz, r.vx);
-    nxt[3 * i + 2] = make_double2(r.vy, r.vz);
+    store_nt(&nxt[3 * i], r.px, r.py);
+    store_nt(&nxt[3 * i + 1], r.pz, r.vx);
+    store_nt(&nxt[3 * i + 2], r.vy, r.vz);
     if (a.write_acc) {
         a.acc[i] = ax;
         a.acc[a.ncap + i] = ay;
@@ -403,6 +438,9 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     const int lane = threadIdx.x & (kChunk - 1);
     const int b = uniform(a.chunk_body[chunk]);
     if (b < 0 || a.b_tiled[b]) return;
+    // read before this kernel's first store: scalar loads
+    const int64_t ebase = a.chunk_ptr[chunk];
+    const int ng = a.chunk_ng[chunk];
     const int s = a.slot;
     const int64_t left = a.bleft[s][b];
     const int64_t nst = a.bn[s][b];
@@ -415,13 +453,18 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     const int p = (int)(nst & 1);
     const double2 *__restrict__ cur = reinterpret_cast<const double2 *>(a.st + (int64_t)p * 6 * n6);
     double2 *__restrict__ nxt = reinterpret_cast<double2 *>(a.st + (int64_t)(p ^ 1) * 6 * n6);
+    // every load that does not depend on another is issued here
     const uint32_t fl = a.mflags[i];
-    if (fl & kPad) return;
     const double2 o0 = cur[3 * i], o1 = cur[3 * i + 1], o2 = cur[3 * i + 2];
+    const int d = a.deg[i];
+    const double m = a.m[i];
+    SpringWalk<UNI, ACT, kGroupGather> w;
+    w.prime(a, ebase, ng, lane);
+    if (fl & kPad) return;
     if (fl & kFixed) {  // src/sim.cu:1100-1101: a fixed mass keeps its state
-        nxt[3 * i] = o0;
-        nxt[3 * i + 1] = o1;
-        nxt[3 * i + 2] = o2;
+        store_nt(&nxt[3 * i], o0.x, o0.y);
+        store_nt(&nxt[3 * i + 1], o1.x, o1.y);
+        store_nt(&nxt[3 * i + 2], o2.x, o2.y);
         return;
     }
     MassRegs r;
@@ -432,33 +475,46 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     r.vy = o2.x;
     r.vz = o2.y;
     // 48 bytes per mass; the arena caps a buffer below 4 GiB
-    spring_forces<UNI, ACT, kGroupGather>(a, r, a.deg[i], a.chunk_ptr[chunk], uniform(a.chunk_ng[chunk]), lane, T,
-                            [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
-                                const uint32_t ob = o * 48u;
-                                q0 = ldo(cur, ob);
-                                q1 = ldo(cur, ob + 16u);
-                                q2 = ldo(cur, ob + 32u);
-                            });
-    const double m = a.m[i];
+    w.run(a, r, d, T, [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
+        const uint32_t ob = o * 48u;
+        q0 = ldo(cur, ob);
+        q1 = ldo(cur, ob + 16u);
+        q2 = ldo(cur, ob + 32u);
+    });
     double ax, ay, az;
     mass_pass(a, r, i, fl, b, m, ax, ay, az);
     store_mass(a, nxt, i, r, ax, ay, az);
 }
 
 // Tiled step: one workgroup per tile of up to 256 masses of one body (masses
-// in Morton order).  The tile's masses and its halo (the other endpoints of
-// its springs outside the tile) are staged in LDS, 48 bytes per mass, after
-// which each lane walks its incidences as the gather step does, reading the
-// other endpoints from LDS rows instead of global memory.
+// in Morton order).  The tile's masses (rows 0..nmass-1) and its halo (the
+// other endpoints of its springs outside the tile, rows nmass..) are staged
+// in LDS, 48 bytes per row, after which each lane walks its incidences as
+// the gather step does, reading the other endpoints from LDS rows instead
+// of global memory.
 template <bool UNI, bool ACT>
 __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     extern __shared__ double2 S[];  // 3 per LDS row
     const int64_t tile = xcd_block(blockIdx.x, gridDim.x);
     if (tile >= a.ntiles) return;
     const int t = threadIdx.x;
+    // uniform fields first: read before this kernel's first store, they
+    // are scalar loads
     const TileDesc *__restrict__ td = a.tiles + tile;
     const int b = td->body;
     if (b < 0) return;
+    const int nmass = td->nmass, nhalo = td->nhalo;
+    const int64_t halo_off = td->halo_off;
+    const int64_t chunk = (int64_t)td->chunk0 + uniform(t >> 6);
+    // a wave that holds tile masses loads for all its lanes (the chunk's
+    // padding slots are allocated)
+    const bool live = uniform(t >> 6) * kChunk < nmass;
+    int64_t ebase = 0;
+    int ng = 0;
+    if (live) {
+        ebase = a.chunk_ptr[chunk];
+        ng = a.chunk_ng[chunk];
+    }
     const int s = a.slot;
     const int64_t left = a.bleft[s][b];
     const int64_t nst = a.bn[s][b];
@@ -470,35 +526,55 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     const int p = (int)(nst & 1);
     const double2 *__restrict__ cur = reinterpret_cast<const double2 *>(a.st + (int64_t)p * 6 * n6);
     double2 *__restrict__ nxt = reinterpret_cast<double2 *>(a.st + (int64_t)(p ^ 1) * 6 * n6);
-    const int nmass = td->nmass, nhalo = td->nhalo;
-    const int64_t chunk = (int64_t)td->chunk0 + (t >> 6);
     const int64_t i = chunk * kChunk + (t & (kChunk - 1));
     const bool mine = t < nmass;
+    // every load that does not depend on another is issued before the
+    // staging waits: the mass's state and scalars and its first incidence
+    // groups
     uint32_t fl = kPad;
     double2 o0 = make_double2(0.0, 0.0), o1 = o0, o2 = o0;
-    if (mine) {
+    int d = 0;
+    double m = 1.0;
+    SpringWalk<UNI, ACT, kGroupTiled> w;
+    if (live) {
         fl = a.mflags[i];
         o0 = cur[3 * i];
         o1 = cur[3 * i + 1];
         o2 = cur[3 * i + 2];
+        d = a.deg[i];
+        m = a.m[i];
+        w.prime(a, ebase, ng, t & (kChunk - 1));
     }
-    S[3 * t] = o0;
-    S[3 * t + 1] = o1;
-    S[3 * t + 2] = o2;
-    const int32_t *__restrict__ hl = a.halo + td->halo_off;
-    for (int h = t; h < nhalo; h += kTileMasses) {
-        const uint32_t ob = (uint32_t)hl[h] * 48u;
-        const int q = 3 * (kTileMasses + h);
-        S[q] = ldo(cur, ob);
-        S[q + 1] = ldo(cur, ob + 16u);
-        S[q + 2] = ldo(cur, ob + 32u);
+    // halo rows, two per thread per round: both indices, then both states.
+    // Loads past the halo are clamped to its last entry and stored to the
+    // spare row a.ldim, so that no load sits behind a branch.
+    const int32_t *__restrict__ hl = a.halo + halo_off;
+    for (int h0 = 0; h0 < nhalo; h0 += 2 * kTileMasses) {
+        const int ha = h0 + t, hb = ha + kTileMasses;
+        const uint32_t ia = (uint32_t)hl[min(ha, nhalo - 1)] * 48u;
+        const uint32_t ib = (uint32_t)hl[min(hb, nhalo - 1)] * 48u;
+        const double2 x0 = ldo(cur, ia), x1 = ldo(cur, ia + 16u), x2 = ldo(cur, ia + 32u);
+        const double2 y0 = ldo(cur, ib), y1 = ldo(cur, ib + 16u), y2 = ldo(cur, ib + 32u);
+        const int qa = 3 * (ha < nhalo ? nmass + ha : a.ldim);
+        const int qb = 3 * (hb < nhalo ? nmass + hb : a.ldim);
+        S[qa] = x0;
+        S[qa + 1] = x1;
+        S[qa + 2] = x2;
+        S[qb] = y0;
+        S[qb + 1] = y1;
+        S[qb + 2] = y2;
+    }
+    if (mine) {
+        S[3 * t] = o0;
+        S[3 * t + 1] = o1;
+        S[3 * t + 2] = o2;
     }
     __syncthreads();  // the last barrier: lanes may leave from here on
     if (fl & kPad) return;
     if (fl & kFixed) {  // src/sim.cu:1100-1101
-        nxt[3 * i] = o0;
-        nxt[3 * i + 1] = o1;
-        nxt[3 * i + 2] = o2;
+        store_nt(&nxt[3 * i], o0.x, o0.y);
+        store_nt(&nxt[3 * i + 1], o1.x, o1.y);
+        store_nt(&nxt[3 * i + 2], o2.x, o2.y);
         return;
     }
     MassRegs r;
@@ -508,13 +584,11 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     r.vx = o1.y;
     r.vy = o2.x;
     r.vz = o2.y;
-    spring_forces<UNI, ACT, kGroupTiled>(a, r, a.deg[i], a.chunk_ptr[chunk], uniform(a.chunk_ng[chunk]), t & (kChunk - 1), T,
-                            [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
-                                q0 = S[3 * o];
-                                q1 = S[3 * o + 1];
-                                q2 = S[3 * o + 2];
-                            });
-    const double m = a.m[i];
+    w.run(a, r, d, T, [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
+        q0 = S[3 * o];
+        q1 = S[3 * o + 1];
+        q2 = S[3 * o + 2];
+    });
     double ax, ay, az;
     mass_pass(a, r, i, fl, b, m, ax, ay, az);
     store_mass(a, nxt, i, r, ax, ay, az);
@@ -565,7 +639,8 @@ hipError_t launch_step(const StepArgs &a, bool uni, bool act, hipStream_t s) {
     return hipGetLastError();
 }
 
-size_t tiles_lds_bytes(int ldim) { return (size_t)48 * ldim; }
+// the tile rows plus one spare row (k_step_tiles stores unused halo loads there)
+size_t tiles_lds_bytes(int ldim) { return (size_t)48 * (ldim + 1); }
 
 hipError_t launch_step_tiles(const StepArgs &a, bool uni, bool act, hipStream_t s) {
     if (a.ntiles <= 0) return hipSuccess;
@@ -579,7 +654,7 @@ hipError_t launch_step_tiles(const StepArgs &a, bool uni, bool act, hipStream_t 
 }
 
 hipError_t set_tiles_lds_limit() {
-    const int bytes = (int)tiles_lds_bytes(kTileMasses + kMaxHalo);
+    const int bytes = (int)tiles_lds_bytes(kMaxTileRows);
     hipError_t e = hipSuccess;
     e = hipFuncSetAttribute((const void *)k_step_tiles<true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e != hipSuccess) return e;

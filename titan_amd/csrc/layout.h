@@ -35,18 +35,21 @@ constexpr uint32_t kDrag = 8u;      // non-zero integer drag coefficient
 constexpr uint32_t kPad = 16u;      // padding slot of a chunk (never read)
 
 // Tiled layout (k_step_tiles): a body's masses are ordered along a Morton
-// curve and cut into tiles of up to 4 chunks (256 masses).  A tile's halo is
-// the sorted list of masses outside it that share a spring with one of its
-// masses.  The kernel stages the tile's and the halo's state in LDS, so a
-// tiled body's incidences name the other endpoint by its LDS row: 0..255 for
-// a tile mass, 256 + h for halo mass h.
+// curve and cut into tiles of 1 to 4 whole chunks (up to 256 masses).  A
+// tile's halo is the sorted list of masses outside it that share a spring
+// with one of its masses.  The kernel stages the tile's and the halo's state
+// in LDS, so a tiled body's incidences name the other endpoint by its LDS
+// row: 0..nmass-1 for a tile mass, nmass + h for halo mass h.  A tile takes
+// as many chunks as keep nmass + nhalo within kTileRows (a single chunk may
+// exceed that, up to kMaxTileRows).  Measured on the 1M-mass lattice, full
+// 256-mass tiles (4 workgroups per CU) beat tiles capped at 682 rows (5 per
+// CU), whose extra halo costs more than the occupancy gains.
 constexpr int kTileMasses = 256;
-constexpr int kMaxHalo = 2048;  // LDS budget: (256 + 2048) x 48 bytes = 108 KiB
-
-// incidences per SELL group: gather (four uint32 slots in one 16-byte load)
-// and tiled (two uint16 LDS rows in one 4-byte load)
-constexpr int kGroupGather = 4;
-constexpr int kGroupTiled = 2;
+#ifndef TITAN_TILE_ROWS
+#define TITAN_TILE_ROWS 2304
+#endif
+constexpr int kTileRows = TITAN_TILE_ROWS;
+constexpr int kMaxTileRows = 2304;  // LDS budget: 2304 x 48 bytes = 108 KiB
 
 struct TileDesc {
     int32_t body;
@@ -57,6 +60,11 @@ struct TileDesc {
     int32_t first;      // 1 for the body's first tile (advances the body clock)
     int32_t pad;
 };
+
+// incidences per SELL group: gather (four uint32 slots in one 16-byte load)
+// and tiled (two uint16 LDS rows in one 4-byte load)
+constexpr int kGroupGather = 4;
+constexpr int kGroupTiled = 2;
 
 // local constraint entry (a reference CudaContactPlane / CudaBall /
 // CudaConstraintPlane / CudaDirection, src/object.cu:53-141)
@@ -113,7 +121,7 @@ struct StepArgs {
     const TileDesc *tiles;
     int64_t ntiles;
     const int32_t *halo;      // arena mass slots
-    int32_t ldim;             // LDS rows: kTileMasses + largest halo
+    int32_t ldim;             // LDS rows: largest nmass + nhalo of a tile
     const uint8_t *b_tiled;   // per body: 1 = stepped by k_step_tiles
 };
 

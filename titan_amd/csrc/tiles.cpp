@@ -75,32 +75,59 @@ bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb) {
             other[fill[L]++] = R;
         }
     }
-    const int64_t ntiles = (n + kTileMasses - 1) / kTileMasses;
     tb.tiles.clear();
     tb.halo.clear();
     tb.hmax = 0;
-    std::vector<int32_t> halo;
-    for (int64_t t = 0; t < ntiles; ++t) {
-        const int64_t m0 = t * kTileMasses, m1 = std::min(n, m0 + kTileMasses);
-        halo.clear();
-        for (int64_t j = row[m0]; j < row[m1]; ++j)
-            if (other[j] < m0 || other[j] >= m1) halo.push_back(other[j]);
-        std::sort(halo.begin(), halo.end());
-        halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
-        if ((int64_t)halo.size() > kMaxHalo) {
-            tb.why = "a tile has more than " + std::to_string(kMaxHalo) + " halo masses";
+    tb.rmax = 0;
+    std::vector<int32_t> halo, best;
+    std::vector<int64_t> stamp(n, -1);
+    int64_t mark = 0;
+    auto halo_of = [&](int64_t m0, int64_t m1, std::vector<int32_t> &h) {
+        h.clear();
+        ++mark;
+        for (int64_t j = row[m0]; j < row[m1]; ++j) {
+            const int32_t o = other[j];
+            if ((o < m0 || o >= m1) && stamp[o] != mark) {
+                stamp[o] = mark;
+                h.push_back(o);
+            }
+        }
+        std::sort(h.begin(), h.end());
+    };
+    for (int64_t m0 = 0; m0 < n;) {
+        // the most chunks (up to 4) whose rows fit kTileRows; at least one
+        int64_t m1 = std::min(n, m0 + kChunk);
+        halo_of(m0, m1, best);
+        for (int k = 2; k <= kTileMasses / kChunk && m0 + (int64_t)(k - 1) * kChunk < n; ++k) {
+            const int64_t e = std::min(n, m0 + (int64_t)k * kChunk);
+            halo_of(m0, e, halo);
+            if (e - m0 + (int64_t)halo.size() > kTileRows) break;
+            m1 = e;
+            best.swap(halo);
+        }
+        const int64_t rows = m1 - m0 + (int64_t)best.size();
+        if (rows > kMaxTileRows) {
+            tb.why = "a chunk and its halo need more than " + std::to_string(kMaxTileRows) + " LDS rows";
             return false;
         }
-        tb.hmax = std::max<int32_t>(tb.hmax, (int32_t)halo.size());
+        tb.hmax = std::max<int32_t>(tb.hmax, (int32_t)best.size());
+        tb.rmax = std::max<int32_t>(tb.rmax, (int32_t)rows);
         TileDesc d{};
         d.body = -1;
         d.chunk0 = (int32_t)(m0 / kChunk);
         d.nmass = (int32_t)(m1 - m0);
-        d.nhalo = (int32_t)halo.size();
+        d.nhalo = (int32_t)best.size();
         d.halo_off = (int64_t)tb.halo.size();
-        d.first = t == 0;
-        tb.halo.insert(tb.halo.end(), halo.begin(), halo.end());
+        d.first = m0 == 0;
+        tb.halo.insert(tb.halo.end(), best.begin(), best.end());
         tb.tiles.push_back(d);
+        m0 = m1;
+    }
+    // the tile of every chunk
+    tb.tile_of_chunk.assign((n + kChunk - 1) / kChunk, -1);
+    for (size_t q = 0; q < tb.tiles.size(); ++q) {
+        const TileDesc &d = tb.tiles[q];
+        for (int64_t c = 0; c < (d.nmass + kChunk - 1) / kChunk; ++c) tb.tile_of_chunk[d.chunk0 + c] = (int32_t)q;
     }
     return true;
 }
@@ -125,11 +152,11 @@ void build_sell(int64_t n, const SpringsIn &sp, bool per_spring_kd, const TileBu
     const int64_t ne = out.coff[nch];
     // the LDS row of mass `o` in the tile of mass `me`
     auto row_of = [&](int64_t me, int64_t o) -> uint32_t {
-        const int64_t t = me / kTileMasses, m0 = t * kTileMasses;
-        if (o >= m0 && o < m0 + kTileMasses) return (uint32_t)(o - m0);
-        const TileDesc &d = tb->tiles[t];
+        const TileDesc &d = tb->tiles[tb->tile_of_chunk[me / kChunk]];
+        const int64_t m0 = (int64_t)d.chunk0 * kChunk;
+        if (o >= m0 && o < m0 + d.nmass) return (uint32_t)(o - m0);
         const int32_t *h = tb->halo.data() + d.halo_off;
-        return (uint32_t)(kTileMasses + (std::lower_bound(h, h + d.nhalo, (int32_t)o) - h));
+        return (uint32_t)(d.nmass + (std::lower_bound(h, h + d.nhalo, (int32_t)o) - h));
     };
     // entry e of a chunk holds incidence j of lane l: e = ((j / G) * 64 + l) * G + j % G
     auto entry = [&](int64_t me, int64_t j) -> int64_t {
@@ -157,11 +184,12 @@ void build_sell(int64_t n, const SpringsIn &sp, bool per_spring_kd, const TileBu
         out.damp.assign(ne, 0.0);
     }
     if (sp.act) out.act.assign(ne, -1);
-    // padding entries: the mass itself
+    // padding entries: the mass itself (tiled: row 0 for slots past the body)
     for (int64_t c = 0; c < nch; ++c)
         for (int64_t e = out.coff[c]; e < out.coff[c + 1]; ++e) {
             const int64_t me = c * kChunk + ((e - out.coff[c]) / G) % kChunk;
-            put_nbr(me, e, me);
+            if (tb && me >= n) put_nbr(me, e, (int64_t)tb->tiles[tb->tile_of_chunk[c]].chunk0 * kChunk);
+            else put_nbr(me, e, me);
         }
     std::vector<int32_t> fill(nch * kChunk, 0);
     auto place = [&](int64_t me, int64_t o, int64_t s) {

@@ -23,7 +23,9 @@ struct SpringsIn {
 struct TileBuild {
     std::vector<TileDesc> tiles;   // chunk0 and halo_off relative to this body
     std::vector<int32_t> halo;     // body-local mass indices (device order), ascending per tile
-    int32_t hmax = 0;
+    std::vector<int32_t> tile_of_chunk;
+    int32_t hmax = 0;              // largest halo
+    int32_t rmax = 0;              // largest nmass + nhalo (LDS rows)
     std::string why;               // reason when the layout does not apply
 };
 
@@ -48,8 +50,9 @@ struct SellBuild {
 // each).  Returns perm with perm[new] = old; ties keep the original order.
 std::vector<int32_t> morton_order(const double *pos, int64_t n);
 
-// Cut a body of n masses (device order) into tiles of kTileMasses and list
-// each tile's halo.  Returns false (with tb.why) if a halo exceeds kMaxHalo.
+// Cut a body of n masses (device order) into tiles of whole chunks (see
+// layout.h) and list each tile's halo.  Returns false (with tb.why) if one
+// chunk and its halo exceed kMaxTileRows.
 bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb);
 
 // Build the incidence lists.  With tb == nullptr (gather layout) an
