@@ -128,6 +128,17 @@ int titan_step(titan_ctx *ctx, int64_t n_steps);
 int titan_step_until(titan_ctx *ctx, double t, int64_t *steps_done);
 int titan_time(const titan_ctx *ctx, double *T);   /* Simulation::time, src/sim.cu:1082-1084 */
 int titan_sync(titan_ctx *ctx);
+/* Per-body stepping, for batched rollouts and serving: advance each listed
+ * body by its own number of steps (n_steps[k] for body_ids[k]); the other
+ * bodies do not move and the context clock (titan_time) does not advance.
+ * Generalises the driver loop of Simulation::execute (src/sim.cu:1503-1617),
+ * which has one clock for the whole simulation.  With count == 1 the
+ * launches cover that body's tiles or chunks only (one body per launch);
+ * otherwise one launch per step covers every body.  Returns after the steps
+ * are enqueued (count == 1) or done (count > 1). */
+int titan_step_bodies(titan_ctx *ctx, const int32_t *body_ids, const int64_t *n_steps, int32_t count);
+/* A body's own clock (T += dt per step it took) and step count. */
+int titan_body_time(const titan_ctx *ctx, int32_t body_id, double *T, int64_t *steps);
 
 /* ---- readback / write-back (replace getAll/get/setAll/set,
  *      src/sim.cu:588-764,940-1005) ---- */

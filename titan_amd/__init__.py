@@ -33,7 +33,8 @@ EXPORTS = (
     "titan_body_create", "titan_body_destroy", "titan_body_count", "titan_step", "titan_step_until",
     "titan_time", "titan_sync", "titan_body_download", "titan_body_upload", "titan_timer_start",
     "titan_timer_stop", "titan_mem_used", "titan_kernel_name", "titan_lattice_spring_count",
-    "titan_lattice", "titan_debug_inject_failure", "titan_body_layout",
+    "titan_lattice", "titan_debug_inject_failure", "titan_body_layout", "titan_step_bodies",
+    "titan_body_time",
 )
 
 LAYOUT_AUTO, LAYOUT_GATHER, LAYOUT_TILED = 0, 1, 2
@@ -99,6 +100,8 @@ def lib() -> ctypes.CDLL:
                                                                    POINTER(c_int32), POINTER(c_double)]),
         "titan_debug_inject_failure": (c_int, [vp, c_int32]),
         "titan_body_layout": (c_int, [vp, c_int32, POINTER(c_int32)]),
+        "titan_step_bodies": (c_int, [vp, vp, vp, c_int32]),
+        "titan_body_time": (c_int, [vp, c_int32, POINTER(c_double), POINTER(c_int64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -244,6 +247,19 @@ class Context:
     # -- stepping --
     def step(self, n: int):
         self._chk(self._L.titan_step(self._h, int(n)))
+
+    def step_bodies(self, ids, steps):
+        """Advance each body in `ids` by its own number of steps (the others do not move)."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        steps = np.ascontiguousarray(np.broadcast_to(steps, ids.shape), dtype=np.int64)
+        self._chk(self._L.titan_step_bodies(self._h, ids.ctypes.data_as(ctypes.c_void_p),
+                                             steps.ctypes.data_as(ctypes.c_void_p), int(ids.size)))
+
+    def body_time(self, body: int):
+        """(T, steps) of one body's own clock."""
+        t, n = c_double(), c_int64()
+        self._chk(self._L.titan_body_time(self._h, int(body), ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
 
     def step_until(self, t: float) -> int:
         k = c_int64()

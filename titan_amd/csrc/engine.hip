@@ -124,6 +124,7 @@ struct titan_ctx {
 
     Ranges chunks, incs, acts, lcs, bids, tiles, halos;
     std::vector<BodyH> bodies;
+    std::vector<int64_t> left_h;  // staging for per-body step budgets
 
     // mass arena (ncap = chunks.cap() * 64)
     int64_t ncap = 0;
@@ -392,7 +393,34 @@ int kind_rank(int kind) {
     }
 }
 
-StepArgs make_args(titan_ctx *c, int slot, int write_acc) {
+// The chunks and tiles one launch covers, and which kernels it needs.
+struct Range {
+    int64_t c0 = 0, c1 = 0, t0 = 0, t1 = 0;
+    bool gather = false, tiled = false, all = true;
+};
+
+Range full_range(const titan_ctx *c) {
+    Range r;
+    r.c1 = c->chunks.high_water();
+    r.t1 = c->tiles.high_water();
+    r.gather = any_body(c, false);
+    r.tiled = any_body(c, true);
+    return r;
+}
+
+Range body_range(const BodyH &b) {
+    Range r;
+    r.c0 = b.chunk0;
+    r.c1 = b.chunk0 + b.nchunks;
+    r.t0 = b.tile0;
+    r.t1 = b.tile0 + b.ntiles;
+    r.gather = !b.tiled;
+    r.tiled = b.tiled;
+    r.all = false;
+    return r;
+}
+
+StepArgs make_args(titan_ctx *c, int slot, const Range &rg) {
     StepArgs a{};
     a.st = c->st;
     a.ncap = c->ncap;
@@ -406,7 +434,8 @@ StepArgs make_args(titan_ctx *c, int slot, int write_acc) {
     a.chunk_body = c->chunk_body;
     a.chunk_ptr = c->chunk_ptr;
     a.chunk_ng = c->chunk_ng;
-    a.nchunks = c->chunks.high_water();
+    a.chunk_base = rg.c0;
+    a.nchunks = rg.c1;
     a.inc_nbr = c->inc_nbr;
     a.inc_rest = c->inc_rest;
     a.inc_k = c->inc_k;
@@ -433,39 +462,38 @@ StepArgs make_args(titan_ctx *c, int slot, int write_acc) {
     a.gz = c->g[2];
     a.dt = c->dt;
     a.slot = slot;
-    a.write_acc = write_acc;
     a.tiles = c->tile;
-    a.ntiles = c->tiles.high_water();
+    a.tile_base = rg.t0;
+    a.ntiles = rg.t1;
     a.halo = c->halo;
     a.ldim = tiled_rows(c);
     a.b_tiled = c->b_tiled;
     return a;
 }
 
-// One step of every body: the gather kernel over gather-layout bodies and
-// the tiled kernel over tiled bodies (disjoint, both read launch slot
-// `slot` and write slot ^ 1).
-hipError_t launch_both(titan_ctx *c, int slot, int write_acc) {
-    StepArgs a = make_args(c, slot, write_acc);
+// One step of the bodies in a range: the gather kernel over gather-layout
+// bodies and the tiled kernel over tiled bodies (disjoint; both read launch
+// slot `slot` and write slot ^ 1).
+hipError_t launch_range(titan_ctx *c, int slot, const Range &rg) {
+    StepArgs a = make_args(c, slot, rg);
     const bool act = c->has_act;
-    if (any_body(c, false)) {
+    if (rg.gather) {
         hipError_t e = launch_step(a, c->uniform_kd, act, c->stream);
         if (e != hipSuccess) return e;
     }
-    if (any_body(c, true)) return launch_step_tiles(a, c->uniform_kd, act, c->stream);
+    if (rg.tiled) return launch_step_tiles(a, c->uniform_kd, act, c->stream);
     return hipSuccess;
 }
-
-void launch(titan_ctx *c, int slot, int write_acc) { HIPCHK(launch_both(c, slot, write_acc)); }
 
 hipGraphExec_t graph_for(titan_ctx *c, int slot) {
     for (auto &g : c->graphs)
         if (g.version == c->version && g.slot == slot) return g.exec;
+    const Range rg = full_range(c);
     hipGraph_t graph;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int s = slot;
     for (int i = 0; i < kGraphSteps; ++i) {
-        hipError_t e = launch_both(c, s, 0);
+        hipError_t e = launch_range(c, s, rg);
         if (e != hipSuccess) {
             hipGraph_t junk;
             (void)hipStreamEndCapture(c->stream, &junk);
@@ -483,8 +511,7 @@ hipGraphExec_t graph_for(titan_ctx *c, int slot) {
 
 int64_t live_body_hw(titan_ctx *c) { return c->bids.high_water(); }
 
-void do_step(titan_ctx *c, int64_t n) {
-    if (n < 0) fail(TITAN_E_INVALID, "step: negative step count");
+void maybe_inject(titan_ctx *c) {
     if (c->inject >= 0) {
         if (c->inject == 0) {
             c->inject = -1;
@@ -492,26 +519,44 @@ void do_step(titan_ctx *c, int64_t n) {
         }
         c->inject--;
     }
+}
+
+// Run n launches over range rg; every body's steps-left for this call are
+// already in bleft[c->slot].  Bodies whose budget ends early stop there
+// (and store their acceleration on their last step).  A range of one body
+// leaves the other bodies' launch-slot fields alone, so when n is odd they
+// are first copied to the slot the call ends on.
+void run_launches(titan_ctx *c, int64_t n, const Range &rg) {
+    const int64_t nb = live_body_hw(c);
+    int s = c->slot;
+    if (!rg.all && (n & 1)) {
+        HIPCHK(hipMemcpyAsync(c->bT[s ^ 1], c->bT[s], sizeof(double) * nb, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->bn[s ^ 1], c->bn[s], sizeof(int64_t) * nb, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->bleft[s ^ 1], c->bleft[s], sizeof(int64_t) * nb, hipMemcpyDeviceToDevice, c->stream));
+    }
+    int64_t rest = n;
+    if (rg.all && rest >= kGraphSteps) {
+        hipGraphExec_t g = graph_for(c, s);  // an even number of steps: ends on slot s
+        for (int64_t r = 0; r < rest / kGraphSteps; ++r) HIPCHK(hipGraphLaunch(g, c->stream));
+        rest %= kGraphSteps;
+    }
+    for (int64_t r = 0; r < rest; ++r) {
+        HIPCHK(launch_range(c, s, rg));
+        s ^= 1;
+    }
+    HIPCHK(launch_advance_T(c->Tm, c->mflags, c->chunk_body, c->bn[s], c->bnsync, rg.c0, rg.c1, nb, c->dt,
+                            c->stream));
+    c->slot = s;
+}
+
+void do_step(titan_ctx *c, int64_t n) {
+    if (n < 0) fail(TITAN_E_INVALID, "step: negative step count");
+    maybe_inject(c);
     if (n == 0) return;
     const int64_t nb = live_body_hw(c);
     if (nb > 0 && c->chunks.high_water() > 0) {
         HIPCHK(launch_set_left(c->bleft[c->slot], c->b_chunk0, nb, n, c->stream));
-        int64_t rest = n - 1;
-        if (rest >= kGraphSteps) {
-            hipGraphExec_t g = graph_for(c, c->slot);
-            for (int64_t r = 0; r < rest / kGraphSteps; ++r) HIPCHK(hipGraphLaunch(g, c->stream));
-            rest %= kGraphSteps;
-        }
-        int s = c->slot;
-        for (int64_t r = 0; r < rest; ++r) {
-            launch(c, s, 0);
-            s ^= 1;
-        }
-        launch(c, s, 1);
-        s ^= 1;
-        HIPCHK(launch_advance_T(c->Tm, c->mflags, c->chunk_body, c->bn[s], c->bnsync,
-                                c->chunks.high_water(), nb, c->dt, c->stream));
-        c->slot = s;
+        run_launches(c, n, full_range(c));
     }
     for (auto &b : c->bodies) {
         if (!b.live) continue;
@@ -519,6 +564,40 @@ void do_step(titan_ctx *c, int64_t n) {
         b.n += n;
     }
     for (int64_t r = 0; r < n; ++r) c->T += c->dt;  // src/sim.cu:1590
+}
+
+BodyH &body_of(titan_ctx *c, int32_t id);
+
+// Advance each listed body by its own number of steps; the others do not
+// move.  One body alone is launched over its own tiles or chunks only.
+void do_step_bodies(titan_ctx *c, const int32_t *ids, const int64_t *steps, int32_t count) {
+    if (count < 0 || (count > 0 && (!ids || !steps))) fail(TITAN_E_INVALID, "step_bodies: bad arguments");
+    std::vector<int64_t> left(live_body_hw(c), 0);
+    int64_t n = 0;
+    for (int32_t k = 0; k < count; ++k) {
+        body_of(c, ids[k]);  // validates
+        if (steps[k] < 0) fail(TITAN_E_INVALID, "step_bodies: negative step count");
+        if (left[ids[k]] != 0) fail(TITAN_E_INVALID, "step_bodies: a body is listed twice");
+        left[ids[k]] = steps[k];
+        n = std::max(n, steps[k]);
+    }
+    maybe_inject(c);
+    if (n == 0) return;
+    if (count == 1) {
+        // the budget travels as a kernel argument: nothing to keep alive
+        HIPCHK(launch_set_i64(c->bleft[c->slot] + ids[0], left[ids[0]], c->stream));
+        run_launches(c, n, body_range(c->bodies[ids[0]]));
+    } else {
+        c->left_h = left;
+        h2d(c, c->bleft[c->slot], c->left_h.data(), (int64_t)left.size());
+        run_launches(c, n, full_range(c));
+        HIPCHK(hipStreamSynchronize(c->stream));  // the host copy of the budgets is released
+    }
+    for (int32_t k = 0; k < count; ++k) {
+        BodyH &b = c->bodies[ids[k]];
+        for (int64_t r = 0; r < steps[k]; ++r) b.T += c->dt;
+        b.n += steps[k];
+    }
 }
 
 BodyH &body_of(titan_ctx *c, int32_t id) {
@@ -1217,6 +1296,23 @@ int titan_time(const titan_ctx *c, double *T) {
     if (!T) return TITAN_E_INVALID;
     *T = c->T;
     return TITAN_OK;
+}
+
+int titan_step_bodies(titan_ctx *c, const int32_t *body_ids, const int64_t *n_steps, int32_t count) {
+    NEED_CTX(c);
+    API_BEGIN
+    do_step_bodies(c, body_ids, n_steps, count);
+    API_END(c)
+}
+
+int titan_body_time(const titan_ctx *c, int32_t body_id, double *T, int64_t *steps) {
+    NEED_CTX(c);
+    titan_ctx *m = const_cast<titan_ctx *>(c);
+    API_BEGIN
+    const BodyH &b = body_of(m, body_id);
+    if (T) *T = b.T;
+    if (steps) *steps = b.n;
+    API_END(m)
 }
 
 int titan_sync(titan_ctx *c) {

@@ -14,10 +14,11 @@ const char *step_tiles_kernel_name(bool uniform_kd, bool actuated);
 size_t tiles_lds_bytes(int ldim);
 hipError_t set_tiles_lds_limit();
 hipError_t launch_advance_T(double *Tm, const uint32_t *mflags, const int32_t *chunk_body,
-                            const int64_t *bn, int64_t *bnsync, int64_t nchunks, int64_t nb,
-                            double dt, hipStream_t s);
+                            const int64_t *bn, int64_t *bnsync, int64_t chunk_begin, int64_t chunk_end,
+                            int64_t nb, double dt, hipStream_t s);
 hipError_t launch_set_left(int64_t *bleft, const int32_t *b_chunk0, int64_t nb, int64_t n,
                            hipStream_t s);
 hipError_t launch_fill_f64(double *p, int64_t n, double v, hipStream_t s);
+hipError_t launch_set_i64(int64_t *p, int64_t v, hipStream_t s);
 
 }  // namespace titan_mi355x

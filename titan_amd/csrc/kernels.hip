@@ -413,12 +413,14 @@ __device__ __forceinline__ void store_nt(double2 *p, double x, double y) {
     __builtin_nontemporal_store((f64x2){x, y}, reinterpret_cast<f64x2 *>(p));
 }
 
+// The acceleration is stored on the body's last step of a stepping call
+// (left == 1), where the caller can read it.
 __device__ __forceinline__ void store_mass(const StepArgs &a, double2 *__restrict__ nxt, int64_t i,
-                                           const MassRegs &r, double ax, double ay, double az) {
+                                           const MassRegs &r, double ax, double ay, double az, bool last) {
     store_nt(&nxt[3 * i], r.px, r.py);
     store_nt(&nxt[3 * i + 1], r.pz, r.vx);
     store_nt(&nxt[3 * i + 2], r.vy, r.vz);
-    if (a.write_acc) {
+    if (last) {
         a.acc[i] = ax;
         a.acc[a.ncap + i] = ay;
         a.acc[2 * a.ncap + i] = az;
@@ -433,7 +435,7 @@ __device__ __forceinline__ void store_mass(const StepArgs &a, double2 *__restric
 template <bool UNI, bool ACT>
 __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const int64_t chunk = blk * kWavesPerBlock + uniform(threadIdx.x >> 6);
+    const int64_t chunk = a.chunk_base + blk * kWavesPerBlock + uniform(threadIdx.x >> 6);
     if (chunk >= a.nchunks) return;
     const int lane = threadIdx.x & (kChunk - 1);
     const int b = uniform(a.chunk_body[chunk]);
@@ -483,7 +485,7 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     });
     double ax, ay, az;
     mass_pass(a, r, i, fl, b, m, ax, ay, az);
-    store_mass(a, nxt, i, r, ax, ay, az);
+    store_mass(a, nxt, i, r, ax, ay, az, left == 1);
 }
 
 // Tiled step: one workgroup per tile of up to 256 masses of one body (masses
@@ -495,7 +497,7 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
 template <bool UNI, bool ACT>
 __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     extern __shared__ double2 S[];  // 3 per LDS row
-    const int64_t tile = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t tile = a.tile_base + xcd_block(blockIdx.x, gridDim.x);
     if (tile >= a.ntiles) return;
     const int t = threadIdx.x;
     // uniform fields first: read before this kernel's first store, they
@@ -591,17 +593,17 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     });
     double ax, ay, az;
     mass_pass(a, r, i, fl, b, m, ax, ay, az);
-    store_mass(a, nxt, i, r, ax, ay, az);
+    store_mass(a, nxt, i, r, ax, ay, az, left == 1);
 }
 
 // Per-mass local time: T += dt once per step the body took since the last
 // sync (src/sim.cu:1165), for every non-fixed mass.
 __global__ void k_advance_T(double *Tm, const uint32_t *mflags, const int32_t *chunk_body,
-                            const int64_t *bn, const int64_t *bnsync, int64_t nchunks,
-                            double dt) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                            const int64_t *bn, const int64_t *bnsync, int64_t chunk_begin,
+                            int64_t chunk_end, double dt) {
+    const int64_t i = chunk_begin * kChunk + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t chunk = i / kChunk;
-    if (chunk >= nchunks) return;
+    if (chunk >= chunk_end) return;
     const int b = chunk_body[chunk];
     if (b < 0) return;
     if (mflags[i] & (kFixed | kPad)) return;
@@ -621,6 +623,8 @@ __global__ void k_set_left(int64_t *bleft, const int32_t *b_chunk0, int64_t nb, 
     if (b < nb) bleft[b] = b_chunk0[b] >= 0 ? n : 0;
 }
 
+__global__ void k_set_i64(int64_t *p, int64_t v) { *p = v; }
+
 __global__ void k_fill_f64(double *p, int64_t n, double v) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -629,7 +633,7 @@ __global__ void k_fill_f64(double *p, int64_t n, double v) {
 // ---- launchers ----
 
 hipError_t launch_step(const StepArgs &a, bool uni, bool act, hipStream_t s) {
-    const int64_t blocks = (a.nchunks + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int64_t blocks = (a.nchunks - a.chunk_base + kWavesPerBlock - 1) / kWavesPerBlock;
     if (blocks <= 0) return hipSuccess;
     const dim3 g((unsigned)blocks), blk(kBlock);
     if (uni && !act) hipLaunchKernelGGL((k_step_gather<true, false>), g, blk, 0, s, a);
@@ -643,9 +647,9 @@ hipError_t launch_step(const StepArgs &a, bool uni, bool act, hipStream_t s) {
 size_t tiles_lds_bytes(int ldim) { return (size_t)48 * (ldim + 1); }
 
 hipError_t launch_step_tiles(const StepArgs &a, bool uni, bool act, hipStream_t s) {
-    if (a.ntiles <= 0) return hipSuccess;
+    if (a.ntiles <= a.tile_base) return hipSuccess;
     const size_t lds = tiles_lds_bytes(a.ldim);
-    const dim3 g((unsigned)a.ntiles), blk(kTileMasses);
+    const dim3 g((unsigned)(a.ntiles - a.tile_base)), blk(kTileMasses);
     if (uni && !act) hipLaunchKernelGGL((k_step_tiles<true, false>), g, blk, lds, s, a);
     else if (uni && act) hipLaunchKernelGGL((k_step_tiles<true, true>), g, blk, lds, s, a);
     else if (!uni && !act) hipLaunchKernelGGL((k_step_tiles<false, false>), g, blk, lds, s, a);
@@ -680,14 +684,19 @@ const char *step_kernel_name(bool uni, bool act) {
 }
 
 hipError_t launch_advance_T(double *Tm, const uint32_t *mflags, const int32_t *chunk_body,
-                            const int64_t *bn, int64_t *bnsync, int64_t nchunks, int64_t nb,
-                            double dt, hipStream_t s) {
-    const int64_t n = nchunks * kChunk;
+                            const int64_t *bn, int64_t *bnsync, int64_t chunk_begin, int64_t chunk_end,
+                            int64_t nb, double dt, hipStream_t s) {
+    const int64_t n = (chunk_end - chunk_begin) * kChunk;
     if (n > 0) {
         hipLaunchKernelGGL(k_advance_T, dim3((n + 255) / 256), dim3(256), 0, s, Tm, mflags,
-                           chunk_body, bn, (const int64_t *)bnsync, nchunks, dt);
+                           chunk_body, bn, (const int64_t *)bnsync, chunk_begin, chunk_end, dt);
     }
     if (nb > 0) hipLaunchKernelGGL(k_sync_bodies, dim3((nb + 255) / 256), dim3(256), 0, s, bn, bnsync, nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_i64(int64_t *p, int64_t v, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_i64, dim3(1), dim3(1), 0, s, p, v);
     return hipGetLastError();
 }
 

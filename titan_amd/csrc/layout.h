@@ -92,7 +92,7 @@ struct StepArgs {
     const int32_t *chunk_body;
     const int64_t *chunk_ptr;  // first incidence entry of the chunk
     const int32_t *chunk_ng;   // incidence groups of every lane of the chunk
-    int64_t nchunks;
+    int64_t chunk_base, nchunks;  // launch range: chunks [chunk_base, nchunks)
     // incidences
     const uint32_t *inc_nbr;
     const double *inc_rest;
@@ -116,10 +116,9 @@ struct StepArgs {
     int32_t nballs;
     double gx, gy, gz, dt;
     int32_t slot;
-    int32_t write_acc;
     // tiled path
     const TileDesc *tiles;
-    int64_t ntiles;
+    int64_t tile_base, ntiles;    // launch range: tiles [tile_base, ntiles)
     const int32_t *halo;      // arena mass slots
     int32_t ldim;             // LDS rows: largest nmass + nhalo of a tile
     const uint8_t *b_tiled;   // per body: 1 = stepped by k_step_tiles
