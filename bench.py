@@ -13,7 +13,9 @@ region; a step is one launch of the fused step kernel over the whole lattice.
 Prints one JSON line: mass-steps/s, the roofline fraction of the step kernel
 (algorithmic bytes per step / average step time from HIP events on the
 engine's stream, against 8 TB/s), and the CPU oracle (OpenMP) timed on a
-bounded sample of the same workload.
+bounded sample of the same workload.  Unless --no-extra, the same line also
+carries "batched": a 4096-body robot population stepped together against the
+same GPU stepping one body per launch.
 """
 from __future__ import annotations
 
@@ -83,6 +85,86 @@ def cpu_baseline(scene, budget_s: float = 12.0) -> dict:
             "ms_per_step": 1e3 * dt / n}
 
 
+def bench_batched(n_bodies: int, seed: int, rollout: int, sample: int) -> dict:
+    """Batched uneven bodies: a robot population (lattices of side U{3..12},
+    30% of springs actuated, each body with its own contact plane, dt 1e-4)
+    stepped in one launch per step with graph-captured rollouts, against the
+    same GPU stepping one body per launch.  The schedule (which bodies are
+    tiled, the tile row budget) is picked by the library's autotuner on a
+    sample of the population.  The one-body-per-launch figure is measured on
+    the first `sample` bodies; both runs are checked to give identical states."""
+    import numpy as np
+    from titan_amd import Context, scenes
+
+    t0 = time.perf_counter()
+    sc = scenes.robot_population(n_bodies, seed=seed)
+    n_m, n_s = sc.n_masses, sc.n_springs
+    log(f"batched: {n_bodies} bodies, {n_m} masses, {n_s} springs, {sc.n_actuated} actuated "
+        f"({time.perf_counter() - t0:.1f} s)")
+    sub = scenes.Scene(bodies=sc.bodies[:sample], planes=sc.planes, dt=sc.dt)
+    tune_set = sc.bodies[:64]
+    with Context() as ctx:
+        ctx.set_dt(sc.dt)
+        ctx.set_global_acc(sc.g)
+        t0 = time.perf_counter()
+        best, rates = ctx.autotune(tune_set, replicas=8, steps=20)
+        log(f"batched: autotune {time.perf_counter() - t0:.1f} s -> tiled_min_masses {best[0]}, "
+            f"tile_rows {best[1]}")
+        t0 = time.perf_counter()
+        ids = [ctx.add_body(**b) for b in sc.bodies]
+        ctx.sync()
+        log(f"batched: uploaded in {time.perf_counter() - t0:.1f} s, device bytes {ctx.mem_used()}")
+        ctx.step(rollout)  # warm-up rollout (captures the graph)
+        ctx.sync()
+        ctx.timer_start()
+        ctx.step(rollout)
+        ms_batched = ctx.timer_stop()
+        del ids
+    rate_batched = n_m * rollout / (ms_batched * 1e-3)
+    log(f"batched: {rollout}-step rollout {ms_batched:.2f} ms, {rate_batched:.3e} mass-steps/s")
+
+    # the same schedule, the sample bodies: batched vs one body per launch, same states
+    def run(one_per_launch: bool):
+        with Context() as c:
+            c.set_dt(sc.dt)
+            c.set_global_acc(sc.g)
+            c.set_schedule(*best)
+            bid = [c.add_body(**b) for b in sub.bodies]
+            c.sync()
+            c.timer_start()
+            if one_per_launch:
+                for b in bid:
+                    c.step_bodies([b], [rollout])
+            else:
+                c.step(rollout)
+            ms = c.timer_stop()
+            st = [c.download(b) for b in bid]
+        return ms, st
+    run(True)  # warm-up
+    ms_one, st_one = run(True)
+    _, st_bat = run(False)
+    same = all(np.array_equal(a["pos"], b["pos"]) and np.array_equal(a["vel"], b["vel"])
+               for a, b in zip(st_one, st_bat))
+    sub_m = sub.n_masses
+    rate_one = sub_m * rollout / (ms_one * 1e-3)
+    log(f"batched: one body per launch on {sample} bodies ({sub_m} masses): {ms_one:.1f} ms, "
+        f"{rate_one:.3e} mass-steps/s; identical states: {same}")
+    return {
+        "workload": f"{n_bodies} lattice robots, side U{{3..12}} (seed {seed}), {n_m} masses, {n_s} springs, "
+                    f"{sc.n_actuated} actuated (a~U(0,0.2), omega~U(1,20), phi~U(0,2pi)), own contact plane "
+                    f"each, dt=1e-4, FP64, {rollout}-step graph-captured rollouts, one launch per step",
+        "value": rate_batched, "unit": "mass-steps/s", "ms_per_rollout": ms_batched,
+        "schedule": {"tiled_min_masses": best[0], "tile_rows": best[1],
+                     "autotune": [{"tiled_min_masses": c[0], "tile_rows": c[1], "mass_steps_per_s": r}
+                                  for c, r in rates]},
+        "one_body_per_launch": {"value": rate_one, "unit": "mass-steps/s",
+                                "sample": f"first {sample} bodies ({sub_m} masses), {rollout} steps each, "
+                                          f"one launch per body per step over its own tiles or chunks"},
+        "speedup_vs_one_body_per_launch": rate_batched / rate_one,
+        "identical_to_one_body_per_launch": same,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +175,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--layout", choices=["auto", "gather", "tiled"], default="auto",
                     help="device layout of the body (results are identical; speed differs)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the batched, serving and irregular-graph workloads")
+    ap.add_argument("--only", choices=["lattice", "batched"], default=None,
+                    help="run one workload (its figures go in the same JSON line)")
+    ap.add_argument("--bodies", type=int, default=4096, help="batched workload: robot count")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times (the "
                          "reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -124,6 +211,20 @@ def main() -> int:
                                    "large body would need a halo exchange that is not built)"}))
         return 2
 
+    extra = {}
+    if args.only == "batched" or (args.only is None and not args.no_extra):
+        extra["batched"] = bench_batched(args.bodies, args.seed, rollout=100, sample=min(512, args.bodies))
+    if args.only == "batched":
+        print(json.dumps(extra), flush=True)
+        return 0
+    out = bench_lattice(args)
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_lattice(args) -> dict:
+    """The 1M-mass lattice: the headline metric, its roofline and the CPU baseline."""
     from titan_amd import Context, scenes
 
     log(f"building scene lattice {args.side}^3 seed {args.seed}")
@@ -190,8 +291,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
-    return 0
+    return out
 
 
 if __name__ == "__main__":

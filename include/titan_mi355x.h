@@ -86,9 +86,10 @@ typedef struct {
     const double *lc_num;
     /* body time at creation: the first step's actuation time */
     double t0;
-    /* device layout: TITAN_LAYOUT_AUTO (tiled when the body's tiles fit the
-     * kernel's limits, else gather), TITAN_LAYOUT_GATHER or TITAN_LAYOUT_TILED
-     * (fails if a tile does not fit).  Results are identical either way. */
+    /* device layout: TITAN_LAYOUT_AUTO (the context's schedule decides, see
+     * titan_schedule; tiled only when the body's tiles fit the kernel's
+     * limits), TITAN_LAYOUT_GATHER or TITAN_LAYOUT_TILED (fails if a tile does
+     * not fit).  Results are identical either way. */
     int32_t layout;
 } titan_body_desc;
 
@@ -163,6 +164,32 @@ int titan_mem_used(const titan_ctx *ctx, uint64_t *bytes);
 int titan_kernel_name(const titan_ctx *ctx, char *buf, int64_t len);
 /* Layout a body was given: TITAN_LAYOUT_GATHER or TITAN_LAYOUT_TILED. */
 int titan_body_layout(const titan_ctx *ctx, int32_t body_id, int32_t *layout);
+
+/* ---- schedule and autotuning (no counterpart in the reference, which has
+ *      one fixed launch shape per kernel, src/sim.cu:1569-1590) ---- */
+/* How bodies created with TITAN_LAYOUT_AUTO are laid out: bodies with at
+ * least tiled_min_masses masses get the tiled layout, smaller ones the
+ * gather layout; tile_rows bounds a tile's LDS rows (masses + halo, 64 ..
+ * 2304; fewer rows allow more workgroups per CU, more rows bigger tiles).
+ * Default: tiled_min_masses 0, tile_rows 2304.  Applies to bodies created
+ * afterwards. */
+typedef struct titan_schedule {
+    int64_t tiled_min_masses;
+    int32_t tile_rows;
+    int32_t pad;
+} titan_schedule;
+int titan_set_schedule(titan_ctx *ctx, const titan_schedule *sched);
+int titan_get_schedule(const titan_ctx *ctx, titan_schedule *sched);
+/* Pick the schedule by measurement: for each candidate (tiled_min_masses in
+ * {0, 128, 256, 512, 1024, never}, tile_rows in {682, 2304}) a scratch
+ * context on the same device with this context's dt, gravity, planes and
+ * balls holds `replicas` copies of the n_samples sample bodies and is timed
+ * over `steps` steps (after a warm-up of the same length).  The fastest in
+ * mass-steps/s becomes this context's schedule and is returned in *best
+ * (may be NULL); *rate_out (may be NULL) receives each candidate's rate, in
+ * candidate order (12 entries). */
+int titan_autotune(titan_ctx *ctx, const titan_body_desc *samples, int32_t n_samples, int32_t replicas,
+                   int32_t steps, titan_schedule *best, double *rate_out);
 
 /* Host-side lattice generator with the reference's mass and spring order
  * (Lattice::Lattice, src/object.cu:232-293).  pos: 3*nx*ny*nz; left/right/

@@ -104,9 +104,10 @@ static void check(int64_t n, const std::vector<int32_t> &L, const std::vector<in
     SellBuild sg;
     build_sell(n, sp, kd, nullptr, 4096, sg);
     check_sell(n, L, R, k, rest, damp, kd ? act : noact, kd, nullptr, 4096, sg);
-    // tiled layout: tiles, halos and LDS rows
+    // tiled layout: tiles, halos and LDS rows, under two row budgets
+    for (int rows : {kTileRows, 682}) {
     TileBuild tb;
-    CHECK(build_tiles(n, sp, tb));
+    CHECK(build_tiles(n, sp, tb, rows));
     int32_t hmax = 0, rmax = 0;
     int64_t next = 0;  // tiles cover the masses in order, chunk-aligned
     for (size_t t = 0; t < tb.tiles.size(); ++t) {
@@ -125,7 +126,7 @@ static void check(int64_t n, const std::vector<int32_t> &L, const std::vector<in
         }
         CHECK(d.nhalo == (int32_t)halo.size());
         // within the row budget, unless the tile is a single chunk
-        CHECK(d.nmass + d.nhalo <= kTileRows || d.nmass <= kChunk);
+        CHECK(d.nmass + d.nhalo <= rows || d.nmass <= kChunk);
         CHECK(d.nmass + d.nhalo <= kMaxTileRows);
         hmax = std::max<int32_t>(hmax, d.nhalo);
         rmax = std::max<int32_t>(rmax, d.nmass + d.nhalo);
@@ -139,6 +140,7 @@ static void check(int64_t n, const std::vector<int32_t> &L, const std::vector<in
     SellBuild st;
     build_sell(n, sp, kd, &tb, 0, st);
     check_sell(n, L, R, k, rest, damp, kd ? act : noact, kd, &tb, 0, st);
+    }
 }
 
 int main() {

@@ -34,7 +34,7 @@ EXPORTS = (
     "titan_time", "titan_sync", "titan_body_download", "titan_body_upload", "titan_timer_start",
     "titan_timer_stop", "titan_mem_used", "titan_kernel_name", "titan_lattice_spring_count",
     "titan_lattice", "titan_debug_inject_failure", "titan_body_layout", "titan_step_bodies",
-    "titan_body_time",
+    "titan_body_time", "titan_set_schedule", "titan_get_schedule", "titan_autotune",
 )
 
 LAYOUT_AUTO, LAYOUT_GATHER, LAYOUT_TILED = 0, 1, 2
@@ -102,6 +102,9 @@ def lib() -> ctypes.CDLL:
         "titan_body_layout": (c_int, [vp, c_int32, POINTER(c_int32)]),
         "titan_step_bodies": (c_int, [vp, vp, vp, c_int32]),
         "titan_body_time": (c_int, [vp, c_int32, POINTER(c_double), POINTER(c_int64)]),
+        "titan_set_schedule": (c_int, [vp, vp]),
+        "titan_get_schedule": (c_int, [vp, vp]),
+        "titan_autotune": (c_int, [vp, vp, c_int32, c_int32, c_int32, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -144,6 +147,47 @@ def lattice(center, dims, nx: int, ny: int, nz: int):
     if rc != 0:
         raise TitanError(rc, "titan_lattice failed")
     return pos, left, right, rest
+
+
+def body_desc(pos, m, *, vel=None, T=None, ext=None, fixed=None, drag=None, left=None, right=None,
+              k=None, rest=None, damping=None, type=None, omega=None, act_a=None, act_phi=None,
+              lc_ptr=None, lc_type=None, lc_vec=None, lc_num=None, t0: float = 0.0,
+              layout: int = LAYOUT_AUTO):
+    """A titan_body_desc over contiguous copies of the given arrays.  Returns
+    (desc, arrays): keep `arrays` alive while the descriptor is in use."""
+    pos = _f64(pos, (-1, 3))
+    n = pos.shape[0]
+    m = _f64(np.broadcast_to(np.asarray(m, dtype=np.float64), (n,)))
+    keep = dict(pos=pos, m=m, vel=_f64(vel, (-1, 3)), T=_f64(T), ext=_f64(ext, (-1, 3)),
+                fixed=None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8),
+                drag=_i32(drag), left=_i32(left), right=_i32(right))
+    s = 0 if keep["left"] is None else keep["left"].shape[0]
+    for name, v in (("k", k), ("rest", rest), ("damping", damping), ("omega", omega),
+                    ("act_a", act_a), ("act_phi", act_phi)):
+        keep[name] = None if v is None else _f64(np.broadcast_to(np.asarray(v, dtype=np.float64), (s,)))
+    keep["type"] = None if type is None else _i32(np.broadcast_to(np.asarray(type), (s,)))
+    keep["lc_ptr"] = None if lc_ptr is None else np.ascontiguousarray(lc_ptr, dtype=np.int64)
+    keep["lc_type"] = _i32(lc_type)
+    keep["lc_vec"] = _f64(lc_vec, (-1, 3)) if lc_vec is not None else None
+    keep["lc_num"] = _f64(lc_num)
+    d = BodyDesc()
+    d.n_masses = n
+    d.n_springs = s
+    d.t0 = float(t0)
+    d.layout = int(layout)
+    ctypes_of = dict(fixed=c_uint8, drag=c_int32, left=c_int32, right=c_int32, type=c_int32,
+                     lc_ptr=c_int64, lc_type=c_int32)
+    for name, arr in keep.items():
+        setattr(d, name, _ptr(arr, ctypes_of.get(name, c_double)))
+    return d, keep
+
+
+class Schedule(ctypes.Structure):
+    _fields_ = [("tiled_min_masses", c_int64), ("tile_rows", c_int32), ("pad", c_int32)]
+
+
+# titan_autotune's candidates, in the order of its rate_out (min -1: never tiled)
+AUTOTUNE_CANDIDATES = [(mn, rw) for mn in (0, 128, 256, 512, 1024, -1) for rw in (682, 2304)]
 
 
 class Context:
@@ -202,38 +246,38 @@ class Context:
         self._chk(self._L.titan_clear_planes_and_balls(self._h))
 
     # -- bodies --
-    def add_body(self, pos, m, *, vel=None, T=None, ext=None, fixed=None, drag=None,
-                 left=None, right=None, k=None, rest=None, damping=None, type=None, omega=None,
-                 act_a=None, act_phi=None, lc_ptr=None, lc_type=None, lc_vec=None, lc_num=None,
-                 t0: float = 0.0, layout: int = LAYOUT_AUTO) -> int:
-        pos = _f64(pos, (-1, 3))
-        n = pos.shape[0]
-        m = _f64(np.broadcast_to(np.asarray(m, dtype=np.float64), (n,)))
-        keep = dict(pos=pos, m=m, vel=_f64(vel, (-1, 3)), T=_f64(T), ext=_f64(ext, (-1, 3)),
-                    fixed=None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8),
-                    drag=_i32(drag), left=_i32(left), right=_i32(right))
-        s = 0 if keep["left"] is None else keep["left"].shape[0]
-        for name, v in (("k", k), ("rest", rest), ("damping", damping), ("omega", omega),
-                        ("act_a", act_a), ("act_phi", act_phi)):
-            keep[name] = None if v is None else _f64(np.broadcast_to(np.asarray(v, dtype=np.float64), (s,)))
-        keep["type"] = None if type is None else _i32(np.broadcast_to(np.asarray(type), (s,)))
-        keep["lc_ptr"] = None if lc_ptr is None else np.ascontiguousarray(lc_ptr, dtype=np.int64)
-        keep["lc_type"] = _i32(lc_type)
-        keep["lc_vec"] = _f64(lc_vec, (-1, 3)) if lc_vec is not None else None
-        keep["lc_num"] = _f64(lc_num)
-        d = BodyDesc()
-        d.n_masses = n
-        d.n_springs = s
-        d.t0 = float(t0)
-        d.layout = int(layout)
-        ctypes_of = dict(fixed=c_uint8, drag=c_int32, left=c_int32, right=c_int32, type=c_int32,
-                         lc_ptr=c_int64, lc_type=c_int32)
-        for name, arr in keep.items():
-            setattr(d, name, _ptr(arr, ctypes_of.get(name, c_double)))
+    def add_body(self, pos, m, *, layout: int = LAYOUT_AUTO, **fields) -> int:
+        """Create a body from titan_body_desc fields (see body_desc)."""
+        d, keep = body_desc(pos, m, layout=layout, **fields)
         bid = c_int32()
         self._chk(self._L.titan_body_create(self._h, ctypes.byref(d), ctypes.byref(bid)))
-        self._bodies[bid.value] = n
+        del keep
+        self._bodies[bid.value] = d.n_masses
         return bid.value
+
+    # -- schedule --
+    def set_schedule(self, tiled_min_masses: int = 0, tile_rows: int = 2304):
+        s = Schedule(int(tiled_min_masses), int(tile_rows), 0)
+        self._chk(self._L.titan_set_schedule(self._h, ctypes.byref(s)))
+
+    def schedule(self) -> tuple[int, int]:
+        s = Schedule()
+        self._chk(self._L.titan_get_schedule(self._h, ctypes.byref(s)))
+        return s.tiled_min_masses, s.tile_rows
+
+    def autotune(self, samples, replicas: int, steps: int):
+        """Time the candidate schedules on `replicas` copies of the sample
+        bodies (dicts of body fields); the fastest becomes this context's
+        schedule.  Returns ((tiled_min_masses, tile_rows), [((min, rows), rate)])
+        with min -1 for "never tiled"."""
+        built = [body_desc(**b) for b in samples]
+        arr = (BodyDesc * len(built))(*[d for d, _ in built])
+        best = Schedule()
+        rates = (c_double * len(AUTOTUNE_CANDIDATES))()
+        self._chk(self._L.titan_autotune(self._h, arr, len(built), int(replicas), int(steps),
+                                         ctypes.byref(best), rates))
+        del built
+        return (best.tiled_min_masses, best.tile_rows), list(zip(AUTOTUNE_CANDIDATES, list(rates)))
 
     def destroy_body(self, bid: int):
         self._chk(self._L.titan_body_destroy(self._h, bid))

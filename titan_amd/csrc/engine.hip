@@ -125,6 +125,7 @@ struct titan_ctx {
     Ranges chunks, incs, acts, lcs, bids, tiles, halos;
     std::vector<BodyH> bodies;
     std::vector<int64_t> left_h;  // staging for per-body step budgets
+    titan_schedule sched{0, kTileRows, 0};
 
     // mass arena (ncap = chunks.cap() * 64)
     int64_t ncap = 0;
@@ -703,7 +704,9 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     TileBuild tb;
     bool tiled = false;
     std::vector<int32_t> nl, nr;  // endpoints in device order
-    if (d->layout != TITAN_LAYOUT_GATHER && N > kTileMasses) perm = morton_order(d->pos, N);
+    const bool want_tiles = d->layout == TITAN_LAYOUT_TILED ||
+                            (d->layout == TITAN_LAYOUT_AUTO && N >= c->sched.tiled_min_masses);
+    if (want_tiles && N > kTileMasses) perm = morton_order(d->pos, N);
     {
         std::vector<int32_t> inv;
         if (!perm.empty()) {
@@ -726,8 +729,8 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     sp.rest = d->rest;
     sp.damping = d->damping;
     sp.act = n_act > 0 ? act_local.data() : nullptr;
-    if (d->layout != TITAN_LAYOUT_GATHER) {
-        tiled = build_tiles(N, sp, tb);
+    if (want_tiles) {
+        tiled = build_tiles(N, sp, tb, c->sched.tile_rows);
         if (!tiled && d->layout == TITAN_LAYOUT_TILED)
             fail(TITAN_E_INVALID, "body_create: tiled layout does not fit: " + tb.why);
         if (!tiled && !perm.empty()) {  // fall back to the caller's order
@@ -1296,6 +1299,106 @@ int titan_time(const titan_ctx *c, double *T) {
     if (!T) return TITAN_E_INVALID;
     *T = c->T;
     return TITAN_OK;
+}
+
+int titan_set_schedule(titan_ctx *c, const titan_schedule *s) {
+    NEED_CTX(c);
+    API_BEGIN
+    if (!s) fail(TITAN_E_INVALID, "set_schedule: null schedule");
+    if (s->tile_rows < kChunk || s->tile_rows > kMaxTileRows)
+        fail(TITAN_E_INVALID, "set_schedule: tile_rows must be in [64, " + std::to_string(kMaxTileRows) + "]");
+    if (s->tiled_min_masses < 0) fail(TITAN_E_INVALID, "set_schedule: negative tiled_min_masses");
+    c->sched = *s;
+    c->sched.pad = 0;
+    API_END(c)
+}
+
+int titan_get_schedule(const titan_ctx *c, titan_schedule *s) {
+    NEED_CTX(c);
+    if (!s) return TITAN_E_INVALID;
+    *s = c->sched;
+    return TITAN_OK;
+}
+
+namespace {
+
+// One autotuning trial: a scratch context with c's globals, `replicas`
+// copies of the samples under schedule `sc`, timed over `steps` steps.
+double time_schedule(titan_ctx *c, const titan_body_desc *samples, int32_t n_samples, int32_t replicas,
+                     int32_t steps, const titan_schedule &sc) {
+    titan_ctx *t = nullptr;
+    auto chk = [&](int rc, const char *what) {
+        if (rc != TITAN_OK) {
+            std::string msg = std::string("autotune: ") + what + ": " + titan_last_error(t);
+            titan_ctx_destroy(t);
+            fail(rc, msg);
+        }
+    };
+    if (titan_ctx_create(c->device, c->budget, &t) != TITAN_OK)
+        fail(TITAN_E_HIP, std::string("autotune: scratch context: ") + titan_last_error(nullptr));
+    chk(titan_set_dt(t, c->dt), "set_dt");
+    chk(titan_set_global_acc(t, c->g[0], c->g[1], c->g[2]), "set_global_acc");
+    for (size_t q = 0; q < c->planes_h.size(); q += 8) {
+        const double *P = c->planes_h.data() + q;
+        chk(titan_add_plane(t, P[0], P[1], P[2], P[3], P[4], P[5]), "add_plane");
+    }
+    for (size_t q = 0; q < c->balls_h.size(); q += 4) {
+        const double *B = c->balls_h.data() + q;
+        chk(titan_add_ball(t, B[0], B[1], B[2], B[3]), "add_ball");
+    }
+    chk(titan_set_schedule(t, &sc), "set_schedule");
+    int64_t masses = 0;
+    for (int32_t r = 0; r < replicas; ++r)
+        for (int32_t k = 0; k < n_samples; ++k) {
+            titan_body_desc d = samples[k];
+            d.layout = TITAN_LAYOUT_AUTO;
+            int32_t id = -1;
+            chk(titan_body_create(t, &d, &id), "body_create");
+            masses += d.n_masses;
+        }
+    chk(titan_step(t, steps), "warm-up");
+    chk(titan_sync(t), "sync");
+    float ms = 0;
+    chk(titan_timer_start(t), "timer");
+    chk(titan_step(t, steps), "step");
+    chk(titan_timer_stop(t, &ms), "timer");
+    titan_ctx_destroy(t);
+    return ms > 0 ? (double)masses * steps / (ms * 1e-3) : 0.0;
+}
+
+}  // namespace
+
+int titan_autotune(titan_ctx *c, const titan_body_desc *samples, int32_t n_samples, int32_t replicas,
+                   int32_t steps, titan_schedule *best, double *rate_out) {
+    NEED_CTX(c);
+    API_BEGIN
+    if (!samples || n_samples <= 0 || replicas <= 0 || steps <= 0)
+        fail(TITAN_E_INVALID, "autotune: need samples, replicas > 0 and steps > 0");
+    const int64_t mins[] = {0, 128, 256, 512, 1024, INT64_MAX};
+    const int32_t rows[] = {682, kMaxTileRows};
+    titan_schedule win{0, kTileRows, 0};
+    double win_rate = -1.0, never_rate = -1.0;
+    int q = 0;
+    for (int64_t mn : mins)
+        for (int32_t rw : rows) {
+            const titan_schedule sc{mn, rw, 0};
+            double rate;
+            if (mn == INT64_MAX && never_rate >= 0) {
+                rate = never_rate;  // no body is tiled: the row budget is moot
+            } else {
+                rate = time_schedule(c, samples, n_samples, replicas, steps, sc);
+                if (mn == INT64_MAX) never_rate = rate;
+            }
+            if (rate_out) rate_out[q] = rate;
+            ++q;
+            if (rate > win_rate) {
+                win_rate = rate;
+                win = sc;
+            }
+        }
+    c->sched = win;
+    if (best) *best = win;
+    API_END(c)
 }
 
 int titan_step_bodies(titan_ctx *c, const int32_t *body_ids, const int64_t *n_steps, int32_t count) {

@@ -40,16 +40,14 @@ constexpr uint32_t kPad = 16u;      // padding slot of a chunk (never read)
 // with one of its masses.  The kernel stages the tile's and the halo's state
 // in LDS, so a tiled body's incidences name the other endpoint by its LDS
 // row: 0..nmass-1 for a tile mass, nmass + h for halo mass h.  A tile takes
-// as many chunks as keep nmass + nhalo within kTileRows (a single chunk may
-// exceed that, up to kMaxTileRows).  Measured on the 1M-mass lattice, full
-// 256-mass tiles (4 workgroups per CU) beat tiles capped at 682 rows (5 per
-// CU), whose extra halo costs more than the occupancy gains.
+// as many chunks as keep nmass + nhalo within the context's tile row budget
+// (a single chunk may exceed that, up to kMaxTileRows).  Measured on the
+// 1M-mass lattice, full 256-mass tiles (4 workgroups per CU) beat tiles
+// capped at 682 rows (5 per CU), whose extra halo costs more than the
+// occupancy gains; the default budget is therefore the maximum.
 constexpr int kTileMasses = 256;
-#ifndef TITAN_TILE_ROWS
-#define TITAN_TILE_ROWS 2304
-#endif
-constexpr int kTileRows = TITAN_TILE_ROWS;
 constexpr int kMaxTileRows = 2304;  // LDS budget: 2304 x 48 bytes = 108 KiB
+constexpr int kTileRows = kMaxTileRows;
 
 struct TileDesc {
     int32_t body;

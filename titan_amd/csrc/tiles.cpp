@@ -55,7 +55,7 @@ std::vector<int32_t> morton_order(const double *pos, int64_t n) {
     return perm;
 }
 
-bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb) {
+bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb, int tile_rows) {
     const int64_t S = sp.n_springs;
     // other endpoints per mass (CSR), to collect each tile's halo
     std::vector<int64_t> row(n + 1, 0);
@@ -95,13 +95,13 @@ bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb) {
         std::sort(h.begin(), h.end());
     };
     for (int64_t m0 = 0; m0 < n;) {
-        // the most chunks (up to 4) whose rows fit kTileRows; at least one
+        // the most chunks (up to 4) whose rows fit tile_rows; at least one
         int64_t m1 = std::min(n, m0 + kChunk);
         halo_of(m0, m1, best);
         for (int k = 2; k <= kTileMasses / kChunk && m0 + (int64_t)(k - 1) * kChunk < n; ++k) {
             const int64_t e = std::min(n, m0 + (int64_t)k * kChunk);
             halo_of(m0, e, halo);
-            if (e - m0 + (int64_t)halo.size() > kTileRows) break;
+            if (e - m0 + (int64_t)halo.size() > tile_rows) break;
             m1 = e;
             best.swap(halo);
         }

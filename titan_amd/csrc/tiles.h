@@ -51,9 +51,10 @@ struct SellBuild {
 std::vector<int32_t> morton_order(const double *pos, int64_t n);
 
 // Cut a body of n masses (device order) into tiles of whole chunks (see
-// layout.h) and list each tile's halo.  Returns false (with tb.why) if one
-// chunk and its halo exceed kMaxTileRows.
-bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb);
+// layout.h): each tile takes the most chunks (up to 4) whose masses and
+// halo fit tile_rows LDS rows.  Lists each tile's halo.  Returns false (with
+// tb.why) if one chunk and its halo exceed kMaxTileRows.
+bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb, int tile_rows = kTileRows);
 
 // Build the incidence lists.  With tb == nullptr (gather layout) an
 // incidence names the other endpoint's arena slot, g0 + its body index;
