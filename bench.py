@@ -15,7 +15,9 @@ Prints one JSON line: mass-steps/s, the roofline fraction of the step kernel
 engine's stream, against 8 TB/s), and the CPU oracle (OpenMP) timed on a
 bounded sample of the same workload.  Unless --no-extra, the same line also
 carries "batched": a 4096-body robot population stepped together against the
-same GPU stepping one body per launch.
+same GPU stepping one body per launch, and "serving": Poisson streams of
+rollout requests served with continuous batching (p50/p99 latency and
+mass-steps/s per arrival rate) against serving them one at a time.
 """
 from __future__ import annotations
 
@@ -165,6 +167,97 @@ def bench_batched(n_bodies: int, seed: int, rollout: int, sample: int) -> dict:
     }
 
 
+def bench_serving(seed: int, n_req: int, lams, quantum: int, n_base: int) -> dict:
+    """Serving: rollout requests (lattices of side U{3..40} or kNN-12 clouds of
+    1e2..1e5 masses, 10-30% of springs actuated, horizons U{100..10000} steps,
+    a ground plane with friction, dt 1e-4) arriving as a Poisson process at
+    each rate lambda, admitted and stepped together by the library's serving
+    loop (continuous batching, `quantum` steps per pump).  Reports p50/p99
+    completion latency and mass-steps/s per lambda, against the same requests
+    served one at a time with one launch per step."""
+    import numpy as np
+    from titan_amd import Context, scenes
+
+    t0 = time.perf_counter()
+    reqs = scenes.serving_requests(n_req, seed=seed)
+    total = sum(len(b["pos"]) * h for b, h in reqs)
+    log(f"serving: {n_req} requests, {sum(len(b['pos']) for b, _ in reqs)} masses, {total / 1e9:.2f}e9 "
+        f"mass-steps ({time.perf_counter() - t0:.1f} s to build)")
+    rng = np.random.default_rng(seed + 1)
+
+    def setup(ctx, sched=None):
+        ctx.set_dt(1e-4)
+        ctx.add_plane((0.0, 0.0, 1.0), 0.0, 0.8, 1.0)
+        if sched is not None:
+            ctx.set_schedule(*sched)
+
+    with Context() as ctx:  # the schedule, by the autotuner, on a sample of the requests
+        setup(ctx)
+        t0 = time.perf_counter()
+        sched, _ = ctx.autotune([b for b, _ in reqs[:6]], replicas=2, steps=16)
+        log(f"serving: autotune {time.perf_counter() - t0:.1f} s -> tiled_min_masses {sched[0]}, "
+            f"tile_rows {sched[1]}")
+
+    with Context() as ctx:  # warm-up: load the kernels, build a graph
+        setup(ctx, sched)
+        ctx.serve_config(quantum=quantum)
+        ctx.serve_submit(reqs[0][0], 2 * quantum, want_result=False)
+        while ctx.serve_pump() != (0, 0):
+            pass
+    per_lam = []
+    for lam in lams:
+        arr = np.cumsum(rng.exponential(1.0 / lam, n_req))
+        with Context() as ctx:
+            setup(ctx, sched)
+            ctx.serve_config(quantum=quantum)
+            t0 = time.perf_counter()
+            i = pumps = 0
+            while True:
+                now = time.perf_counter() - t0
+                while i < n_req and arr[i] <= now:
+                    ctx.serve_submit(reqs[i][0], reqs[i][1], want_result=False)
+                    i += 1
+                running, queued = ctx.serve_pump()
+                pumps += 1
+                if running == 0 and queued == 0:
+                    if i == n_req:
+                        break
+                    time.sleep(max(0.0, arr[i] - (time.perf_counter() - t0)))
+            wall = time.perf_counter() - t0
+            st = ctx.serve_stats()
+        per_lam.append({"lambda_req_per_s": lam, "requests": n_req, "completed": st["completed"],
+                        "p50_latency_s": st["p50_latency_s"], "p99_latency_s": st["p99_latency_s"],
+                        "max_latency_s": st["max_latency_s"], "mass_steps_per_s": st["mass_steps_per_s"],
+                        "wall_s": wall, "pumps": pumps})
+        log(f"serving: lambda {lam}/s: p50 {st['p50_latency_s']:.3f} s, p99 {st['p99_latency_s']:.3f} s, "
+            f"{st['mass_steps_per_s']:.3e} mass-steps/s, {pumps} pumps, {wall:.1f} s")
+    # the same requests served one at a time, one launch per step
+    base = reqs[:n_base]
+    with Context() as ctx:
+        setup(ctx, sched)
+        ids = [ctx.add_body(**b) for b, _ in base]
+        ctx.sync()
+        ctx.timer_start()
+        for bid, (_, h) in zip(ids, base):
+            ctx.step_bodies([bid], [h])
+        ms = ctx.timer_stop()
+    base_rate = sum(len(b["pos"]) * h for b, h in base) / (ms * 1e-3)
+    best = max(x["mass_steps_per_s"] for x in per_lam)
+    log(f"serving: one request at a time, one launch per step: {base_rate:.3e} mass-steps/s "
+        f"({n_base} requests, {ms:.0f} ms)")
+    return {
+        "workload": f"{n_req} rollout requests per lambda (seed {seed}): lattices side U{{3..40}} or kNN-12 "
+                    f"clouds of 1e2..1e5 masses, 10-30% actuated, horizon U{{100..10000}} steps, ground "
+                    f"plane with friction, dt=1e-4, FP64; Poisson arrivals; continuous batching with "
+                    f"{quantum}-step pumps",
+        "per_lambda": per_lam,
+        "one_request_at_a_time": {"value": base_rate, "unit": "mass-steps/s",
+                                  "sample": f"first {n_base} requests, one launch per step each"},
+        "speedup_vs_one_request_at_a_time": best / base_rate,
+        "schedule": {"tiled_min_masses": sched[0], "tile_rows": sched[1]},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,7 +270,7 @@ def main() -> int:
                     help="device layout of the body (results are identical; speed differs)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the batched, serving and irregular-graph workloads")
-    ap.add_argument("--only", choices=["lattice", "batched"], default=None,
+    ap.add_argument("--only", choices=["lattice", "batched", "serving"], default=None,
                     help="run one workload (its figures go in the same JSON line)")
     ap.add_argument("--bodies", type=int, default=4096, help="batched workload: robot count")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -214,7 +307,10 @@ def main() -> int:
     extra = {}
     if args.only == "batched" or (args.only is None and not args.no_extra):
         extra["batched"] = bench_batched(args.bodies, args.seed, rollout=100, sample=min(512, args.bodies))
-    if args.only == "batched":
+    if args.only == "serving" or (args.only is None and not args.no_extra):
+        extra["serving"] = bench_serving(args.seed, n_req=64, lams=(50, 200, 1000, 2000), quantum=64,
+                                         n_base=8)
+    if args.only in ("batched", "serving"):
         print(json.dumps(extra), flush=True)
         return 0
     out = bench_lattice(args)

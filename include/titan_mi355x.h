@@ -135,8 +135,8 @@ int titan_sync(titan_ctx *ctx);
  * Generalises the driver loop of Simulation::execute (src/sim.cu:1503-1617),
  * which has one clock for the whole simulation.  With count == 1 the
  * launches cover that body's tiles or chunks only (one body per launch);
- * otherwise one launch per step covers every body.  Returns after the steps
- * are enqueued (count == 1) or done (count > 1). */
+ * otherwise one launch per step covers every body.  Returns once the steps
+ * are enqueued. */
 int titan_step_bodies(titan_ctx *ctx, const int32_t *body_ids, const int64_t *n_steps, int32_t count);
 /* A body's own clock (T += dt per step it took) and step count. */
 int titan_body_time(const titan_ctx *ctx, int32_t body_id, double *T, int64_t *steps);
@@ -190,6 +190,55 @@ int titan_get_schedule(const titan_ctx *ctx, titan_schedule *sched);
  * candidate order (12 entries). */
 int titan_autotune(titan_ctx *ctx, const titan_body_desc *samples, int32_t n_samples, int32_t replicas,
                    int32_t steps, titan_schedule *best, double *rate_out);
+
+/* ---- serving: continuous batching of rollout requests (no counterpart in
+ *      the reference, whose one Simulation runs one scene on a driver
+ *      thread, src/sim.cu:1389,1503-1617) ----
+ * A request is a body and a horizon (steps).  titan_serve_submit copies the
+ * body and queues it.  titan_serve_pump, called by the host loop (the
+ * library starts no thread), admits queued requests in arrival order while
+ * the running set stays within max_running requests and the serving HBM
+ * budget (estimated bytes per body; a body that does not fit waits until
+ * others finish), steps every running request together, up to `quantum`
+ * steps each and never past its horizon, in one launch per step, and
+ * retires the finished ones: their final positions and velocities go to the
+ * request's out buffers and their device memory is released.  A request's
+ * result does not depend on what runs beside it.  Latency is measured from
+ * submit to retirement on the host's steady clock. */
+#define TITAN_REQ_QUEUED 0
+#define TITAN_REQ_RUNNING 1
+#define TITAN_REQ_DONE 2
+#define TITAN_REQ_CANCELLED 3
+#define TITAN_REQ_FAILED 4     /* its body was rejected by titan_body_create */
+
+typedef struct titan_request {
+    titan_body_desc body;   /* copied by titan_serve_submit */
+    int64_t horizon;        /* steps to run, >= 0 */
+    double *out_pos;        /* optional: 3*n_masses, final positions; must stay valid until done */
+    double *out_vel;        /* optional: 3*n_masses, final velocities */
+} titan_request;
+
+typedef struct titan_serve_report {
+    int64_t submitted, completed, cancelled, failed, running, queued;
+    double p50_latency_s, p99_latency_s, max_latency_s;  /* over completed requests */
+    double mass_steps;          /* of completed requests */
+    double mass_steps_per_s;    /* mass_steps / (last retirement - first submit) */
+} titan_serve_report;
+
+/* max_running: requests on the device at once (>= 1); hbm_bytes: serving
+ * budget (0: the context's budget, or unlimited); quantum: steps per pump
+ * (>= 1).  Defaults: 4096, 0, 64. */
+int titan_serve_config(titan_ctx *ctx, int32_t max_running, uint64_t hbm_bytes, int64_t quantum);
+int titan_serve_submit(titan_ctx *ctx, const titan_request *req, int64_t *ticket);
+/* One round of admit / step / retire; *running and *queued (may be NULL)
+ * receive the counts afterwards. */
+int titan_serve_pump(titan_ctx *ctx, int32_t *running, int32_t *queued);
+int titan_serve_poll(titan_ctx *ctx, int64_t ticket, int32_t *state, int64_t *steps_done);
+/* Drops a queued or running request (its out buffers are not written). */
+int titan_serve_cancel(titan_ctx *ctx, int64_t ticket);
+int titan_serve_stats(titan_ctx *ctx, titan_serve_report *out);
+/* Forget completed/cancelled requests and reset the statistics. */
+int titan_serve_reset_stats(titan_ctx *ctx);
 
 /* Host-side lattice generator with the reference's mass and spring order
  * (Lattice::Lattice, src/object.cu:232-293).  pos: 3*nx*ny*nz; left/right/

@@ -280,3 +280,21 @@ def test_auto_layout_prefers_tiles():
     with Context() as ctx:
         (bid,) = sc.load(ctx)
         assert ctx.body_layout(bid) == LAYOUT_TILED
+
+
+@LAYOUTS
+def test_many_balls_culled_by_grid_bitexact(layout):
+    # 300 balls (above the grid threshold) scattered through and around a
+    # falling lattice: the engine tests only each mass's cell list, the oracle
+    # every ball in creation order
+    rng = np.random.default_rng(7)
+    b = scenes.lattice_body((0.0, 0.0, 1.2), (2.0, 2.0, 2.0), 12, damping=0.1)
+    balls = [(*rng.uniform(-1.5, 1.5, 2), rng.uniform(0.0, 2.5), rng.uniform(0.05, 0.35)) for _ in range(300)]
+    sc = scenes.Scene(bodies=[b], planes=[(0.0, 0.0, 1.0, 0.0, 0.8, 1.0)], balls=balls, dt=1e-4)
+    _say(f"case 300 balls culled by grid, {sc.n_masses} masses, 200 steps, layout {layout}")
+    eng, _ = run_engine(sc, 200, layout=layout)
+    orc = run_oracle(sc, 200)
+    assert_bitexact(eng, orc)
+    # the balls act: the run differs from the same scene without them
+    free, _ = run_engine(scenes.Scene(bodies=[b], planes=sc.planes, dt=1e-4), 200, layout=layout)
+    assert not np.array_equal(free["pos"], eng["pos"])

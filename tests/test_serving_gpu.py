@@ -1,0 +1,148 @@
+"""Serving (titan_serve_*): continuous batching of rollout requests.
+
+Each request's result must be the same whether it is served alone or in a
+mix that arrives while others run (bit for bit: a body's arithmetic never
+depends on the others), and the same as the oracle run of that body for its
+horizon (bit for bit for passive bodies, 1e-10 normwise relative for
+actuated ones, whose sin is the device's).  Admission respects the request
+limit and the serving HBM budget, cancellation frees the slot, and the
+statistics count what was served."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from titan_amd import REQ_CANCELLED, REQ_DONE, REQ_QUEUED, REQ_RUNNING, Context, scenes
+from helpers import normwise_rel, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests():
+    robots = scenes.robot_population(5, side_lo=3, side_hi=9, seed=11)
+    lat = scenes.readme_lattice(2.45)
+    bodies = robots.bodies[:4] + [lat.bodies[0], robots.bodies[4]]
+    horizons = [150, 37, 400, 64, 129, 250]
+    return bodies, horizons, lat.planes
+
+
+def _serve(bodies, horizons, planes, *, order=None, quantum=64, max_running=4096, hbm=0, arrive_every=0):
+    """Serve the requests; with arrive_every > 0, request q is submitted before
+    pump number q * arrive_every (later arrivals join a running batch)."""
+    order = list(range(len(bodies))) if order is None else order
+    with Context() as ctx:
+        ctx.set_dt(1e-4)
+        for p in planes:
+            ctx.add_plane(p[:3], p[3], p[4], p[5])
+        ctx.serve_config(max_running=max_running, hbm_bytes=hbm, quantum=quantum)
+        tickets, pumps, peak = {}, 0, 0
+        pending = list(order)
+        while pending or any(ctx.serve_poll(t)[0] in (REQ_QUEUED, REQ_RUNNING) for t in tickets.values()):
+            while pending and (arrive_every == 0 or pumps >= arrive_every * len(tickets)):
+                q = pending.pop(0)
+                tickets[q] = ctx.serve_submit(bodies[q], horizons[q])
+            running, _ = ctx.serve_pump()
+            peak = max(peak, running)
+            pumps += 1
+            assert pumps < 10_000
+        res = {q: ctx.serve_result(t) for q, t in tickets.items()}
+        for q, t in tickets.items():
+            assert ctx.serve_poll(t) == (REQ_DONE, horizons[q])
+        stats = ctx.serve_stats()
+    return res, peak, stats
+
+
+def test_each_result_is_the_same_alone_and_in_the_mix():
+    bodies, horizons, planes = _requests()
+    print(f"case serving alone vs mix: {len(bodies)} requests, horizons {horizons}", flush=True)
+    alone = {}
+    for q in range(len(bodies)):
+        r, _, _ = _serve([bodies[q]], [horizons[q]], planes)
+        alone[q] = r[0]
+    mixes = [dict(quantum=64), dict(quantum=16, arrive_every=3, order=[2, 0, 5, 1, 4, 3]),
+             dict(quantum=33, max_running=2, order=[5, 4, 3, 2, 1, 0])]
+    for kw in mixes:
+        mix, _, stats = _serve(bodies, horizons, planes, **kw)
+        assert stats["completed"] == len(bodies)
+        for q in range(len(bodies)):
+            np.testing.assert_array_equal(mix[q][0], alone[q][0])
+            np.testing.assert_array_equal(mix[q][1], alone[q][1])
+    # and against the oracle
+    for q in range(len(bodies)):
+        o = to_oracle(scenes.Scene(bodies=[bodies[q]], planes=planes, dt=1e-4))
+        o.run(horizons[q])
+        if bodies[q].get("type") is not None and np.any(np.asarray(bodies[q]["type"]) >= 2):
+            assert normwise_rel(alone[q][0], o.pos) <= 1e-10
+            assert normwise_rel(alone[q][1], o.vel) <= 1e-10
+        else:
+            np.testing.assert_array_equal(alone[q][0], o.pos)
+            np.testing.assert_array_equal(alone[q][1], o.vel)
+
+
+def _est(b):
+    # the serving layer's documented per-body estimate (titan_amd/csrc/serve.cpp)
+    return 256 * len(b["pos"]) + 96 * len(b["left"]) + (1 << 16)
+
+
+def _replay(bodies, horizons, budget, max_running, quantum):
+    """Peak running count, as titan_serve_pump reports it (after retiring), of
+    its admission rule: in arrival order, within max_running and the byte
+    budget (a request is always admitted when nothing else is admitted or
+    running).  A pump admits first only when nothing runs; otherwise it steps
+    the running set by up to `quantum` steps each, admits arrivals for the
+    next pump, then retires the finished."""
+    queue, running, peak = list(range(len(bodies))), [], 0
+    left = list(horizons)
+
+    def admit(fresh):
+        while queue and len(running) + len(fresh) < max_running:
+            busy = running + fresh
+            if budget and busy and sum(_est(bodies[r]) for r in busy) + _est(bodies[queue[0]]) > budget:
+                break
+            fresh.append(queue.pop(0))
+
+    while queue or running:
+        fresh = []
+        if not running:
+            admit(fresh)
+            running, fresh = fresh, []
+        for r in running:
+            left[r] -= min(quantum, left[r])
+        admit(fresh)
+        running = [r for r in running if left[r] > 0] + fresh
+        peak = max(peak, len(running))
+    return peak
+
+
+def test_admission_respects_request_limit_and_budget():
+    bodies, horizons, planes = _requests()
+    _, peak, stats = _serve(bodies, horizons, planes, max_running=2, quantum=32)
+    assert peak == 2 and stats["completed"] == len(bodies)
+    big = max(_est(b) for b in bodies)
+    for budget in (big, big + 2 * min(_est(b) for b in bodies)):
+        want = _replay(bodies, horizons, budget, 4096, 32)
+        _, peak, stats = _serve(bodies, horizons, planes, hbm=budget, quantum=32)
+        assert peak == want and stats["completed"] == len(bodies)
+    assert _replay(bodies, horizons, big, 4096, 32) < len(bodies)
+
+
+def test_cancel_and_stats():
+    bodies, horizons, planes = _requests()
+    with Context() as ctx:
+        ctx.set_dt(1e-4)
+        ctx.serve_config(max_running=2, quantum=16)
+        t = [ctx.serve_submit(b, h) for b, h in zip(bodies, horizons)]
+        ctx.serve_pump()
+        assert ctx.serve_poll(t[0])[0] == REQ_RUNNING and ctx.serve_poll(t[5])[0] == REQ_QUEUED
+        ctx.serve_cancel(t[0])  # running
+        ctx.serve_cancel(t[5])  # queued
+        assert ctx.serve_poll(t[0])[0] == REQ_CANCELLED and ctx.serve_poll(t[5])[0] == REQ_CANCELLED
+        while any(ctx.serve_poll(x)[0] in (REQ_QUEUED, REQ_RUNNING) for x in t):
+            ctx.serve_pump()
+        s = ctx.serve_stats()
+        assert (s["submitted"], s["completed"], s["cancelled"], s["running"], s["queued"]) == (6, 4, 2, 0, 0)
+        want = sum(len(bodies[q]["pos"]) * horizons[q] for q in (1, 2, 3, 4))
+        assert s["mass_steps"] == want
+        assert 0 < s["p50_latency_s"] <= s["p99_latency_s"] <= s["max_latency_s"]
+        ctx.serve_reset_stats()
+        assert ctx.serve_stats()["completed"] == 0

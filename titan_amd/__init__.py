@@ -35,7 +35,11 @@ EXPORTS = (
     "titan_timer_stop", "titan_mem_used", "titan_kernel_name", "titan_lattice_spring_count",
     "titan_lattice", "titan_debug_inject_failure", "titan_body_layout", "titan_step_bodies",
     "titan_body_time", "titan_set_schedule", "titan_get_schedule", "titan_autotune",
+    "titan_serve_config", "titan_serve_submit", "titan_serve_pump", "titan_serve_poll",
+    "titan_serve_cancel", "titan_serve_stats", "titan_serve_reset_stats",
 )
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
 
 LAYOUT_AUTO, LAYOUT_GATHER, LAYOUT_TILED = 0, 1, 2
 
@@ -105,6 +109,13 @@ def lib() -> ctypes.CDLL:
         "titan_set_schedule": (c_int, [vp, vp]),
         "titan_get_schedule": (c_int, [vp, vp]),
         "titan_autotune": (c_int, [vp, vp, c_int32, c_int32, c_int32, vp, vp]),
+        "titan_serve_config": (c_int, [vp, c_int32, c_uint64, c_int64]),
+        "titan_serve_submit": (c_int, [vp, vp, POINTER(c_int64)]),
+        "titan_serve_pump": (c_int, [vp, POINTER(c_int32), POINTER(c_int32)]),
+        "titan_serve_poll": (c_int, [vp, c_int64, POINTER(c_int32), POINTER(c_int64)]),
+        "titan_serve_cancel": (c_int, [vp, c_int64]),
+        "titan_serve_stats": (c_int, [vp, vp]),
+        "titan_serve_reset_stats": (c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -182,6 +193,18 @@ def body_desc(pos, m, *, vel=None, T=None, ext=None, fixed=None, drag=None, left
     return d, keep
 
 
+class Request(ctypes.Structure):
+    _fields_ = [("body", BodyDesc), ("horizon", c_int64), ("out_pos", POINTER(c_double)),
+                ("out_vel", POINTER(c_double))]
+
+
+class ServeReport(ctypes.Structure):
+    _fields_ = [("submitted", c_int64), ("completed", c_int64), ("cancelled", c_int64), ("failed", c_int64),
+                ("running", c_int64), ("queued", c_int64), ("p50_latency_s", c_double),
+                ("p99_latency_s", c_double), ("max_latency_s", c_double), ("mass_steps", c_double),
+                ("mass_steps_per_s", c_double)]
+
+
 class Schedule(ctypes.Structure):
     _fields_ = [("tiled_min_masses", c_int64), ("tile_rows", c_int32), ("pad", c_int32)]
 
@@ -201,6 +224,7 @@ class Context:
             raise TitanError(rc, self._L.titan_last_error(None).decode())
         self._h = h
         self._bodies: dict[int, int] = {}
+        self._serve_out: dict[int, tuple] = {}
 
     # -- plumbing --
     def _chk(self, rc: int):
@@ -278,6 +302,57 @@ class Context:
                                          ctypes.byref(best), rates))
         del built
         return (best.tiled_min_masses, best.tile_rows), list(zip(AUTOTUNE_CANDIDATES, list(rates)))
+
+    # -- serving --
+    def serve_config(self, max_running: int = 4096, hbm_bytes: int = 0, quantum: int = 64):
+        self._chk(self._L.titan_serve_config(self._h, int(max_running), int(hbm_bytes), int(quantum)))
+
+    def serve_submit(self, body: dict, horizon: int, want_result: bool = True) -> int:
+        """Queue a rollout request (the body's arrays are copied); returns its ticket.
+        With want_result, the final pos/vel land in arrays read by serve_result."""
+        d, keep = body_desc(**body)
+        req = Request()
+        req.body = d
+        req.horizon = int(horizon)
+        out = None
+        if want_result:
+            out = (np.zeros((d.n_masses, 3)), np.zeros((d.n_masses, 3)))
+            req.out_pos = _ptr(out[0], c_double)
+            req.out_vel = _ptr(out[1], c_double)
+        t = c_int64()
+        self._chk(self._L.titan_serve_submit(self._h, ctypes.byref(req), ctypes.byref(t)))
+        del keep
+        self._serve_out[t.value] = out
+        return t.value
+
+    def serve_pump(self) -> tuple[int, int]:
+        """One admit / step / retire round; returns (running, queued)."""
+        r, q = c_int32(), c_int32()
+        self._chk(self._L.titan_serve_pump(self._h, ctypes.byref(r), ctypes.byref(q)))
+        return r.value, q.value
+
+    def serve_poll(self, ticket: int) -> tuple[int, int]:
+        st, n = c_int32(), c_int64()
+        self._chk(self._L.titan_serve_poll(self._h, int(ticket), ctypes.byref(st), ctypes.byref(n)))
+        return st.value, n.value
+
+    def serve_result(self, ticket: int):
+        """(pos, vel) of a completed request."""
+        st, _ = self.serve_poll(ticket)
+        if st != REQ_DONE:
+            raise RuntimeError(f"request {ticket} is not done (state {st})")
+        return self._serve_out[ticket]
+
+    def serve_cancel(self, ticket: int):
+        self._chk(self._L.titan_serve_cancel(self._h, int(ticket)))
+
+    def serve_stats(self) -> dict:
+        r = ServeReport()
+        self._chk(self._L.titan_serve_stats(self._h, ctypes.byref(r)))
+        return {name: getattr(r, name) for name, _ in ServeReport._fields_}
+
+    def serve_reset_stats(self):
+        self._chk(self._L.titan_serve_reset_stats(self._h))
 
     def destroy_body(self, bid: int):
         self._chk(self._L.titan_body_destroy(self._h, bid))

@@ -124,7 +124,6 @@ struct titan_ctx {
 
     Ranges chunks, incs, acts, lcs, bids, tiles, halos;
     std::vector<BodyH> bodies;
-    std::vector<int64_t> left_h;  // staging for per-body step budgets
     titan_schedule sched{0, kTileRows, 0};
 
     // mass arena (ncap = chunks.cap() * 64)
@@ -153,20 +152,34 @@ struct titan_ctx {
     int32_t *b_chunk0 = nullptr, *b_lc = nullptr, *b_lcn = nullptr;
     double *planes = nullptr, *balls = nullptr;
     int64_t planes_cap = 0, balls_cap = 0;
+    // ball grid (rebuilt before the next launch after the balls change)
+    bool ball_grid_dirty = false;
+    int32_t *bg_start = nullptr, *bg_list = nullptr;
+    double bg_x0 = 0, bg_y0 = 0, bg_z0 = 0, bg_inv_h = 0;
+    int32_t bg_n[3] = {0, 0, 0};
 
     bool kd_known = false, uniform_kd = true;
     bool has_act = false;  // some spring is actuated: launch the actuated variants
     double k_uni = 0, damp_uni = 0;
     int slot = 0;
 
-    // captured multi-step graphs, valid while `version` is unchanged
+    // captured multi-step graphs, valid while `version` is unchanged, keyed
+    // by the launch shape they were captured with (most recent last)
     uint64_t version = 1;
     struct Graph {
         hipGraphExec_t exec;
         uint64_t version;
         int slot;
+        int64_t c1, t1;
+        int32_t ldim;
+        bool gather, tiled;
     };
     std::vector<Graph> graphs;
+    // pinned staging of per-body step budgets, two buffers used in turn
+    int64_t *left_pin[2] = {nullptr, nullptr};
+    hipEvent_t left_ev[2] = {nullptr, nullptr};
+    int64_t left_cap = 0;
+    int left_k = 0;
     int32_t inject = -1;
 
     std::map<void *, size_t> allocs;
@@ -175,6 +188,7 @@ struct titan_ctx {
 namespace {
 
 constexpr int kGraphSteps = 64;  // even: a replay leaves the launch slot unchanged
+constexpr size_t kGraphCache = 8;  // captured graphs kept per context
 thread_local std::string g_create_err;
 
 template <class T>
@@ -200,6 +214,15 @@ void dfree(titan_ctx *c, void *p) {
         c->allocs.erase(it);
     }
     (void)hipFree(p);
+}
+
+// Fail before an arena grows several arrays unless all of the new arrays fit
+// the budget together (the old ones are freed one by one as they are
+// replaced), so that running out of budget never leaves an arena half-grown.
+void check_growth(titan_ctx *c, size_t bytes) {
+    if (c->budget && c->used + bytes > c->budget)
+        fail(TITAN_E_NOMEM, "HBM budget exhausted: growing an arena needs " + std::to_string(bytes) +
+                                " bytes, " + std::to_string(c->budget - c->used) + " left");
 }
 
 // Reallocate a K-component SoA array from cap_old to cap_new entries per
@@ -228,8 +251,11 @@ void d2h(titan_ctx *c, T *dst, const T *src, int64_t n) {
 }
 
 void drop_graphs(titan_ctx *c) {
-    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
-    c->graphs.clear();
+    if (!c->graphs.empty()) {
+        (void)hipStreamSynchronize(c->stream);  // none of them may still be running
+        for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+        c->graphs.clear();
+    }
     c->version++;
 }
 
@@ -242,6 +268,9 @@ void ensure_chunks(titan_ctx *c, int64_t need) {
     if (need > max_chunks) fail(TITAN_E_NOMEM, "mass arena limit: at most 2^26 mass slots");
     nc = std::min(nc, max_chunks);
     const int64_t on = old * kChunk, nn = nc * kChunk;
+    check_growth(c, (size_t)nn * (96 + 8 + 8 + 24 + 4 + 4 + (c->ext ? 24 : 0) + (c->drag ? 4 : 0) +
+                                  (c->mlc ? 8 : 0)) +
+                        (size_t)nc * (4 + 8 + 4));
     grow_soa(c, c->st, 2, 6 * on, 6 * nn);
     grow_soa(c, c->m, 1, on, nn);
     grow_soa(c, c->Tm, 1, on, nn);
@@ -269,6 +298,7 @@ void ensure_incs(titan_ctx *c, int64_t need) {
     if (c->incs.cap() >= need) return;
     const int64_t old = c->incs.cap();
     const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 4096));
+    check_growth(c, (size_t)nc * (4 + 8 + (c->inc_k ? 16 : 0) + (c->inc_act ? 4 : 0)));
     grow_soa(c, c->inc_nbr, 1, old, nc);
     grow_soa(c, c->inc_rest, 1, old, nc);
     if (c->inc_k) grow_soa(c, c->inc_k, 1, old, nc);
@@ -307,6 +337,7 @@ void ensure_bodies(titan_ctx *c, int64_t need) {
     if (c->bcap >= need) return;
     const int64_t old = c->bcap;
     const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 64));
+    check_growth(c, (size_t)nc * (2 * (8 + 8 + 8) + 8 + 4 + 4 + 1 + 4));
     for (int s = 0; s < 2; ++s) {
         grow_soa(c, c->bT[s], 1, old, nc);
         grow_soa(c, c->bn[s], 1, old, nc);
@@ -333,6 +364,14 @@ void ensure_tiles(titan_ctx *c, int64_t need) {
     const int64_t old = c->tiles.cap();
     const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 64));
     grow_soa(c, c->tile, 1, old, nc);
+    // launches cover the whole arena: unused tiles are marked free
+    std::vector<TileDesc> dead(nc - old);
+    for (auto &t : dead) {
+        t = TileDesc{};
+        t.body = -1;
+    }
+    h2d(c, c->tile + old, dead.data(), nc - old);
+    HIPCHK(hipStreamSynchronize(c->stream));
     c->tiles.grow(nc);
     drop_graphs(c);
 }
@@ -352,6 +391,7 @@ void make_per_incidence_kd(titan_ctx *c) {
     // at least one entry, so that the arrays exist (and grow with their
     // arena) even while the arena is empty
     const int64_t n = std::max<int64_t>(c->incs.cap(), 1);
+    check_growth(c, (size_t)n * 16);
     c->inc_k = dalloc<double>(c, n);
     c->inc_damp = dalloc<double>(c, n);
     HIPCHK(launch_fill_f64(c->inc_k, n, c->k_uni, c->stream));
@@ -369,6 +409,7 @@ void make_act(titan_ctx *c) {
     drop_graphs(c);
 }
 
+// LDS rows of the largest live tile
 int32_t tiled_rows(const titan_ctx *c) {
     int32_t h = 0;
     for (auto &b : c->bodies)
@@ -400,6 +441,8 @@ struct Range {
     bool gather = false, tiled = false, all = true;
 };
 
+// Every live chunk and tile (up to the arenas' high-water marks; free ones
+// are marked and their blocks exit at once).
 Range full_range(const titan_ctx *c) {
     Range r;
     r.c1 = c->chunks.high_water();
@@ -458,6 +501,15 @@ StepArgs make_args(titan_ctx *c, int slot, const Range &rg) {
     a.nplanes = (int32_t)(c->planes_h.size() / 8);
     a.balls = c->balls;
     a.nballs = (int32_t)(c->balls_h.size() / 4);
+    a.bg_start = c->bg_start;
+    a.bg_list = c->bg_list;
+    a.bg_x0 = c->bg_x0;
+    a.bg_y0 = c->bg_y0;
+    a.bg_z0 = c->bg_z0;
+    a.bg_inv_h = c->bg_inv_h;
+    a.bg_nx = c->bg_n[0];
+    a.bg_ny = c->bg_n[1];
+    a.bg_nz = c->bg_n[2];
     a.gx = c->g[0];
     a.gy = c->g[1];
     a.gz = c->g[2];
@@ -487,9 +539,17 @@ hipError_t launch_range(titan_ctx *c, int slot, const Range &rg) {
 }
 
 hipGraphExec_t graph_for(titan_ctx *c, int slot) {
-    for (auto &g : c->graphs)
-        if (g.version == c->version && g.slot == slot) return g.exec;
     const Range rg = full_range(c);
+    const int32_t ldim = tiled_rows(c);
+    for (auto &g : c->graphs)
+        if (g.version == c->version && g.slot == slot && g.c1 == rg.c1 && g.t1 == rg.t1 && g.ldim == ldim &&
+            g.gather == rg.gather && g.tiled == rg.tiled)
+            return g.exec;
+    if (c->graphs.size() >= kGraphCache) {  // evict the oldest
+        HIPCHK(hipStreamSynchronize(c->stream));
+        (void)hipGraphExecDestroy(c->graphs.front().exec);
+        c->graphs.erase(c->graphs.begin());
+    }
     hipGraph_t graph;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int s = slot;
@@ -506,11 +566,102 @@ hipGraphExec_t graph_for(titan_ctx *c, int slot) {
     hipGraphExec_t exec;
     HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     (void)hipGraphDestroy(graph);
-    c->graphs.push_back({exec, c->version, slot});
+    c->graphs.push_back({exec, c->version, slot, rg.c1, rg.t1, ldim, rg.gather, rg.tiled});
     return exec;
 }
 
 int64_t live_body_hw(titan_ctx *c) { return c->bids.high_water(); }
+
+// Build the ball grid: cells of side h over the balls' bounding box, each
+// listing (ascending) the balls whose sphere, grown by a relative 1e-9, meets
+// the cell grown by 1e-6 h.  A point's computed distance to a ball can fall
+// within its radius only if the true distance is within r (1 + 1e-15), and
+// the cell index computed from a point's coordinates is off by at most a
+// rounding error, far below 1e-6 h; so a ball missing from a point's cell
+// list contributes a zero vector.  Points outside the grid touch no ball.
+void build_ball_grid(titan_ctx *c) {
+    c->ball_grid_dirty = false;
+    dfree(c, c->bg_start);
+    dfree(c, c->bg_list);
+    c->bg_start = c->bg_list = nullptr;
+    const int64_t nb = (int64_t)c->balls_h.size() / 4;
+    if (nb < kBallGridMin) return;
+    const double *B = c->balls_h.data();
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY}, rsum = 0;
+    for (int64_t b = 0; b < nb; ++b) {
+        const double rad = std::fabs(B[4 * b + 3]);
+        if (!std::isfinite(rad) || !std::isfinite(B[4 * b]) || !std::isfinite(B[4 * b + 1]) ||
+            !std::isfinite(B[4 * b + 2]))
+            return;  // a non-finite ball: test every ball, as the reference does
+        rsum += rad;
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], B[4 * b + k] - rad);
+            hi[k] = std::max(hi[k], B[4 * b + k] + rad);
+        }
+    }
+    double ext = 0;
+    for (int k = 0; k < 3; ++k) ext = std::max(ext, hi[k] - lo[k]);
+    if (!(ext > 0)) return;
+    for (int k = 0; k < 3; ++k) {  // grow the box so that rounding never leaves it
+        const double g = 1e-6 * ext + 1e-300;
+        lo[k] -= g;
+        hi[k] += g;
+    }
+    const double vol = (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+    double h = std::max(2.0 * rsum / (double)nb, std::cbrt(vol / (double)nb));
+    int64_t n[3];
+    for (;;) {
+        for (int k = 0; k < 3; ++k) n[k] = std::max<int64_t>(1, (int64_t)std::ceil((hi[k] - lo[k]) / h));
+        if (n[0] * n[1] * n[2] <= kBallGridMaxCells) break;
+        h *= 1.26;
+    }
+    const int64_t ncell = n[0] * n[1] * n[2];
+    const double delta = 1e-6 * h;
+    std::vector<std::vector<int32_t>> cells(ncell);
+    for (int64_t b = 0; b < nb; ++b) {
+        const double cx[3] = {B[4 * b], B[4 * b + 1], B[4 * b + 2]};
+        const double R = std::fabs(B[4 * b + 3]) * (1.0 + 1e-9) + delta;
+        int64_t i0[3], i1[3];
+        for (int k = 0; k < 3; ++k) {
+            i0[k] = std::max<int64_t>(0, (int64_t)std::floor((cx[k] - R - lo[k]) / h) - 1);
+            i1[k] = std::min<int64_t>(n[k] - 1, (int64_t)std::floor((cx[k] + R - lo[k]) / h) + 1);
+        }
+        for (int64_t iz = i0[2]; iz <= i1[2]; ++iz)
+            for (int64_t iy = i0[1]; iy <= i1[1]; ++iy)
+                for (int64_t ix = i0[0]; ix <= i1[0]; ++ix) {
+                    const int64_t id[3] = {ix, iy, iz};
+                    double d2 = 0;  // squared distance from the centre to the grown cell
+                    for (int k = 0; k < 3; ++k) {
+                        const double a0 = lo[k] + (double)id[k] * h - delta, a1 = lo[k] + (double)(id[k] + 1) * h + delta;
+                        const double e = cx[k] < a0 ? a0 - cx[k] : (cx[k] > a1 ? cx[k] - a1 : 0.0);
+                        d2 += e * e;
+                    }
+                    if (d2 <= R * R) cells[(iz * n[1] + iy) * n[0] + ix].push_back((int32_t)b);
+                }
+    }
+    std::vector<int32_t> start(ncell + 1, 0), list;
+    for (int64_t q = 0; q < ncell; ++q) {
+        start[q + 1] = start[q] + (int32_t)cells[q].size();
+        list.insert(list.end(), cells[q].begin(), cells[q].end());
+    }
+    c->bg_start = dalloc<int32_t>(c, ncell + 1);
+    c->bg_list = dalloc<int32_t>(c, std::max<int64_t>(1, (int64_t)list.size()));
+    h2d(c, c->bg_start, start.data(), ncell + 1);
+    h2d(c, c->bg_list, list.data(), (int64_t)list.size());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->bg_x0 = lo[0];
+    c->bg_y0 = lo[1];
+    c->bg_z0 = lo[2];
+    c->bg_inv_h = 1.0 / h;
+    for (int k = 0; k < 3; ++k) c->bg_n[k] = (int32_t)n[k];
+}
+
+void prepare_launches(titan_ctx *c) {
+    if (c->ball_grid_dirty) {
+        build_ball_grid(c);
+        drop_graphs(c);
+    }
+}
 
 void maybe_inject(titan_ctx *c) {
     if (c->inject >= 0) {
@@ -556,6 +707,7 @@ void do_step(titan_ctx *c, int64_t n) {
     if (n == 0) return;
     const int64_t nb = live_body_hw(c);
     if (nb > 0 && c->chunks.high_water() > 0) {
+        prepare_launches(c);
         HIPCHK(launch_set_left(c->bleft[c->slot], c->b_chunk0, nb, n, c->stream));
         run_launches(c, n, full_range(c));
     }
@@ -584,15 +736,35 @@ void do_step_bodies(titan_ctx *c, const int32_t *ids, const int64_t *steps, int3
     }
     maybe_inject(c);
     if (n == 0) return;
+    prepare_launches(c);
     if (count == 1) {
         // the budget travels as a kernel argument: nothing to keep alive
         HIPCHK(launch_set_i64(c->bleft[c->slot] + ids[0], left[ids[0]], c->stream));
         run_launches(c, n, body_range(c->bodies[ids[0]]));
     } else {
-        c->left_h = left;
-        h2d(c, c->bleft[c->slot], c->left_h.data(), (int64_t)left.size());
+        // budgets go through one of two pinned buffers, so the call returns
+        // once the steps are enqueued; a buffer is rewritten only after the
+        // copy that last read it has run
+        const int64_t nb = (int64_t)left.size();
+        if (nb > c->left_cap) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (int k = 0; k < 2; ++k) {
+                if (c->left_pin[k]) (void)hipHostFree(c->left_pin[k]);
+                c->left_pin[k] = nullptr;
+            }
+            const int64_t cap = std::max<int64_t>(nb, 2 * c->left_cap);
+            for (int k = 0; k < 2; ++k)
+                HIPCHK(hipHostMalloc((void **)&c->left_pin[k], sizeof(int64_t) * cap, hipHostMallocDefault));
+            c->left_cap = cap;
+        }
+        const int k = c->left_k;
+        c->left_k ^= 1;
+        HIPCHK(hipEventSynchronize(c->left_ev[k]));
+        std::copy(left.begin(), left.end(), c->left_pin[k]);
+        HIPCHK(hipMemcpyAsync(c->bleft[c->slot], c->left_pin[k], sizeof(int64_t) * nb, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipEventRecord(c->left_ev[k], c->stream));
         run_launches(c, n, full_range(c));
-        HIPCHK(hipStreamSynchronize(c->stream));  // the host copy of the budgets is released
     }
     for (int32_t k = 0; k < count; ++k) {
         BodyH &b = c->bodies[ids[k]];
@@ -958,7 +1130,6 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     b.n = 0;
     b.flags.assign(fl.begin(), fl.begin() + N);
     b.perm = std::move(perm);
-    drop_graphs(c);  // the launch grids cover the new chunks and tiles
     return (int32_t)bid;
 }
 
@@ -988,7 +1159,6 @@ void body_destroy(titan_ctx *c, int32_t id) {
     c->halos.release(b.halo0, b.nhalo);
     c->bids.release(id, 1);
     b = BodyH{};
-    drop_graphs(c);
 }
 
 // device order -> caller's order for a (N, K) host array read from K SoA rows
@@ -1141,6 +1311,7 @@ int titan_ctx_create(int device, uint64_t hbm_budget_bytes, titan_ctx **out) {
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
+    for (int k = 0; k < 2; ++k) HIPCHK(hipEventCreateWithFlags(&c->left_ev[k], hipEventDisableTiming));
     ensure_bodies(c, 64);
     c->planes = dalloc<double>(c, 8 * 16);
     c->planes_cap = 16;
@@ -1162,12 +1333,20 @@ int titan_ctx_create(int device, uint64_t hbm_budget_bytes, titan_ctx **out) {
     return TITAN_OK;
 }
 
+// serve.cpp: drop the context's request table
+extern "C" __attribute__((visibility("hidden"))) void titan_internal_serve_forget(const titan_ctx *ctx);
+
 int titan_ctx_destroy(titan_ctx *c) {
     if (!c) return TITAN_OK;
+    titan_internal_serve_forget(c);
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
     for (auto &a : c->allocs) (void)hipFree(a.first);
+    for (int k = 0; k < 2; ++k) {
+        if (c->left_ev[k]) (void)hipEventDestroy(c->left_ev[k]);
+        if (c->left_pin[k]) (void)hipHostFree(c->left_pin[k]);
+    }
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1233,6 +1412,7 @@ int titan_add_ball(titan_ctx *c, double cx, double cy, double cz, double r) {
     }
     h2d(c, c->balls, c->balls_h.data(), (int64_t)c->balls_h.size());
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->ball_grid_dirty = true;
     drop_graphs(c);
     API_END(c)
 }
@@ -1242,6 +1422,7 @@ int titan_clear_planes_and_balls(titan_ctx *c) {
     API_BEGIN
     c->planes_h.clear();
     c->balls_h.clear();
+    c->ball_grid_dirty = true;
     drop_graphs(c);
     API_END(c)
 }

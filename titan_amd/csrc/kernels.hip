@@ -170,9 +170,26 @@ __device__ __forceinline__ void mass_pass(const StepArgs &a, MassRegs &r, int64_
         const double *P = a.planes + 8 * q;
         contact_plane(r, P[0], P[1], P[2], P[3], P[4], P[5]);
     }
-    for (int q = 0; q < a.nballs; ++q) {                       // :1113-1115
-        const double *B = a.balls + 4 * q;
-        ball(r, B[0], B[1], B[2], B[3]);
+    if (a.bg_start) {                                          // :1113-1115
+        // only the balls listed for the mass's grid cell, in creation order:
+        // any other ball is farther than its radius from every point of the
+        // cell (with margins for rounding), so it would add a zero vector
+        const double fx = (r.px - a.bg_x0) * a.bg_inv_h;
+        const double fy = (r.py - a.bg_y0) * a.bg_inv_h;
+        const double fz = (r.pz - a.bg_z0) * a.bg_inv_h;
+        if (fx >= 0.0 && fx < (double)a.bg_nx && fy >= 0.0 && fy < (double)a.bg_ny && fz >= 0.0 &&
+            fz < (double)a.bg_nz) {
+            const int64_t cell = ((int64_t)(int)fz * a.bg_ny + (int)fy) * a.bg_nx + (int)fx;
+            for (int32_t q = a.bg_start[cell]; q < a.bg_start[cell + 1]; ++q) {
+                const double *B = a.balls + 4 * (int64_t)a.bg_list[q];
+                ball(r, B[0], B[1], B[2], B[3]);
+            }
+        }
+    } else {
+        for (int q = 0; q < a.nballs; ++q) {
+            const double *B = a.balls + 4 * q;
+            ball(r, B[0], B[1], B[2], B[3]);
+        }
     }
     int64_t lb, ln;                                            // :1118-1132
     if (fl & kLocalLC) {
@@ -290,12 +307,15 @@ __device__ __forceinline__ void add_spring(const StepArgs &a, MassRegs &r, const
     const double dvx = r.vx - nb[1].y, dvy = r.vy - nb[2].x, dvz = r.vz - nb[2].y;
     double rl = cur.rest[q];
     if (ACT) {
-        // evaluated for every lane (a lane without an actuator reads
-        // actuator 0 and keeps rl), so the group stays free of branches
+        // skipped when no lane of the wave has an actuator here (a uniform
+        // branch); otherwise evaluated for every lane (a lane without an
+        // actuator reads actuator 0 and keeps rl), free of divergent branches
         const int32_t ai = cur.act[q];
-        const double *A = a.act + 4 * (int64_t)(ai >= 0 ? ai : 0);
-        const double ra = rl * (1 + A[0] * sin(A[1] * T + A[2]));
-        rl = ai >= 0 ? ra : rl;
+        if (__any(ai >= 0)) {
+            const double *A = a.act + 4 * (int64_t)(ai >= 0 ? ai : 0);
+            const double ra = rl * (1 + A[0] * sin(A[1] * T + A[2]));
+            rl = ai >= 0 ? ra : rl;
+        }
     }
     const double k = UNI ? a.k_uni : cur.k[q];
     const double dmp = UNI ? a.damp_uni : cur.damp[q];

@@ -73,7 +73,9 @@ struct LocalC {
     int32_t pad;
 };
 
-constexpr int64_t kMaxMassSlots = (int64_t)1 << 26;  // 48 * slots < 4 GiB per buffer
+constexpr int64_t kMaxMassSlots = (int64_t)1 << 26;
+constexpr int kBallGridMin = 8;               // balls from which a cell grid culls them
+constexpr int64_t kBallGridMaxCells = 1 << 21;  // 48 * slots < 4 GiB per buffer
 
 struct StepArgs {
     // mass arena
@@ -112,6 +114,12 @@ struct StepArgs {
     int32_t nplanes;
     const double *balls;      // 4 per ball: cx cy cz r
     int32_t nballs;
+    // uniform grid over the balls (null below kBallGridMin balls): cell
+    // (ix, iy, iz) lists, in creation order, the balls that can touch it
+    const int32_t *bg_start;  // nx*ny*nz + 1 offsets into bg_list
+    const int32_t *bg_list;
+    double bg_x0, bg_y0, bg_z0, bg_inv_h;
+    int32_t bg_nx, bg_ny, bg_nz;
     double gx, gy, gz, dt;
     int32_t slot;
     // tiled path

@@ -1,0 +1,355 @@
+// Serving layer: continuous batching of rollout requests over the C ABI.
+//
+// Requests are bodies with a horizon.  The host loop calls titan_serve_pump;
+// each pump admits queued requests in arrival order while they fit the
+// running-set limits, steps all running requests together through
+// titan_step_bodies (one launch per step covers every body; each body stops
+// at its own horizon), and retires the finished ones.  Nothing here runs on
+// its own thread.  The engine keeps every body's arithmetic independent of
+// the others, so a request's result is the same alone and in any mix.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "titan_mi355x.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double now_s() {
+    return std::chrono::duration<double>(Clock::now().time_since_epoch()).count();
+}
+
+template <class T>
+std::vector<T> copy_of(const T *p, int64_t n) {
+    return p && n > 0 ? std::vector<T>(p, p + n) : std::vector<T>();
+}
+template <class T>
+const T *ptr_or_null(const std::vector<T> &v) {
+    return v.empty() ? nullptr : v.data();
+}
+
+// A request with its own copy of the body's arrays.
+struct Request {
+    int64_t ticket = 0;
+    int32_t state = TITAN_REQ_QUEUED;
+    int64_t n = 0, s = 0, horizon = 0, done = 0;
+    int32_t body = -1;
+    double t_submit = 0, t_done = 0;
+    uint64_t est_bytes = 0;
+    double *out_pos = nullptr, *out_vel = nullptr;
+    titan_body_desc desc{};
+    std::vector<double> pos, vel, m, T, ext, k, rest, damping, omega, act_a, act_phi, lc_vec, lc_num;
+    std::vector<uint8_t> fixed;
+    std::vector<int32_t> drag, left, right, type, lc_type;
+    std::vector<int64_t> lc_ptr;
+
+    void copy_from(const titan_body_desc &d) {
+        n = d.n_masses;
+        s = d.n_springs;
+        pos = copy_of(d.pos, 3 * n);
+        vel = copy_of(d.vel, 3 * n);
+        m = copy_of(d.m, n);
+        T = copy_of(d.T, n);
+        ext = copy_of(d.ext, 3 * n);
+        fixed = copy_of(d.fixed, n);
+        drag = copy_of(d.drag, n);
+        left = copy_of(d.left, s);
+        right = copy_of(d.right, s);
+        k = copy_of(d.k, s);
+        rest = copy_of(d.rest, s);
+        damping = copy_of(d.damping, s);
+        type = copy_of(d.type, s);
+        omega = copy_of(d.omega, s);
+        act_a = copy_of(d.act_a, s);
+        act_phi = copy_of(d.act_phi, s);
+        lc_ptr = copy_of(d.lc_ptr, d.lc_ptr ? n + 1 : 0);
+        const int64_t nl = lc_ptr.empty() ? 0 : lc_ptr[n];
+        lc_type = copy_of(d.lc_type, nl);
+        lc_vec = copy_of(d.lc_vec, 3 * nl);
+        lc_num = copy_of(d.lc_num, nl);
+        desc = d;
+        desc.pos = ptr_or_null(pos);
+        desc.vel = ptr_or_null(vel);
+        desc.m = ptr_or_null(m);
+        desc.T = ptr_or_null(T);
+        desc.ext = ptr_or_null(ext);
+        desc.fixed = ptr_or_null(fixed);
+        desc.drag = ptr_or_null(drag);
+        desc.left = ptr_or_null(left);
+        desc.right = ptr_or_null(right);
+        desc.k = ptr_or_null(k);
+        desc.rest = ptr_or_null(rest);
+        desc.damping = ptr_or_null(damping);
+        desc.type = ptr_or_null(type);
+        desc.omega = ptr_or_null(omega);
+        desc.act_a = ptr_or_null(act_a);
+        desc.act_phi = ptr_or_null(act_phi);
+        desc.lc_ptr = ptr_or_null(lc_ptr);
+        desc.lc_type = ptr_or_null(lc_type);
+        desc.lc_vec = ptr_or_null(lc_vec);
+        desc.lc_num = ptr_or_null(lc_num);
+    }
+
+    void release_arrays() {
+        for (auto *v : {&pos, &vel, &m, &T, &ext, &k, &rest, &damping, &omega, &act_a, &act_phi, &lc_vec, &lc_num})
+            std::vector<double>().swap(*v);
+        std::vector<uint8_t>().swap(fixed);
+        for (auto *v : {&drag, &left, &right, &type, &lc_type}) std::vector<int32_t>().swap(*v);
+        std::vector<int64_t>().swap(lc_ptr);
+    }
+};
+
+// Device bytes a body needs, estimated from its size: state, side arrays and
+// incidences with their SELL padding and tile halos, rounded up.
+uint64_t estimate_bytes(int64_t n, int64_t s) {
+    return (uint64_t)(256 * n + 96 * s + (1 << 16));
+}
+
+struct Server {
+    int32_t max_running = 4096;
+    uint64_t hbm_bytes = 0;
+    int64_t quantum = 64;
+    int64_t next_ticket = 1;
+    std::map<int64_t, std::unique_ptr<Request>> reqs;
+    std::deque<Request *> queue;
+    std::vector<Request *> running;
+    uint64_t running_bytes = 0;
+    // statistics
+    int64_t submitted = 0, completed = 0, cancelled = 0, failed = 0;
+    std::vector<double> latencies;
+    double mass_steps = 0, first_submit = -1, last_done = -1;
+};
+
+std::mutex g_mu;
+std::map<const titan_ctx *, std::unique_ptr<Server>> g_servers;
+
+Server &server_of(titan_ctx *ctx) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto &p = g_servers[ctx];
+    if (!p) p.reset(new Server);
+    return *p;
+}
+
+void finish(Server &sv, Request *r, int32_t state, double t) {
+    r->state = state;
+    r->t_done = t;
+    r->release_arrays();
+    if (state == TITAN_REQ_DONE) {
+        sv.completed++;
+        sv.latencies.push_back(t - r->t_submit);
+        sv.mass_steps += (double)r->n * (double)r->horizon;
+        sv.last_done = t;
+    } else if (state == TITAN_REQ_CANCELLED) {
+        sv.cancelled++;
+    } else {
+        sv.failed++;
+    }
+}
+
+int retire(titan_ctx *ctx, Server &sv, Request *r) {
+    if (r->out_pos || r->out_vel) {
+        const int rc = titan_body_download(ctx, r->body, r->out_pos, r->out_vel, nullptr, nullptr);
+        if (rc != TITAN_OK) return rc;
+    }
+    const int rc = titan_body_destroy(ctx, r->body);
+    if (rc != TITAN_OK) return rc;
+    r->body = -1;
+    sv.running_bytes -= r->est_bytes;
+    finish(sv, r, TITAN_REQ_DONE, now_s());
+    return TITAN_OK;
+}
+
+}  // namespace
+
+// Called by titan_ctx_destroy (internal to the library).
+extern "C" __attribute__((visibility("hidden"))) void titan_internal_serve_forget(const titan_ctx *ctx) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_servers.erase(ctx);
+}
+
+int titan_serve_config(titan_ctx *ctx, int32_t max_running, uint64_t hbm_bytes, int64_t quantum) {
+    if (!ctx || max_running < 1 || quantum < 1) return TITAN_E_INVALID;
+    Server &sv = server_of(ctx);
+    sv.max_running = max_running;
+    sv.hbm_bytes = hbm_bytes;
+    sv.quantum = quantum;
+    return TITAN_OK;
+}
+
+int titan_serve_submit(titan_ctx *ctx, const titan_request *req, int64_t *ticket) {
+    if (!ctx || !req || req->horizon < 0 || req->body.n_masses <= 0 || !req->body.pos || !req->body.m)
+        return TITAN_E_INVALID;
+    Server &sv = server_of(ctx);
+    std::unique_ptr<Request> r(new Request);
+    try {
+        r->copy_from(req->body);
+    } catch (const std::bad_alloc &) {
+        return TITAN_E_NOMEM;
+    }
+    r->ticket = sv.next_ticket++;
+    r->horizon = req->horizon;
+    r->out_pos = req->out_pos;
+    r->out_vel = req->out_vel;
+    r->t_submit = now_s();
+    r->est_bytes = estimate_bytes(r->n, r->s);
+    if (sv.first_submit < 0) sv.first_submit = r->t_submit;
+    sv.submitted++;
+    sv.queue.push_back(r.get());
+    if (ticket) *ticket = r->ticket;
+    sv.reqs[r->ticket] = std::move(r);
+    return TITAN_OK;
+}
+
+int titan_serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
+    if (!ctx) return TITAN_E_INVALID;
+    Server &sv = server_of(ctx);
+    std::vector<Request *> fresh;  // admitted during this pump's quantum: they join the next one
+    // admit, in arrival order, while the running set has room
+    auto admit = [&]() -> int {
+        while (!sv.queue.empty() && (int32_t)(sv.running.size() + fresh.size()) < sv.max_running) {
+            Request *r = sv.queue.front();
+            const bool busy = !sv.running.empty() || !fresh.empty();
+            if (sv.hbm_bytes && busy && sv.running_bytes + r->est_bytes > sv.hbm_bytes) break;
+            int32_t id = -1;
+            const int rc = titan_body_create(ctx, &r->desc, &id);
+            if (rc == TITAN_E_NOMEM && busy) break;  // wait for memory to free up
+            sv.queue.pop_front();
+            if (rc != TITAN_OK) {  // the body is invalid or can never fit: it fails alone
+                finish(sv, r, TITAN_REQ_FAILED, now_s());
+                continue;
+            }
+            r->body = id;
+            r->state = TITAN_REQ_RUNNING;
+            sv.running_bytes += r->est_bytes;
+            if (r->horizon == 0) {
+                const int rr = retire(ctx, sv, r);
+                if (rr != TITAN_OK) return rr;
+                continue;
+            }
+            fresh.push_back(r);
+        }
+        return TITAN_OK;
+    };
+    int rc = TITAN_OK;
+    if (sv.running.empty()) {  // nothing on the device: this pump steps the new arrivals
+        rc = admit();
+        if (rc != TITAN_OK) return rc;
+        sv.running.swap(fresh);
+    }
+    // step every running request together, each up to the quantum and its
+    // horizon; the call returns once the launches are enqueued
+    if (!sv.running.empty()) {
+        std::vector<int32_t> ids;
+        std::vector<int64_t> steps;
+        for (Request *r : sv.running) {
+            ids.push_back(r->body);
+            steps.push_back(std::min(sv.quantum, r->horizon - r->done));
+        }
+        rc = titan_step_bodies(ctx, ids.data(), steps.data(), (int32_t)ids.size());
+        if (rc != TITAN_OK) return rc;
+        for (size_t q = 0; q < ids.size(); ++q) sv.running[q]->done += steps[q];
+    }
+    // admit more while the device works (their layouts are built on the host meanwhile)
+    rc = admit();
+    if (rc != TITAN_OK) return rc;
+    // retire the finished ones (their downloads wait for the quantum)
+    std::vector<Request *> still;
+    for (Request *r : sv.running) {
+        if (r->done < r->horizon) {
+            still.push_back(r);
+            continue;
+        }
+        rc = retire(ctx, sv, r);
+        if (rc != TITAN_OK) return rc;
+    }
+    still.insert(still.end(), fresh.begin(), fresh.end());
+    sv.running.swap(still);
+    if (n_running) *n_running = (int32_t)sv.running.size();
+    if (n_queued) *n_queued = (int32_t)sv.queue.size();
+    return TITAN_OK;
+}
+
+int titan_serve_poll(titan_ctx *ctx, int64_t ticket, int32_t *state, int64_t *steps_done) {
+    if (!ctx) return TITAN_E_INVALID;
+    Server &sv = server_of(ctx);
+    auto it = sv.reqs.find(ticket);
+    if (it == sv.reqs.end()) return TITAN_E_NOTFOUND;
+    if (state) *state = it->second->state;
+    if (steps_done) *steps_done = it->second->done;
+    return TITAN_OK;
+}
+
+int titan_serve_cancel(titan_ctx *ctx, int64_t ticket) {
+    if (!ctx) return TITAN_E_INVALID;
+    Server &sv = server_of(ctx);
+    auto it = sv.reqs.find(ticket);
+    if (it == sv.reqs.end()) return TITAN_E_NOTFOUND;
+    Request *r = it->second.get();
+    if (r->state == TITAN_REQ_QUEUED) {
+        sv.queue.erase(std::find(sv.queue.begin(), sv.queue.end(), r));
+    } else if (r->state == TITAN_REQ_RUNNING) {
+        const int rc = titan_body_destroy(ctx, r->body);
+        if (rc != TITAN_OK) return rc;
+        r->body = -1;
+        sv.running_bytes -= r->est_bytes;
+        sv.running.erase(std::find(sv.running.begin(), sv.running.end(), r));
+    } else {
+        return TITAN_OK;  // already finished
+    }
+    finish(sv, r, TITAN_REQ_CANCELLED, now_s());
+    return TITAN_OK;
+}
+
+int titan_serve_stats(titan_ctx *ctx, titan_serve_report *out) {
+    if (!ctx || !out) return TITAN_E_INVALID;
+    Server &sv = server_of(ctx);
+    std::memset(out, 0, sizeof(*out));
+    out->submitted = sv.submitted;
+    out->completed = sv.completed;
+    out->cancelled = sv.cancelled;
+    out->failed = sv.failed;
+    out->running = (int64_t)sv.running.size();
+    out->queued = (int64_t)sv.queue.size();
+    out->mass_steps = sv.mass_steps;
+    if (!sv.latencies.empty()) {
+        std::vector<double> l = sv.latencies;
+        std::sort(l.begin(), l.end());
+        // nearest-rank percentiles
+        auto pct = [&](double p) {
+            const size_t r = (size_t)std::ceil(p / 100.0 * (double)l.size());
+            return l[std::min(l.size() - 1, r == 0 ? 0 : r - 1)];
+        };
+        out->p50_latency_s = pct(50);
+        out->p99_latency_s = pct(99);
+        out->max_latency_s = l.back();
+    }
+    if (sv.last_done > sv.first_submit && sv.first_submit >= 0)
+        out->mass_steps_per_s = sv.mass_steps / (sv.last_done - sv.first_submit);
+    return TITAN_OK;
+}
+
+int titan_serve_reset_stats(titan_ctx *ctx) {
+    if (!ctx) return TITAN_E_INVALID;
+    Server &sv = server_of(ctx);
+    for (auto it = sv.reqs.begin(); it != sv.reqs.end();) {
+        const int32_t st = it->second->state;
+        if (st == TITAN_REQ_DONE || st == TITAN_REQ_CANCELLED || st == TITAN_REQ_FAILED) it = sv.reqs.erase(it);
+        else ++it;
+    }
+    sv.submitted = (int64_t)sv.reqs.size();
+    sv.completed = sv.cancelled = sv.failed = 0;
+    sv.latencies.clear();
+    sv.mass_steps = 0;
+    sv.first_submit = sv.reqs.empty() ? -1 : now_s();
+    sv.last_done = -1;
+    return TITAN_OK;
+}

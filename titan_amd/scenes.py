@@ -128,3 +128,63 @@ def robot_population(n_bodies: int = 4096, side_lo: int = 3, side_hi: int = 12, 
         b["lc_num"] = np.zeros(n)
         bodies.append(b)
     return Scene(bodies=bodies, dt=dt, name=f"robots{n_bodies}_seed{seed}")
+
+
+def knn_cloud(n: int, *, k_nn: int = 12, box: float = 10.0, seed: int = 0, k_lo: float = 1e3,
+              k_hi: float = 1e5, damping: float = 0.05, m: float = DEFAULT_MASS, z0: float = 0.0):
+    """A seeded uniform point cloud of n masses in a box (side `box`, bottom at
+    z0) joined by springs to each point's k_nn nearest neighbours, deduplicated
+    (pair i < j once, in ascending (i, j) order), so degrees vary; rest =
+    initial distance, k ~ LogUniform(k_lo, k_hi), damping constant.  Returns
+    the body dict."""
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(0.0, box, size=(n, 3))
+    pos[:, 0] -= box / 2
+    pos[:, 1] -= box / 2
+    pos[:, 2] += z0
+    kk = min(k_nn, n - 1)
+    _, nbr = cKDTree(pos).query(pos, k=kk + 1)
+    i = np.repeat(np.arange(n), kk)
+    j = nbr[:, 1:].reshape(-1)
+    a, b = np.minimum(i, j), np.maximum(i, j)
+    pairs = np.unique(a.astype(np.int64) * n + b)
+    left = (pairs // n).astype(np.int32)
+    right = (pairs % n).astype(np.int32)
+    rest = rest_lengths(pos, left, right)
+    ks = np.exp(rng.uniform(np.log(k_lo), np.log(k_hi), size=len(left)))
+    return dict(pos=pos, m=np.full(n, m), left=left, right=right, k=ks, rest=rest,
+                damping=np.full(len(left), damping))
+
+
+def actuate(body: dict, frac: float, rng) -> dict:
+    """Make a fraction of a body's springs actuated with the general law
+    rest * (1 + a sin(omega t + phi)), a ~ U(0, 0.2), omega ~ U(1, 20), phi ~ U(0, 2 pi)."""
+    ns = len(body["left"])
+    act = rng.random(ns) < frac
+    body["type"] = np.where(act, 4, 0).astype(np.int32)
+    body["omega"] = np.where(act, rng.uniform(1.0, 20.0, ns), 0.0)
+    body["act_a"] = np.where(act, rng.uniform(0.0, 0.2, ns), 0.0)
+    body["act_phi"] = np.where(act, rng.uniform(0.0, 2 * np.pi, ns), 0.0)
+    return body
+
+
+def serving_requests(n: int, seed: int = 0, side_lo: int = 3, side_hi: int = 40, knn_lo: float = 1e2,
+                     knn_hi: float = 1e5, h_lo: int = 100, h_hi: int = 10000):
+    """Rollout requests: each a seeded body, half lattices of side U{side_lo..side_hi}
+    (1 m cubes, damping 0.1) and half kNN-12 clouds of LogUniform(knn_lo, knn_hi)
+    masses (1 m box), with 10-30% of springs actuated and a horizon of
+    U{h_lo..h_hi} steps.  Returns [(body, horizon)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(n):
+        if rng.random() < 0.5:
+            s = int(rng.integers(side_lo, side_hi + 1))
+            b = lattice_body((0.0, 0.0, 0.6), (1.0, 1.0, 1.0), s, damping=0.1)
+        else:
+            nm = int(np.exp(rng.uniform(np.log(knn_lo), np.log(knn_hi))))
+            b = knn_cloud(nm, box=1.0, seed=int(rng.integers(1 << 30)), z0=0.1)
+        actuate(b, rng.uniform(0.1, 0.3), rng)
+        out.append((b, int(rng.integers(h_lo, h_hi + 1))))
+    return out
