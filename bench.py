@@ -17,7 +17,8 @@ bounded sample of the same workload.  Unless --no-extra, the same line also
 carries "batched": a 4096-body robot population stepped together against the
 same GPU stepping one body per launch, and "serving": Poisson streams of
 rollout requests served with continuous batching (p50/p99 latency and
-mass-steps/s per arrival rate) against serving them one at a time.
+mass-steps/s per arrival rate) against serving them one at a time, and
+"irregular": a 2M-mass kNN-12 cloud with 1000 contact balls.
 """
 from __future__ import annotations
 
@@ -258,6 +259,47 @@ def bench_serving(seed: int, n_req: int, lams, quantum: int, n_base: int) -> dic
     }
 
 
+def bench_irregular(n: int, n_balls: int, seed: int, steps: int, warmup: int) -> dict:
+    """Irregular spring graph: a 2M-mass kNN-12 cloud (variable degree,
+    per-spring k), 1000 contact balls and a ground plane, dt 1e-5, FP64; one
+    launch per step over the whole cloud."""
+    import numpy as np
+    from titan_amd import Context, scenes
+
+    t0 = time.perf_counter()
+    sc = scenes.knn_scene(n, n_balls=n_balls, seed=seed)
+    log(f"irregular: {sc.n_masses} masses, {sc.n_springs} springs, {n_balls} balls "
+        f"({time.perf_counter() - t0:.1f} s to build)")
+    with Context() as ctx:
+        t0 = time.perf_counter()
+        (bid,) = sc.load(ctx)
+        ctx.sync()
+        log(f"irregular: uploaded in {time.perf_counter() - t0:.1f} s, device bytes {ctx.mem_used()}, "
+            f"kernel {ctx.kernel_name()}")
+        ctx.step(warmup)
+        ctx.sync()
+        ctx.timer_start()
+        ctx.step(steps)
+        ms = ctx.timer_stop()
+        st = ctx.download(bid)
+        kernel = ctx.kernel_name()
+    finite = bool(np.isfinite(st["pos"]).all() and np.isfinite(st["vel"]).all())
+    step_s = ms * 1e-3 / steps
+    bytes_step = sc.algorithmic_bytes_per_step()
+    achieved = bytes_step / step_s / 1e9
+    log(f"irregular: {steps} steps {ms:.1f} ms, {step_s * 1e6:.1f} us/step, "
+        f"{sc.n_masses * steps / (ms * 1e-3):.3e} mass-steps/s, {achieved / HBM_PEAK_GBS:.3f} of HBM peak")
+    return {
+        "workload": f"kNN-12 cloud of {sc.n_masses} masses (seed {seed}) in a 10 m box, {sc.n_springs} springs "
+                    f"after dedup, k~LogUniform(1e3,1e5), damping 0.05, {n_balls} contact balls r~U(0.1,0.5), "
+                    f"ground plane with friction, dt=1e-5, FP64, {steps} steps after {warmup}",
+        "value": sc.n_masses * steps / (ms * 1e-3), "unit": "mass-steps/s", "us_per_step": step_s * 1e6,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "bytes_per_launch": bytes_step, "kernel": kernel},
+        "finite_after_run": finite,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,7 +312,7 @@ def main() -> int:
                     help="device layout of the body (results are identical; speed differs)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the batched, serving and irregular-graph workloads")
-    ap.add_argument("--only", choices=["lattice", "batched", "serving"], default=None,
+    ap.add_argument("--only", choices=["lattice", "batched", "serving", "irregular"], default=None,
                     help="run one workload (its figures go in the same JSON line)")
     ap.add_argument("--bodies", type=int, default=4096, help="batched workload: robot count")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -310,7 +352,9 @@ def main() -> int:
     if args.only == "serving" or (args.only is None and not args.no_extra):
         extra["serving"] = bench_serving(args.seed, n_req=64, lams=(50, 200, 1000, 2000), quantum=64,
                                          n_base=8)
-    if args.only in ("batched", "serving"):
+    if args.only == "irregular" or (args.only is None and not args.no_extra):
+        extra["irregular"] = bench_irregular(2_000_000, 1000, args.seed, steps=500, warmup=20)
+    if args.only in ("batched", "serving", "irregular"):
         print(json.dumps(extra), flush=True)
         return 0
     out = bench_lattice(args)

@@ -102,7 +102,7 @@ static void check(int64_t n, const std::vector<int32_t> &L, const std::vector<in
     const std::vector<int32_t> noact;
     // gather layout: arena slots
     SellBuild sg;
-    build_sell(n, sp, kd, nullptr, 4096, sg);
+    build_sell(n, sp, kd, kd, nullptr, 4096, sg);
     check_sell(n, L, R, k, rest, damp, kd ? act : noact, kd, nullptr, 4096, sg);
     // tiled layout: tiles, halos and LDS rows, under two row budgets
     for (int rows : {kTileRows, 682}) {
@@ -138,7 +138,7 @@ static void check(int64_t n, const std::vector<int32_t> &L, const std::vector<in
     CHECK(tb.rmax == rmax);
     CHECK(tb.hmax == hmax);
     SellBuild st;
-    build_sell(n, sp, kd, &tb, 0, st);
+    build_sell(n, sp, kd, kd, &tb, 0, st);
     check_sell(n, L, R, k, rest, damp, kd ? act : noact, kd, &tb, 0, st);
     }
 }

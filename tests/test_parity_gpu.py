@@ -298,3 +298,33 @@ def test_many_balls_culled_by_grid_bitexact(layout):
     # the balls act: the run differs from the same scene without them
     free, _ = run_engine(scenes.Scene(bodies=[b], planes=sc.planes, dt=1e-4), 200, layout=layout)
     assert not np.array_equal(free["pos"], eng["pos"])
+
+
+@LAYOUTS
+def test_knn_cloud_with_balls_bitexact(layout):
+    # irregular degrees, per-spring k, balls through the cloud and the ground plane
+    sc = scenes.knn_scene(3000, n_balls=40, seed=2, dt=1e-5)
+    _say(f"case kNN cloud {sc.n_masses} masses, {sc.n_springs} springs, 40 balls, 300 steps, layout {layout}")
+    degs = np.bincount(np.concatenate([sc.bodies[0]["left"], sc.bodies[0]["right"]]))
+    assert degs.min() < degs.max()
+    eng, _ = run_engine(sc, 300, layout=layout)
+    orc = run_oracle(sc, 300)
+    assert_bitexact(eng, orc)
+
+
+@LAYOUTS
+def test_per_body_constants_in_one_context_bitexact(layout):
+    # four bodies: k and damping shared or per spring, in every combination
+    rng = np.random.default_rng(4)
+    bodies = []
+    for q, (pk, pd) in enumerate([(False, False), (True, False), (False, True), (True, True)]):
+        b = scenes.lattice_body((3.0 * q, 0.0, 2.4), (2.0, 2.0, 2.0), 7, damping=0.1)
+        s = len(b["left"])
+        b["k"] = rng.uniform(5e3, 2e4, s) if pk else np.full(s, 1e4 + 500.0 * q)
+        b["damping"] = rng.uniform(0.0, 0.3, s) if pd else np.full(s, 0.05 * q)
+        bodies.append(b)
+    sc = scenes.Scene(bodies=bodies, planes=[(0.0, 0.0, 1.0, 0.0, 0.8, 1.0)], dt=1e-4)
+    _say(f"case per-body constants, 4 bodies, 300 steps, layout {layout}")
+    eng, _ = run_engine(sc, 300, layout=layout)
+    orc = run_oracle(sc, 300)
+    assert_bitexact(eng, orc)

@@ -150,6 +150,8 @@ struct titan_ctx {
     double *bT[2] = {nullptr, nullptr};
     int64_t *bn[2] = {nullptr, nullptr}, *bleft[2] = {nullptr, nullptr}, *bnsync = nullptr;
     int32_t *b_chunk0 = nullptr, *b_lc = nullptr, *b_lcn = nullptr;
+    double *b_k = nullptr, *b_d = nullptr;  // the body's k / damping when all its springs share it
+    uint8_t *b_kd = nullptr;                 // bit 0: k per incidence, bit 1: damping per incidence
     double *planes = nullptr, *balls = nullptr;
     int64_t planes_cap = 0, balls_cap = 0;
     // ball grid (rebuilt before the next launch after the balls change)
@@ -158,9 +160,7 @@ struct titan_ctx {
     double bg_x0 = 0, bg_y0 = 0, bg_z0 = 0, bg_inv_h = 0;
     int32_t bg_n[3] = {0, 0, 0};
 
-    bool kd_known = false, uniform_kd = true;
     bool has_act = false;  // some spring is actuated: launch the actuated variants
-    double k_uni = 0, damp_uni = 0;
     int slot = 0;
 
     // captured multi-step graphs, valid while `version` is unchanged, keyed
@@ -298,7 +298,7 @@ void ensure_incs(titan_ctx *c, int64_t need) {
     if (c->incs.cap() >= need) return;
     const int64_t old = c->incs.cap();
     const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 4096));
-    check_growth(c, (size_t)nc * (4 + 8 + (c->inc_k ? 16 : 0) + (c->inc_act ? 4 : 0)));
+    check_growth(c, (size_t)nc * (4 + 8 + (c->inc_k ? 8 : 0) + (c->inc_damp ? 8 : 0) + (c->inc_act ? 4 : 0)));
     grow_soa(c, c->inc_nbr, 1, old, nc);
     grow_soa(c, c->inc_rest, 1, old, nc);
     if (c->inc_k) grow_soa(c, c->inc_k, 1, old, nc);
@@ -337,7 +337,7 @@ void ensure_bodies(titan_ctx *c, int64_t need) {
     if (c->bcap >= need) return;
     const int64_t old = c->bcap;
     const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 64));
-    check_growth(c, (size_t)nc * (2 * (8 + 8 + 8) + 8 + 4 + 4 + 1 + 4));
+    check_growth(c, (size_t)nc * (2 * (8 + 8 + 8) + 8 + 4 + 4 + 1 + 4 + 8 + 8 + 1));
     for (int s = 0; s < 2; ++s) {
         grow_soa(c, c->bT[s], 1, old, nc);
         grow_soa(c, c->bn[s], 1, old, nc);
@@ -347,6 +347,9 @@ void ensure_bodies(titan_ctx *c, int64_t need) {
     grow_soa(c, c->b_lc, 1, old, nc);
     grow_soa(c, c->b_lcn, 1, old, nc);
     grow_soa(c, c->b_tiled, 1, old, nc);
+    grow_soa(c, c->b_k, 1, old, nc);
+    grow_soa(c, c->b_d, 1, old, nc);
+    grow_soa(c, c->b_kd, 1, old, nc);
     int32_t *q = dalloc<int32_t>(c, nc);
     HIPCHK(hipMemsetAsync(q, 0xff, sizeof(int32_t) * nc, c->stream));
     if (c->b_chunk0) HIPCHK(hipMemcpyAsync(q, c->b_chunk0, sizeof(int32_t) * old, hipMemcpyDeviceToDevice, c->stream));
@@ -385,18 +388,13 @@ void ensure_halos(titan_ctx *c, int64_t need) {
     drop_graphs(c);
 }
 
-// Switch from by-value spring constants to per-incidence arrays.
-void make_per_incidence_kd(titan_ctx *c) {
-    if (!c->uniform_kd) return;
-    // at least one entry, so that the arrays exist (and grow with their
-    // arena) even while the arena is empty
+// Create the per-incidence spring constant (or damping) array the first time
+// a body needs one.  At least one entry, so that it exists (and grows with
+// the arena) even while the arena is empty.
+void make_inc_array(titan_ctx *c, double *&p) {
+    if (p) return;
     const int64_t n = std::max<int64_t>(c->incs.cap(), 1);
-    check_growth(c, (size_t)n * 16);
-    c->inc_k = dalloc<double>(c, n);
-    c->inc_damp = dalloc<double>(c, n);
-    HIPCHK(launch_fill_f64(c->inc_k, n, c->k_uni, c->stream));
-    HIPCHK(launch_fill_f64(c->inc_damp, n, c->damp_uni, c->stream));
-    c->uniform_kd = false;
+    p = dalloc<double>(c, n);
     drop_graphs(c);
 }
 
@@ -484,10 +482,11 @@ StepArgs make_args(titan_ctx *c, int slot, const Range &rg) {
     a.inc_rest = c->inc_rest;
     a.inc_k = c->inc_k;
     a.inc_damp = c->inc_damp;
+    a.b_k = c->b_k;
+    a.b_d = c->b_d;
+    a.b_kd = c->b_kd;
     a.inc_act = c->inc_act;
     a.act = c->act;
-    a.k_uni = c->k_uni;
-    a.damp_uni = c->damp_uni;
     for (int s = 0; s < 2; ++s) {
         a.bT[s] = c->bT[s];
         a.bn[s] = c->bn[s];
@@ -531,10 +530,10 @@ hipError_t launch_range(titan_ctx *c, int slot, const Range &rg) {
     StepArgs a = make_args(c, slot, rg);
     const bool act = c->has_act;
     if (rg.gather) {
-        hipError_t e = launch_step(a, c->uniform_kd, act, c->stream);
+        hipError_t e = launch_step(a, act, c->stream);
         if (e != hipSuccess) return e;
     }
-    if (rg.tiled) return launch_step_tiles(a, c->uniform_kd, act, c->stream);
+    if (rg.tiled) return launch_step_tiles(a, act, c->stream);
     return hipSuccess;
 }
 
@@ -793,9 +792,9 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     const int64_t nch = (N + kChunk - 1) / kChunk;
     const int64_t npad = nch * kChunk;
 
-    // ---- springs: validation, uniform constants, actuators (caller's order)
+    // ---- springs: validation, shared constants, actuators (caller's order)
     int64_t n_act = 0;
-    bool kd_uni = true, first = true;
+    bool k_same = true, d_same = true, first = true;
     double k0 = 0, d0 = 0;
     std::vector<int32_t> act_local(S > 0 ? S : 0, -1);
     std::vector<double> acth;
@@ -823,16 +822,13 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
             k0 = ks;
             d0 = ds;
             first = false;
-        } else if (!same_bits(ks, k0) || !same_bits(ds, d0)) {
-            kd_uni = false;
+        } else {
+            k_same = k_same && same_bits(ks, k0);
+            d_same = d_same && same_bits(ds, d0);
         }
     }
-    const bool has_springs = !first;
-    // will the context still pass k and damping by value after this body?
-    const bool uniform_after =
-        !has_springs ? c->uniform_kd
-                     : (kd_uni && (!c->kd_known || (c->uniform_kd && same_bits(k0, c->k_uni) &&
-                                                    same_bits(d0, c->damp_uni))));
+    // constants every spring of the body shares travel by value
+    const bool per_k = !k_same, per_d = !d_same;
 
     // ---- local constraints, sorted stably into application order
     std::vector<std::vector<LocalC>> lists;
@@ -918,7 +914,7 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     // incidences of both layouts; the gather layout's name arena slots,
     // which are known once the chunks are placed (built below)
     SellBuild sb;
-    if (tiled) build_sell(N, sp, !uniform_after, &tb, 0, sb);
+    if (tiled) build_sell(N, sp, per_k, per_d, &tb, 0, sb);
 
     // ---- reserve space; on failure release whatever was taken
     const int64_t n_tiles = tiled ? (int64_t)tb.tiles.size() : 0;
@@ -937,7 +933,7 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
         if ((ch0 + nch) * kChunk > kMaxMassSlots)
             fail(TITAN_E_NOMEM, "body_create: mass arena limit (2^26 slots) reached");
         if (!tiled) {  // gather incidences name arena slots: build them now
-            build_sell(N, sp, !uniform_after, nullptr, ch0 * kChunk, sb);
+            build_sell(N, sp, per_k, per_d, nullptr, ch0 * kChunk, sb);
         }
         const int64_t n_inc_all = sb.coff[nch];
         inc0 = take(c->incs, n_inc_all, ensure_incs);
@@ -946,17 +942,8 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
         tl0 = take(c->tiles, n_tiles, ensure_tiles);
         hl0 = take(c->halos, n_halo, ensure_halos);
         bid = take(c->bids, 1, ensure_bodies);
-        if (has_springs) {
-            if (!c->kd_known) {
-                c->kd_known = true;
-                c->k_uni = k0;
-                c->damp_uni = d0;
-                c->uniform_kd = true;
-                if (!kd_uni) make_per_incidence_kd(c);
-            } else if (!uniform_after) {
-                make_per_incidence_kd(c);
-            }
-        }
+        if (per_k) make_inc_array(c, c->inc_k);
+        if (per_d) make_inc_array(c, c->inc_damp);
         if (n_act > 0) make_act(c);
     } catch (...) {
         if (ch0 >= 0) c->chunks.release(ch0, nch);
@@ -1077,10 +1064,8 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     if (n_inc > 0) {
         h2d(c, c->inc_nbr + inc0, sb.nbr.data(), n_inc);
         h2d(c, c->inc_rest + inc0, sb.rest.data(), n_inc);
-        if (!c->uniform_kd) {
-            h2d(c, c->inc_k + inc0, sb.k.data(), n_inc);
-            h2d(c, c->inc_damp + inc0, sb.damp.data(), n_inc);
-        }
+        if (per_k) h2d(c, c->inc_k + inc0, sb.k.data(), n_inc);
+        if (per_d) h2d(c, c->inc_damp + inc0, sb.damp.data(), n_inc);
         if (c->inc_act) {
             if (!sb.act.empty()) h2d(c, c->inc_act + inc0, sb.act.data(), n_inc);
             else HIPCHK(hipMemsetAsync(c->inc_act + inc0, 0xff, sizeof(int32_t) * n_inc, c->stream));
@@ -1106,6 +1091,10 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     h2d(c, c->b_lc + bid, &lcb, 1);
     h2d(c, c->b_lcn + bid, &lcn, 1);
     h2d(c, c->b_tiled + bid, &tflag, 1);
+    const uint8_t kdf = (uint8_t)((per_k ? 1 : 0) | (per_d ? 2 : 0));
+    h2d(c, c->b_k + bid, &k0, 1);
+    h2d(c, c->b_d + bid, &d0, 1);
+    h2d(c, c->b_kd + bid, &kdf, 1);
     HIPCHK(hipStreamSynchronize(c->stream));
 
     BodyH &b = c->bodies[bid];
@@ -1654,8 +1643,8 @@ int titan_kernel_name(const titan_ctx *c, char *buf, int64_t len) {
     if (!buf || len <= 0) return TITAN_E_INVALID;
     const bool act = c->has_act;
     std::string n;
-    if (any_body(c, false)) n = step_kernel_name(c->uniform_kd, act);
-    if (any_body(c, true)) n += std::string(n.empty() ? "" : "+") + step_tiles_kernel_name(c->uniform_kd, act);
+    if (any_body(c, false)) n = step_kernel_name(act);
+    if (any_body(c, true)) n += std::string(n.empty() ? "" : "+") + step_tiles_kernel_name(act);
     std::snprintf(buf, (size_t)len, "%s", n.c_str());
     return TITAN_OK;
 }

@@ -7,10 +7,10 @@
 
 namespace titan_mi355x {
 
-hipError_t launch_step(const StepArgs &a, bool uniform_kd, bool actuated, hipStream_t s);
-const char *step_kernel_name(bool uniform_kd, bool actuated);
-hipError_t launch_step_tiles(const StepArgs &a, bool uniform_kd, bool actuated, hipStream_t s);
-const char *step_tiles_kernel_name(bool uniform_kd, bool actuated);
+hipError_t launch_step(const StepArgs &a, bool actuated, hipStream_t s);
+const char *step_kernel_name(bool actuated);
+hipError_t launch_step_tiles(const StepArgs &a, bool actuated, hipStream_t s);
+const char *step_tiles_kernel_name(bool actuated);
 size_t tiles_lds_bytes(int ldim);
 hipError_t set_tiles_lds_limit();
 hipError_t launch_advance_T(double *Tm, const uint32_t *mflags, const int32_t *chunk_body,
@@ -18,7 +18,6 @@ hipError_t launch_advance_T(double *Tm, const uint32_t *mflags, const int32_t *c
                             int64_t nb, double dt, hipStream_t s);
 hipError_t launch_set_left(int64_t *bleft, const int32_t *b_chunk0, int64_t nb, int64_t n,
                            hipStream_t s);
-hipError_t launch_fill_f64(double *p, int64_t n, double v, hipStream_t s);
 hipError_t launch_set_i64(int64_t *p, int64_t v, hipStream_t s);
 
 }  // namespace titan_mi355x

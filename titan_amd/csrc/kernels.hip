@@ -266,14 +266,23 @@ struct Names<2> {
     }
 };
 
+// The body's spring constant and damping: by value when every spring of
+// the body shares them, else per incidence (kd bit 0: k, bit 1: damping;
+// uniform over the wave, so the loads it selects need no divergent branch).
+struct BodyKD {
+    double k, d;
+    uint32_t kd;
+};
+
 // One group's incidence data in registers: the names, rest lengths,
-// constants (per-spring constants only) and actuator ids (actuated only).
-template <bool UNI, bool ACT, int G>
+// per-spring constants (bodies that have them) and actuator ids (actuated
+// only).
+template <bool ACT, int G>
 struct Group {
     Names<G> n;
     double rest[G], k[G], damp[G];
     int32_t act[G];
-    __device__ __forceinline__ void load(const StepArgs &a, int64_t ebase, int lane, int g) {
+    __device__ __forceinline__ void load(const StepArgs &a, int64_t ebase, int lane, int g, const BodyKD &bkd) {
         constexpr int P = G / 2;  // 16-byte pairs per group and lane
         n = Names<G>::load(a.inc_nbr, ebase, lane, g);
         const int64_t e2 = ebase / 2 + ((int64_t)g * kChunk + lane) * P;  // double2 index
@@ -282,13 +291,19 @@ struct Group {
             const double2 rr = reinterpret_cast<const double2 *>(a.inc_rest)[e2 + h];
             rest[2 * h] = rr.x;
             rest[2 * h + 1] = rr.y;
-            if (!UNI) {
+            if (bkd.kd & 1u) {
                 const double2 kk = reinterpret_cast<const double2 *>(a.inc_k)[e2 + h];
-                const double2 dd = reinterpret_cast<const double2 *>(a.inc_damp)[e2 + h];
                 k[2 * h] = kk.x;
                 k[2 * h + 1] = kk.y;
+            } else {
+                k[2 * h] = k[2 * h + 1] = bkd.k;
+            }
+            if (bkd.kd & 2u) {
+                const double2 dd = reinterpret_cast<const double2 *>(a.inc_damp)[e2 + h];
                 damp[2 * h] = dd.x;
                 damp[2 * h + 1] = dd.y;
+            } else {
+                damp[2 * h] = damp[2 * h + 1] = bkd.d;
             }
         }
         if (ACT) {
@@ -300,7 +315,7 @@ struct Group {
 };
 
 // Add -G of one incident spring (axis nrm, u) to the mass's force.
-template <bool UNI, bool ACT, class Grp>
+template <bool ACT, class Grp>
 __device__ __forceinline__ void add_spring(const StepArgs &a, MassRegs &r, const Grp &cur, int q, double T,
                                            const double2 (&nb)[3], double nrm, double ux, double uy,
                                            double uz) {
@@ -317,8 +332,8 @@ __device__ __forceinline__ void add_spring(const StepArgs &a, MassRegs &r, const
             rl = ai >= 0 ? ra : rl;
         }
     }
-    const double k = UNI ? a.k_uni : cur.k[q];
-    const double dmp = UNI ? a.damp_uni : cur.damp[q];
+    const double k = cur.k[q];
+    const double dmp = cur.damp[q];
     const double mag = k * (rl - nrm);
     const double dd = (dvx * ux + dvy * uy + dvz * uz) * dmp;
     r.fx = r.fx - (ux * mag + ux * dd);
@@ -331,8 +346,8 @@ __device__ __forceinline__ void add_spring(const StepArgs &a, MassRegs &r, const
 // wave has all four and they are in the fast range, the four evaluations
 // are straight-line code that interleaves; otherwise each incidence that
 // exists takes the general path.
-template <bool UNI, bool ACT, int G, int K, class Fetch>
-__device__ __forceinline__ void eval_set(const StepArgs &a, MassRegs &r, const Group<UNI, ACT, G> (&set)[K],
+template <bool ACT, int G, int K, class Fetch>
+__device__ __forceinline__ void eval_set(const StepArgs &a, MassRegs &r, const Group<ACT, G> (&set)[K],
                                          int g0, int d, double T, Fetch &fetch) {
     constexpr int N = G * K;
     double2 nb[N][3];
@@ -359,7 +374,7 @@ __device__ __forceinline__ void eval_set(const StepArgs &a, MassRegs &r, const G
             const double nrm = fast_sqrt(n2[e]);
             const double y = fast_recip(nrm);
             const double ux = fast_div(tx[e], nrm, y), uy = fast_div(ty[e], nrm, y), uz = fast_div(tz[e], nrm, y);
-            add_spring<UNI, ACT>(a, r, set[e / G], e % G, T, nb[e], nrm, ux, uy, uz);
+            add_spring<ACT>(a, r, set[e / G], e % G, T, nb[e], nrm, ux, uy, uz);
         }
     } else {
 #pragma unroll
@@ -367,7 +382,7 @@ __device__ __forceinline__ void eval_set(const StepArgs &a, MassRegs &r, const G
             if (G * g0 + e >= d) break;
             double nrm, ux, uy, uz;
             unit_axis(tx[e], ty[e], tz[e], nrm, ux, uy, uz);
-            add_spring<UNI, ACT>(a, r, set[e / G], e % G, T, nb[e], nrm, ux, uy, uz);
+            add_spring<ACT>(a, r, set[e / G], e % G, T, nb[e], nrm, ux, uy, uz);
         }
     }
 }
@@ -380,21 +395,23 @@ __device__ __forceinline__ void eval_set(const StepArgs &a, MassRegs &r, const G
 // clamped to it and re-reads a group already loaded).  prime() issues the
 // first set's loads, so that a kernel can start them before it waits for
 // anything else.
-template <bool UNI, bool ACT, int G>
+template <bool ACT, int G>
 struct SpringWalk {
     static_assert(G == 2 || G == 4, "groups of 2 or 4");
     static constexpr int K = 4 / G;  // groups per register set: four incidences
-    Group<UNI, ACT, G> A[K], B[K];
+    Group<ACT, G> A[K], B[K];
     int64_t ebase;
     int ng, lane;
+    BodyKD bkd;
 
-    __device__ __forceinline__ void prime(const StepArgs &a, int64_t eb, int n, int ln) {
+    __device__ __forceinline__ void prime(const StepArgs &a, int64_t eb, int n, int ln, const BodyKD &kd) {
         ebase = eb;
         ng = n;
         lane = ln;
+        bkd = kd;
         if (ng <= 0) return;
 #pragma unroll
-        for (int j = 0; j < K; ++j) A[j].load(a, ebase, lane, min(j, ng - 1));
+        for (int j = 0; j < K; ++j) A[j].load(a, ebase, lane, min(j, ng - 1), bkd);
     }
 
     template <class Fetch>
@@ -405,12 +422,12 @@ struct SpringWalk {
         const int last = ng - 1;
         for (int g = 0; g < ng; g += 2 * K) {
 #pragma unroll
-            for (int j = 0; j < K; ++j) B[j].load(a, ebase, lane, min(g + K + j, last));
-            eval_set<UNI, ACT, G, K>(a, r, A, g, d, T, fetch);
+            for (int j = 0; j < K; ++j) B[j].load(a, ebase, lane, min(g + K + j, last), bkd);
+            eval_set<ACT, G, K>(a, r, A, g, d, T, fetch);
             if (g + K >= ng) break;
 #pragma unroll
-            for (int j = 0; j < K; ++j) A[j].load(a, ebase, lane, min(g + 2 * K + j, last));
-            eval_set<UNI, ACT, G, K>(a, r, B, g + K, d, T, fetch);
+            for (int j = 0; j < K; ++j) A[j].load(a, ebase, lane, min(g + 2 * K + j, last), bkd);
+            eval_set<ACT, G, K>(a, r, B, g + K, d, T, fetch);
         }
     }
 };
@@ -449,10 +466,8 @@ __device__ __forceinline__ void store_mass(const StepArgs &a, double2 *__restric
 
 // Gather step: one lane per mass slot, one wavefront per 64-mass chunk, four
 // chunks per workgroup; the other endpoints' states are read from the
-// current state buffer (global memory).  UNI: spring constant and damping
-// are the same for every spring (passed by value).  ACT: some springs are
-// actuated.
-template <bool UNI, bool ACT>
+// current state buffer (global memory).  ACT: some springs are actuated.
+template <bool ACT>
 __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int64_t chunk = a.chunk_base + blk * kWavesPerBlock + uniform(threadIdx.x >> 6);
@@ -463,6 +478,7 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     // read before this kernel's first store: scalar loads
     const int64_t ebase = a.chunk_ptr[chunk];
     const int ng = a.chunk_ng[chunk];
+    const BodyKD bkd{a.b_k[b], a.b_d[b], a.b_kd[b]};
     const int s = a.slot;
     const int64_t left = a.bleft[s][b];
     const int64_t nst = a.bn[s][b];
@@ -480,8 +496,8 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     const double2 o0 = cur[3 * i], o1 = cur[3 * i + 1], o2 = cur[3 * i + 2];
     const int d = a.deg[i];
     const double m = a.m[i];
-    SpringWalk<UNI, ACT, kGroupGather> w;
-    w.prime(a, ebase, ng, lane);
+    SpringWalk<ACT, kGroupGather> w;
+    w.prime(a, ebase, ng, lane, bkd);
     if (fl & kPad) return;
     if (fl & kFixed) {  // src/sim.cu:1100-1101: a fixed mass keeps its state
         store_nt(&nxt[3 * i], o0.x, o0.y);
@@ -514,7 +530,7 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
 // in LDS, 48 bytes per row, after which each lane walks its incidences as
 // the gather step does, reading the other endpoints from LDS rows instead
 // of global memory.
-template <bool UNI, bool ACT>
+template <bool ACT>
 __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     extern __shared__ double2 S[];  // 3 per LDS row
     const int64_t tile = a.tile_base + xcd_block(blockIdx.x, gridDim.x);
@@ -537,6 +553,7 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
         ebase = a.chunk_ptr[chunk];
         ng = a.chunk_ng[chunk];
     }
+    const BodyKD bkd{a.b_k[b], a.b_d[b], a.b_kd[b]};
     const int s = a.slot;
     const int64_t left = a.bleft[s][b];
     const int64_t nst = a.bn[s][b];
@@ -557,7 +574,7 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     double2 o0 = make_double2(0.0, 0.0), o1 = o0, o2 = o0;
     int d = 0;
     double m = 1.0;
-    SpringWalk<UNI, ACT, kGroupTiled> w;
+    SpringWalk<ACT, kGroupTiled> w;
     if (live) {
         fl = a.mflags[i];
         o0 = cur[3 * i];
@@ -565,7 +582,7 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
         o2 = cur[3 * i + 2];
         d = a.deg[i];
         m = a.m[i];
-        w.prime(a, ebase, ng, t & (kChunk - 1));
+        w.prime(a, ebase, ng, t & (kChunk - 1), bkd);
     }
     // halo rows, two per thread per round: both indices, then both states.
     // Loads past the halo are clamped to its last entry and stored to the
@@ -645,63 +662,40 @@ __global__ void k_set_left(int64_t *bleft, const int32_t *b_chunk0, int64_t nb, 
 
 __global__ void k_set_i64(int64_t *p, int64_t v) { *p = v; }
 
-__global__ void k_fill_f64(double *p, int64_t n, double v) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
 // ---- launchers ----
 
-hipError_t launch_step(const StepArgs &a, bool uni, bool act, hipStream_t s) {
+hipError_t launch_step(const StepArgs &a, bool act, hipStream_t s) {
     const int64_t blocks = (a.nchunks - a.chunk_base + kWavesPerBlock - 1) / kWavesPerBlock;
     if (blocks <= 0) return hipSuccess;
     const dim3 g((unsigned)blocks), blk(kBlock);
-    if (uni && !act) hipLaunchKernelGGL((k_step_gather<true, false>), g, blk, 0, s, a);
-    else if (uni && act) hipLaunchKernelGGL((k_step_gather<true, true>), g, blk, 0, s, a);
-    else if (!uni && !act) hipLaunchKernelGGL((k_step_gather<false, false>), g, blk, 0, s, a);
-    else hipLaunchKernelGGL((k_step_gather<false, true>), g, blk, 0, s, a);
+    if (act) hipLaunchKernelGGL((k_step_gather<true>), g, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_step_gather<false>), g, blk, 0, s, a);
     return hipGetLastError();
 }
 
 // the tile rows plus one spare row (k_step_tiles stores unused halo loads there)
 size_t tiles_lds_bytes(int ldim) { return (size_t)48 * (ldim + 1); }
 
-hipError_t launch_step_tiles(const StepArgs &a, bool uni, bool act, hipStream_t s) {
+hipError_t launch_step_tiles(const StepArgs &a, bool act, hipStream_t s) {
     if (a.ntiles <= a.tile_base) return hipSuccess;
     const size_t lds = tiles_lds_bytes(a.ldim);
     const dim3 g((unsigned)(a.ntiles - a.tile_base)), blk(kTileMasses);
-    if (uni && !act) hipLaunchKernelGGL((k_step_tiles<true, false>), g, blk, lds, s, a);
-    else if (uni && act) hipLaunchKernelGGL((k_step_tiles<true, true>), g, blk, lds, s, a);
-    else if (!uni && !act) hipLaunchKernelGGL((k_step_tiles<false, false>), g, blk, lds, s, a);
-    else hipLaunchKernelGGL((k_step_tiles<false, true>), g, blk, lds, s, a);
+    if (act) hipLaunchKernelGGL((k_step_tiles<true>), g, blk, lds, s, a);
+    else hipLaunchKernelGGL((k_step_tiles<false>), g, blk, lds, s, a);
     return hipGetLastError();
 }
 
 hipError_t set_tiles_lds_limit() {
     const int bytes = (int)tiles_lds_bytes(kMaxTileRows);
-    hipError_t e = hipSuccess;
-    e = hipFuncSetAttribute((const void *)k_step_tiles<true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    const hipError_t e =
+        hipFuncSetAttribute((const void *)k_step_tiles<false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void *)k_step_tiles<true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void *)k_step_tiles<false, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void *)k_step_tiles<false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    return hipFuncSetAttribute((const void *)k_step_tiles<true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
-const char *step_tiles_kernel_name(bool uni, bool act) {
-    if (uni && !act) return "k_step_tiles<uniform_kd,passive>";
-    if (uni && act) return "k_step_tiles<uniform_kd,actuated>";
-    if (!uni && !act) return "k_step_tiles<per_spring_kd,passive>";
-    return "k_step_tiles<per_spring_kd,actuated>";
-}
+const char *step_tiles_kernel_name(bool act) { return act ? "k_step_tiles<actuated>" : "k_step_tiles<passive>"; }
 
-const char *step_kernel_name(bool uni, bool act) {
-    if (uni && !act) return "k_step_gather<uniform_kd,passive>";
-    if (uni && act) return "k_step_gather<uniform_kd,actuated>";
-    if (!uni && !act) return "k_step_gather<per_spring_kd,passive>";
-    return "k_step_gather<per_spring_kd,actuated>";
-}
+const char *step_kernel_name(bool act) { return act ? "k_step_gather<actuated>" : "k_step_gather<passive>"; }
 
 hipError_t launch_advance_T(double *Tm, const uint32_t *mflags, const int32_t *chunk_body,
                             const int64_t *bn, int64_t *bnsync, int64_t chunk_begin, int64_t chunk_end,
@@ -726,9 +720,5 @@ hipError_t launch_set_left(int64_t *bleft, const int32_t *b_chunk0, int64_t nb, 
     return hipGetLastError();
 }
 
-hipError_t launch_fill_f64(double *p, int64_t n, double v, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_fill_f64, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
-    return hipGetLastError();
-}
 
 }  // namespace titan_mi355x

@@ -96,16 +96,19 @@ struct StepArgs {
     // incidences
     const uint32_t *inc_nbr;
     const double *inc_rest;
-    const double *inc_k;      // null when uniform
-    const double *inc_damp;   // null when uniform
+    const double *inc_k;      // per incidence, for bodies with per-spring k (else null)
+    const double *inc_damp;   // per incidence, for bodies with per-spring damping (else null)
     const int32_t *inc_act;   // null when no actuators
     const double *act;        // 4 per actuator: a, omega, phi, 0
-    double k_uni, damp_uni;
     // bodies: dynamic fields double-buffered by launch slot
     double *bT[2];
     int64_t *bn[2];
     int64_t *bleft[2];
     const int32_t *b_chunk0;
+    // per body: spring constant and damping when every spring of the body
+    // shares them, else read per incidence (b_kd bit 0: k, bit 1: damping)
+    const double *b_k, *b_d;
+    const uint8_t *b_kd;
     const int32_t *b_lc;      // shared local list: begin
     const int32_t *b_lcn;     // shared local list: count
     const LocalC *lc;

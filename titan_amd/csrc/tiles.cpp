@@ -132,7 +132,7 @@ bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb, int tile_rows) {
     return true;
 }
 
-void build_sell(int64_t n, const SpringsIn &sp, bool per_spring_kd, const TileBuild *tb, int64_t g0,
+void build_sell(int64_t n, const SpringsIn &sp, bool per_k, bool per_d, const TileBuild *tb, int64_t g0,
                 SellBuild &out) {
     const int64_t S = sp.n_springs;
     const int64_t nch = (n + kChunk - 1) / kChunk;
@@ -179,10 +179,8 @@ void build_sell(int64_t n, const SpringsIn &sp, bool per_spring_kd, const TileBu
     out.k.clear();
     out.damp.clear();
     out.act.clear();
-    if (per_spring_kd) {
-        out.k.assign(ne, 0.0);
-        out.damp.assign(ne, 0.0);
-    }
+    if (per_k) out.k.assign(ne, 0.0);
+    if (per_d) out.damp.assign(ne, 0.0);
     if (sp.act) out.act.assign(ne, -1);
     // padding entries: the mass itself (tiled: row 0 for slots past the body)
     for (int64_t c = 0; c < nch; ++c)
@@ -196,10 +194,8 @@ void build_sell(int64_t n, const SpringsIn &sp, bool per_spring_kd, const TileBu
         const int64_t e = entry(me, fill[me]++);
         put_nbr(me, e, o);
         out.rest[e] = sp.rest[s];
-        if (per_spring_kd) {
-            out.k[e] = sp.k[s];
-            out.damp[e] = sp.damping ? sp.damping[s] : 0.0;
-        }
+        if (per_k) out.k[e] = sp.k[s];
+        if (per_d) out.damp[e] = sp.damping ? sp.damping[s] : 0.0;
         if (sp.act) out.act[e] = sp.act[s];
     };
     for (int64_t s = 0; s < S; ++s) {

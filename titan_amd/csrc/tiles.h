@@ -42,7 +42,7 @@ struct SellBuild {
     // coff[c] + (e - coff[c]) / 2 holds the LDS rows of one group's two
     // entries (low half: the even entry); the chunk's other half is unused
     std::vector<uint32_t> nbr;
-    std::vector<double> rest, k, damp;  // k and damp only with per-spring constants
+    std::vector<double> rest, k, damp;  // k, damp: only when built per incidence
     std::vector<int32_t> act;      // body-local actuator index or -1, only if sp.act
 };
 
@@ -59,7 +59,8 @@ bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb, int tile_rows = 
 // Build the incidence lists.  With tb == nullptr (gather layout) an
 // incidence names the other endpoint's arena slot, g0 + its body index;
 // with a tile build it names the endpoint's LDS row in the mass's tile.
-void build_sell(int64_t n, const SpringsIn &sp, bool per_spring_kd, const TileBuild *tb, int64_t g0,
+// per_k / per_d: also list each incidence's spring constant / damping.
+void build_sell(int64_t n, const SpringsIn &sp, bool per_k, bool per_d, const TileBuild *tb, int64_t g0,
                 SellBuild &out);
 
 }  // namespace titan_mi355x

@@ -145,7 +145,7 @@ def knn_cloud(n: int, *, k_nn: int = 12, box: float = 10.0, seed: int = 0, k_lo:
     pos[:, 1] -= box / 2
     pos[:, 2] += z0
     kk = min(k_nn, n - 1)
-    _, nbr = cKDTree(pos).query(pos, k=kk + 1)
+    _, nbr = cKDTree(pos).query(pos, k=kk + 1, workers=-1)
     i = np.repeat(np.arange(n), kk)
     j = nbr[:, 1:].reshape(-1)
     a, b = np.minimum(i, j), np.maximum(i, j)
@@ -188,3 +188,19 @@ def serving_requests(n: int, seed: int = 0, side_lo: int = 3, side_hi: int = 40,
         actuate(b, rng.uniform(0.1, 0.3), rng)
         out.append((b, int(rng.integers(h_lo, h_hi + 1))))
     return out
+
+
+def knn_scene(n: int = 2_000_000, n_balls: int = 1000, seed: int = 0, dt: float = 1e-5) -> Scene:
+    """The irregular-graph workload: a seeded uniform cloud of n masses in a
+    10 m box (bottom at z = 0), kNN-12 springs deduplicated (variable
+    degree), k ~ LogUniform(1e3, 1e5), damping 0.05, n_balls rigid contact
+    balls (centres uniform in the box, radius U(0.1, 0.5) m), a ground plane
+    z = 0 with friction, dt = 1e-5."""
+    b = knn_cloud(n, box=10.0, seed=seed)
+    rng = np.random.default_rng(seed + 7)
+    c = rng.uniform(0.0, 10.0, size=(n_balls, 3))
+    c[:, :2] -= 5.0
+    balls = [(float(x), float(y), float(z), float(r))
+             for (x, y, z), r in zip(c, rng.uniform(0.1, 0.5, n_balls))]
+    return Scene(bodies=[b], planes=[(0.0, 0.0, 1.0, 0.0, 0.8, 1.0)], balls=balls, dt=dt,
+                 name=f"knn{n}_balls{n_balls}_seed{seed}")
