@@ -70,12 +70,10 @@ def cpu_baseline(scene, budget_s: float = 12.0) -> dict:
 
     threads = cpu_threads()
     o = to_oracle(scene, threads=threads)
-    o.run(3)  # warm-up (page-in, thread pool)
+    o.run(5)  # warm-up (page-in, thread pool)
+    k = 10    # calibration: the per-step time after warm-up sizes the timed sample
     t0 = time.perf_counter()
-    k = 0
-    while time.perf_counter() - t0 < 1.0:
-        o.run(1)
-        k += 1
+    o.run(k)
     per = (time.perf_counter() - t0) / k
     n = max(3, int(budget_s / max(per, 1e-6)))
     t0 = time.perf_counter()
@@ -84,7 +82,7 @@ def cpu_baseline(scene, budget_s: float = 12.0) -> dict:
     value = scene.n_masses * n / dt
     return {"value": value, "unit": "mass-steps/s", "cores": threads, "kind": "port",
             "sample": f"oracle (C++17/OpenMP, {threads} threads, {cpu_model()}) on the same "
-                      f"1M-mass lattice, {n} timed steps after {3 + k} untimed ones, {dt:.1f} s",
+                      f"1M-mass lattice, {n} timed steps after {5 + k} untimed ones, {dt:.1f} s",
             "ms_per_step": 1e3 * dt / n}
 
 
