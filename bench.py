@@ -110,7 +110,7 @@ def bench_batched(n_bodies: int, seed: int, rollout: int, sample: int) -> dict:
         t0 = time.perf_counter()
         best, rates = ctx.autotune(tune_set, replicas=8, steps=20)
         log(f"batched: autotune {time.perf_counter() - t0:.1f} s -> tiled_min_masses {best[0]}, "
-            f"tile_rows {best[1]}")
+            f"tile_rows {best[1]}, resident_max_masses {best[2]}")
         t0 = time.perf_counter()
         ids = [ctx.add_body(**b) for b in sc.bodies]
         ctx.sync()
@@ -155,8 +155,9 @@ def bench_batched(n_bodies: int, seed: int, rollout: int, sample: int) -> dict:
                     f"{sc.n_actuated} actuated (a~U(0,0.2), omega~U(1,20), phi~U(0,2pi)), own contact plane "
                     f"each, dt=1e-4, FP64, {rollout}-step graph-captured rollouts, one launch per step",
         "value": rate_batched, "unit": "mass-steps/s", "ms_per_rollout": ms_batched,
-        "schedule": {"tiled_min_masses": best[0], "tile_rows": best[1],
-                     "autotune": [{"tiled_min_masses": c[0], "tile_rows": c[1], "mass_steps_per_s": r}
+        "schedule": {"tiled_min_masses": best[0], "tile_rows": best[1], "resident_max_masses": best[2],
+                     "autotune": [{"tiled_min_masses": c[0], "tile_rows": c[1], "resident_max_masses": c[2],
+                                   "mass_steps_per_s": r}
                                   for c, r in rates]},
         "one_body_per_launch": {"value": rate_one, "unit": "mass-steps/s",
                                 "sample": f"first {sample} bodies ({sub_m} masses), {rollout} steps each, "
@@ -195,7 +196,7 @@ def bench_serving(seed: int, n_req: int, lams, quantum: int, n_base: int) -> dic
         t0 = time.perf_counter()
         sched, _ = ctx.autotune([b for b, _ in reqs[:6]], replicas=2, steps=16)
         log(f"serving: autotune {time.perf_counter() - t0:.1f} s -> tiled_min_masses {sched[0]}, "
-            f"tile_rows {sched[1]}")
+            f"tile_rows {sched[1]}, resident_max_masses {sched[2]}")
 
     with Context() as ctx:  # warm-up: load the kernels, build a graph
         setup(ctx, sched)
@@ -253,7 +254,7 @@ def bench_serving(seed: int, n_req: int, lams, quantum: int, n_base: int) -> dic
         "one_request_at_a_time": {"value": base_rate, "unit": "mass-steps/s",
                                   "sample": f"first {n_base} requests, one launch per step each"},
         "speedup_vs_one_request_at_a_time": best / base_rate,
-        "schedule": {"tiled_min_masses": sched[0], "tile_rows": sched[1]},
+        "schedule": {"tiled_min_masses": sched[0], "tile_rows": sched[1], "resident_max_masses": sched[2]},
     }
 
 

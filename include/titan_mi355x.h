@@ -50,6 +50,7 @@ extern "C" {
 #define TITAN_LAYOUT_AUTO 0
 #define TITAN_LAYOUT_GATHER 1
 #define TITAN_LAYOUT_TILED 2
+#define TITAN_LAYOUT_RESIDENT 3
 
 typedef struct titan_ctx titan_ctx;
 
@@ -88,8 +89,10 @@ typedef struct {
     double t0;
     /* device layout: TITAN_LAYOUT_AUTO (the context's schedule decides, see
      * titan_schedule; tiled only when the body's tiles fit the kernel's
-     * limits), TITAN_LAYOUT_GATHER or TITAN_LAYOUT_TILED (fails if a tile does
-     * not fit).  Results are identical either way. */
+     * limits), TITAN_LAYOUT_GATHER, TITAN_LAYOUT_TILED (fails if a tile does
+     * not fit) or TITAN_LAYOUT_RESIDENT (at most 1024 masses: the body is
+     * held whole in one workgroup's LDS and all its steps of a stepping call
+     * run in one launch).  Results are identical either way. */
     int32_t layout;
 } titan_body_desc;
 
@@ -162,32 +165,36 @@ int titan_mem_used(const titan_ctx *ctx, uint64_t *bytes);
 /* Names of the step kernel variants the next titan_step will launch
  * ("+"-separated when both layouts are in use). */
 int titan_kernel_name(const titan_ctx *ctx, char *buf, int64_t len);
-/* Layout a body was given: TITAN_LAYOUT_GATHER or TITAN_LAYOUT_TILED. */
+/* Layout a body was given: TITAN_LAYOUT_GATHER, _TILED or _RESIDENT. */
 int titan_body_layout(const titan_ctx *ctx, int32_t body_id, int32_t *layout);
 
 /* ---- schedule and autotuning (no counterpart in the reference, which has
  *      one fixed launch shape per kernel, src/sim.cu:1569-1590) ---- */
 /* How bodies created with TITAN_LAYOUT_AUTO are laid out: bodies with at
- * least tiled_min_masses masses get the tiled layout, smaller ones the
- * gather layout; tile_rows bounds a tile's LDS rows (masses + halo, 64 ..
- * 2304; fewer rows allow more workgroups per CU, more rows bigger tiles).
- * Default: tiled_min_masses 0, tile_rows 2304.  Applies to bodies created
+ * most resident_max_masses masses (0 .. 1024; 0 = none) get the resident
+ * layout; of the others, those with at least tiled_min_masses masses get the
+ * tiled layout and smaller ones the gather layout; tile_rows bounds a tile's
+ * LDS rows (masses + halo, 64 .. 2304; fewer rows allow more workgroups per
+ * CU, more rows bigger tiles).  Default: resident_max_masses 1024,
+ * tiled_min_masses 0, tile_rows 2304.  Applies to bodies created
  * afterwards. */
 typedef struct titan_schedule {
     int64_t tiled_min_masses;
     int32_t tile_rows;
-    int32_t pad;
+    int32_t resident_max_masses;
 } titan_schedule;
 int titan_set_schedule(titan_ctx *ctx, const titan_schedule *sched);
 int titan_get_schedule(const titan_ctx *ctx, titan_schedule *sched);
-/* Pick the schedule by measurement: for each candidate (tiled_min_masses in
- * {0, 128, 256, 512, 1024, never}, tile_rows in {682, 2304}) a scratch
+/* Pick the schedule by measurement: for each candidate (resident_max_masses
+ * in {0, 1024}, tiled_min_masses in {0, 128, 256, 512, 1024, never},
+ * tile_rows in {682, 2304}; 24 in that order, the last index fastest) a scratch
  * context on the same device with this context's dt, gravity, planes and
  * balls holds `replicas` copies of the n_samples sample bodies and is timed
  * over `steps` steps (after a warm-up of the same length).  The fastest in
  * mass-steps/s becomes this context's schedule and is returned in *best
  * (may be NULL); *rate_out (may be NULL) receives each candidate's rate, in
- * candidate order (12 entries). */
+ * candidate order (24 entries; candidates that lay out every sample alike
+ * are timed once and share the rate). */
 int titan_autotune(titan_ctx *ctx, const titan_body_desc *samples, int32_t n_samples, int32_t replicas,
                    int32_t steps, titan_schedule *best, double *rate_out);
 

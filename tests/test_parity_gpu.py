@@ -13,11 +13,14 @@ import sys
 import numpy as np
 import pytest
 
-from titan_amd import LAYOUT_GATHER, LAYOUT_TILED, Context, scenes
+from titan_amd import LAYOUT_GATHER, LAYOUT_RESIDENT, LAYOUT_TILED, Context, scenes
 from helpers import normwise_rel, to_oracle
 
 pytestmark = pytest.mark.gpu
 LAYOUTS = pytest.mark.parametrize("layout", [LAYOUT_GATHER, LAYOUT_TILED], ids=["gather", "tiled"])
+# bodies of at most 1024 masses also run in the resident layout
+LAYOUTS3 = pytest.mark.parametrize("layout", [LAYOUT_GATHER, LAYOUT_TILED, LAYOUT_RESIDENT],
+                                   ids=["gather", "tiled", "resident"])
 
 TOL_ACT = 1e-10
 
@@ -54,7 +57,7 @@ def assert_bitexact(eng, orc):
     np.testing.assert_array_equal(eng["vel"], orc.vel)
 
 
-@LAYOUTS
+@LAYOUTS3
 def test_readme_lattice_free_fall_bitexact(layout):
     sc = scenes.readme_lattice(10.0)
     _say(f"case readme lattice cz=10, {sc.n_masses} masses, {sc.n_springs} springs, 100 steps")
@@ -66,7 +69,7 @@ def test_readme_lattice_free_fall_bitexact(layout):
     assert t == orc.t
 
 
-@LAYOUTS
+@LAYOUTS3
 def test_readme_lattice_contact_bitexact(layout):
     sc = scenes.readme_lattice(2.45)
     _say("case readme lattice cz=2.45 (contact + friction from step 0), 1000 steps")
@@ -77,7 +80,7 @@ def test_readme_lattice_contact_bitexact(layout):
     assert np.min(sc.bodies[0]["pos"][:, 2]) < 0  # in contact from the first step
 
 
-@LAYOUTS
+@LAYOUTS3
 def test_graph_path_matches_direct_launches(layout):
     sc = scenes.readme_lattice(2.45)
     _say("case graph replay vs single launches, 300 steps")
@@ -87,7 +90,7 @@ def test_graph_path_matches_direct_launches(layout):
     np.testing.assert_array_equal(a["vel"], b["vel"])
 
 
-@LAYOUTS
+@LAYOUTS3
 def test_actuated_reference_kinds_and_general_law(layout):
     sc = scenes.readme_lattice(2.45)
     b = sc.bodies[0]
@@ -105,7 +108,7 @@ def test_actuated_reference_kinds_and_general_law(layout):
     assert normwise_rel(eng["vel"], orc.vel) <= TOL_ACT
 
 
-@LAYOUTS
+@LAYOUTS3
 def test_fixed_ext_drag_balls_per_spring_k(layout):
     sc = scenes.readme_lattice(2.45)
     b = sc.bodies[0]
@@ -126,7 +129,7 @@ def test_fixed_ext_drag_balls_per_spring_k(layout):
     np.testing.assert_array_equal(eng["T"], orc.T)
 
 
-@LAYOUTS
+@LAYOUTS3
 def test_local_constraints_shared_and_per_mass(layout):
     sc = scenes.readme_lattice(2.0)
     b = sc.bodies[0]
@@ -159,7 +162,7 @@ def test_local_constraints_shared_and_per_mass(layout):
     assert_bitexact(eng, orc)
 
 
-@LAYOUTS
+@LAYOUTS3
 def test_bodies_alone_equal_bodies_together(layout):
     s1 = scenes.readme_lattice(2.45)
     s2 = scenes.readme_lattice(3.0)
@@ -275,11 +278,64 @@ def test_mixed_layouts_in_one_context():
     np.testing.assert_array_equal(gb["pos"], ob.pos)
 
 
-def test_auto_layout_prefers_tiles():
-    sc = scenes.readme_lattice(2.45)
+def test_auto_layout_by_size():
+    small = scenes.readme_lattice(2.45)
+    big = scenes.lattice_body((0.0, 0.0, 0.55), (1.2, 1.2, 1.2), 11, damping=0.1)  # 1331 masses
     with Context() as ctx:
-        (bid,) = sc.load(ctx)
-        assert ctx.body_layout(bid) == LAYOUT_TILED
+        (a,) = small.load(ctx)
+        b = ctx.add_body(**big)
+        assert ctx.body_layout(a) == LAYOUT_RESIDENT and ctx.body_layout(b) == LAYOUT_TILED
+        ctx.set_schedule(0, 2304, 0)  # no resident bodies
+        assert ctx.body_layout(ctx.add_body(**small.bodies[0])) == LAYOUT_TILED
+        with pytest.raises(RuntimeError):
+            ctx.add_body(**big, layout=LAYOUT_RESIDENT)  # more than 1024 masses
+
+
+def test_resident_beside_gather_and_tiled_bodies():
+    # one body per layout in one context, stepped in calls of odd, even and
+    # graph-sized (>= 64) counts: each must match its own oracle run
+    rng = np.random.default_rng(11)
+    bodies = []
+    for q, side in enumerate([10, 6, 11]):  # 1000 (resident), 216 (gather), 1331 (tiled)
+        b = scenes.lattice_body((4.0 * q, 0.0, 0.58), (1.2, 1.2, 1.2), side, damping=0.1)
+        b["pos"] = b["pos"] + rng.normal(0, 1e-3, b["pos"].shape)
+        b["vel"] = rng.normal(0, 0.2, b["pos"].shape)
+        bodies.append(b)
+    sc = scenes.Scene(bodies=bodies, planes=[(0.0, 0.0, 1.0, 0.0, 0.8, 1.0)], dt=1e-4)
+    lays = [LAYOUT_RESIDENT, LAYOUT_GATHER, LAYOUT_TILED]
+    calls = [1, 7, 64, 130, 2]
+    _say(f"case resident + gather + tiled bodies, calls {calls}")
+    with Context() as ctx:
+        ids = sc.load(ctx, layout=lays)
+        assert [ctx.body_layout(i) for i in ids] == lays
+        assert "k_rollout_resident" in ctx.kernel_name()
+        for n in calls:
+            ctx.step(n)
+        got = [ctx.download(i, acc=True, T=True) for i in ids]
+    for q in range(3):
+        o = to_oracle(scenes.Scene(bodies=[bodies[q]], planes=sc.planes, dt=sc.dt))
+        o.run(sum(calls))
+        np.testing.assert_array_equal(got[q]["pos"], o.pos)
+        np.testing.assert_array_equal(got[q]["vel"], o.vel)
+        np.testing.assert_array_equal(got[q]["acc"], o.acc)
+        np.testing.assert_array_equal(got[q]["T"], o.T)
+
+
+def test_resident_many_balls_and_actuation():
+    # the ball grid and actuated springs inside the resident kernel
+    rng = np.random.default_rng(8)
+    b = scenes.lattice_body((0.0, 0.0, 1.2), (2.0, 2.0, 2.0), 10, damping=0.1)
+    b = scenes.actuate(b, 0.3, rng)
+    balls = [(*rng.uniform(-1.5, 1.5, 2), rng.uniform(0.0, 2.5), rng.uniform(0.05, 0.35)) for _ in range(200)]
+    sc = scenes.Scene(bodies=[b], planes=[(0.0, 0.0, 1.0, 0.0, 0.8, 1.0)], balls=balls, dt=1e-4)
+    _say("case resident body, 200 balls, 30% actuated, 150 steps")
+    eng, _ = run_engine(sc, 150, layout=LAYOUT_RESIDENT)
+    ref, _ = run_engine(sc, 150, layout=LAYOUT_GATHER)
+    np.testing.assert_array_equal(eng["pos"], ref["pos"])  # same device sin on both paths
+    np.testing.assert_array_equal(eng["vel"], ref["vel"])
+    orc = run_oracle(sc, 150)
+    assert normwise_rel(eng["pos"], orc.pos) <= TOL_ACT
+    assert normwise_rel(eng["vel"], orc.vel) <= TOL_ACT
 
 
 @LAYOUTS
@@ -312,7 +368,7 @@ def test_knn_cloud_with_balls_bitexact(layout):
     assert_bitexact(eng, orc)
 
 
-@LAYOUTS
+@LAYOUTS3
 def test_per_body_constants_in_one_context_bitexact(layout):
     # four bodies: k and damping shared or per spring, in every combination
     rng = np.random.default_rng(4)

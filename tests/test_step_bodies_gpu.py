@@ -11,7 +11,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from titan_amd import LAYOUT_GATHER, LAYOUT_TILED, Context, scenes
+from titan_amd import LAYOUT_GATHER, LAYOUT_RESIDENT, LAYOUT_TILED, Context, scenes
 from helpers import to_oracle
 
 pytestmark = pytest.mark.gpu
@@ -44,14 +44,18 @@ def _check(ctx, bid, scene, i, n):
     assert k == n and T == t
 
 
-@pytest.mark.parametrize("layout", [0, LAYOUT_GATHER], ids=["auto", "gather"])
+MIXED = [LAYOUT_RESIDENT, LAYOUT_GATHER, LAYOUT_TILED]
+
+
+@pytest.mark.parametrize("layout", [0, LAYOUT_GATHER, LAYOUT_TILED, MIXED],
+                         ids=["auto", "gather", "tiled", "mixed"])
 def test_step_bodies_own_counts_then_all(layout):
     sc = _bodies()
     print(f"case step_bodies layout {layout}: masses {[len(b['pos']) for b in sc.bodies]}", flush=True)
     with Context() as ctx:
         ids = sc.load(ctx, layout=layout)
-        if layout == 0:
-            assert ctx.body_layout(ids[2]) == LAYOUT_TILED
+        want = MIXED if layout == MIXED else [LAYOUT_RESIDENT] * 3 if layout == 0 else [layout] * 3
+        assert [ctx.body_layout(i) for i in ids] == want
         # several bodies per launch, uneven budgets; body 1 listed with 0
         ctx.step_bodies([ids[0], ids[1], ids[2]], [7, 0, 13])
         for i, n in zip(range(3), (7, 0, 13)):

@@ -41,7 +41,8 @@ EXPORTS = (
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
 
-LAYOUT_AUTO, LAYOUT_GATHER, LAYOUT_TILED = 0, 1, 2
+LAYOUT_AUTO, LAYOUT_GATHER, LAYOUT_TILED, LAYOUT_RESIDENT = 0, 1, 2, 3
+RESIDENT_MAX = 1024
 
 
 class TitanError(RuntimeError):
@@ -206,11 +207,13 @@ class ServeReport(ctypes.Structure):
 
 
 class Schedule(ctypes.Structure):
-    _fields_ = [("tiled_min_masses", c_int64), ("tile_rows", c_int32), ("pad", c_int32)]
+    _fields_ = [("tiled_min_masses", c_int64), ("tile_rows", c_int32), ("resident_max_masses", c_int32)]
 
 
-# titan_autotune's candidates, in the order of its rate_out (min -1: never tiled)
-AUTOTUNE_CANDIDATES = [(mn, rw) for mn in (0, 128, 256, 512, 1024, -1) for rw in (682, 2304)]
+# titan_autotune's candidates (tiled_min_masses, tile_rows, resident_max_masses),
+# in the order of its rate_out (min -1: never tiled)
+AUTOTUNE_CANDIDATES = [(mn, rw, rs) for rs in (0, RESIDENT_MAX) for mn in (0, 128, 256, 512, 1024, -1)
+                       for rw in (682, 2304)]
 
 
 class Context:
@@ -280,20 +283,23 @@ class Context:
         return bid.value
 
     # -- schedule --
-    def set_schedule(self, tiled_min_masses: int = 0, tile_rows: int = 2304):
-        s = Schedule(int(tiled_min_masses), int(tile_rows), 0)
+    def set_schedule(self, tiled_min_masses: int = 0, tile_rows: int = 2304,
+                     resident_max_masses: int = RESIDENT_MAX):
+        s = Schedule(int(tiled_min_masses), int(tile_rows), int(resident_max_masses))
         self._chk(self._L.titan_set_schedule(self._h, ctypes.byref(s)))
 
-    def schedule(self) -> tuple[int, int]:
+    def schedule(self) -> tuple[int, int, int]:
+        """(tiled_min_masses, tile_rows, resident_max_masses)"""
         s = Schedule()
         self._chk(self._L.titan_get_schedule(self._h, ctypes.byref(s)))
-        return s.tiled_min_masses, s.tile_rows
+        return s.tiled_min_masses, s.tile_rows, s.resident_max_masses
 
     def autotune(self, samples, replicas: int, steps: int):
         """Time the candidate schedules on `replicas` copies of the sample
         bodies (dicts of body fields); the fastest becomes this context's
-        schedule.  Returns ((tiled_min_masses, tile_rows), [((min, rows), rate)])
-        with min -1 for "never tiled"."""
+        schedule.  Returns the winner (tiled_min_masses, tile_rows,
+        resident_max_masses) and [(candidate, rate)] in AUTOTUNE_CANDIDATES
+        order, with min -1 for "never tiled"."""
         built = [body_desc(**b) for b in samples]
         arr = (BodyDesc * len(built))(*[d for d, _ in built])
         best = Schedule()
@@ -301,7 +307,8 @@ class Context:
         self._chk(self._L.titan_autotune(self._h, arr, len(built), int(replicas), int(steps),
                                          ctypes.byref(best), rates))
         del built
-        return (best.tiled_min_masses, best.tile_rows), list(zip(AUTOTUNE_CANDIDATES, list(rates)))
+        return ((best.tiled_min_masses, best.tile_rows, best.resident_max_masses),
+                list(zip(AUTOTUNE_CANDIDATES, list(rates))))
 
     # -- serving --
     def serve_config(self, max_running: int = 4096, hbm_bytes: int = 0, quantum: int = 64):

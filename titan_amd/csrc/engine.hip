@@ -97,6 +97,8 @@ class Ranges {
 struct BodyH {
     bool live = false;
     bool tiled = false;
+    bool resident = false;
+    int64_t res = -1;  // resident descriptor (resident layout)
     int64_t n_masses = 0, chunk0 = 0, nchunks = 0;
     int64_t inc0 = 0, inc_len = 0, act0 = 0, act_len = 0, lc0 = 0, lc_len = 0;
     int64_t tile0 = 0, ntiles = 0, halo0 = 0, nhalo = 0;
@@ -122,9 +124,9 @@ struct titan_ctx {
     double T = 0.0;                      // Simulation::T
     std::vector<double> planes_h, balls_h;
 
-    Ranges chunks, incs, acts, lcs, bids, tiles, halos;
+    Ranges chunks, incs, acts, lcs, bids, tiles, halos, rdescs;
     std::vector<BodyH> bodies;
-    titan_schedule sched{0, kTileRows, 0};
+    titan_schedule sched{0, kTileRows, kResidentMax};
 
     // mass arena (ncap = chunks.cap() * 64)
     int64_t ncap = 0;
@@ -135,16 +137,20 @@ struct titan_ctx {
     int32_t *chunk_body = nullptr;
     int64_t *chunk_ptr = nullptr;
     int32_t *chunk_ng = nullptr;   // incidence groups per lane (uniform over the chunk)
+    int64_t *chunk_act0 = nullptr; // the chunk's share of its body's actuators
+    int32_t *chunk_nact = nullptr;
     // incidence arena
     uint32_t *inc_nbr = nullptr;
     double *inc_rest = nullptr, *inc_k = nullptr, *inc_damp = nullptr;
-    int32_t *inc_act = nullptr;
     double *act = nullptr;
+    double *ract[2] = {nullptr, nullptr};  // actuation factors by launch slot
     LocalC *lc = nullptr;
     // tiled layout arenas
     TileDesc *tile = nullptr;
     int32_t *halo = nullptr;
-    uint8_t *b_tiled = nullptr;
+    // resident layout: one descriptor per body
+    ResidentDesc *rdesc = nullptr;
+    uint8_t *b_mode = nullptr;  // per body: kModeGather, kModeTiled, kModeResident
     // body table
     int64_t bcap = 0;
     double *bT[2] = {nullptr, nullptr};
@@ -270,7 +276,7 @@ void ensure_chunks(titan_ctx *c, int64_t need) {
     const int64_t on = old * kChunk, nn = nc * kChunk;
     check_growth(c, (size_t)nn * (96 + 8 + 8 + 24 + 4 + 4 + (c->ext ? 24 : 0) + (c->drag ? 4 : 0) +
                                   (c->mlc ? 8 : 0)) +
-                        (size_t)nc * (4 + 8 + 4));
+                        (size_t)nc * (4 + 8 + 4 + 8 + 4));
     grow_soa(c, c->st, 2, 6 * on, 6 * nn);
     grow_soa(c, c->m, 1, on, nn);
     grow_soa(c, c->Tm, 1, on, nn);
@@ -289,6 +295,8 @@ void ensure_chunks(titan_ctx *c, int64_t need) {
     c->chunk_body = cb;
     grow_soa(c, c->chunk_ptr, 1, old, nc);
     grow_soa(c, c->chunk_ng, 1, old, nc);
+    grow_soa(c, c->chunk_act0, 1, old, nc);
+    grow_soa(c, c->chunk_nact, 1, old, nc);
     c->chunks.grow(nc);
     c->ncap = nn;
     drop_graphs(c);
@@ -298,19 +306,11 @@ void ensure_incs(titan_ctx *c, int64_t need) {
     if (c->incs.cap() >= need) return;
     const int64_t old = c->incs.cap();
     const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 4096));
-    check_growth(c, (size_t)nc * (4 + 8 + (c->inc_k ? 8 : 0) + (c->inc_damp ? 8 : 0) + (c->inc_act ? 4 : 0)));
+    check_growth(c, (size_t)nc * (4 + 8 + (c->inc_k ? 8 : 0) + (c->inc_damp ? 8 : 0)));
     grow_soa(c, c->inc_nbr, 1, old, nc);
     grow_soa(c, c->inc_rest, 1, old, nc);
     if (c->inc_k) grow_soa(c, c->inc_k, 1, old, nc);
     if (c->inc_damp) grow_soa(c, c->inc_damp, 1, old, nc);
-    if (c->inc_act) {
-        int32_t *q = dalloc<int32_t>(c, nc);
-        HIPCHK(hipMemsetAsync(q, 0xff, sizeof(int32_t) * nc, c->stream));
-        HIPCHK(hipMemcpyAsync(q, c->inc_act, sizeof(int32_t) * old, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        dfree(c, c->inc_act);
-        c->inc_act = q;
-    }
     c->incs.grow(nc);
     drop_graphs(c);
 }
@@ -320,6 +320,10 @@ void ensure_acts(titan_ctx *c, int64_t need) {
     const int64_t old = c->acts.cap();
     const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 1024));
     grow_soa(c, c->act, 1, 4 * old, 4 * nc);
+    if (c->ract[0]) {
+        grow_soa(c, c->ract[0], 1, old, nc);
+        grow_soa(c, c->ract[1], 1, old, nc);
+    }
     c->acts.grow(nc);
     drop_graphs(c);
 }
@@ -346,7 +350,7 @@ void ensure_bodies(titan_ctx *c, int64_t need) {
     grow_soa(c, c->bnsync, 1, old, nc);
     grow_soa(c, c->b_lc, 1, old, nc);
     grow_soa(c, c->b_lcn, 1, old, nc);
-    grow_soa(c, c->b_tiled, 1, old, nc);
+    grow_soa(c, c->b_mode, 1, old, nc);
     grow_soa(c, c->b_k, 1, old, nc);
     grow_soa(c, c->b_d, 1, old, nc);
     grow_soa(c, c->b_kd, 1, old, nc);
@@ -379,6 +383,18 @@ void ensure_tiles(titan_ctx *c, int64_t need) {
     drop_graphs(c);
 }
 
+void ensure_rdescs(titan_ctx *c, int64_t need) {
+    if (c->rdescs.cap() >= need) return;
+    const int64_t old = c->rdescs.cap();
+    const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 64));
+    grow_soa(c, c->rdesc, 1, old, nc);
+    std::vector<ResidentDesc> dead(nc - old, ResidentDesc{-1, 0, 0, 0, 0});
+    h2d(c, c->rdesc + old, dead.data(), nc - old);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->rdescs.grow(nc);
+    drop_graphs(c);
+}
+
 void ensure_halos(titan_ctx *c, int64_t need) {
     if (c->halos.cap() >= need) return;
     const int64_t old = c->halos.cap();
@@ -401,9 +417,8 @@ void make_inc_array(titan_ctx *c, double *&p) {
 void make_act(titan_ctx *c) {
     if (c->has_act) return;
     c->has_act = true;
-    const int64_t n = std::max<int64_t>(c->incs.cap(), 1);
-    c->inc_act = dalloc<int32_t>(c, n);
-    HIPCHK(hipMemsetAsync(c->inc_act, 0xff, sizeof(int32_t) * n, c->stream));
+    const int64_t na = std::max<int64_t>(c->acts.cap(), 1);
+    for (int k = 0; k < 2; ++k) c->ract[k] = dalloc<double>(c, na);
     drop_graphs(c);
 }
 
@@ -415,9 +430,19 @@ int32_t tiled_rows(const titan_ctx *c) {
     return h;
 }
 
-bool any_body(const titan_ctx *c, bool tiled) {
+// LDS rows per buffer of the resident kernel: the largest live resident body
+int32_t resident_rows(const titan_ctx *c) {
+    int32_t h = 0;
     for (auto &b : c->bodies)
-        if (b.live && b.tiled == tiled) return true;
+        if (b.live && b.resident) h = std::max(h, (int32_t)b.n_masses);
+    return h;
+}
+
+uint8_t mode_of(const BodyH &b) { return b.resident ? kModeResident : b.tiled ? kModeTiled : kModeGather; }
+
+bool any_body(const titan_ctx *c, uint8_t mode) {
+    for (auto &b : c->bodies)
+        if (b.live && mode_of(b) == mode) return true;
     return false;
 }
 
@@ -433,10 +458,11 @@ int kind_rank(int kind) {
     }
 }
 
-// The chunks and tiles one launch covers, and which kernels it needs.
+// The chunks, tiles and resident descriptors one stepping call covers, and
+// which kernels it needs.
 struct Range {
-    int64_t c0 = 0, c1 = 0, t0 = 0, t1 = 0;
-    bool gather = false, tiled = false, all = true;
+    int64_t c0 = 0, c1 = 0, t0 = 0, t1 = 0, r0 = 0, r1 = 0;
+    bool gather = false, tiled = false, resident = false, all = true;
 };
 
 // Every live chunk and tile (up to the arenas' high-water marks; free ones
@@ -445,8 +471,10 @@ Range full_range(const titan_ctx *c) {
     Range r;
     r.c1 = c->chunks.high_water();
     r.t1 = c->tiles.high_water();
-    r.gather = any_body(c, false);
-    r.tiled = any_body(c, true);
+    r.r1 = c->rdescs.high_water();
+    r.gather = any_body(c, kModeGather);
+    r.tiled = any_body(c, kModeTiled);
+    r.resident = any_body(c, kModeResident);
     return r;
 }
 
@@ -456,8 +484,11 @@ Range body_range(const BodyH &b) {
     r.c1 = b.chunk0 + b.nchunks;
     r.t0 = b.tile0;
     r.t1 = b.tile0 + b.ntiles;
-    r.gather = !b.tiled;
-    r.tiled = b.tiled;
+    r.r0 = b.resident ? b.res : 0;
+    r.r1 = b.resident ? b.res + 1 : 0;
+    r.gather = mode_of(b) == kModeGather;
+    r.tiled = mode_of(b) == kModeTiled;
+    r.resident = b.resident;
     r.all = false;
     return r;
 }
@@ -485,8 +516,11 @@ StepArgs make_args(titan_ctx *c, int slot, const Range &rg) {
     a.b_k = c->b_k;
     a.b_d = c->b_d;
     a.b_kd = c->b_kd;
-    a.inc_act = c->inc_act;
     a.act = c->act;
+    a.ract[0] = c->ract[0];
+    a.ract[1] = c->ract[1];
+    a.chunk_act0 = c->chunk_act0;
+    a.chunk_nact = c->chunk_nact;
     for (int s = 0; s < 2; ++s) {
         a.bT[s] = c->bT[s];
         a.bn[s] = c->bn[s];
@@ -519,7 +553,11 @@ StepArgs make_args(titan_ctx *c, int slot, const Range &rg) {
     a.ntiles = rg.t1;
     a.halo = c->halo;
     a.ldim = tiled_rows(c);
-    a.b_tiled = c->b_tiled;
+    a.b_mode = c->b_mode;
+    a.rdesc = c->rdesc;
+    a.res_base = rg.r0;
+    a.nres = rg.r1;
+    a.res_rows = resident_rows(c);
     return a;
 }
 
@@ -685,7 +723,15 @@ void run_launches(titan_ctx *c, int64_t n, const Range &rg) {
         HIPCHK(hipMemcpyAsync(c->bn[s ^ 1], c->bn[s], sizeof(int64_t) * nb, hipMemcpyDeviceToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(c->bleft[s ^ 1], c->bleft[s], sizeof(int64_t) * nb, hipMemcpyDeviceToDevice, c->stream));
     }
-    int64_t rest = n;
+    if (rg.resident) {  // every step of the resident bodies, in one launch
+        const StepArgs a = make_args(c, s, rg);
+        HIPCHK(launch_rollout_resident(a, c->has_act, s ^ (int)(n & 1), c->stream));
+    }
+    int64_t rest = (rg.gather || rg.tiled) ? n : 0;
+    if (rest > 0 && c->has_act) {  // the first step's actuation factors
+        const StepArgs a = make_args(c, s, rg);
+        HIPCHK(launch_actuate(a, c->stream));
+    }
     if (rg.all && rest >= kGraphSteps) {
         hipGraphExec_t g = graph_for(c, s);  // an even number of steps: ends on slot s
         for (int64_t r = 0; r < rest / kGraphSteps; ++r) HIPCHK(hipGraphLaunch(g, c->stream));
@@ -695,6 +741,7 @@ void run_launches(titan_ctx *c, int64_t n, const Range &rg) {
         HIPCHK(launch_range(c, s, rg));
         s ^= 1;
     }
+    s = c->slot ^ (int)(n & 1);
     HIPCHK(launch_advance_T(c->Tm, c->mflags, c->chunk_body, c->bn[s], c->bnsync, rg.c0, rg.c1, nb, c->dt,
                             c->stream));
     c->slot = s;
@@ -787,7 +834,7 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     if (!d->pos || !d->m) fail(TITAN_E_INVALID, "body_create: pos and m are required");
     if (S > 0 && (!d->left || !d->right || !d->k || !d->rest))
         fail(TITAN_E_INVALID, "body_create: springs need left, right, k and rest");
-    if (d->layout < TITAN_LAYOUT_AUTO || d->layout > TITAN_LAYOUT_TILED)
+    if (d->layout < TITAN_LAYOUT_AUTO || d->layout > TITAN_LAYOUT_RESIDENT)
         fail(TITAN_E_INVALID, "body_create: bad layout");
     const int64_t nch = (N + kChunk - 1) / kChunk;
     const int64_t npad = nch * kChunk;
@@ -815,7 +862,7 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
                               : ty == TITAN_ACTIVE_EXPAND_THEN_CONTRACT ? 0.2 : d->act_a[s];
             const double ph = ty == TITAN_SPRING_GENERAL ? d->act_phi[s] : 0.0;
             act_local[s] = (int32_t)n_act++;
-            acth.insert(acth.end(), {aa, d->omega[s], ph, 0.0});
+            acth.insert(acth.end(), {aa, d->omega[s], ph, d->rest[s]});
         }
         const double ks = d->k[s], ds = d->damping ? d->damping[s] : 0.0;
         if (first) {
@@ -872,8 +919,13 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     TileBuild tb;
     bool tiled = false;
     std::vector<int32_t> nl, nr;  // endpoints in device order
-    const bool want_tiles = d->layout == TITAN_LAYOUT_TILED ||
-                            (d->layout == TITAN_LAYOUT_AUTO && N >= c->sched.tiled_min_masses);
+    if (d->layout == TITAN_LAYOUT_RESIDENT && (N < 1 || N > kResidentMax))
+        fail(TITAN_E_INVALID, "body_create: the resident layout holds 1 to " + std::to_string(kResidentMax) +
+                                  " masses, not " + std::to_string(N));
+    const bool resident = d->layout == TITAN_LAYOUT_RESIDENT ||
+                          (d->layout == TITAN_LAYOUT_AUTO && N >= 1 && N <= c->sched.resident_max_masses);
+    const bool want_tiles = !resident && (d->layout == TITAN_LAYOUT_TILED ||
+                                          (d->layout == TITAN_LAYOUT_AUTO && N >= c->sched.tiled_min_masses));
     if (want_tiles && N > kTileMasses) perm = morton_order(d->pos, N);
     {
         std::vector<int32_t> inv;
@@ -915,11 +967,21 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     // which are known once the chunks are placed (built below)
     SellBuild sb;
     if (tiled) build_sell(N, sp, per_k, per_d, &tb, 0, sb);
+    if (resident) {  // the whole body is one tile with no halo: rows = body indices
+        TileBuild one;
+        TileDesc t{};
+        t.nmass = (int32_t)N;
+        t.first = 1;
+        one.tiles.push_back(t);
+        one.tile_of_chunk.assign(nch, 0);
+        one.rmax = (int32_t)N;
+        build_sell(N, sp, per_k, per_d, &one, 0, sb);
+    }
 
     // ---- reserve space; on failure release whatever was taken
     const int64_t n_tiles = tiled ? (int64_t)tb.tiles.size() : 0;
     const int64_t n_halo = tiled ? (int64_t)tb.halo.size() : 0;
-    int64_t ch0 = -1, inc0 = -1, act0 = -1, lc0 = -1, bid = -1, tl0 = -1, hl0 = -1;
+    int64_t ch0 = -1, inc0 = -1, act0 = -1, lc0 = -1, bid = -1, tl0 = -1, hl0 = -1, rs0 = -1;
     auto take = [&](Ranges &r, int64_t len, void (*grow)(titan_ctx *, int64_t)) -> int64_t {
         int64_t at = r.alloc(len);
         if (at < 0) {
@@ -932,7 +994,7 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
         ch0 = take(c->chunks, nch, ensure_chunks);
         if ((ch0 + nch) * kChunk > kMaxMassSlots)
             fail(TITAN_E_NOMEM, "body_create: mass arena limit (2^26 slots) reached");
-        if (!tiled) {  // gather incidences name arena slots: build them now
+        if (!tiled && !resident) {  // gather incidences name arena slots: build them now
             build_sell(N, sp, per_k, per_d, nullptr, ch0 * kChunk, sb);
         }
         const int64_t n_inc_all = sb.coff[nch];
@@ -941,6 +1003,7 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
         lc0 = take(c->lcs, n_lc, ensure_lcs);
         tl0 = take(c->tiles, n_tiles, ensure_tiles);
         hl0 = take(c->halos, n_halo, ensure_halos);
+        if (resident) rs0 = take(c->rdescs, 1, ensure_rdescs);
         bid = take(c->bids, 1, ensure_bodies);
         if (per_k) make_inc_array(c, c->inc_k);
         if (per_d) make_inc_array(c, c->inc_damp);
@@ -952,6 +1015,7 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
         if (lc0 >= 0) c->lcs.release(lc0, n_lc);
         if (tl0 >= 0) c->tiles.release(tl0, n_tiles);
         if (hl0 >= 0) c->halos.release(hl0, n_halo);
+        if (rs0 >= 0) c->rdescs.release(rs0, 1);
         if (bid >= 0) c->bids.release(bid, 1);
         throw;
     }
@@ -1021,10 +1085,9 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
         }
     }
 
-    // ---- incidences: actuator indices to the arena; tiles rebased
+    // ---- incidences: actuated ones marked (with arena actuator indices); tiles rebased
     const int64_t n_inc = sb.coff[nch];
-    for (auto &x : sb.act)
-        if (x >= 0) x += (int32_t)act0;
+    mark_actuated(sb, tiled || resident, act0);
     if (tiled) {
         for (auto &t : tb.tiles) {
             t.body = (int32_t)bid;
@@ -1037,8 +1100,16 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     // ---- upload (buffer parity 0: a new body has taken no steps)
     std::vector<int32_t> cbody(nch, (int32_t)bid);
     std::vector<int64_t> cptr(nch);
-    std::vector<int32_t> cng(nch);
-    const int64_t gsz = (int64_t)(tiled ? kGroupTiled : kGroupGather) * kChunk;
+    std::vector<int32_t> cng(nch), cna(nch, 0);
+    std::vector<int64_t> ca0(nch, 0);
+    {  // each chunk takes an equal share of the body's actuators
+        const int64_t per = nch > 0 ? (n_act + nch - 1) / nch : 0;
+        for (int64_t ch = 0; ch < nch; ++ch) {
+            ca0[ch] = act0 + ch * per;
+            cna[ch] = (int32_t)std::max<int64_t>(0, std::min(per, n_act - ch * per));
+        }
+    }
+    const int64_t gsz = (int64_t)(tiled || resident ? kGroupTiled : kGroupGather) * kChunk;
     for (int64_t ch = 0; ch < nch; ++ch) {
         cptr[ch] = inc0 + sb.coff[ch];
         cng[ch] = (int32_t)((sb.coff[ch + 1] - sb.coff[ch]) / gsz);
@@ -1061,26 +1132,26 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     h2d(c, c->chunk_body + ch0, cbody.data(), nch);
     h2d(c, c->chunk_ptr + ch0, cptr.data(), nch);
     h2d(c, c->chunk_ng + ch0, cng.data(), nch);
+    h2d(c, c->chunk_act0 + ch0, ca0.data(), nch);
+    h2d(c, c->chunk_nact + ch0, cna.data(), nch);
     if (n_inc > 0) {
         h2d(c, c->inc_nbr + inc0, sb.nbr.data(), n_inc);
         h2d(c, c->inc_rest + inc0, sb.rest.data(), n_inc);
         if (per_k) h2d(c, c->inc_k + inc0, sb.k.data(), n_inc);
         if (per_d) h2d(c, c->inc_damp + inc0, sb.damp.data(), n_inc);
-        if (c->inc_act) {
-            if (!sb.act.empty()) h2d(c, c->inc_act + inc0, sb.act.data(), n_inc);
-            else HIPCHK(hipMemsetAsync(c->inc_act + inc0, 0xff, sizeof(int32_t) * n_inc, c->stream));
-        }
     }
     if (tiled) {
         h2d(c, c->tile + tl0, tb.tiles.data(), n_tiles);
         h2d(c, c->halo + hl0, tb.halo.data(), n_halo);
     }
+    const ResidentDesc rdh{(int32_t)bid, (int32_t)ch0, (int32_t)N, (int32_t)n_act, act0};
+    if (resident) h2d(c, c->rdesc + rs0, &rdh, 1);
     if (!acth.empty()) h2d(c, c->act + 4 * act0, acth.data(), (int64_t)acth.size());
     if (!lch.empty()) h2d(c, c->lc + lc0, lch.data(), (int64_t)lch.size());
     const double t0 = d->t0;
     const int64_t zero = 0;
     const int32_t c0 = (int32_t)ch0, lcb = (int32_t)(shared ? lc0 : 0), lcn = (int32_t)(shared ? n_lc : 0);
-    const uint8_t tflag = tiled ? 1 : 0;
+    const uint8_t mflag = resident ? kModeResident : tiled ? kModeTiled : kModeGather;
     for (int s = 0; s < 2; ++s) {
         h2d(c, c->bT[s] + bid, &t0, 1);
         h2d(c, c->bn[s] + bid, &zero, 1);
@@ -1090,7 +1161,7 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     h2d(c, c->b_chunk0 + bid, &c0, 1);
     h2d(c, c->b_lc + bid, &lcb, 1);
     h2d(c, c->b_lcn + bid, &lcn, 1);
-    h2d(c, c->b_tiled + bid, &tflag, 1);
+    h2d(c, c->b_mode + bid, &mflag, 1);
     const uint8_t kdf = (uint8_t)((per_k ? 1 : 0) | (per_d ? 2 : 0));
     h2d(c, c->b_k + bid, &k0, 1);
     h2d(c, c->b_d + bid, &d0, 1);
@@ -1101,6 +1172,8 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     b = BodyH{};
     b.live = true;
     b.tiled = tiled;
+    b.resident = resident;
+    b.res = rs0;
     b.n_masses = N;
     b.chunk0 = ch0;
     b.nchunks = nch;
@@ -1135,10 +1208,14 @@ void body_destroy(titan_ctx *c, int32_t id) {
         }
         h2d(c, c->tile + b.tile0, dead.data(), b.ntiles);
     }
+    if (b.resident) {
+        const ResidentDesc dead{-1, 0, 0, 0, 0};
+        h2d(c, c->rdesc + b.res, &dead, 1);
+    }
     const int32_t m1 = -1;
-    const uint8_t z = 0;
+    const uint8_t z = kModeGather;
     h2d(c, c->b_chunk0 + id, &m1, 1);
-    h2d(c, c->b_tiled + id, &z, 1);
+    h2d(c, c->b_mode + id, &z, 1);
     HIPCHK(hipStreamSynchronize(c->stream));
     c->chunks.release(b.chunk0, b.nchunks);
     c->incs.release(b.inc0, b.inc_len);
@@ -1146,6 +1223,7 @@ void body_destroy(titan_ctx *c, int32_t id) {
     c->lcs.release(b.lc0, b.lc_len);
     c->tiles.release(b.tile0, b.ntiles);
     c->halos.release(b.halo0, b.nhalo);
+    if (b.resident) c->rdescs.release(b.res, 1);
     c->bids.release(id, 1);
     b = BodyH{};
 }
@@ -1478,8 +1556,9 @@ int titan_set_schedule(titan_ctx *c, const titan_schedule *s) {
     if (s->tile_rows < kChunk || s->tile_rows > kMaxTileRows)
         fail(TITAN_E_INVALID, "set_schedule: tile_rows must be in [64, " + std::to_string(kMaxTileRows) + "]");
     if (s->tiled_min_masses < 0) fail(TITAN_E_INVALID, "set_schedule: negative tiled_min_masses");
+    if (s->resident_max_masses < 0 || s->resident_max_masses > kResidentMax)
+        fail(TITAN_E_INVALID, "set_schedule: resident_max_masses must be in [0, " + std::to_string(kResidentMax) + "]");
     c->sched = *s;
-    c->sched.pad = 0;
     API_END(c)
 }
 
@@ -1544,28 +1623,42 @@ int titan_autotune(titan_ctx *c, const titan_body_desc *samples, int32_t n_sampl
     API_BEGIN
     if (!samples || n_samples <= 0 || replicas <= 0 || steps <= 0)
         fail(TITAN_E_INVALID, "autotune: need samples, replicas > 0 and steps > 0");
+    const int32_t res[] = {0, kResidentMax};
     const int64_t mins[] = {0, 128, 256, 512, 1024, INT64_MAX};
     const int32_t rows[] = {682, kMaxTileRows};
-    titan_schedule win{0, kTileRows, 0};
-    double win_rate = -1.0, never_rate = -1.0;
+    titan_schedule win{0, kTileRows, kResidentMax};
+    double win_rate = -1.0;
+    // candidates that lay out every sample alike are timed once
+    std::map<std::string, double> seen;
     int q = 0;
-    for (int64_t mn : mins)
-        for (int32_t rw : rows) {
-            const titan_schedule sc{mn, rw, 0};
-            double rate;
-            if (mn == INT64_MAX && never_rate >= 0) {
-                rate = never_rate;  // no body is tiled: the row budget is moot
-            } else {
-                rate = time_schedule(c, samples, n_samples, replicas, steps, sc);
-                if (mn == INT64_MAX) never_rate = rate;
+    for (int32_t rs : res)
+        for (int64_t mn : mins)
+            for (int32_t rw : rows) {
+                const titan_schedule sc{mn, rw, rs};
+                std::string sig;
+                bool any_tiled = false;
+                for (int32_t k = 0; k < n_samples; ++k) {
+                    const int64_t n = samples[k].n_masses;
+                    const char m = (n >= 1 && n <= rs) ? 'r' : (n >= mn ? 't' : 'g');
+                    any_tiled |= m == 't';
+                    sig += m;
+                }
+                if (any_tiled) sig += std::to_string(rw);
+                double rate;
+                auto it = seen.find(sig);
+                if (it != seen.end()) {
+                    rate = it->second;
+                } else {
+                    rate = time_schedule(c, samples, n_samples, replicas, steps, sc);
+                    seen[sig] = rate;
+                }
+                if (rate_out) rate_out[q] = rate;
+                ++q;
+                if (rate > win_rate) {
+                    win_rate = rate;
+                    win = sc;
+                }
             }
-            if (rate_out) rate_out[q] = rate;
-            ++q;
-            if (rate > win_rate) {
-                win_rate = rate;
-                win = sc;
-            }
-        }
     c->sched = win;
     if (best) *best = win;
     API_END(c)
@@ -1643,8 +1736,9 @@ int titan_kernel_name(const titan_ctx *c, char *buf, int64_t len) {
     if (!buf || len <= 0) return TITAN_E_INVALID;
     const bool act = c->has_act;
     std::string n;
-    if (any_body(c, false)) n = step_kernel_name(act);
-    if (any_body(c, true)) n += std::string(n.empty() ? "" : "+") + step_tiles_kernel_name(act);
+    if (any_body(c, kModeGather)) n = step_kernel_name(act);
+    if (any_body(c, kModeTiled)) n += std::string(n.empty() ? "" : "+") + step_tiles_kernel_name(act);
+    if (any_body(c, kModeResident)) n += std::string(n.empty() ? "" : "+") + resident_kernel_name(act);
     std::snprintf(buf, (size_t)len, "%s", n.c_str());
     return TITAN_OK;
 }
@@ -1653,7 +1747,8 @@ int titan_body_layout(const titan_ctx *c, int32_t body_id, int32_t *layout) {
     NEED_CTX(c);
     if (!layout) return TITAN_E_INVALID;
     if (body_id < 0 || body_id >= (int64_t)c->bodies.size() || !c->bodies[body_id].live) return TITAN_E_NOTFOUND;
-    *layout = c->bodies[body_id].tiled ? TITAN_LAYOUT_TILED : TITAN_LAYOUT_GATHER;
+    const BodyH &b = c->bodies[body_id];
+    *layout = b.resident ? TITAN_LAYOUT_RESIDENT : b.tiled ? TITAN_LAYOUT_TILED : TITAN_LAYOUT_GATHER;
     return TITAN_OK;
 }
 

@@ -12,6 +12,10 @@ const char *step_kernel_name(bool actuated);
 hipError_t launch_step_tiles(const StepArgs &a, bool actuated, hipStream_t s);
 const char *step_tiles_kernel_name(bool actuated);
 size_t tiles_lds_bytes(int ldim);
+// Actuation factors for the first step of a stepping call (launch slot a.slot).
+hipError_t launch_actuate(const StepArgs &a, hipStream_t s);
+hipError_t launch_rollout_resident(const StepArgs &a, bool actuated, int final_slot, hipStream_t s);
+const char *resident_kernel_name(bool actuated);
 hipError_t set_tiles_lds_limit();
 hipError_t launch_advance_T(double *Tm, const uint32_t *mflags, const int32_t *chunk_body,
                             const int64_t *bn, int64_t *bnsync, int64_t chunk_begin, int64_t chunk_end,

@@ -248,10 +248,14 @@ struct Names<4> {
         return {reinterpret_cast<const uint4 *>(base + ebase)[(int64_t)g * kChunk + lane]};
     }
     __device__ void get(uint32_t (&o)[4]) const {
-        o[0] = v.x;
-        o[1] = v.y;
-        o[2] = v.z;
-        o[3] = v.w;
+        o[0] = v.x & ~kActBit32;
+        o[1] = v.y & ~kActBit32;
+        o[2] = v.z & ~kActBit32;
+        o[3] = v.w & ~kActBit32;
+    }
+    __device__ bool actuated(int q) const {
+        const uint32_t w = q == 0 ? v.x : q == 1 ? v.y : q == 2 ? v.z : v.w;
+        return (w & kActBit32) != 0;
     }
 };
 template <>
@@ -261,9 +265,10 @@ struct Names<2> {
         return {base[ebase + (int64_t)g * kChunk + lane]};
     }
     __device__ void get(uint32_t (&o)[2]) const {
-        o[0] = v & 0xffffu;
-        o[1] = v >> 16;
+        o[0] = v & (0xffffu & ~kActBit16);
+        o[1] = (v >> 16) & ~kActBit16;
     }
+    __device__ bool actuated(int q) const { return ((v >> (16 * q)) & kActBit16) != 0; }
 };
 
 // The body's spring constant and damping: by value when every spring of
@@ -274,14 +279,13 @@ struct BodyKD {
     uint32_t kd;
 };
 
-// One group's incidence data in registers: the names, rest lengths,
-// per-spring constants (bodies that have them) and actuator ids (actuated
-// only).
+// One group's incidence data in registers: the names, rest lengths (or
+// actuator indices), per-spring constants (bodies that have them) and the
+// actuated rest lengths (actuated only).
 template <bool ACT, int G>
 struct Group {
     Names<G> n;
-    double rest[G], k[G], damp[G];
-    int32_t act[G];
+    double rest[G], k[G], damp[G], f[G];
     __device__ __forceinline__ void load(const StepArgs &a, int64_t ebase, int lane, int g, const BodyKD &bkd) {
         constexpr int P = G / 2;  // 16-byte pairs per group and lane
         n = Names<G>::load(a.inc_nbr, ebase, lane, g);
@@ -306,32 +310,43 @@ struct Group {
                 damp[2 * h] = damp[2 * h + 1] = bkd.d;
             }
         }
-        if (ACT) {
-            const int64_t e = ebase + ((int64_t)g * kChunk + lane) * G;
+    }
+    // The actuated rest lengths of the group's actuated incidences (rc: this
+    // step's), skipped for an incidence slot where no lane of the wave has
+    // one.  Issued before the next group's loads, so that waiting for them
+    // does not wait for those.
+    __device__ __forceinline__ void fetch_f(const double *__restrict__ rc) {
 #pragma unroll
-            for (int q = 0; q < G; ++q) act[q] = a.inc_act[e + q];
+        for (int q = 0; q < G; ++q) {
+            const bool on = n.actuated(q);
+            f[q] = rest[q];
+            if (__any(on)) {
+                const double v = rc[on ? __double_as_longlong(rest[q]) : 0];
+                f[q] = on ? v : rest[q];
+            }
         }
     }
 };
 
+// Actuated rest lengths at time T of actuators [first, first + n), one per
+// lane in turn: rest * (1 + a sin(omega T + phi)) (src/sim.cu:1058-1066, the
+// reference's two kinds being a = -0.2 and 0.2 with phi = 0), evaluated in
+// the reference's order.
+__device__ __forceinline__ void actuate_slice(const StepArgs &a, int64_t first, int n, double T,
+                                              double *__restrict__ dst, int lane, int stride) {
+    const double2 *__restrict__ A = reinterpret_cast<const double2 *>(a.act);
+    for (int q = lane; q < n; q += stride) {
+        const double2 aw = A[2 * (first + q)], pr = A[2 * (first + q) + 1];  // (a, omega), (phi, rest)
+        dst[first + q] = pr.y * (1 + aw.x * sin(aw.y * T + pr.x));
+    }
+}
+
 // Add -G of one incident spring (axis nrm, u) to the mass's force.
 template <bool ACT, class Grp>
-__device__ __forceinline__ void add_spring(const StepArgs &a, MassRegs &r, const Grp &cur, int q, double T,
-                                           const double2 (&nb)[3], double nrm, double ux, double uy,
-                                           double uz) {
+__device__ __forceinline__ void add_spring(MassRegs &r, const Grp &cur, int q, const double2 (&nb)[3],
+                                           double nrm, double ux, double uy, double uz) {
     const double dvx = r.vx - nb[1].y, dvy = r.vy - nb[2].x, dvz = r.vz - nb[2].y;
-    double rl = cur.rest[q];
-    if (ACT) {
-        // skipped when no lane of the wave has an actuator here (a uniform
-        // branch); otherwise evaluated for every lane (a lane without an
-        // actuator reads actuator 0 and keeps rl), free of divergent branches
-        const int32_t ai = cur.act[q];
-        if (__any(ai >= 0)) {
-            const double *A = a.act + 4 * (int64_t)(ai >= 0 ? ai : 0);
-            const double ra = rl * (1 + A[0] * sin(A[1] * T + A[2]));
-            rl = ai >= 0 ? ra : rl;
-        }
-    }
+    const double rl = ACT ? cur.f[q] : cur.rest[q];  // ACT: the actuated rest length where marked
     const double k = cur.k[q];
     const double dmp = cur.damp[q];
     const double mag = k * (rl - nrm);
@@ -347,8 +362,8 @@ __device__ __forceinline__ void add_spring(const StepArgs &a, MassRegs &r, const
 // are straight-line code that interleaves; otherwise each incidence that
 // exists takes the general path.
 template <bool ACT, int G, int K, class Fetch>
-__device__ __forceinline__ void eval_set(const StepArgs &a, MassRegs &r, const Group<ACT, G> (&set)[K],
-                                         int g0, int d, double T, Fetch &fetch) {
+__device__ __forceinline__ void eval_set(MassRegs &r, const Group<ACT, G> (&set)[K], int g0, int d,
+                                         Fetch &fetch) {
     constexpr int N = G * K;
     double2 nb[N][3];
 #pragma unroll
@@ -374,7 +389,7 @@ __device__ __forceinline__ void eval_set(const StepArgs &a, MassRegs &r, const G
             const double nrm = fast_sqrt(n2[e]);
             const double y = fast_recip(nrm);
             const double ux = fast_div(tx[e], nrm, y), uy = fast_div(ty[e], nrm, y), uz = fast_div(tz[e], nrm, y);
-            add_spring<ACT>(a, r, set[e / G], e % G, T, nb[e], nrm, ux, uy, uz);
+            add_spring<ACT>(r, set[e / G], e % G, nb[e], nrm, ux, uy, uz);
         }
     } else {
 #pragma unroll
@@ -382,7 +397,7 @@ __device__ __forceinline__ void eval_set(const StepArgs &a, MassRegs &r, const G
             if (G * g0 + e >= d) break;
             double nrm, ux, uy, uz;
             unit_axis(tx[e], ty[e], tz[e], nrm, ux, uy, uz);
-            add_spring<ACT>(a, r, set[e / G], e % G, T, nb[e], nrm, ux, uy, uz);
+            add_spring<ACT>(r, set[e / G], e % G, nb[e], nrm, ux, uy, uz);
         }
     }
 }
@@ -414,20 +429,29 @@ struct SpringWalk {
         for (int j = 0; j < K; ++j) A[j].load(a, ebase, lane, min(j, ng - 1), bkd);
     }
 
+    // rc: this step's actuation factors (actuated only)
     template <class Fetch>
-    __device__ __forceinline__ void run(const StepArgs &a, MassRegs &r, int d, double T, Fetch fetch) {
+    __device__ __forceinline__ void run(const StepArgs &a, MassRegs &r, int d, const double *rc, Fetch fetch) {
         r.fx = 0.0;
         r.fy = 0.0;
         r.fz = 0.0;
         const int last = ng - 1;
         for (int g = 0; g < ng; g += 2 * K) {
+            if (ACT) {
+#pragma unroll
+                for (int j = 0; j < K; ++j) A[j].fetch_f(rc);
+            }
 #pragma unroll
             for (int j = 0; j < K; ++j) B[j].load(a, ebase, lane, min(g + K + j, last), bkd);
-            eval_set<ACT, G, K>(a, r, A, g, d, T, fetch);
+            eval_set<ACT, G, K>(r, A, g, d, fetch);
             if (g + K >= ng) break;
+            if (ACT) {
+#pragma unroll
+                for (int j = 0; j < K; ++j) B[j].fetch_f(rc);
+            }
 #pragma unroll
             for (int j = 0; j < K; ++j) A[j].load(a, ebase, lane, min(g + 2 * K + j, last), bkd);
-            eval_set<ACT, G, K>(a, r, B, g + K, d, T, fetch);
+            eval_set<ACT, G, K>(r, B, g + K, d, fetch);
         }
     }
 };
@@ -474,7 +498,7 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     if (chunk >= a.nchunks) return;
     const int lane = threadIdx.x & (kChunk - 1);
     const int b = uniform(a.chunk_body[chunk]);
-    if (b < 0 || a.b_tiled[b]) return;
+    if (b < 0 || a.b_mode[b] != kModeGather) return;
     // read before this kernel's first store: scalar loads
     const int64_t ebase = a.chunk_ptr[chunk];
     const int ng = a.chunk_ng[chunk];
@@ -498,6 +522,9 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     const double m = a.m[i];
     SpringWalk<ACT, kGroupGather> w;
     w.prime(a, ebase, ng, lane, bkd);
+    // the chunk's share of its body's actuation factors for the next step
+    // (the body's clock then reads T + dt), while the loads are in flight
+    if (ACT && left > 1) actuate_slice(a, a.chunk_act0[chunk], a.chunk_nact[chunk], T + a.dt, a.ract[s ^ 1], lane, kChunk);
     if (fl & kPad) return;
     if (fl & kFixed) {  // src/sim.cu:1100-1101: a fixed mass keeps its state
         store_nt(&nxt[3 * i], o0.x, o0.y);
@@ -513,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     r.vy = o2.x;
     r.vz = o2.y;
     // 48 bytes per mass; the arena caps a buffer below 4 GiB
-    w.run(a, r, d, T, [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
+    w.run(a, r, d, a.ract[s], [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
         const uint32_t ob = o * 48u;
         q0 = ldo(cur, ob);
         q1 = ldo(cur, ob + 16u);
@@ -583,6 +610,9 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
         d = a.deg[i];
         m = a.m[i];
         w.prime(a, ebase, ng, t & (kChunk - 1), bkd);
+        // the chunk's share of its body's actuation factors for the next step
+        if (ACT && left > 1)
+            actuate_slice(a, a.chunk_act0[chunk], a.chunk_nact[chunk], T + a.dt, a.ract[s ^ 1], t & (kChunk - 1), kChunk);
     }
     // halo rows, two per thread per round: both indices, then both states.
     // Loads past the halo are clamped to its last entry and stored to the
@@ -623,7 +653,7 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     r.vx = o1.y;
     r.vy = o2.x;
     r.vz = o2.y;
-    w.run(a, r, d, T, [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
+    w.run(a, r, d, a.ract[s], [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
         q0 = S[3 * o];
         q1 = S[3 * o + 1];
         q2 = S[3 * o + 2];
@@ -631,6 +661,133 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     double ax, ay, az;
     mass_pass(a, r, i, fl, b, m, ax, ay, az);
     store_mass(a, nxt, i, r, ax, ay, az, left == 1);
+}
+
+// Actuation factors for the first step of a stepping call: each wave takes
+// one chunk's share, at its body's clock, for bodies with steps left that
+// the gather or tiled kernels step (a resident body computes its own).
+__global__ __launch_bounds__(kBlock) void k_actuate(const StepArgs a) {
+    const int64_t chunk = a.chunk_base + (int64_t)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock +
+                          uniform(threadIdx.x >> 6);
+    if (chunk >= a.nchunks) return;
+    const int b = uniform(a.chunk_body[chunk]);
+    if (b < 0 || a.b_mode[b] == kModeResident) return;
+    const int s = a.slot;
+    if (a.bleft[s][b] <= 0) return;
+    actuate_slice(a, a.chunk_act0[chunk], a.chunk_nact[chunk], a.bT[s][b], a.ract[s], threadIdx.x & (kChunk - 1),
+                  kChunk);
+}
+
+// Resident rollout: one workgroup per body of at most kResidentMax masses,
+// which runs the body's whole budget of steps for this call in one launch.
+// The body's state lives in LDS, double-buffered: step k reads buffer k & 1
+// and writes the other, with one barrier per step, so it never returns to
+// HBM until the last step.  Each wave walks the body's chunks w, w + 4, ...;
+// every mass is computed exactly as the other kernels do (the same spring
+// walk, mass pass and Euler update, T += dt per step), so results are
+// identical.  The body's clock fields are written to launch slot
+// final_slot, where the stepping call ends.
+template <bool ACT>
+__global__ __launch_bounds__(kResidentBlock) void k_rollout_resident(const StepArgs a, int final_slot) {
+    extern __shared__ double2 S[];  // two buffers of 3 x res_rows pairs
+    const int64_t rid = a.res_base + blockIdx.x;
+    if (rid >= a.nres) return;
+    const ResidentDesc *__restrict__ rd = a.rdesc + rid;
+    const int b = rd->body;
+    if (b < 0) return;
+    const int nmass = rd->nmass, chunk0 = rd->chunk0, nact = rd->nact;
+    const int64_t act0 = rd->act0;
+    const int nch = (nmass + kChunk - 1) / kChunk;
+    const BodyKD bkd{a.b_k[b], a.b_d[b], a.b_kd[b]};
+    const int s0 = a.slot;
+    const int64_t K = a.bleft[s0][b];
+    const int64_t nst = a.bn[s0][b];
+    double T = a.bT[s0][b];
+    const int t = threadIdx.x, wave = uniform(t >> 6), lane = t & (kChunk - 1);
+    if (K <= 0) {  // not stepped in this call: only its clock fields move to final_slot
+        if (t == 0) {
+            a.bT[final_slot][b] = T;
+            a.bn[final_slot][b] = nst;
+            a.bleft[final_slot][b] = 0;
+        }
+        return;
+    }
+    const int64_t g0 = (int64_t)chunk0 * kChunk;
+    const int64_t n6 = a.ncap;
+    double2 *__restrict__ B0 = S, *__restrict__ B1 = S + 3 * a.res_rows;
+    // actuation factors: step k reads ract[k & 1] (this block's actuators
+    // only), written before the barrier that precedes the step
+    if (ACT) actuate_slice(a, act0, nact, T, a.ract[0], t, kResidentBlock);
+    {
+        const double2 *__restrict__ src = reinterpret_cast<const double2 *>(a.st + (int64_t)(nst & 1) * 6 * n6);
+        for (int q = t; q < nmass; q += kResidentBlock) {
+            const int64_t i = g0 + q;
+            B0[3 * q] = src[3 * i];
+            B0[3 * q + 1] = src[3 * i + 1];
+            B0[3 * q + 2] = src[3 * i + 2];
+        }
+    }
+    __syncthreads();
+    for (int64_t k = 0; k < K; ++k) {
+        const double2 *__restrict__ cur = (k & 1) ? B1 : B0;
+        double2 *__restrict__ nxt = (k & 1) ? B0 : B1;
+        for (int c = wave; c < nch; c += kResidentBlock / kChunk) {
+            const int q = c * kChunk + lane;
+            const int64_t i = g0 + q;
+            const uint32_t fl = a.mflags[i];
+            if (fl & kPad) continue;
+            const double2 o0 = cur[3 * q], o1 = cur[3 * q + 1], o2 = cur[3 * q + 2];
+            if (fl & kFixed) {  // src/sim.cu:1100-1101
+                nxt[3 * q] = o0;
+                nxt[3 * q + 1] = o1;
+                nxt[3 * q + 2] = o2;
+                continue;
+            }
+            MassRegs r;
+            r.px = o0.x;
+            r.py = o0.y;
+            r.pz = o1.x;
+            r.vx = o1.y;
+            r.vy = o2.x;
+            r.vz = o2.y;
+            SpringWalk<ACT, kGroupTiled> w;
+            w.prime(a, a.chunk_ptr[chunk0 + c], a.chunk_ng[chunk0 + c], lane, bkd);
+            w.run(a, r, a.deg[i], a.ract[k & 1], [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
+                q0 = cur[3 * o];
+                q1 = cur[3 * o + 1];
+                q2 = cur[3 * o + 2];
+            });
+            double ax, ay, az;
+            mass_pass(a, r, i, fl, b, a.m[i], ax, ay, az);
+            nxt[3 * q] = make_double2(r.px, r.py);
+            nxt[3 * q + 1] = make_double2(r.pz, r.vx);
+            nxt[3 * q + 2] = make_double2(r.vy, r.vz);
+            if (k == K - 1) {
+                a.acc[i] = ax;
+                a.acc[n6 + i] = ay;
+                a.acc[2 * n6 + i] = az;
+            }
+        }
+        T = T + a.dt;  // src/sim.cu:1590
+        if (ACT && k + 1 < K) actuate_slice(a, act0, nact, T, a.ract[(k + 1) & 1], t, kResidentBlock);
+        __syncthreads();
+    }
+    // the final state to the state buffer of the body's new step parity
+    {
+        const double2 *__restrict__ fin = (K & 1) ? B1 : B0;
+        double2 *__restrict__ dst = reinterpret_cast<double2 *>(a.st + (int64_t)((nst + K) & 1) * 6 * n6);
+        for (int q = t; q < nmass; q += kResidentBlock) {
+            const int64_t i = g0 + q;
+            store_nt(&dst[3 * i], fin[3 * q].x, fin[3 * q].y);
+            store_nt(&dst[3 * i + 1], fin[3 * q + 1].x, fin[3 * q + 1].y);
+            store_nt(&dst[3 * i + 2], fin[3 * q + 2].x, fin[3 * q + 2].y);
+        }
+    }
+    if (t == 0) {
+        a.bT[final_slot][b] = T;
+        a.bn[final_slot][b] = nst + K;
+        a.bleft[final_slot][b] = 0;
+    }
 }
 
 // Per-mass local time: T += dt once per step the body took since the last
@@ -685,7 +842,36 @@ hipError_t launch_step_tiles(const StepArgs &a, bool act, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_actuate(const StepArgs &a, hipStream_t s) {
+    const int64_t n = a.nchunks - a.chunk_base;
+    if (n <= 0) return hipSuccess;
+    const dim3 g((unsigned)((n + kWavesPerBlock - 1) / kWavesPerBlock)), blk(kBlock);
+    hipLaunchKernelGGL(k_actuate, g, blk, 0, s, a);
+    return hipGetLastError();
+}
+
+size_t resident_lds_bytes(int rows) { return (size_t)2 * 48 * rows; }
+
+hipError_t launch_rollout_resident(const StepArgs &a, bool act, int final_slot, hipStream_t s) {
+    if (a.nres <= a.res_base) return hipSuccess;
+    const size_t lds = resident_lds_bytes(a.res_rows);
+    const dim3 g((unsigned)(a.nres - a.res_base)), blk(kResidentBlock);
+    if (act) hipLaunchKernelGGL((k_rollout_resident<true>), g, blk, lds, s, a, final_slot);
+    else hipLaunchKernelGGL((k_rollout_resident<false>), g, blk, lds, s, a, final_slot);
+    return hipGetLastError();
+}
+
+const char *resident_kernel_name(bool act) {
+    return act ? "k_rollout_resident<actuated>" : "k_rollout_resident<passive>";
+}
+
 hipError_t set_tiles_lds_limit() {
+    const int rb = (int)resident_lds_bytes(kResidentMax);
+    hipError_t er = hipFuncSetAttribute((const void *)k_rollout_resident<false>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, rb);
+    if (er != hipSuccess) return er;
+    er = hipFuncSetAttribute((const void *)k_rollout_resident<true>, hipFuncAttributeMaxDynamicSharedMemorySize, rb);
+    if (er != hipSuccess) return er;
     const int bytes = (int)tiles_lds_bytes(kMaxTileRows);
     const hipError_t e =
         hipFuncSetAttribute((const void *)k_step_tiles<false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);

@@ -59,10 +59,36 @@ struct TileDesc {
     int32_t pad;
 };
 
+// Resident layout (k_rollout_resident): a body of at most kResidentMax
+// masses is held whole in one workgroup's LDS, double-buffered, for every
+// step of a stepping call; its incidences name the other endpoint by its
+// index in the body (uint16, as the tiled layout's LDS rows, with the whole
+// body as one tile and no halo).
+constexpr int kResidentMax = 1024;
+constexpr int kResidentBlock = 256;
+
+struct ResidentDesc {
+    int32_t body;     // -1: free
+    int32_t chunk0;   // first chunk (arena index)
+    int32_t nmass;
+    int32_t nact;     // the body's actuators: [act0, act0 + nact)
+    int64_t act0;
+};
+
+// per-body layout (b_mode)
+constexpr uint8_t kModeGather = 0, kModeTiled = 1, kModeResident = 2;
+
 // incidences per SELL group: gather (four uint32 slots in one 16-byte load)
 // and tiled (two uint16 LDS rows in one 4-byte load)
 constexpr int kGroupGather = 4;
 constexpr int kGroupTiled = 2;
+
+// An actuated incidence is marked by the top bit of its name (bit 31 of a
+// gather slot, bit 15 of a 16-bit LDS row); its rest-length entry then holds
+// the actuator's arena index (an int64 in the same 8 bytes), whose actuated
+// rest length the step reads from the factor buffer instead.
+constexpr uint32_t kActBit32 = 0x80000000u;
+constexpr uint32_t kActBit16 = 0x8000u;
 
 // local constraint entry (a reference CudaContactPlane / CudaBall /
 // CudaConstraintPlane / CudaDirection, src/object.cu:53-141)
@@ -98,8 +124,14 @@ struct StepArgs {
     const double *inc_rest;
     const double *inc_k;      // per incidence, for bodies with per-spring k (else null)
     const double *inc_damp;   // per incidence, for bodies with per-spring damping (else null)
-    const int32_t *inc_act;   // null when no actuators
-    const double *act;        // 4 per actuator: a, omega, phi, 0
+    const double *act;        // 4 per actuator: a, omega, phi, rest
+    // actuated rest lengths rest (1 + a sin(omega T + phi)), one per
+    // actuator, for the step that reads launch slot s (ract[s]); each chunk
+    // computes its share of its body's actuators, [chunk_act0, chunk_act0 +
+    // chunk_nact), for the next step
+    double *ract[2];
+    const int64_t *chunk_act0;
+    const int32_t *chunk_nact;
     // bodies: dynamic fields double-buffered by launch slot
     double *bT[2];
     int64_t *bn[2];
@@ -130,7 +162,11 @@ struct StepArgs {
     int64_t tile_base, ntiles;    // launch range: tiles [tile_base, ntiles)
     const int32_t *halo;      // arena mass slots
     int32_t ldim;             // LDS rows: largest nmass + nhalo of a tile
-    const uint8_t *b_tiled;   // per body: 1 = stepped by k_step_tiles
+    const uint8_t *b_mode;    // per body: kModeGather, kModeTiled or kModeResident
+    // resident path
+    const ResidentDesc *rdesc;
+    int64_t res_base, nres;   // launch range: descriptors [res_base, nres)
+    int32_t res_rows;         // LDS rows per buffer: the largest resident body
 };
 
 }  // namespace titan_mi355x

@@ -2,6 +2,7 @@
 #include "tiles.h"
 
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <numeric>
 
@@ -204,6 +205,23 @@ void build_sell(int64_t n, const SpringsIn &sp, bool per_k, bool per_d, const Ti
         place(R, L, s);  // a self-spring is listed for its right end first
         place(L, R, s);
     }
+}
+
+void mark_actuated(SellBuild &sb, bool packed, int64_t act0) {
+    if (sb.act.empty()) return;
+    const int64_t nch = (int64_t)sb.coff.size() - 1;
+    for (int64_t c = 0; c < nch; ++c)
+        for (int64_t e = sb.coff[c]; e < sb.coff[c + 1]; ++e) {
+            if (sb.act[e] < 0) continue;
+            const int64_t idx = act0 + sb.act[e];
+            std::memcpy(&sb.rest[e], &idx, sizeof(idx));
+            if (packed) {
+                const int64_t rel = e - sb.coff[c];
+                sb.nbr[sb.coff[c] + rel / 2] |= kActBit16 << (16 * (int)(rel % 2));
+            } else {
+                sb.nbr[e] |= kActBit32;
+            }
+        }
 }
 
 }  // namespace titan_mi355x

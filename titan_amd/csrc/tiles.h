@@ -46,6 +46,12 @@ struct SellBuild {
     std::vector<int32_t> act;      // body-local actuator index or -1, only if sp.act
 };
 
+// Mark the actuated incidences (sb.act[e] >= 0) for the kernels: the name's
+// top bit is set (bit 31 of a gather slot; bit 15 of a packed 16-bit row
+// when `packed`) and the rest-length entry takes the actuator's arena index,
+// act0 + sb.act[e], as an int64 in its 8 bytes.
+void mark_actuated(SellBuild &sb, bool packed, int64_t act0);
+
 // Order of the masses along a Morton curve of their positions (3 doubles
 // each).  Returns perm with perm[new] = old; ties keep the original order.
 std::vector<int32_t> morton_order(const double *pos, int64_t n);
