@@ -314,6 +314,9 @@ def main() -> int:
     ap.add_argument("--only", choices=["lattice", "batched", "serving", "irregular"], default=None,
                     help="run one workload (its figures go in the same JSON line)")
     ap.add_argument("--bodies", type=int, default=4096, help="batched workload: robot count")
+    ap.add_argument("--requests", type=int, default=256, help="serving workload: requests per arrival rate")
+    ap.add_argument("--lambdas", default="50,200,1000,2000",
+                    help="serving workload: Poisson arrival rates (requests/s), comma-separated")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times (the "
                          "reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -349,8 +352,9 @@ def main() -> int:
     if args.only == "batched" or (args.only is None and not args.no_extra):
         extra["batched"] = bench_batched(args.bodies, args.seed, rollout=100, sample=min(512, args.bodies))
     if args.only == "serving" or (args.only is None and not args.no_extra):
-        extra["serving"] = bench_serving(args.seed, n_req=64, lams=(50, 200, 1000, 2000), quantum=64,
-                                         n_base=8)
+        lams = tuple(float(x) for x in args.lambdas.split(","))
+        extra["serving"] = bench_serving(args.seed, n_req=args.requests, lams=lams,
+                                         quantum=64, n_base=16)
     if args.only == "irregular" or (args.only is None and not args.no_extra):
         extra["irregular"] = bench_irregular(2_000_000, 1000, args.seed, steps=500, warmup=20)
     if args.only in ("batched", "serving", "irregular"):

@@ -18,10 +18,12 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
 
+#include "engine_internal.h"
 #include "kernels.h"
 #include "layout.h"
 #include "tiles.h"
@@ -113,7 +115,8 @@ struct BodyH {
 
 struct titan_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // every launch, in order
+    hipStream_t up = nullptr;      // uploads of new bodies into free arena space
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timing = false;
     std::string err;
@@ -189,6 +192,14 @@ struct titan_ctx {
     int32_t inject = -1;
 
     std::map<void *, size_t> allocs;
+    // destroyed bodies whose slots are released once `ev` (recorded after
+    // their deactivation on the step stream) has completed
+    struct Pending {
+        BodyH b;
+        int32_t id;
+        hipEvent_t ev;
+    };
+    std::vector<Pending> pending;
 };
 
 namespace {
@@ -825,8 +836,26 @@ BodyH &body_of(titan_ctx *c, int32_t id) {
     return c->bodies[id];
 }
 
-int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
+// A body's layout and arrays built on the host, before any device work.
+struct PreparedBody {
+    int64_t N = 0, S = 0, nch = 0, npad = 0, n_act = 0, n_lc = 0;
+    bool per_k = false, per_d = false, any_lc = false, shared = false, tiled = false, resident = false;
+    bool any_ext = false, any_drag = false;
+    double k0 = 0, d0 = 0, t0 = 0;
+    std::vector<double> acth;       // 4 per actuator: a, omega, phi, rest
+    std::vector<int32_t> perm;      // device order -> caller's order (empty = identity)
+    TileBuild tb;                   // tiled layout, body-relative
+    SellBuild sb;                   // gather names body-relative (arena slot = chunk0 * 64 + name)
+    std::vector<double> st, mm, Tm, ext;
+    std::vector<uint32_t> fl;
+    std::vector<int32_t> drag;
+    std::vector<LocalC> lch;        // the shared list, or every per-mass list in device order
+    std::vector<int64_t> mlc;       // per-mass lists: (offset in lch << 20) | count
+};
+
+PreparedBody prepare_body(const titan_schedule &sched, const titan_body_desc *d) {
     if (!d) fail(TITAN_E_INVALID, "body_create: null descriptor");
+    PreparedBody P;
     const int64_t N = d->n_masses, S = d->n_springs;
     if (N <= 0) fail(TITAN_E_INVALID, "body_create: a body needs at least one mass");
     if (N > kMaxMassSlots) fail(TITAN_E_INVALID, "body_create: too many masses in one body");
@@ -838,13 +867,18 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
         fail(TITAN_E_INVALID, "body_create: bad layout");
     const int64_t nch = (N + kChunk - 1) / kChunk;
     const int64_t npad = nch * kChunk;
+    P.N = N;
+    P.S = S;
+    P.nch = nch;
+    P.npad = npad;
+    P.t0 = d->t0;
 
     // ---- springs: validation, shared constants, actuators (caller's order)
     int64_t n_act = 0;
     bool k_same = true, d_same = true, first = true;
     double k0 = 0, d0 = 0;
     std::vector<int32_t> act_local(S > 0 ? S : 0, -1);
-    std::vector<double> acth;
+    std::vector<double> &acth = P.acth;
     for (int64_t s = 0; s < S; ++s) {
         const int64_t L = d->left[s], R = d->right[s];
         if (L >= N || R >= N) fail(TITAN_E_INVALID, "body_create: spring " + std::to_string(s) + " endpoint out of range");
@@ -876,6 +910,11 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     }
     // constants every spring of the body shares travel by value
     const bool per_k = !k_same, per_d = !d_same;
+    P.n_act = n_act;
+    P.per_k = per_k;
+    P.per_d = per_d;
+    P.k0 = k0;
+    P.d0 = d0;
 
     // ---- local constraints, sorted stably into application order
     std::vector<std::vector<LocalC>> lists;
@@ -908,24 +947,21 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
             }
         }
     }
-    int64_t n_lc = 0;
-    if (any_lc) {
-        if (shared) n_lc = (int64_t)lists[0].size();
-        else for (auto &l : lists) n_lc += (int64_t)l.size();
-    }
+    P.any_lc = any_lc;
+    P.shared = shared;
 
-    // ---- layout: tiled (masses along a Morton curve) or gather (caller's order)
-    std::vector<int32_t> perm;  // device order -> caller's order
-    TileBuild tb;
+    // ---- layout: resident, tiled (masses along a Morton curve) or gather (caller's order)
+    std::vector<int32_t> &perm = P.perm;
+    TileBuild &tb = P.tb;
     bool tiled = false;
     std::vector<int32_t> nl, nr;  // endpoints in device order
     if (d->layout == TITAN_LAYOUT_RESIDENT && (N < 1 || N > kResidentMax))
         fail(TITAN_E_INVALID, "body_create: the resident layout holds 1 to " + std::to_string(kResidentMax) +
                                   " masses, not " + std::to_string(N));
     const bool resident = d->layout == TITAN_LAYOUT_RESIDENT ||
-                          (d->layout == TITAN_LAYOUT_AUTO && N >= 1 && N <= c->sched.resident_max_masses);
+                          (d->layout == TITAN_LAYOUT_AUTO && N >= 1 && N <= sched.resident_max_masses);
     const bool want_tiles = !resident && (d->layout == TITAN_LAYOUT_TILED ||
-                                          (d->layout == TITAN_LAYOUT_AUTO && N >= c->sched.tiled_min_masses));
+                                          (d->layout == TITAN_LAYOUT_AUTO && N >= sched.tiled_min_masses));
     if (want_tiles && N > kTileMasses) perm = morton_order(d->pos, N);
     {
         std::vector<int32_t> inv;
@@ -950,7 +986,7 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     sp.damping = d->damping;
     sp.act = n_act > 0 ? act_local.data() : nullptr;
     if (want_tiles) {
-        tiled = build_tiles(N, sp, tb, c->sched.tile_rows);
+        tiled = build_tiles(N, sp, tb, sched.tile_rows);
         if (!tiled && d->layout == TITAN_LAYOUT_TILED)
             fail(TITAN_E_INVALID, "body_create: tiled layout does not fit: " + tb.why);
         if (!tiled && !perm.empty()) {  // fall back to the caller's order
@@ -962,12 +998,14 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
             }
         }
     }
+    if (!tiled) tb = TileBuild{};
+    P.tiled = tiled;
+    P.resident = resident;
     auto orig = [&](int64_t q) -> int64_t { return perm.empty() ? q : perm[q]; };
-    // incidences of both layouts; the gather layout's name arena slots,
-    // which are known once the chunks are placed (built below)
-    SellBuild sb;
-    if (tiled) build_sell(N, sp, per_k, per_d, &tb, 0, sb);
-    if (resident) {  // the whole body is one tile with no halo: rows = body indices
+    // incidences: LDS rows (tiled, resident) or body-relative slots (gather)
+    if (tiled) {
+        build_sell(N, sp, per_k, per_d, &tb, 0, P.sb);
+    } else if (resident) {  // the whole body is one tile with no halo: rows = body indices
         TileBuild one;
         TileDesc t{};
         t.nmass = (int32_t)N;
@@ -975,15 +1013,131 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
         one.tiles.push_back(t);
         one.tile_of_chunk.assign(nch, 0);
         one.rmax = (int32_t)N;
-        build_sell(N, sp, per_k, per_d, &one, 0, sb);
+        build_sell(N, sp, per_k, per_d, &one, 0, P.sb);
+    } else {
+        build_sell(N, sp, per_k, per_d, nullptr, 0, P.sb);
     }
 
+    // ---- mass arrays in device order
+    P.st.assign(6 * npad, 0.0);  // 6 per mass: pos, vel
+    P.mm.assign(npad, 1.0);
+    P.Tm.assign(npad, 0.0);
+    P.fl.assign(npad, kPad);
+    for (int64_t q = 0; q < N; ++q) {
+        const int64_t i = orig(q);
+        for (int k = 0; k < 3; ++k) {
+            P.st[6 * q + k] = d->pos[3 * i + k];
+            P.st[6 * q + 3 + k] = d->vel ? d->vel[3 * i + k] : 0.0;
+        }
+        P.mm[q] = d->m[i];
+        P.Tm[q] = d->T ? d->T[i] : 0.0;
+        uint32_t f = 0;
+        if (d->fixed && d->fixed[i]) f |= kFixed;
+        if (d->ext && (d->ext[3 * i] != 0 || d->ext[3 * i + 1] != 0 || d->ext[3 * i + 2] != 0)) {
+            f |= kExt;
+            P.any_ext = true;
+        }
+        if (d->drag && d->drag[i] != 0) {
+            f |= kDrag;
+            P.any_drag = true;
+        }
+        P.fl[q] = f;
+    }
+    if (P.any_ext) {
+        P.ext.assign(3 * npad, 0.0);
+        for (int64_t q = 0; q < N; ++q)
+            for (int k = 0; k < 3; ++k) P.ext[k * npad + q] = d->ext[3 * orig(q) + k];
+    }
+    if (P.any_drag) {
+        P.drag.assign(npad, 0);
+        for (int64_t q = 0; q < N; ++q) P.drag[q] = d->drag[orig(q)];
+    }
+    if (any_lc) {
+        if (shared) {
+            P.lch = lists[0];
+        } else {
+            P.mlc.assign(npad, 0);
+            for (int64_t q = 0; q < N; ++q) {
+                const auto &l = lists[orig(q)];
+                if (l.empty()) continue;
+                P.mlc[q] = ((int64_t)P.lch.size() << 20) | (int64_t)l.size();
+                P.fl[q] |= kLocalLC;
+                P.lch.insert(P.lch.end(), l.begin(), l.end());
+            }
+        }
+        P.n_lc = (int64_t)P.lch.size();
+    }
+    return P;
+}
+
+// Index of the body table and arenas the activation kernel writes.
+BodyTable body_table(titan_ctx *c) {
+    BodyTable t{};
+    t.chunk_body = c->chunk_body;
+    t.tiles = c->tile;
+    t.rdesc = c->rdesc;
+    for (int s = 0; s < 2; ++s) {
+        t.bT[s] = c->bT[s];
+        t.bn[s] = c->bn[s];
+        t.bleft[s] = c->bleft[s];
+    }
+    t.bnsync = c->bnsync;
+    t.b_chunk0 = c->b_chunk0;
+    t.b_lc = c->b_lc;
+    t.b_lcn = c->b_lcn;
+    t.b_mode = c->b_mode;
+    t.b_k = c->b_k;
+    t.b_d = c->b_d;
+    t.b_kd = c->b_kd;
+    return t;
+}
+
+// Ranges of a destroyed body, released once the launches enqueued before its
+// deactivation have run (until then they may still read its slots).
+void harvest_released(titan_ctx *c, bool wait) {
+    std::vector<titan_ctx::Pending> keep;
+    for (auto &p : c->pending) {
+        if (!wait && hipEventQuery(p.ev) == hipErrorNotReady) {
+            keep.push_back(p);
+            continue;
+        }
+        if (wait) HIPCHK(hipEventSynchronize(p.ev));
+        const BodyH &b = p.b;
+        c->chunks.release(b.chunk0, b.nchunks);
+        c->incs.release(b.inc0, b.inc_len);
+        c->acts.release(b.act0, b.act_len);
+        c->lcs.release(b.lc0, b.lc_len);
+        c->tiles.release(b.tile0, b.ntiles);
+        c->halos.release(b.halo0, b.nhalo);
+        if (b.resident) c->rdescs.release(b.res, 1);
+        c->bids.release(p.id, 1);
+        (void)hipEventDestroy(p.ev);
+    }
+    c->pending.swap(keep);
+}
+
+// Place a prepared body on the device.  Its arrays are copied on the upload
+// stream into free arena space, which the launches in flight do not read;
+// the body then joins the launches through a small kernel on the step
+// stream, after the launches enqueued before it.  Nothing here waits for
+// the step stream (only growing an arena does).
+int32_t commit_body(titan_ctx *c, PreparedBody &P) {
+    harvest_released(c, false);
+    const int64_t N = P.N, nch = P.nch, npad = P.npad, n_act = P.n_act, n_lc = P.n_lc;
+    const bool tiled = P.tiled, resident = P.resident;
+    SellBuild &sb = P.sb;
+    TileBuild &tb = P.tb;
+    const int64_t n_tiles = (int64_t)tb.tiles.size(), n_halo = (int64_t)tb.halo.size();
+    const int64_t n_inc = sb.coff[nch];
+
     // ---- reserve space; on failure release whatever was taken
-    const int64_t n_tiles = tiled ? (int64_t)tb.tiles.size() : 0;
-    const int64_t n_halo = tiled ? (int64_t)tb.halo.size() : 0;
     int64_t ch0 = -1, inc0 = -1, act0 = -1, lc0 = -1, bid = -1, tl0 = -1, hl0 = -1, rs0 = -1;
     auto take = [&](Ranges &r, int64_t len, void (*grow)(titan_ctx *, int64_t)) -> int64_t {
         int64_t at = r.alloc(len);
+        if (at < 0 && !c->pending.empty()) {  // slots of destroyed bodies first
+            harvest_released(c, true);
+            at = r.alloc(len);
+        }
         if (at < 0) {
             grow(c, r.high_water() + len);
             at = r.alloc(len);
@@ -994,23 +1148,40 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
         ch0 = take(c->chunks, nch, ensure_chunks);
         if ((ch0 + nch) * kChunk > kMaxMassSlots)
             fail(TITAN_E_NOMEM, "body_create: mass arena limit (2^26 slots) reached");
-        if (!tiled && !resident) {  // gather incidences name arena slots: build them now
-            build_sell(N, sp, per_k, per_d, nullptr, ch0 * kChunk, sb);
-        }
-        const int64_t n_inc_all = sb.coff[nch];
-        inc0 = take(c->incs, n_inc_all, ensure_incs);
+        inc0 = take(c->incs, n_inc, ensure_incs);
         act0 = take(c->acts, n_act, ensure_acts);
         lc0 = take(c->lcs, n_lc, ensure_lcs);
         tl0 = take(c->tiles, n_tiles, ensure_tiles);
         hl0 = take(c->halos, n_halo, ensure_halos);
         if (resident) rs0 = take(c->rdescs, 1, ensure_rdescs);
         bid = take(c->bids, 1, ensure_bodies);
-        if (per_k) make_inc_array(c, c->inc_k);
-        if (per_d) make_inc_array(c, c->inc_damp);
+        if (P.per_k) make_inc_array(c, c->inc_k);
+        if (P.per_d) make_inc_array(c, c->inc_damp);
         if (n_act > 0) make_act(c);
+        // side arrays created on first use, zeroed before any upload into them
+        bool made = false;
+        if (P.any_ext && !c->ext) {
+            c->ext = dalloc<double>(c, 3 * c->ncap);
+            HIPCHK(hipMemsetAsync(c->ext, 0, sizeof(double) * 3 * c->ncap, c->stream));
+            made = true;
+        }
+        if (P.any_drag && !c->drag) {
+            c->drag = dalloc<int32_t>(c, c->ncap);
+            HIPCHK(hipMemsetAsync(c->drag, 0, sizeof(int32_t) * c->ncap, c->stream));
+            made = true;
+        }
+        if (P.any_lc && !P.shared && !c->mlc) {
+            c->mlc = dalloc<int64_t>(c, c->ncap);
+            HIPCHK(hipMemsetAsync(c->mlc, 0, sizeof(int64_t) * c->ncap, c->stream));
+            made = true;
+        }
+        if (made) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            drop_graphs(c);
+        }
     } catch (...) {
         if (ch0 >= 0) c->chunks.release(ch0, nch);
-        if (inc0 >= 0) c->incs.release(inc0, sb.coff[nch]);
+        if (inc0 >= 0) c->incs.release(inc0, n_inc);
         if (act0 >= 0) c->acts.release(act0, n_act);
         if (lc0 >= 0) c->lcs.release(lc0, n_lc);
         if (tl0 >= 0) c->tiles.release(tl0, n_tiles);
@@ -1021,87 +1192,20 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     }
     const int64_t g0 = ch0 * kChunk;
 
-    // ---- mass arrays in device order
-    std::vector<double> st(6 * npad, 0.0), mm(npad, 1.0), Tm(npad, 0.0);  // st: 6 per mass
-    std::vector<uint32_t> fl(npad, kPad);
-    std::vector<double> ext;
-    std::vector<int32_t> drag;
-    std::vector<int64_t> mlc;
-    bool any_ext = false, any_drag = false;
-    for (int64_t q = 0; q < N; ++q) {
-        const int64_t i = orig(q);
-        for (int k = 0; k < 3; ++k) {
-            st[6 * q + k] = d->pos[3 * i + k];
-            st[6 * q + 3 + k] = d->vel ? d->vel[3 * i + k] : 0.0;
-        }
-        mm[q] = d->m[i];
-        Tm[q] = d->T ? d->T[i] : 0.0;
-        uint32_t f = 0;
-        if (d->fixed && d->fixed[i]) f |= kFixed;
-        if (d->ext && (d->ext[3 * i] != 0 || d->ext[3 * i + 1] != 0 || d->ext[3 * i + 2] != 0)) {
-            f |= kExt;
-            any_ext = true;
-        }
-        if (d->drag && d->drag[i] != 0) {
-            f |= kDrag;
-            any_drag = true;
-        }
-        fl[q] = f;
-    }
-    if (any_ext) {
-        if (!c->ext) {
-            c->ext = dalloc<double>(c, 3 * c->ncap);
-            HIPCHK(hipMemsetAsync(c->ext, 0, sizeof(double) * 3 * c->ncap, c->stream));
-        }
-        ext.assign(3 * npad, 0.0);
-        for (int64_t q = 0; q < N; ++q)
-            for (int k = 0; k < 3; ++k) ext[k * npad + q] = d->ext[3 * orig(q) + k];
-    }
-    if (any_drag) {
-        if (!c->drag) {
-            c->drag = dalloc<int32_t>(c, c->ncap);
-            HIPCHK(hipMemsetAsync(c->drag, 0, sizeof(int32_t) * c->ncap, c->stream));
-        }
-        drag.assign(npad, 0);
-        for (int64_t q = 0; q < N; ++q) drag[q] = d->drag[orig(q)];
-    }
-    std::vector<LocalC> lch;
-    if (any_lc) {
-        if (shared) {
-            lch = lists[0];
-        } else {
-            if (!c->mlc) {
-                c->mlc = dalloc<int64_t>(c, c->ncap);
-                HIPCHK(hipMemsetAsync(c->mlc, 0, sizeof(int64_t) * c->ncap, c->stream));
-            }
-            mlc.assign(npad, 0);
-            for (int64_t q = 0; q < N; ++q) {
-                const auto &l = lists[orig(q)];
-                if (l.empty()) continue;
-                mlc[q] = ((lc0 + (int64_t)lch.size()) << 20) | (int64_t)l.size();
-                fl[q] |= kLocalLC;
-                lch.insert(lch.end(), l.begin(), l.end());
-            }
-        }
-    }
-
-    // ---- incidences: actuated ones marked (with arena actuator indices); tiles rebased
-    const int64_t n_inc = sb.coff[nch];
+    // ---- arena addresses: gather slots, actuated incidences, tiles, local lists
+    if (!tiled && !resident)
+        for (auto &x : sb.nbr) x += (uint32_t)g0;
     mark_actuated(sb, tiled || resident, act0);
-    if (tiled) {
-        for (auto &t : tb.tiles) {
-            t.body = (int32_t)bid;
-            t.chunk0 += (int32_t)ch0;
-            t.halo_off += hl0;
-        }
-        for (auto &h : tb.halo) h += (int32_t)g0;
+    for (auto &t : tb.tiles) {
+        t.body = -1;  // set by the activation kernel
+        t.chunk0 += (int32_t)ch0;
+        t.halo_off += hl0;
     }
-
-    // ---- upload (buffer parity 0: a new body has taken no steps)
-    std::vector<int32_t> cbody(nch, (int32_t)bid);
-    std::vector<int64_t> cptr(nch);
+    for (auto &h : tb.halo) h += (int32_t)g0;
+    for (auto &v : P.mlc)
+        if (v) v += lc0 << 20;
+    std::vector<int64_t> cptr(nch), ca0(nch, 0);
     std::vector<int32_t> cng(nch), cna(nch, 0);
-    std::vector<int64_t> ca0(nch, 0);
     {  // each chunk takes an equal share of the body's actuators
         const int64_t per = nch > 0 ? (n_act + nch - 1) / nch : 0;
         for (int64_t ch = 0; ch < nch; ++ch) {
@@ -1114,59 +1218,60 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
         cptr[ch] = inc0 + sb.coff[ch];
         cng[ch] = (int32_t)((sb.coff[ch + 1] - sb.coff[ch]) / gsz);
     }
-    h2d(c, c->st + 6 * g0, st.data(), 6 * npad);
-    h2d(c, c->m + g0, mm.data(), npad);
-    h2d(c, c->Tm + g0, Tm.data(), npad);
-    h2d(c, c->mflags + g0, fl.data(), npad);
-    h2d(c, c->deg + g0, sb.deg.data(), npad);
+
+    // ---- upload on the upload stream (buffer parity 0: a new body has taken no steps)
+    auto up = [&](auto *dst, const auto *src, int64_t n) {
+        if (n > 0) HIPCHK(hipMemcpyAsync(dst, src, sizeof(*src) * n, hipMemcpyHostToDevice, c->up));
+    };
+    up(c->st + 6 * g0, P.st.data(), 6 * npad);
+    up(c->m + g0, P.mm.data(), npad);
+    up(c->Tm + g0, P.Tm.data(), npad);
+    up(c->mflags + g0, P.fl.data(), npad);
+    up(c->deg + g0, sb.deg.data(), npad);
     std::vector<double> zeros(3 * npad, 0.0);
-    for (int k = 0; k < 3; ++k) h2d(c, c->acc + k * c->ncap + g0, zeros.data(), npad);
-    if (c->ext) {
+    for (int k = 0; k < 3; ++k) up(c->acc + k * c->ncap + g0, zeros.data(), npad);
+    if (c->ext)
         for (int k = 0; k < 3; ++k)
-            h2d(c, c->ext + k * c->ncap + g0, any_ext ? ext.data() + k * npad : zeros.data(), npad);
-    }
+            up(c->ext + k * c->ncap + g0, P.any_ext ? P.ext.data() + k * npad : zeros.data(), npad);
     std::vector<int32_t> zi(npad, 0);
     std::vector<int64_t> zl(npad, 0);
-    if (c->drag) h2d(c, c->drag + g0, any_drag ? drag.data() : zi.data(), npad);
-    if (c->mlc) h2d(c, c->mlc + g0, mlc.empty() ? zl.data() : mlc.data(), npad);
-    h2d(c, c->chunk_body + ch0, cbody.data(), nch);
-    h2d(c, c->chunk_ptr + ch0, cptr.data(), nch);
-    h2d(c, c->chunk_ng + ch0, cng.data(), nch);
-    h2d(c, c->chunk_act0 + ch0, ca0.data(), nch);
-    h2d(c, c->chunk_nact + ch0, cna.data(), nch);
+    if (c->drag) up(c->drag + g0, P.any_drag ? P.drag.data() : zi.data(), npad);
+    if (c->mlc) up(c->mlc + g0, P.mlc.empty() ? zl.data() : P.mlc.data(), npad);
+    up(c->chunk_ptr + ch0, cptr.data(), nch);
+    up(c->chunk_ng + ch0, cng.data(), nch);
+    up(c->chunk_act0 + ch0, ca0.data(), nch);
+    up(c->chunk_nact + ch0, cna.data(), nch);
     if (n_inc > 0) {
-        h2d(c, c->inc_nbr + inc0, sb.nbr.data(), n_inc);
-        h2d(c, c->inc_rest + inc0, sb.rest.data(), n_inc);
-        if (per_k) h2d(c, c->inc_k + inc0, sb.k.data(), n_inc);
-        if (per_d) h2d(c, c->inc_damp + inc0, sb.damp.data(), n_inc);
+        up(c->inc_nbr + inc0, sb.nbr.data(), n_inc);
+        up(c->inc_rest + inc0, sb.rest.data(), n_inc);
+        if (P.per_k) up(c->inc_k + inc0, sb.k.data(), n_inc);
+        if (P.per_d) up(c->inc_damp + inc0, sb.damp.data(), n_inc);
     }
-    if (tiled) {
-        h2d(c, c->tile + tl0, tb.tiles.data(), n_tiles);
-        h2d(c, c->halo + hl0, tb.halo.data(), n_halo);
-    }
-    const ResidentDesc rdh{(int32_t)bid, (int32_t)ch0, (int32_t)N, (int32_t)n_act, act0};
-    if (resident) h2d(c, c->rdesc + rs0, &rdh, 1);
-    if (!acth.empty()) h2d(c, c->act + 4 * act0, acth.data(), (int64_t)acth.size());
-    if (!lch.empty()) h2d(c, c->lc + lc0, lch.data(), (int64_t)lch.size());
-    const double t0 = d->t0;
-    const int64_t zero = 0;
-    const int32_t c0 = (int32_t)ch0, lcb = (int32_t)(shared ? lc0 : 0), lcn = (int32_t)(shared ? n_lc : 0);
-    const uint8_t mflag = resident ? kModeResident : tiled ? kModeTiled : kModeGather;
-    for (int s = 0; s < 2; ++s) {
-        h2d(c, c->bT[s] + bid, &t0, 1);
-        h2d(c, c->bn[s] + bid, &zero, 1);
-        h2d(c, c->bleft[s] + bid, &zero, 1);
-    }
-    h2d(c, c->bnsync + bid, &zero, 1);
-    h2d(c, c->b_chunk0 + bid, &c0, 1);
-    h2d(c, c->b_lc + bid, &lcb, 1);
-    h2d(c, c->b_lcn + bid, &lcn, 1);
-    h2d(c, c->b_mode + bid, &mflag, 1);
-    const uint8_t kdf = (uint8_t)((per_k ? 1 : 0) | (per_d ? 2 : 0));
-    h2d(c, c->b_k + bid, &k0, 1);
-    h2d(c, c->b_d + bid, &d0, 1);
-    h2d(c, c->b_kd + bid, &kdf, 1);
-    HIPCHK(hipStreamSynchronize(c->stream));
+    up(c->tile + tl0, tb.tiles.data(), n_tiles);
+    up(c->halo + hl0, tb.halo.data(), n_halo);
+    const ResidentDesc rdh{-1, (int32_t)ch0, (int32_t)N, (int32_t)n_act, act0};
+    if (resident) up(c->rdesc + rs0, &rdh, 1);
+    up(c->act + 4 * act0, P.acth.data(), (int64_t)P.acth.size());
+    up(c->lc + lc0, P.lch.data(), n_lc);
+    HIPCHK(hipStreamSynchronize(c->up));  // the copies only: the host arrays may go
+
+    // ---- join the launches
+    BodyInit bi{};
+    bi.bid = (int32_t)bid;
+    bi.chunk0 = ch0;
+    bi.nchunks = nch;
+    bi.tile0 = tl0;
+    bi.ntiles = n_tiles;
+    bi.res = resident ? rs0 : -1;
+    bi.t0 = P.t0;
+    bi.k = P.k0;
+    bi.d = P.d0;
+    bi.lcb = (int32_t)(P.shared ? lc0 : 0);
+    bi.lcn = (int32_t)(P.shared ? n_lc : 0);
+    bi.mode = resident ? kModeResident : tiled ? kModeTiled : kModeGather;
+    bi.kd = (uint8_t)((P.per_k ? 1 : 0) | (P.per_d ? 2 : 0));
+    bi.on = 1;
+    HIPCHK(launch_set_body(body_table(c), bi, c->stream));
 
     BodyH &b = c->bodies[bid];
     b = BodyH{};
@@ -1188,43 +1293,35 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
     b.halo0 = hl0;
     b.nhalo = n_halo;
     b.rows = tb.rmax;
-    b.T = t0;
+    b.T = P.t0;
     b.n = 0;
-    b.flags.assign(fl.begin(), fl.begin() + N);
-    b.perm = std::move(perm);
+    b.flags.assign(P.fl.begin(), P.fl.begin() + N);
+    b.perm = std::move(P.perm);
     return (int32_t)bid;
 }
 
+int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
+    PreparedBody P = prepare_body(c->sched, d);
+    return commit_body(c, P);
+}
+
+// The body leaves the launches through a kernel on the step stream; its
+// slots are released once that kernel has run (harvest_released).
 void body_destroy(titan_ctx *c, int32_t id) {
     BodyH &b = body_of(c, id);
-    HIPCHK(hipStreamSynchronize(c->stream));
-    std::vector<int32_t> neg(b.nchunks, -1);
-    h2d(c, c->chunk_body + b.chunk0, neg.data(), b.nchunks);
-    if (b.ntiles > 0) {
-        std::vector<TileDesc> dead(b.ntiles);
-        for (auto &t : dead) {
-            t = TileDesc{};
-            t.body = -1;
-        }
-        h2d(c, c->tile + b.tile0, dead.data(), b.ntiles);
-    }
-    if (b.resident) {
-        const ResidentDesc dead{-1, 0, 0, 0, 0};
-        h2d(c, c->rdesc + b.res, &dead, 1);
-    }
-    const int32_t m1 = -1;
-    const uint8_t z = kModeGather;
-    h2d(c, c->b_chunk0 + id, &m1, 1);
-    h2d(c, c->b_mode + id, &z, 1);
-    HIPCHK(hipStreamSynchronize(c->stream));
-    c->chunks.release(b.chunk0, b.nchunks);
-    c->incs.release(b.inc0, b.inc_len);
-    c->acts.release(b.act0, b.act_len);
-    c->lcs.release(b.lc0, b.lc_len);
-    c->tiles.release(b.tile0, b.ntiles);
-    c->halos.release(b.halo0, b.nhalo);
-    if (b.resident) c->rdescs.release(b.res, 1);
-    c->bids.release(id, 1);
+    BodyInit bi{};
+    bi.bid = id;
+    bi.chunk0 = b.chunk0;
+    bi.nchunks = b.nchunks;
+    bi.tile0 = b.tile0;
+    bi.ntiles = b.ntiles;
+    bi.res = b.resident ? b.res : -1;
+    bi.on = 0;
+    HIPCHK(launch_set_body(body_table(c), bi, c->stream));
+    hipEvent_t ev;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(ev, c->stream));
+    c->pending.push_back({b, id, ev});
     b = BodyH{};
 }
 
@@ -1376,6 +1473,7 @@ int titan_ctx_create(int device, uint64_t hbm_budget_bytes, titan_ctx **out) {
     c->budget = hbm_budget_bytes;
     HIPCHK(set_tiles_lds_limit());
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&c->up, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
     for (int k = 0; k < 2; ++k) HIPCHK(hipEventCreateWithFlags(&c->left_ev[k], hipEventDisableTiming));
@@ -1408,6 +1506,8 @@ int titan_ctx_destroy(titan_ctx *c) {
     titan_internal_serve_forget(c);
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->up) (void)hipStreamSynchronize(c->up);
+    for (auto &p : c->pending) (void)hipEventDestroy(p.ev);
     for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
     for (auto &a : c->allocs) (void)hipFree(a.first);
     for (int k = 0; k < 2; ++k) {
@@ -1417,6 +1517,7 @@ int titan_ctx_destroy(titan_ctx *c) {
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->up) (void)hipStreamDestroy(c->up);
     delete c;
     return TITAN_OK;
 }
@@ -1759,3 +1860,38 @@ int titan_debug_inject_failure(titan_ctx *c, int32_t after) {
 }
 
 }  // extern "C"
+
+namespace titan_mi355x {
+
+struct Prepared {
+    PreparedBody body;
+};
+
+Prepared *prepare_body_host(const titan_ctx *c, const titan_body_desc *d, int *rc, std::string *err) {
+    try {
+        std::unique_ptr<Prepared> p(new Prepared);
+        p->body = prepare_body(c->sched, d);
+        *rc = TITAN_OK;
+        return p.release();
+    } catch (const Fail &f) {
+        *rc = f.code;
+        *err = f.msg;
+    } catch (const std::bad_alloc &) {
+        *rc = TITAN_E_NOMEM;
+        *err = "host allocation failed";
+    } catch (const std::exception &e) {
+        *rc = TITAN_E_INVALID;
+        *err = e.what();
+    }
+    return nullptr;
+}
+
+int commit_body_device(titan_ctx *c, Prepared *p, int32_t *id) {
+    API_BEGIN
+    *id = commit_body(c, p->body);
+    API_END(c)
+}
+
+void free_prepared(Prepared *p) { delete p; }
+
+}  // namespace titan_mi355x

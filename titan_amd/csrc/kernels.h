@@ -12,6 +12,8 @@ const char *step_kernel_name(bool actuated);
 hipError_t launch_step_tiles(const StepArgs &a, bool actuated, hipStream_t s);
 const char *step_tiles_kernel_name(bool actuated);
 size_t tiles_lds_bytes(int ldim);
+// A body joins (bi.on) or leaves the launches.
+hipError_t launch_set_body(const BodyTable &t, const BodyInit &bi, hipStream_t s);
 // Actuation factors for the first step of a stepping call (launch slot a.slot).
 hipError_t launch_actuate(const StepArgs &a, hipStream_t s);
 hipError_t launch_rollout_resident(const StepArgs &a, bool actuated, int final_slot, hipStream_t s);

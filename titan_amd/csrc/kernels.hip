@@ -850,6 +850,40 @@ hipError_t launch_actuate(const StepArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// A body joins the launches (on = 1: its clock fields, shared constants and
+// mode, then its chunks, tiles and resident descriptor name it) or leaves
+// them (on = 0: they name no body).  Its other arrays were uploaded before.
+__global__ __launch_bounds__(kBlock) void k_set_body(const BodyTable t, const BodyInit bi) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t who = bi.on ? bi.bid : -1;
+    if (q < bi.nchunks) t.chunk_body[bi.chunk0 + q] = who;
+    if (q < bi.ntiles) t.tiles[bi.tile0 + q].body = who;
+    if (q != 0) return;
+    if (bi.res >= 0) t.rdesc[bi.res].body = who;
+    const int b = bi.bid;
+    if (bi.on) {
+        for (int s = 0; s < 2; ++s) {
+            t.bT[s][b] = bi.t0;
+            t.bn[s][b] = 0;
+            t.bleft[s][b] = 0;
+        }
+        t.bnsync[b] = 0;
+        t.b_lc[b] = bi.lcb;
+        t.b_lcn[b] = bi.lcn;
+        t.b_k[b] = bi.k;
+        t.b_d[b] = bi.d;
+        t.b_kd[b] = bi.kd;
+    }
+    t.b_chunk0[b] = bi.on ? (int32_t)bi.chunk0 : -1;
+    t.b_mode[b] = bi.on ? bi.mode : kModeGather;
+}
+
+hipError_t launch_set_body(const BodyTable &t, const BodyInit &bi, hipStream_t s) {
+    const int64_t n = std::max<int64_t>(1, std::max(bi.nchunks, bi.ntiles));
+    hipLaunchKernelGGL(k_set_body, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, t, bi);
+    return hipGetLastError();
+}
+
 size_t resident_lds_bytes(int rows) { return (size_t)2 * 48 * rows; }
 
 hipError_t launch_rollout_resident(const StepArgs &a, bool act, int final_slot, hipStream_t s) {

@@ -75,6 +75,26 @@ struct ResidentDesc {
     int64_t act0;
 };
 
+// The body table and the arena fields a body's activation (joining the
+// launches) and deactivation write, on the step stream.
+struct BodyTable {
+    int32_t *chunk_body;
+    TileDesc *tiles;
+    ResidentDesc *rdesc;
+    double *bT[2];
+    int64_t *bn[2], *bleft[2], *bnsync;
+    int32_t *b_chunk0, *b_lc, *b_lcn;
+    uint8_t *b_mode;
+    double *b_k, *b_d;
+    uint8_t *b_kd;
+};
+struct BodyInit {
+    int64_t chunk0, nchunks, tile0, ntiles, res;  // res < 0: not resident
+    double t0, k, d;
+    int32_t bid, lcb, lcn;
+    uint8_t mode, kd, on;  // on: 1 joins the launches, 0 leaves them
+};
+
 // per-body layout (b_mode)
 constexpr uint8_t kModeGather = 0, kModeTiled = 1, kModeResident = 2;
 

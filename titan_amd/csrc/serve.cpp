@@ -4,10 +4,14 @@
 // each pump admits queued requests in arrival order while they fit the
 // running-set limits, steps all running requests together through
 // titan_step_bodies (one launch per step covers every body; each body stops
-// at its own horizon), and retires the finished ones.  Nothing here runs on
-// its own thread.  The engine keeps every body's arithmetic independent of
-// the others, so a request's result is the same alone and in any mix.
+// at its own horizon), and retires the finished ones.  Admission lays out
+// several bodies at once on short-lived host threads, joined before the
+// pump returns, while the device runs the quantum just enqueued; placing a
+// body on the device does not wait for the launches in flight.  The engine
+// keeps every body's arithmetic independent of the others, so a request's
+// result is the same alone and in any mix.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -16,8 +20,10 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
+#include "engine_internal.h"
 #include "titan_mi355x.h"
 
 namespace {
@@ -47,6 +53,11 @@ struct Request {
     uint64_t est_bytes = 0;
     double *out_pos = nullptr, *out_vel = nullptr;
     titan_body_desc desc{};
+    titan_mi355x::Prepared *prep = nullptr;  // laid out on the host, not yet placed
+    int prep_rc = TITAN_OK;
+    std::string prep_err;
+
+    ~Request() { titan_mi355x::free_prepared(prep); }
     std::vector<double> pos, vel, m, T, ext, k, rest, damping, omega, act_a, act_phi, lc_vec, lc_num;
     std::vector<uint8_t> fixed;
     std::vector<int32_t> drag, left, right, type, lc_type;
@@ -100,6 +111,8 @@ struct Request {
     }
 
     void release_arrays() {
+        titan_mi355x::free_prepared(prep);
+        prep = nullptr;
         for (auto *v : {&pos, &vel, &m, &T, &ext, &k, &rest, &damping, &omega, &act_a, &act_phi, &lc_vec, &lc_num})
             std::vector<double>().swap(*v);
         std::vector<uint8_t>().swap(fixed);
@@ -112,6 +125,28 @@ struct Request {
 // incidences with their SELL padding and tile halos, rounded up.
 uint64_t estimate_bytes(int64_t n, int64_t s) {
     return (uint64_t)(256 * n + 96 * s + (1 << 16));
+}
+
+// Lay out the bodies of `rs` on up to `threads` host threads.
+void prepare_all(const titan_ctx *ctx, const std::vector<Request *> &rs, unsigned threads) {
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        for (size_t q; (q = next.fetch_add(1)) < rs.size();) {
+            Request *r = rs[q];
+            if (r->prep) continue;
+            r->prep = titan_mi355x::prepare_body_host(ctx, &r->desc, &r->prep_rc, &r->prep_err);
+        }
+    };
+    const unsigned nt = std::min<unsigned>(threads, (unsigned)rs.size());
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+}
+
+unsigned host_threads() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    return std::max(1u, std::min(16u, hw ? hw : 1u));
 }
 
 struct Server {
@@ -213,35 +248,53 @@ int titan_serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
     if (!ctx) return TITAN_E_INVALID;
     Server &sv = server_of(ctx);
     std::vector<Request *> fresh;  // admitted during this pump's quantum: they join the next one
-    // admit, in arrival order, while the running set has room
-    auto admit = [&]() -> int {
-        while (!sv.queue.empty() && (int32_t)(sv.running.size() + fresh.size()) < sv.max_running) {
-            Request *r = sv.queue.front();
-            const bool busy = !sv.running.empty() || !fresh.empty();
-            if (sv.hbm_bytes && busy && sv.running_bytes + r->est_bytes > sv.hbm_bytes) break;
-            int32_t id = -1;
-            const int rc = titan_body_create(ctx, &r->desc, &id);
-            if (rc == TITAN_E_NOMEM && busy) break;  // wait for memory to free up
-            sv.queue.pop_front();
-            if (rc != TITAN_OK) {  // the body is invalid or can never fit: it fails alone
-                finish(sv, r, TITAN_REQ_FAILED, now_s());
-                continue;
+    // admit, in arrival order, while the running set has room: up to `cap`
+    // requests, laid out together on the host, then placed one by one
+    auto admit = [&](size_t cap) -> int {
+        const unsigned nt = host_threads();
+        while (!sv.queue.empty() && cap > 0) {
+            std::vector<Request *> pick;
+            uint64_t bytes = sv.running_bytes;
+            size_t n_run = sv.running.size() + fresh.size();
+            for (Request *r : sv.queue) {
+                if (pick.size() >= std::min<size_t>(cap, nt)) break;
+                if ((int64_t)(n_run + pick.size()) >= sv.max_running) break;
+                const bool busy = n_run + pick.size() > 0;
+                if (sv.hbm_bytes && busy && bytes + r->est_bytes > sv.hbm_bytes) break;
+                pick.push_back(r);
+                bytes += r->est_bytes;
             }
-            r->body = id;
-            r->state = TITAN_REQ_RUNNING;
-            sv.running_bytes += r->est_bytes;
-            if (r->horizon == 0) {
-                const int rr = retire(ctx, sv, r);
-                if (rr != TITAN_OK) return rr;
-                continue;
+            if (pick.empty()) break;
+            prepare_all(ctx, pick, nt);
+            for (Request *r : pick) {
+                const bool busy = !sv.running.empty() || !fresh.empty();
+                int32_t id = -1;
+                int rc = r->prep ? titan_mi355x::commit_body_device(ctx, r->prep, &id) : r->prep_rc;
+                if (rc == TITAN_E_NOMEM && busy) return TITAN_OK;  // wait for memory to free up
+                sv.queue.pop_front();  // r is the queue's head
+                --cap;
+                if (rc != TITAN_OK) {  // the body is invalid or can never fit: it fails alone
+                    finish(sv, r, TITAN_REQ_FAILED, now_s());
+                    continue;
+                }
+                titan_mi355x::free_prepared(r->prep);
+                r->prep = nullptr;
+                r->body = id;
+                r->state = TITAN_REQ_RUNNING;
+                sv.running_bytes += r->est_bytes;
+                if (r->horizon == 0) {
+                    const int rr = retire(ctx, sv, r);
+                    if (rr != TITAN_OK) return rr;
+                    continue;
+                }
+                fresh.push_back(r);
             }
-            fresh.push_back(r);
         }
         return TITAN_OK;
     };
     int rc = TITAN_OK;
-    if (sv.running.empty()) {  // nothing on the device: this pump steps the new arrivals
-        rc = admit();
+    if (sv.running.empty()) {  // nothing on the device: a first batch starts it
+        rc = admit(host_threads());
         if (rc != TITAN_OK) return rc;
         sv.running.swap(fresh);
     }
@@ -259,7 +312,7 @@ int titan_serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
         for (size_t q = 0; q < ids.size(); ++q) sv.running[q]->done += steps[q];
     }
     // admit more while the device works (their layouts are built on the host meanwhile)
-    rc = admit();
+    rc = admit(SIZE_MAX);
     if (rc != TITAN_OK) return rc;
     // retire the finished ones (their downloads wait for the quantum)
     std::vector<Request *> still;
