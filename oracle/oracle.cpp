@@ -131,8 +131,9 @@ inline bool spring_live(const orc_scene *s, int64_t i) {
 
 inline bool is_fixed(const orc_scene *s, int64_t i) { return s->fixed && s->fixed[i]; }
 
-// massForcesAndUpdate after the spring pass: src/sim.cu:1100-1139,1161-1167.
-inline void mass_update(const orc_scene *s, int64_t i, V force) {
+// massForcesAndUpdate after the spring pass: src/sim.cu:1100-1139, then the
+// integrator (:1141-1167).
+inline void mass_update(const orc_scene *s, int64_t i, V force, int stage) {
     MassView m;
     m.pos = load3(s->px, s->py, s->pz, i);
     m.vel = load3(s->vx, s->vy, s->vz, i);
@@ -170,13 +171,36 @@ inline void mass_update(const orc_scene *s, int64_t i, V force) {
         int c = s->drag ? s->drag[i] : 0;
         m.force = vadd(m.force, vdiv(vscale(m.vel, (double)(-c) * (norm * norm)), norm));
     }
-    V acc = vdiv(m.force, mm);                                         // :1162
-    V vel = vadd(m.vel, vscale(acc, s->dt));                           // :1163
-    V pos = vadd(m.pos, vscale(vel, s->dt));                           // :1164
+    const double dt = s->dt;
+    V acc, vel, pos;
+    if (s->integrator == ORC_RK2) {
+        double *bk = s->rk2_backup + 6 * i;
+        acc = vdiv(m.force, mm);                                       // :1143, :1151
+        if (stage == 0) {
+            bk[0] = m.pos.x; bk[1] = m.pos.y; bk[2] = m.pos.z;         // :1144-1145
+            bk[3] = m.vel.x; bk[4] = m.vel.y; bk[5] = m.vel.z;
+            pos = vadd(m.pos, vscale(vscale(m.vel, 0.5), dt));         // :1147
+            vel = vadd(m.vel, vscale(vscale(acc, 0.5), dt));           // :1148
+        } else {
+            pos = vadd(V{bk[0], bk[1], bk[2]}, vscale(m.vel, dt));     // :1152
+            vel = vadd(V{bk[3], bk[4], bk[5]}, vscale(acc, dt));       // :1153
+        }
+        s->T[i] += 0.5 * dt;                                           // :1149, :1154
+    } else if (s->integrator == ORC_VERLET) {
+        const V prev{s->ax[i], s->ay[i], s->az[i]};
+        vel = vadd(m.vel, vscale(vscale(vadd(prev, vdiv(m.force, mm)), 0.5), dt));  // :1157
+        acc = vdiv(m.force, mm);                                       // :1158
+        pos = vadd(m.pos, vadd(vscale(vel, dt), vscale(vscale(acc, 0.5), dt * dt)));  // :1159
+        s->T[i] += dt;                                                 // :1160
+    } else {
+        acc = vdiv(m.force, mm);                                       // :1162
+        vel = vadd(m.vel, vscale(acc, dt));                            // :1163
+        pos = vadd(m.pos, vscale(vel, dt));                            // :1164
+        s->T[i] += dt;                                                 // :1165
+    }
     s->ax[i] = acc.x; s->ay[i] = acc.y; s->az[i] = acc.z;
     s->vx[i] = vel.x; s->vy[i] = vel.y; s->vz[i] = vel.z;
     s->px[i] = pos.x; s->py[i] = pos.y; s->pz[i] = pos.z;
-    s->T[i] += s->dt;                                                  // :1165
 }
 
 int threads_of(const orc_scene *s) {
@@ -226,7 +250,7 @@ int orc_build_incidence(const orc_scene *s, int64_t *row, int32_t *sid, int8_t *
 }
 
 int orc_step(const orc_scene *s, double t, const int64_t *row, const int32_t *sid,
-             const int8_t *sgn, double *fbuf) {
+             const int8_t *sgn, double *fbuf, int stage) {
     orc_spring_forces(s, t, fbuf);
     const int nt = threads_of(s);
 #pragma omp parallel for schedule(static) num_threads(nt)
@@ -238,7 +262,7 @@ int orc_step(const orc_scene *s, double t, const int64_t *row, const int32_t *si
             V Fv{F[0], F[1], F[2]};
             force = vadd(force, sgn[j] > 0 ? Fv : vneg(Fv));
         }
-        mass_update(s, i, force);
+        mass_update(s, i, force, stage);
     }
     return 0;
 }
@@ -251,9 +275,24 @@ int orc_run(const orc_scene *s, double t0, int64_t n_steps, double *t_out) {
     std::vector<double> fbuf(3 * S + 3);
     if (orc_build_incidence(s, row.data(), sid.data(), sgn.data())) return -1;
     double T = t0;
-    for (int64_t n = 0; n < n_steps; n++) {
-        orc_step(s, T, row.data(), sid.data(), sgn.data(), fbuf.data());
-        T += s->dt;                                                     // src/sim.cu:1590
+    if (s->integrator == ORC_RK2) {
+        orc_scene r = *s;
+        std::vector<double> bk;
+        if (!r.rk2_backup) {
+            bk.assign(6 * N + 6, 0.0);
+            r.rk2_backup = bk.data();
+        }
+        for (int64_t n = 0; n < n_steps; n++) {                         // src/sim.cu:1572-1582
+            orc_step(&r, T, row.data(), sid.data(), sgn.data(), fbuf.data(), 0);
+            T += 0.5 * s->dt;
+            orc_step(&r, T, row.data(), sid.data(), sgn.data(), fbuf.data(), 1);
+            T += 0.5 * s->dt;
+        }
+    } else {
+        for (int64_t n = 0; n < n_steps; n++) {
+            orc_step(s, T, row.data(), sid.data(), sgn.data(), fbuf.data(), 0);
+            T += s->dt;                                                 // src/sim.cu:1590
+        }
     }
     if (t_out) *t_out = T;
     return 0;
@@ -270,7 +309,7 @@ int orc_scatter_step(const orc_scene *s, double t) {
     }
     for (int64_t i = 0; i < N; i++) {
         if (is_fixed(s, i)) continue;
-        mass_update(s, i, force[i]);
+        mass_update(s, i, force[i], 0);
     }
     return 0;
 }

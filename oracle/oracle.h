@@ -10,9 +10,13 @@
  * Functions and the reference code each one restates:
  *   orc_vec_*            include/Titan/vec.h:109-159, src/vec.cu:39-45
  *   orc_spring_forces    src/sim.cu:1046-1080 (computeSpringForces)
- *   orc_step             src/sim.cu:1090-1139,1161-1167 (massForcesAndUpdate,
- *                        Euler branch, CONSTRAINTS on) after the spring pass,
- *                        with forces summed per mass in ascending spring index
+ *   orc_step             src/sim.cu:1090-1139,1141-1167 (massForcesAndUpdate,
+ *                        CONSTRAINTS on) after the spring pass, with forces
+ *                        summed per mass in ascending spring index; the
+ *                        Euler branch (:1161-1167), the VERLET branch
+ *                        (:1156-1160) or one RK2 stage (:1141-1155)
+ *   orc_run              src/sim.cu:1569-1590 (the step driver; RK2: two
+ *                        stages per step, T += 0.5 dt after each)
  *   orc_contact_plane    src/object.cu:26,73-106
  *   orc_ball             src/object.cu:53-56
  *   orc_constraint_plane src/object.cu:115-124
@@ -43,6 +47,10 @@ enum { ORC_PASSIVE_SOFT = 0, ORC_PASSIVE_STIFF = 1, ORC_CONTRACT_THEN_EXPAND = 2
 /* Local constraint kinds, in the reference's enum order
  * (include/Titan/object.h:231-233). */
 enum { ORC_CONSTRAINT_PLANE = 0, ORC_CONTACT_PLANE = 1, ORC_BALL = 2, ORC_DIRECTION = 3 };
+
+/* Integrators: the default Euler branch and the reference's two
+ * compile-time alternatives (CMakeLists.txt:13-14). */
+enum { ORC_EULER = 0, ORC_VERLET = 1, ORC_RK2 = 2 };
 
 typedef struct {
     int64_t n_masses;
@@ -77,20 +85,25 @@ typedef struct {
     double gx, gy, gz;
     double dt;
     int32_t n_threads;             /* OpenMP threads, <=0 = default */
+    int32_t integrator;            /* ORC_EULER, ORC_VERLET or ORC_RK2 */
+    /* RK2: the state saved by a first stage for the second (6 per mass:
+     * pos, vel), n_masses * 6 doubles; unused otherwise */
+    double *rk2_backup;
 } orc_scene;
 
 /* Incident-spring CSR (ascending spring index per mass). Returns 0 on
  * success.  row must hold n_masses+1 entries, sid/sgn 2*n_springs. */
 int orc_build_incidence(const orc_scene *s, int64_t *row, int32_t *sid, int8_t *sgn);
 
-/* One step at simulation time t (the host T before it is advanced). The
- * incidence CSR comes from orc_build_incidence; fbuf holds 3*n_springs
- * doubles of scratch. */
+/* One step (one RK2 stage: stage 0 or 1) at simulation time t (the host T
+ * before it is advanced).  The incidence CSR comes from orc_build_incidence;
+ * fbuf holds 3*n_springs doubles of scratch. */
 int orc_step(const orc_scene *s, double t, const int64_t *row, const int32_t *sid,
-             const int8_t *sgn, double *fbuf);
+             const int8_t *sgn, double *fbuf, int stage);
 
-/* n steps; T is accumulated as T += dt from t0 (src/sim.cu:1590); the
- * final time is returned in *t_out. */
+/* n steps; T is accumulated as T += dt from t0 (src/sim.cu:1590), or for
+ * RK2 as T += 0.5 dt after each stage (:1577,1582); the final time is
+ * returned in *t_out. */
 int orc_run(const orc_scene *s, double t0, int64_t n_steps, double *t_out);
 
 /* Serial reference-order scatter step: forces are added spring by spring in

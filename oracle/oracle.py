@@ -20,6 +20,7 @@ LIB_PATH = HERE / "liboracle.so"
 
 PASSIVE_SOFT, PASSIVE_STIFF, CONTRACT_THEN_EXPAND, EXPAND_THEN_CONTRACT, GENERAL = range(5)
 CONSTRAINT_PLANE, CONTACT_PLANE, BALL, DIRECTION = range(4)
+EULER, VERLET, RK2 = range(3)
 
 
 class _Scene(ctypes.Structure):
@@ -40,7 +41,7 @@ class _Scene(ctypes.Structure):
         ("lc_ptr", POINTER(c_int64)), ("lc_type", POINTER(c_int32)),
         ("lc_vec", POINTER(c_double)), ("lc_num", POINTER(c_double)),
         ("gx", c_double), ("gy", c_double), ("gz", c_double), ("dt", c_double),
-        ("n_threads", c_int32),
+        ("n_threads", c_int32), ("integrator", c_int32), ("rk2_backup", POINTER(c_double)),
     ]
 
 
@@ -88,7 +89,8 @@ class Scene:
     def __init__(self, pos, m, *, vel=None, T=None, ext=None, fixed=None, drag=None,
                  left=None, right=None, k=None, rest=None, damping=None, type=None, omega=None,
                  act_a=None, act_phi=None, planes=(), balls=(), lc_ptr=None, lc_type=None,
-                 lc_vec=None, lc_num=None, g=(0.0, 0.0, -9.81), dt=1e-4, threads=0):
+                 lc_vec=None, lc_num=None, g=(0.0, 0.0, -9.81), dt=1e-4, threads=0,
+                 integrator=EULER):
         pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
         n = pos.shape[0]
         self.n = n
@@ -122,6 +124,8 @@ class Scene:
         self.g = tuple(map(float, g))
         self.dt = float(dt)
         self.threads = int(threads)
+        self.integrator = int(integrator)
+        self.rk2_backup = np.zeros(6 * n + 6)
         self.t = 0.0
 
     def _struct(self) -> _Scene:
@@ -145,10 +149,13 @@ class Scene:
         S.gx, S.gy, S.gz = self.g
         S.dt = self.dt
         S.n_threads = self.threads
+        S.integrator = self.integrator
+        S.rk2_backup = _p(self.rk2_backup, c_double)
         return S
 
     def run(self, n_steps: int) -> float:
-        """n steps with T accumulated by += dt from the scene's current time."""
+        """n steps with T accumulated by += dt (RK2: += 0.5 dt per stage) from
+        the scene's current time."""
         S = self._struct()
         t = c_double()
         rc = lib().orc_run(ctypes.byref(S), self.t, int(n_steps), ctypes.byref(t))

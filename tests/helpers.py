@@ -6,7 +6,7 @@ import numpy as np
 from oracle import oracle as orc
 
 
-def to_oracle(scene, threads: int = 0) -> orc.Scene:
+def to_oracle(scene, threads: int = 0, integrator: int = orc.EULER) -> orc.Scene:
     """Concatenate a scene's bodies into one oracle scene (bodies are
     independent, so concatenation with index offsets changes nothing)."""
     pos, vel, m, T, ext, fixed, drag = [], [], [], [], [], [], []
@@ -63,7 +63,7 @@ def to_oracle(scene, threads: int = 0) -> orc.Scene:
                   lc_num=cat(lc_num) if lc_num else np.zeros(0))
     return orc.Scene(cat(pos), cat(m), vel=cat(vel), T=cat(T), ext=cat(ext) if any_ext else None,
                      fixed=cat(fixed), drag=cat(drag), planes=scene.planes, balls=scene.balls,
-                     g=scene.g, dt=scene.dt, threads=threads, **kw)
+                     g=scene.g, dt=scene.dt, threads=threads, integrator=integrator, **kw)
 
 
 def normwise_rel(a, b) -> float:

@@ -94,6 +94,93 @@ def test_two_mass_oscillator_matches_matrix_power(damping):
     assert s.vel[0, 0] + s.vel[1, 0] == 0.0
 
 
+# --------------------------------------------------------------- integrators
+
+@pytest.mark.parametrize("integ", [orc.VERLET, orc.RK2])
+def test_free_fall_closed_forms_of_the_other_integrators(integ):
+    # from rest under a constant acceleration a:
+    # VERLET (src/sim.cu:1156-1160, acc starts at 0): v_n = (n - 1/2) a dt and
+    #   x_n = x0 + a dt^2 n (n + 1) / 2;
+    # RK2 (src/sim.cu:1141-1155, two stages, T += dt/2 after each): the midpoint
+    #   rule is exact here, v_n = n a dt and x_n = x0 + a dt^2 n^2 / 2
+    m, dt, n = 0.1, 1e-4, 2000
+    s = one_mass((1.0, 0.0, 1.0), m=m, g=(0, 0, -9.8), dt=dt, integrator=integ)
+    s.run(n)
+    a, h = Fraction(-98, 10), Fraction(dt)
+    if integ == orc.VERLET:
+        z, vz = 1 + a * h * h * n * (n + 1) / 2, a * h * (n - Fraction(1, 2))
+    else:
+        z, vz = 1 + a * h * h * n * n / 2, a * h * n
+    assert abs(s.pos[0, 2] - float(z)) < 1e-12
+    assert abs(s.vel[0, 2] - float(vz)) < 1e-12
+    assert s.acc[0, 2] == -9.8 and s.pos[0, 0] == 1.0
+    # the clock: RK2 adds dt/2 twice per step, to the scene time and the mass time
+    t = 0.0
+    for _ in range(n):
+        if integ == orc.RK2:
+            t += 0.5 * dt
+            t += 0.5 * dt
+        else:
+            t += dt
+    assert s.t == t and s.T[0] == t
+
+
+def _integrator_map(integ, k, c, m, dt, y0, n):
+    # relative coordinate y = (xR - xL) - L0 of two equal masses, y'' = a(y, v)
+    # = -w2 y - g v, iterated in exact arithmetic with each integrator's own
+    # update rule (Verlet carries the previous step's acceleration, 0 at first)
+    w2, g, h = Fraction(2) * Fraction(k) / Fraction(m), Fraction(2) * Fraction(c) / Fraction(m), Fraction(dt)
+    acc = lambda y, v: -w2 * y - g * v
+    y, v, ap = Fraction(y0), Fraction(0), Fraction(0)
+    for _ in range(n):
+        if integ == orc.RK2:
+            a1 = acc(y, v)
+            ym, vm = y + h / 2 * v, v + h / 2 * a1
+            a2 = acc(ym, vm)
+            y, v = y + h * vm, v + h * a2
+        else:
+            ac = acc(y, v)
+            v = v + (ap + ac) / 2 * h
+            ap = ac
+            y = y + v * h + ap / 2 * h * h
+    return float(y), float(v), float(ap)
+
+
+@pytest.mark.parametrize("integ", [orc.VERLET, orc.RK2])
+@pytest.mark.parametrize("damping", [0.0, 0.3])
+def test_two_mass_oscillator_follows_each_integrator(integ, damping):
+    k, m, dt, L0, d0, n = 100.0, 0.5, 1e-3, 1.0, 0.05, 200
+    s = orc.Scene(np.array([[0.0, 0, 0], [L0 + d0, 0, 0]]), [m, m], left=[0], right=[1], k=[k],
+                  rest=[L0], damping=[damping], g=(0, 0, 0), dt=dt, integrator=integ)
+    s.run(n)
+    y, v, ap = _integrator_map(integ, k, damping, m, dt, d0, n)
+    assert abs((s.pos[1, 0] - s.pos[0, 0] - L0) - y) < 1e-11
+    assert abs((s.vel[1, 0] - s.vel[0, 0]) - v) < 1e-9
+    if integ == orc.VERLET:
+        assert abs((s.acc[1, 0] - s.acc[0, 0]) - ap) < 1e-7
+    assert s.vel[0, 0] + s.vel[1, 0] == 0.0
+
+
+def test_rk2_is_second_order_and_euler_first():
+    # undamped oscillator against y0 cos(w t) at t = 0.5 s: halving dt divides
+    # the error by about 4 for RK2 and about 2 for Euler (and for this Verlet
+    # variant, whose first step starts from a zero acceleration)
+    import math
+    k, m, L0, d0 = 100.0, 0.5, 1.0, 0.05
+    w = math.sqrt(2 * k / m)
+
+    def err(integ, dt):
+        n = int(round(0.5 / dt))
+        s = orc.Scene(np.array([[0.0, 0, 0], [L0 + d0, 0, 0]]), [m, m], left=[0], right=[1], k=[k],
+                      rest=[L0], g=(0, 0, 0), dt=dt, integrator=integ)
+        s.run(n)
+        return abs((s.pos[1, 0] - s.pos[0, 0] - L0) - d0 * math.cos(w * n * dt))
+
+    for integ, lo, hi in [(orc.RK2, 3.6, 4.4), (orc.EULER, 1.8, 2.2), (orc.VERLET, 1.8, 2.2)]:
+        r = err(integ, 1e-3) / err(integ, 5e-4)
+        assert lo < r < hi, (integ, r)
+
+
 def test_spring_force_direction_and_sign():
     # stretched spring pulls the right endpoint back toward the left one
     s = orc.Scene(np.array([[0.0, 0, 0], [0.0, 2.0, 0]]), [1.0, 1.0], left=[0], right=[1], k=[10.0],
