@@ -168,6 +168,20 @@ int titan_kernel_name(const titan_ctx *ctx, char *buf, int64_t len);
 /* Layout a body was given: TITAN_LAYOUT_GATHER, _TILED or _RESIDENT. */
 int titan_body_layout(const titan_ctx *ctx, int32_t body_id, int32_t *layout);
 
+/* ---- integrator (the reference picks one at build time: the default Euler
+ *      branch, or VERLET / RK2, CMakeLists.txt:13-14, src/sim.cu:1141-1167,
+ *      1572-1590) ---- */
+#define TITAN_INTEGRATOR_EULER 0
+#define TITAN_INTEGRATOR_VERLET 1
+#define TITAN_INTEGRATOR_RK2 2
+/* Set before the context holds any body.  Euler: acc = F/m, vel += acc dt,
+ * pos += vel dt.  Verlet: vel += (acc_prev + F/m) dt / 2, acc = F/m, pos +=
+ * vel dt + acc dt^2 / 2 (acc_prev starts at 0).  RK2: two force evaluations
+ * per step (midpoint), the clock advancing dt / 2 after each.  Step counts
+ * everywhere are whole steps. */
+int titan_set_integrator(titan_ctx *ctx, int32_t kind);
+int titan_get_integrator(const titan_ctx *ctx, int32_t *kind);
+
 /* ---- schedule and autotuning (no counterpart in the reference, which has
  *      one fixed launch shape per kernel, src/sim.cu:1569-1590) ---- */
 /* How bodies created with TITAN_LAYOUT_AUTO are laid out: bodies with at

@@ -36,13 +36,16 @@ EXPORTS = (
     "titan_lattice", "titan_debug_inject_failure", "titan_body_layout", "titan_step_bodies",
     "titan_body_time", "titan_set_schedule", "titan_get_schedule", "titan_autotune",
     "titan_serve_config", "titan_serve_submit", "titan_serve_pump", "titan_serve_poll",
-    "titan_serve_cancel", "titan_serve_stats", "titan_serve_reset_stats",
+    "titan_serve_cancel", "titan_serve_stats", "titan_serve_reset_stats", "titan_set_integrator",
+    "titan_get_integrator",
 )
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
 
 LAYOUT_AUTO, LAYOUT_GATHER, LAYOUT_TILED, LAYOUT_RESIDENT = 0, 1, 2, 3
 RESIDENT_MAX = 1024
+# integrators: the reference's default Euler branch and its VERLET / RK2 build options
+INTEGRATOR_EULER, INTEGRATOR_VERLET, INTEGRATOR_RK2 = 0, 1, 2
 
 
 class TitanError(RuntimeError):
@@ -108,6 +111,8 @@ def lib() -> ctypes.CDLL:
         "titan_step_bodies": (c_int, [vp, vp, vp, c_int32]),
         "titan_body_time": (c_int, [vp, c_int32, POINTER(c_double), POINTER(c_int64)]),
         "titan_set_schedule": (c_int, [vp, vp]),
+        "titan_set_integrator": (c_int, [vp, c_int32]),
+        "titan_get_integrator": (c_int, [vp, POINTER(c_int32)]),
         "titan_get_schedule": (c_int, [vp, vp]),
         "titan_autotune": (c_int, [vp, vp, c_int32, c_int32, c_int32, vp, vp]),
         "titan_serve_config": (c_int, [vp, c_int32, c_uint64, c_int64]),
@@ -254,6 +259,15 @@ class Context:
     # -- globals --
     def set_dt(self, dt: float):
         self._chk(self._L.titan_set_dt(self._h, dt))
+
+    def set_integrator(self, kind: int):
+        """INTEGRATOR_EULER, _VERLET or _RK2; before any body is created."""
+        self._chk(self._L.titan_set_integrator(self._h, int(kind)))
+
+    def integrator(self) -> int:
+        v = c_int32()
+        self._chk(self._L.titan_get_integrator(self._h, ctypes.byref(v)))
+        return v.value
 
     def get_dt(self) -> float:
         v = c_double()

@@ -92,7 +92,18 @@ class Simulation {
     bool running();
     void printPositions();
 
-    Simulation();
+    // The integrator follows the reference's build options as the program
+    // sees them: RK2, else VERLET, else the Euler branch (CMakeLists.txt:13-14,
+    // src/sim.cu:1141-1167).  Simulation(k) picks one explicitly (0 Euler,
+    // 1 Verlet, 2 RK2).
+#if defined(RK2)
+    Simulation() : Simulation(2) {}
+#elif defined(VERLET)
+    Simulation() : Simulation(1) {}
+#else
+    Simulation() : Simulation(0) {}
+#endif
+    explicit Simulation(int integrator);
     ~Simulation();
 
     Spring *getSpringByIndex(int i);
@@ -129,6 +140,7 @@ class Simulation {
 
     titan_ctx *ctx = nullptr;
     int32_t body = -1;
+    int integrator_ = 0;
     bool dirty = false;  // the engine body must be rebuilt before the next step
 };
 

@@ -160,8 +160,9 @@ Beam::Beam(const Vec &center, const Vec &dims, int nx_, int ny_, int nz_)
 
 // ----------------------------------------------------------- Simulation
 
-Simulation::Simulation()
-    : dt(0.0001), T(0), _global_acc(0, 0, -9.81), RUNNING(false), STARTED(false), ENDED(false), FREED(false) {}
+Simulation::Simulation(int integrator)
+    : dt(0.0001), T(0), _global_acc(0, 0, -9.81), RUNNING(false), STARTED(false), ENDED(false), FREED(false),
+      integrator_(integrator) {}
 
 Simulation::~Simulation() {
     if (ctx) titan_ctx_destroy(ctx);
@@ -675,6 +676,7 @@ void Simulation::start() {  // src/sim.cu:1349-1390
     if (!ctx) {
         const char *dev = std::getenv("TITAN_DEVICE");
         check(nullptr, titan_ctx_create(dev ? std::atoi(dev) : 0, 0, &ctx));
+        check(ctx, titan_set_integrator(ctx, integrator_));
     }
     check(ctx, titan_set_dt(ctx, dt));
     check(ctx, titan_set_global_acc(ctx, _global_acc[0], _global_acc[1], _global_acc[2]));

@@ -123,6 +123,8 @@ struct titan_ctx {
     uint64_t budget = 0, used = 0;
 
     double dt = 0.0001;                  // src/sim.cu:78
+    int32_t integ = kIntEuler;           // the reference's build option (CMakeLists.txt:13-14)
+    double *rk = nullptr;                // RK2: state saved by a first stage, 6 per mass slot
     double g[3] = {0.0, 0.0, -9.81};     // src/sim.cu:86
     double T = 0.0;                      // Simulation::T
     std::vector<double> planes_h, balls_h;
@@ -286,9 +288,10 @@ void ensure_chunks(titan_ctx *c, int64_t need) {
     nc = std::min(nc, max_chunks);
     const int64_t on = old * kChunk, nn = nc * kChunk;
     check_growth(c, (size_t)nn * (96 + 8 + 8 + 24 + 4 + 4 + (c->ext ? 24 : 0) + (c->drag ? 4 : 0) +
-                                  (c->mlc ? 8 : 0)) +
+                                  (c->mlc ? 8 : 0) + (c->integ == kIntRK2 ? 48 : 0)) +
                         (size_t)nc * (4 + 8 + 4 + 8 + 4));
     grow_soa(c, c->st, 2, 6 * on, 6 * nn);
+    if (c->integ == kIntRK2) grow_soa(c, c->rk, 1, 6 * on, 6 * nn);
     grow_soa(c, c->m, 1, on, nn);
     grow_soa(c, c->Tm, 1, on, nn);
     grow_soa(c, c->acc, 3, on, nn);
@@ -433,6 +436,11 @@ void make_act(titan_ctx *c) {
     drop_graphs(c);
 }
 
+// Launches per step and the clock advance of one launch (RK2: two stages of
+// dt / 2, src/sim.cu:1572-1582).
+int64_t launches_per_step(const titan_ctx *c) { return c->integ == kIntRK2 ? 2 : 1; }
+double launch_dt(const titan_ctx *c) { return c->integ == kIntRK2 ? 0.5 * c->dt : c->dt; }
+
 // LDS rows of the largest live tile
 int32_t tiled_rows(const titan_ctx *c) {
     int32_t h = 0;
@@ -528,6 +536,9 @@ StepArgs make_args(titan_ctx *c, int slot, const Range &rg) {
     a.b_d = c->b_d;
     a.b_kd = c->b_kd;
     a.act = c->act;
+    a.integ = c->integ;
+    a.hdt = launch_dt(c);
+    a.rk = c->rk;
     a.ract[0] = c->ract[0];
     a.ract[1] = c->ract[1];
     a.chunk_act0 = c->chunk_act0;
@@ -753,7 +764,7 @@ void run_launches(titan_ctx *c, int64_t n, const Range &rg) {
         s ^= 1;
     }
     s = c->slot ^ (int)(n & 1);
-    HIPCHK(launch_advance_T(c->Tm, c->mflags, c->chunk_body, c->bn[s], c->bnsync, rg.c0, rg.c1, nb, c->dt,
+    HIPCHK(launch_advance_T(c->Tm, c->mflags, c->chunk_body, c->bn[s], c->bnsync, rg.c0, rg.c1, nb, launch_dt(c),
                             c->stream));
     c->slot = s;
 }
@@ -762,18 +773,20 @@ void do_step(titan_ctx *c, int64_t n) {
     if (n < 0) fail(TITAN_E_INVALID, "step: negative step count");
     maybe_inject(c);
     if (n == 0) return;
+    const int64_t nl = n * launches_per_step(c);
+    const double h = launch_dt(c);
     const int64_t nb = live_body_hw(c);
     if (nb > 0 && c->chunks.high_water() > 0) {
         prepare_launches(c);
-        HIPCHK(launch_set_left(c->bleft[c->slot], c->b_chunk0, nb, n, c->stream));
-        run_launches(c, n, full_range(c));
+        HIPCHK(launch_set_left(c->bleft[c->slot], c->b_chunk0, nb, nl, c->stream));
+        run_launches(c, nl, full_range(c));
     }
     for (auto &b : c->bodies) {
         if (!b.live) continue;
-        for (int64_t r = 0; r < n; ++r) b.T += c->dt;
+        for (int64_t r = 0; r < nl; ++r) b.T += h;
         b.n += n;
     }
-    for (int64_t r = 0; r < n; ++r) c->T += c->dt;  // src/sim.cu:1590
+    for (int64_t r = 0; r < nl; ++r) c->T += h;  // src/sim.cu:1590 (RK2: :1577,1582)
 }
 
 BodyH &body_of(titan_ctx *c, int32_t id);
@@ -788,8 +801,8 @@ void do_step_bodies(titan_ctx *c, const int32_t *ids, const int64_t *steps, int3
         body_of(c, ids[k]);  // validates
         if (steps[k] < 0) fail(TITAN_E_INVALID, "step_bodies: negative step count");
         if (left[ids[k]] != 0) fail(TITAN_E_INVALID, "step_bodies: a body is listed twice");
-        left[ids[k]] = steps[k];
-        n = std::max(n, steps[k]);
+        left[ids[k]] = steps[k] * launches_per_step(c);
+        n = std::max(n, left[ids[k]]);
     }
     maybe_inject(c);
     if (n == 0) return;
@@ -823,9 +836,10 @@ void do_step_bodies(titan_ctx *c, const int32_t *ids, const int64_t *steps, int3
         HIPCHK(hipEventRecord(c->left_ev[k], c->stream));
         run_launches(c, n, full_range(c));
     }
+    const double h = launch_dt(c);
     for (int32_t k = 0; k < count; ++k) {
         BodyH &b = c->bodies[ids[k]];
-        for (int64_t r = 0; r < steps[k]; ++r) b.T += c->dt;
+        for (int64_t r = 0; r < left[ids[k]]; ++r) b.T += h;
         b.n += steps[k];
     }
 }
@@ -1337,7 +1351,7 @@ void unpermute(const BodyH &b, const std::vector<double> &soa, int K, double *ou
 void body_download(titan_ctx *c, int32_t id, double *pos, double *vel, double *acc, double *T) {
     BodyH &b = body_of(c, id);
     const int64_t N = b.n_masses, g0 = b.chunk0 * kChunk, n6 = c->ncap;
-    const int p = (int)(b.n & 1);
+    const int p = (int)((b.n * launches_per_step(c)) & 1);
     auto dev = [&](int64_t q) -> int64_t { return q; };
     auto host = [&](int64_t q) -> int64_t { return b.perm.empty() ? q : b.perm[q]; };
     (void)dev;
@@ -1368,7 +1382,7 @@ void body_upload(titan_ctx *c, int32_t id, const double *pos, const double *vel,
                  const double *T, const double *ext) {
     BodyH &b = body_of(c, id);
     const int64_t N = b.n_masses, g0 = b.chunk0 * kChunk, n6 = c->ncap;
-    const int p = (int)(b.n & 1);
+    const int p = (int)((b.n * launches_per_step(c)) & 1);
     HIPCHK(hipStreamSynchronize(c->stream));
     auto host = [&](int64_t q) -> int64_t { return b.perm.empty() ? q : b.perm[q]; };
     if (pos || vel) {  // read-modify-write of the 48-byte records
@@ -1634,8 +1648,10 @@ int titan_step_until(titan_ctx *c, double t, int64_t *steps_done) {
     // (src/sim.cu:1505); count the steps with the same T += dt recurrence
     double T = c->T;
     int64_t k = 0;
+    const int64_t lps = launches_per_step(c);
+    const double h = launch_dt(c);
     while (!(t <= T)) {
-        T += c->dt;
+        for (int64_t q = 0; q < lps; ++q) T += h;
         if (++k > (int64_t)1 << 40) fail(TITAN_E_INVALID, "step_until: target time too far ahead");
     }
     do_step(c, k);
@@ -1663,6 +1679,31 @@ int titan_set_schedule(titan_ctx *c, const titan_schedule *s) {
     API_END(c)
 }
 
+int titan_set_integrator(titan_ctx *c, int32_t kind) {
+    NEED_CTX(c);
+    API_BEGIN
+    if (kind < TITAN_INTEGRATOR_EULER || kind > TITAN_INTEGRATOR_RK2) fail(TITAN_E_INVALID, "set_integrator: bad kind");
+    for (const auto &b : c->bodies)
+        if (b.live) fail(TITAN_E_INVALID, "set_integrator: choose the integrator before creating bodies");
+    if (kind == c->integ) return TITAN_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->integ = kind;
+    if (kind == kIntRK2 && !c->rk && c->ncap > 0) {
+        c->rk = dalloc<double>(c, 6 * c->ncap);
+        HIPCHK(hipMemsetAsync(c->rk, 0, sizeof(double) * 6 * c->ncap, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    drop_graphs(c);
+    API_END(c)
+}
+
+int titan_get_integrator(const titan_ctx *c, int32_t *kind) {
+    NEED_CTX(c);
+    if (!kind) return TITAN_E_INVALID;
+    *kind = c->integ;
+    return TITAN_OK;
+}
+
 int titan_get_schedule(const titan_ctx *c, titan_schedule *s) {
     NEED_CTX(c);
     if (!s) return TITAN_E_INVALID;
@@ -1687,6 +1728,7 @@ double time_schedule(titan_ctx *c, const titan_body_desc *samples, int32_t n_sam
     if (titan_ctx_create(c->device, c->budget, &t) != TITAN_OK)
         fail(TITAN_E_HIP, std::string("autotune: scratch context: ") + titan_last_error(nullptr));
     chk(titan_set_dt(t, c->dt), "set_dt");
+    chk(titan_set_integrator(t, c->integ), "set_integrator");
     chk(titan_set_global_acc(t, c->g[0], c->g[1], c->g[2]), "set_global_acc");
     for (size_t q = 0; q < c->planes_h.size(); q += 8) {
         const double *P = c->planes_h.data() + q;

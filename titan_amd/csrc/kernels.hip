@@ -154,10 +154,39 @@ __device__ __forceinline__ void apply_local(MassRegs &r, const LocalC &c) {
     }
 }
 
-// Everything of massForcesAndUpdate after the spring sum (src/sim.cu:1104-1167,
-// Euler branch), applied to registers.  Returns the acceleration.
+// What a launch's integrator reads besides the mass's state: Verlet the
+// previous step's acceleration (p[0..2]), an RK2 second stage the state its
+// first stage left (p[0..5]: pos, vel).  Loaded early, with the other loads.
+struct IntegIn {
+    double p[6];
+};
+__device__ __forceinline__ IntegIn load_integ(const StepArgs &a, int64_t i, int stage) {
+    IntegIn q;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q.p[k] = 0.0;
+    if (a.integ == kIntVerlet) {
+        q.p[0] = a.acc[i];
+        q.p[1] = a.acc[a.ncap + i];
+        q.p[2] = a.acc[2 * a.ncap + i];
+    } else if (a.integ == kIntRK2 && stage == 1) {
+        const double2 *__restrict__ B = reinterpret_cast<const double2 *>(a.rk) + 3 * i;
+        const double2 x = B[0], y = B[1], z = B[2];
+        q.p[0] = x.x;
+        q.p[1] = x.y;
+        q.p[2] = y.x;
+        q.p[3] = y.y;
+        q.p[4] = z.x;
+        q.p[5] = z.y;
+    }
+    return q;
+}
+
+// Everything of massForcesAndUpdate after the spring sum (src/sim.cu:1104-1167),
+// applied to registers, with the context's integrator.  Returns the
+// acceleration.
 __device__ __forceinline__ void mass_pass(const StepArgs &a, MassRegs &r, int64_t i, uint32_t fl,
-                                          int b, double m, double &ax, double &ay, double &az) {
+                                          int b, double m, int stage, const IntegIn &in, double &ax,
+                                          double &ay, double &az) {
     r.fx = r.fx + a.gx * m;                                    // :1104
     r.fy = r.fy + a.gy * m;
     r.fz = r.fz + a.gz * m;
@@ -210,15 +239,46 @@ __device__ __forceinline__ void mass_pass(const StepArgs &a, MassRegs &r, int64_
             r.fz = r.fz + (r.vz * c) / vn;
         }
     }
-    ax = r.fx / m;                                             // :1162-1164
+    const double dt = a.dt;
+    ax = r.fx / m;                                             // :1143, :1151, :1158, :1162
     ay = r.fy / m;
     az = r.fz / m;
-    r.vx = r.vx + ax * a.dt;
-    r.vy = r.vy + ay * a.dt;
-    r.vz = r.vz + az * a.dt;
-    r.px = r.px + r.vx * a.dt;
-    r.py = r.py + r.vy * a.dt;
-    r.pz = r.pz + r.vz * a.dt;
+    if (a.integ == kIntRK2) {
+        if (stage == 0) {                                      // :1142-1149
+            double2 *__restrict__ B = reinterpret_cast<double2 *>(a.rk) + 3 * i;
+            B[0] = make_double2(r.px, r.py);
+            B[1] = make_double2(r.pz, r.vx);
+            B[2] = make_double2(r.vy, r.vz);
+            r.px = r.px + (0.5 * r.vx) * dt;
+            r.py = r.py + (0.5 * r.vy) * dt;
+            r.pz = r.pz + (0.5 * r.vz) * dt;
+            r.vx = r.vx + (0.5 * ax) * dt;
+            r.vy = r.vy + (0.5 * ay) * dt;
+            r.vz = r.vz + (0.5 * az) * dt;
+        } else {                                               // :1150-1155
+            r.px = in.p[0] + r.vx * dt;
+            r.py = in.p[1] + r.vy * dt;
+            r.pz = in.p[2] + r.vz * dt;
+            r.vx = in.p[3] + ax * dt;
+            r.vy = in.p[4] + ay * dt;
+            r.vz = in.p[5] + az * dt;
+        }
+    } else if (a.integ == kIntVerlet) {                        // :1156-1160
+        const double dt2 = dt * dt;                            // pow(dt, 2)
+        r.vx = r.vx + (0.5 * (in.p[0] + ax)) * dt;
+        r.vy = r.vy + (0.5 * (in.p[1] + ay)) * dt;
+        r.vz = r.vz + (0.5 * (in.p[2] + az)) * dt;
+        r.px = r.px + (r.vx * dt + (0.5 * ax) * dt2);
+        r.py = r.py + (r.vy * dt + (0.5 * ay) * dt2);
+        r.pz = r.pz + (r.vz * dt + (0.5 * az) * dt2);
+    } else {                                                   // :1161-1167
+        r.vx = r.vx + ax * dt;
+        r.vy = r.vy + ay * dt;
+        r.vz = r.vz + az * dt;
+        r.px = r.px + r.vx * dt;
+        r.py = r.py + r.vy * dt;
+        r.pz = r.pz + r.vz * dt;
+    }
 }
 
 }  // namespace
@@ -456,12 +516,13 @@ struct SpringWalk {
     }
 };
 
-// The body clock of launch slot s is advanced once per step by the body's
-// first lane: T += dt (src/sim.cu:1590) while steps are left.
+// The body clock of launch slot s is advanced once per launch by the body's
+// first lane: T += dt (src/sim.cu:1590; RK2: T += dt / 2 per stage,
+// :1577,1582) while launches are left.
 __device__ __forceinline__ void advance_body_clock(const StepArgs &a, int b, int s, int64_t left,
                                                    int64_t nst, double T) {
     const bool go = left > 0;
-    a.bT[s ^ 1][b] = go ? T + a.dt : T;
+    a.bT[s ^ 1][b] = go ? T + a.hdt : T;
     a.bn[s ^ 1][b] = go ? nst + 1 : nst;
     a.bleft[s ^ 1][b] = go ? left - 1 : 0;
 }
@@ -481,7 +542,7 @@ __device__ __forceinline__ void store_mass(const StepArgs &a, double2 *__restric
     store_nt(&nxt[3 * i], r.px, r.py);
     store_nt(&nxt[3 * i + 1], r.pz, r.vx);
     store_nt(&nxt[3 * i + 2], r.vy, r.vz);
-    if (last) {
+    if (last || a.integ == kIntVerlet) {  // Verlet reads it back next step
         a.acc[i] = ax;
         a.acc[a.ncap + i] = ay;
         a.acc[2 * a.ncap + i] = az;
@@ -520,11 +581,13 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     const double2 o0 = cur[3 * i], o1 = cur[3 * i + 1], o2 = cur[3 * i + 2];
     const int d = a.deg[i];
     const double m = a.m[i];
+    const int stage = (int)(nst & 1);
+    const IntegIn in = load_integ(a, i, stage);
     SpringWalk<ACT, kGroupGather> w;
     w.prime(a, ebase, ng, lane, bkd);
     // the chunk's share of its body's actuation factors for the next step
     // (the body's clock then reads T + dt), while the loads are in flight
-    if (ACT && left > 1) actuate_slice(a, a.chunk_act0[chunk], a.chunk_nact[chunk], T + a.dt, a.ract[s ^ 1], lane, kChunk);
+    if (ACT && left > 1) actuate_slice(a, a.chunk_act0[chunk], a.chunk_nact[chunk], T + a.hdt, a.ract[s ^ 1], lane, kChunk);
     if (fl & kPad) return;
     if (fl & kFixed) {  // src/sim.cu:1100-1101: a fixed mass keeps its state
         store_nt(&nxt[3 * i], o0.x, o0.y);
@@ -547,7 +610,7 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
         q2 = ldo(cur, ob + 32u);
     });
     double ax, ay, az;
-    mass_pass(a, r, i, fl, b, m, ax, ay, az);
+    mass_pass(a, r, i, fl, b, m, stage, in, ax, ay, az);
     store_mass(a, nxt, i, r, ax, ay, az, left == 1);
 }
 
@@ -601,9 +664,12 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     double2 o0 = make_double2(0.0, 0.0), o1 = o0, o2 = o0;
     int d = 0;
     double m = 1.0;
+    const int stage = (int)(nst & 1);
+    IntegIn in{};
     SpringWalk<ACT, kGroupTiled> w;
     if (live) {
         fl = a.mflags[i];
+        in = load_integ(a, i, stage);
         o0 = cur[3 * i];
         o1 = cur[3 * i + 1];
         o2 = cur[3 * i + 2];
@@ -612,7 +678,7 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
         w.prime(a, ebase, ng, t & (kChunk - 1), bkd);
         // the chunk's share of its body's actuation factors for the next step
         if (ACT && left > 1)
-            actuate_slice(a, a.chunk_act0[chunk], a.chunk_nact[chunk], T + a.dt, a.ract[s ^ 1], t & (kChunk - 1), kChunk);
+            actuate_slice(a, a.chunk_act0[chunk], a.chunk_nact[chunk], T + a.hdt, a.ract[s ^ 1], t & (kChunk - 1), kChunk);
     }
     // halo rows, two per thread per round: both indices, then both states.
     // Loads past the halo are clamped to its last entry and stored to the
@@ -659,7 +725,7 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
         q2 = S[3 * o + 2];
     });
     double ax, ay, az;
-    mass_pass(a, r, i, fl, b, m, ax, ay, az);
+    mass_pass(a, r, i, fl, b, m, stage, in, ax, ay, az);
     store_mass(a, nxt, i, r, ax, ay, az, left == 1);
 }
 
@@ -743,6 +809,8 @@ __global__ __launch_bounds__(kResidentBlock) void k_rollout_resident(const StepA
                 nxt[3 * q + 2] = o2;
                 continue;
             }
+            const int stage = (int)((nst + k) & 1);
+            const IntegIn in = load_integ(a, i, stage);
             MassRegs r;
             r.px = o0.x;
             r.py = o0.y;
@@ -758,17 +826,17 @@ __global__ __launch_bounds__(kResidentBlock) void k_rollout_resident(const StepA
                 q2 = cur[3 * o + 2];
             });
             double ax, ay, az;
-            mass_pass(a, r, i, fl, b, a.m[i], ax, ay, az);
+            mass_pass(a, r, i, fl, b, a.m[i], stage, in, ax, ay, az);
             nxt[3 * q] = make_double2(r.px, r.py);
             nxt[3 * q + 1] = make_double2(r.pz, r.vx);
             nxt[3 * q + 2] = make_double2(r.vy, r.vz);
-            if (k == K - 1) {
+            if (k == K - 1 || a.integ == kIntVerlet) {
                 a.acc[i] = ax;
                 a.acc[n6 + i] = ay;
                 a.acc[2 * n6 + i] = az;
             }
         }
-        T = T + a.dt;  // src/sim.cu:1590
+        T = T + a.hdt;  // src/sim.cu:1590 (RK2: :1577,1582)
         if (ACT && k + 1 < K) actuate_slice(a, act0, nact, T, a.ract[(k + 1) & 1], t, kResidentBlock);
         __syncthreads();
     }

@@ -95,6 +95,12 @@ struct BodyInit {
     uint8_t mode, kd, on;  // on: 1 joins the launches, 0 leaves them
 };
 
+// Integrators (the reference's default Euler branch and its VERLET and RK2
+// build options, src/sim.cu:1141-1167).  RK2 takes two launches per step
+// (stages 0 and 1, by the parity of the body's launch count); every launch
+// advances the clock by hdt (dt, or dt / 2 for RK2).
+constexpr int32_t kIntEuler = 0, kIntVerlet = 1, kIntRK2 = 2;
+
 // per-body layout (b_mode)
 constexpr uint8_t kModeGather = 0, kModeTiled = 1, kModeResident = 2;
 
@@ -145,6 +151,11 @@ struct StepArgs {
     const double *inc_k;      // per incidence, for bodies with per-spring k (else null)
     const double *inc_damp;   // per incidence, for bodies with per-spring damping (else null)
     const double *act;        // 4 per actuator: a, omega, phi, rest
+    // integrator; RK2: the state a first stage leaves for the second (pos,
+    // vel after the constraints: 6 per mass, as the state buffers)
+    int32_t integ;
+    double hdt;
+    double *rk;
     // actuated rest lengths rest (1 + a sin(omega T + phi)), one per
     // actuator, for the step that reads launch slot s (ract[s]); each chunk
     // computes its share of its body's actuators, [chunk_act0, chunk_act0 +
