@@ -104,10 +104,10 @@ static void check(int64_t n, const std::vector<int32_t> &L, const std::vector<in
     SellBuild sg;
     build_sell(n, sp, kd, kd, nullptr, 4096, sg);
     check_sell(n, L, R, k, rest, damp, kd ? act : noact, kd, nullptr, 4096, sg);
-    // tiled layout: tiles, halos and LDS rows, under two row budgets
-    for (int rows : {kTileRows, 682}) {
-    TileBuild tb;
-    CHECK(build_tiles(n, sp, tb, rows));
+    // tiled layout: tiles, halos and LDS rows, under two row budgets, then
+    // with the masses of each tile reordered by degree
+    auto verify_tiles = [&](const std::vector<int32_t> &L, const std::vector<int32_t> &R, const TileBuild &tb,
+                            int rows) {
     int32_t hmax = 0, rmax = 0;
     int64_t next = 0;  // tiles cover the masses in order, chunk-aligned
     for (size_t t = 0; t < tb.tiles.size(); ++t) {
@@ -137,9 +137,42 @@ static void check(int64_t n, const std::vector<int32_t> &L, const std::vector<in
     CHECK(next == n);
     CHECK(tb.rmax == rmax);
     CHECK(tb.hmax == hmax);
+    };
+    for (int rows : {kTileRows, 682}) {
+    TileBuild tb;
+    CHECK(build_tiles(n, sp, tb, rows));
+    verify_tiles(L, R, tb, rows);
     SellBuild st;
     build_sell(n, sp, kd, kd, &tb, 0, st);
     check_sell(n, L, R, k, rest, damp, kd ? act : noact, kd, &tb, 0, st);
+    // reorder each tile's masses by degree: a permutation inside each tile,
+    // degrees non-increasing there, halos rewritten in the new indices
+    std::vector<int32_t> order = degree_sort_tiles(n, sp, &tb);
+    if (order.empty()) continue;
+    std::vector<int32_t> inv(n, -1), deg(n, 0), L2(S), R2(S);
+    for (int64_t s = 0; s < S; ++s)
+        if (L[s] >= 0 && R[s] >= 0) {
+            deg[L[s]]++;
+            deg[R[s]]++;
+        }
+    for (int64_t q = 0; q < n; ++q) {
+        CHECK(order[q] >= 0 && order[q] < n && inv[order[q]] < 0);
+        CHECK(tb.tile_of_chunk[q / kChunk] == tb.tile_of_chunk[order[q] / kChunk]);
+        inv[order[q]] = (int32_t)q;
+        if (q > 0 && tb.tile_of_chunk[q / kChunk] == tb.tile_of_chunk[(q - 1) / kChunk])
+            CHECK(deg[order[q - 1]] >= deg[order[q]]);
+    }
+    for (int64_t s = 0; s < S; ++s) {
+        L2[s] = L[s] < 0 ? -1 : inv[L[s]];
+        R2[s] = R[s] < 0 ? -1 : inv[R[s]];
+    }
+    verify_tiles(L2, R2, tb, rows);
+    SpringsIn sp2 = sp;
+    sp2.left = L2.data();
+    sp2.right = R2.data();
+    SellBuild s2;
+    build_sell(n, sp2, kd, kd, &tb, 0, s2);
+    check_sell(n, L2, R2, k, rest, damp, kd ? act : noact, kd, &tb, 0, s2);
     }
 }
 

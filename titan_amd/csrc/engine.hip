@@ -1015,6 +1015,21 @@ PreparedBody prepare_body(const titan_schedule &sched, const titan_body_desc *d)
     if (!tiled) tb = TileBuild{};
     P.tiled = tiled;
     P.resident = resident;
+    if (tiled || resident) {  // masses of a tile (or resident body) by descending degree
+        const std::vector<int32_t> order = degree_sort_tiles(N, sp, tiled ? &tb : nullptr);
+        if (!order.empty()) {
+            std::vector<int32_t> np(N), inv(N);
+            for (int64_t q = 0; q < N; ++q) {
+                np[q] = perm.empty() ? order[q] : perm[order[q]];
+                inv[order[q]] = (int32_t)q;
+            }
+            perm.swap(np);
+            for (int64_t s = 0; s < S; ++s) {
+                if (nl[s] >= 0) nl[s] = inv[nl[s]];
+                if (nr[s] >= 0) nr[s] = inv[nr[s]];
+            }
+        }
+    }
     auto orig = [&](int64_t q) -> int64_t { return perm.empty() ? q : perm[q]; };
     // incidences: LDS rows (tiled, resident) or body-relative slots (gather)
     if (tiled) {

@@ -581,8 +581,6 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     const double2 o0 = cur[3 * i], o1 = cur[3 * i + 1], o2 = cur[3 * i + 2];
     const int d = a.deg[i];
     const double m = a.m[i];
-    const int stage = (int)(nst & 1);
-    const IntegIn in = load_integ(a, i, stage);
     SpringWalk<ACT, kGroupGather> w;
     w.prime(a, ebase, ng, lane, bkd);
     // the chunk's share of its body's actuation factors for the next step
@@ -609,6 +607,9 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
         q1 = ldo(cur, ob + 16u);
         q2 = ldo(cur, ob + 32u);
     });
+    // read after the walk, so that nothing is held through it (Euler reads nothing)
+    const int stage = (int)(nst & 1);
+    const IntegIn in = load_integ(a, i, stage);
     double ax, ay, az;
     mass_pass(a, r, i, fl, b, m, stage, in, ax, ay, az);
     store_mass(a, nxt, i, r, ax, ay, az, left == 1);
@@ -664,12 +665,9 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     double2 o0 = make_double2(0.0, 0.0), o1 = o0, o2 = o0;
     int d = 0;
     double m = 1.0;
-    const int stage = (int)(nst & 1);
-    IntegIn in{};
     SpringWalk<ACT, kGroupTiled> w;
     if (live) {
         fl = a.mflags[i];
-        in = load_integ(a, i, stage);
         o0 = cur[3 * i];
         o1 = cur[3 * i + 1];
         o2 = cur[3 * i + 2];
@@ -724,6 +722,8 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
         q1 = S[3 * o + 1];
         q2 = S[3 * o + 2];
     });
+    const int stage = (int)(nst & 1);
+    const IntegIn in = load_integ(a, i, stage);  // after the walk (see the gather step)
     double ax, ay, az;
     mass_pass(a, r, i, fl, b, m, stage, in, ax, ay, az);
     store_mass(a, nxt, i, r, ax, ay, az, left == 1);
@@ -809,8 +809,6 @@ __global__ __launch_bounds__(kResidentBlock) void k_rollout_resident(const StepA
                 nxt[3 * q + 2] = o2;
                 continue;
             }
-            const int stage = (int)((nst + k) & 1);
-            const IntegIn in = load_integ(a, i, stage);
             MassRegs r;
             r.px = o0.x;
             r.py = o0.y;
@@ -826,6 +824,8 @@ __global__ __launch_bounds__(kResidentBlock) void k_rollout_resident(const StepA
                 q2 = cur[3 * o + 2];
             });
             double ax, ay, az;
+            const int stage = (int)((nst + k) & 1);
+            const IntegIn in = load_integ(a, i, stage);
             mass_pass(a, r, i, fl, b, a.m[i], stage, in, ax, ay, az);
             nxt[3 * q] = make_double2(r.px, r.py);
             nxt[3 * q + 1] = make_double2(r.pz, r.vx);

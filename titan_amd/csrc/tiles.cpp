@@ -207,6 +207,39 @@ void build_sell(int64_t n, const SpringsIn &sp, bool per_k, bool per_d, const Ti
     }
 }
 
+std::vector<int32_t> degree_sort_tiles(int64_t n, const SpringsIn &sp, TileBuild *tb) {
+    std::vector<int32_t> deg(n, 0);
+    for (int64_t s = 0; s < sp.n_springs; ++s) {
+        if (sp.left[s] < 0 || sp.right[s] < 0) continue;
+        deg[sp.left[s]]++;
+        deg[sp.right[s]]++;
+    }
+    std::vector<int32_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    auto sort_range = [&](int64_t m0, int64_t m1) {
+        std::stable_sort(order.begin() + m0, order.begin() + m1,
+                         [&](int32_t a, int32_t b) { return deg[a] > deg[b]; });
+    };
+    if (tb) {
+        for (const TileDesc &d : tb->tiles) sort_range((int64_t)d.chunk0 * kChunk, (int64_t)d.chunk0 * kChunk + d.nmass);
+    } else {
+        sort_range(0, n);
+    }
+    bool moved = false;
+    for (int64_t q = 0; q < n && !moved; ++q) moved = order[q] != q;
+    if (!moved) return {};
+    if (tb) {
+        std::vector<int32_t> inv(n);
+        for (int64_t q = 0; q < n; ++q) inv[order[q]] = (int32_t)q;
+        for (const TileDesc &d : tb->tiles) {
+            int32_t *h = tb->halo.data() + d.halo_off;
+            for (int32_t j = 0; j < d.nhalo; ++j) h[j] = inv[h[j]];
+            std::sort(h, h + d.nhalo);
+        }
+    }
+    return order;
+}
+
 void mark_actuated(SellBuild &sb, bool packed, int64_t act0) {
     if (sb.act.empty()) return;
     const int64_t nch = (int64_t)sb.coff.size() - 1;

@@ -62,6 +62,14 @@ std::vector<int32_t> morton_order(const double *pos, int64_t n);
 // tb.why) if one chunk and its halo exceed kMaxTileRows.
 bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb, int tile_rows = kTileRows);
 
+// Reorder the masses inside each tile (tb; null: the whole body as one tile)
+// by descending degree, ties in their current order.  A tile's masses are all
+// in LDS, so their order inside it changes no locality, and chunks of even
+// degree waste fewer SELL padding entries.  Returns the new order (new ->
+// current index; empty when nothing moves) and rewrites tb's halo lists in
+// the new indices (each still ascending); the tiles keep their chunks.
+std::vector<int32_t> degree_sort_tiles(int64_t n, const SpringsIn &sp, TileBuild *tb);
+
 // Build the incidence lists.  With tb == nullptr (gather layout) an
 // incidence names the other endpoint's arena slot, g0 + its body index;
 // with a tile build it names the endpoint's LDS row in the mass's tile.
