@@ -258,6 +258,41 @@ def bench_serving(seed: int, n_req: int, lams, quantum: int, n_base: int) -> dic
     }
 
 
+def bench_readme(steps: int) -> dict:
+    """The README scene (125 masses, 1036 springs, damping 0.1, ground plane
+    mu_k 0.8 / mu_s 1.0, dt 1e-4), both starting heights: latency-bound, so
+    reported as time per step for each layout (graph-launched, the call's
+    steps in one launch for the resident layout), beside the oracle on one
+    host thread."""
+    from oracle import oracle as orc  # CPU baseline only
+    from tests.helpers import to_oracle
+    from titan_amd import LAYOUT_GATHER, LAYOUT_RESIDENT, LAYOUT_TILED, Context, scenes
+
+    out = {}
+    for cz in (10.0, 2.45):
+        sc = scenes.readme_lattice(cz)
+        per = {}
+        for name, lay in (("gather", LAYOUT_GATHER), ("tiled", LAYOUT_TILED), ("resident", LAYOUT_RESIDENT)):
+            with Context() as ctx:
+                sc.load(ctx, layout=lay)
+                ctx.step(steps // 10)
+                ctx.sync()
+                ctx.timer_start()
+                ctx.step(steps)
+                per[name] = 1e3 * ctx.timer_stop() / steps
+        o = to_oracle(sc, threads=1)
+        o.run(steps // 10)
+        t0 = time.perf_counter()
+        o.run(steps)
+        cpu = 1e6 * (time.perf_counter() - t0) / steps
+        log(f"readme cz={cz}: us/step " + ", ".join(f"{k} {v:.2f}" for k, v in per.items()) +
+            f"; CPU oracle (1 thread) {cpu:.2f}")
+        out[f"cz{cz:g}"] = {"us_per_step": per, "cpu_us_per_step_1_thread": cpu}
+    return {"workload": f"README lattice 5x5x5 (125 masses, 1036 springs), cz 10 (free fall) and 2.45 "
+                        f"(in contact from step 0), dt=1e-4, FP64, {steps} timed steps after {steps // 10}",
+            **out}
+
+
 def bench_irregular(n: int, n_balls: int, seed: int, steps: int, warmup: int) -> dict:
     """Irregular spring graph: a 2M-mass kNN-12 cloud (variable degree,
     per-spring k), 1000 contact balls and a ground plane, dt 1e-5, FP64; one
@@ -310,8 +345,8 @@ def main() -> int:
     ap.add_argument("--layout", choices=["auto", "gather", "tiled"], default="auto",
                     help="device layout of the body (results are identical; speed differs)")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the batched, serving and irregular-graph workloads")
-    ap.add_argument("--only", choices=["lattice", "batched", "serving", "irregular"], default=None,
+                    help="skip the README, batched, serving and irregular-graph workloads")
+    ap.add_argument("--only", choices=["lattice", "readme", "batched", "serving", "irregular"], default=None,
                     help="run one workload (its figures go in the same JSON line)")
     ap.add_argument("--bodies", type=int, default=4096, help="batched workload: robot count")
     ap.add_argument("--requests", type=int, default=256, help="serving workload: requests per arrival rate")
@@ -349,6 +384,8 @@ def main() -> int:
         return 2
 
     extra = {}
+    if args.only == "readme" or (args.only is None and not args.no_extra):
+        extra["readme"] = bench_readme(10000)
     if args.only == "batched" or (args.only is None and not args.no_extra):
         extra["batched"] = bench_batched(args.bodies, args.seed, rollout=100, sample=min(512, args.bodies))
     if args.only == "serving" or (args.only is None and not args.no_extra):
@@ -357,7 +394,7 @@ def main() -> int:
                                          quantum=64, n_base=16)
     if args.only == "irregular" or (args.only is None and not args.no_extra):
         extra["irregular"] = bench_irregular(2_000_000, 1000, args.seed, steps=500, warmup=20)
-    if args.only in ("batched", "serving", "irregular"):
+    if args.only in ("readme", "batched", "serving", "irregular"):
         print(json.dumps(extra), flush=True)
         return 0
     out = bench_lattice(args)

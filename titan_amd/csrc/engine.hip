@@ -101,6 +101,7 @@ struct BodyH {
     bool tiled = false;
     bool resident = false;
     int64_t res = -1;  // resident descriptor (resident layout)
+    int32_t res_cache = 0;  // LDS bytes of its incidence copy (resident layout; 0: none)
     int64_t n_masses = 0, chunk0 = 0, nchunks = 0;
     int64_t inc0 = 0, inc_len = 0, act0 = 0, act_len = 0, lc0 = 0, lc_len = 0;
     int64_t tile0 = 0, ntiles = 0, halo0 = 0, nhalo = 0;
@@ -172,6 +173,7 @@ struct titan_ctx {
     int32_t bg_n[3] = {0, 0, 0};
 
     bool has_act = false;  // some spring is actuated: launch the actuated variants
+    int res_lds_max = 0;   // LDS a resident workgroup may have
     int slot = 0;
 
     // captured multi-step graphs, valid while `version` is unchanged, keyed
@@ -402,7 +404,7 @@ void ensure_rdescs(titan_ctx *c, int64_t need) {
     const int64_t old = c->rdescs.cap();
     const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 64));
     grow_soa(c, c->rdesc, 1, old, nc);
-    std::vector<ResidentDesc> dead(nc - old, ResidentDesc{-1, 0, 0, 0, 0});
+    std::vector<ResidentDesc> dead(nc - old, ResidentDesc{-1, 0, 0, 0, 0, 0, 0, 0});
     h2d(c, c->rdesc + old, dead.data(), nc - old);
     HIPCHK(hipStreamSynchronize(c->stream));
     c->rdescs.grow(nc);
@@ -454,6 +456,16 @@ int32_t resident_rows(const titan_ctx *c) {
     int32_t h = 0;
     for (auto &b : c->bodies)
         if (b.live && b.resident) h = std::max(h, (int32_t)b.n_masses);
+    return h;
+}
+
+// The resident launch's incidence-copy region: the largest copy of a live
+// resident body, when it fits beside the state buffers (else none copies)
+int32_t resident_cache(const titan_ctx *c) {
+    int32_t h = 0;
+    for (auto &b : c->bodies)
+        if (b.live && b.resident) h = std::max(h, b.res_cache);
+    if ((int64_t)resident_lds_bytes(resident_rows(c)) + h > c->res_lds_max) return 0;
     return h;
 }
 
@@ -580,6 +592,7 @@ StepArgs make_args(titan_ctx *c, int slot, const Range &rg) {
     a.res_base = rg.r0;
     a.nres = rg.r1;
     a.res_rows = resident_rows(c);
+    a.res_cache = resident_cache(c);
     return a;
 }
 
@@ -1278,7 +1291,11 @@ int32_t commit_body(titan_ctx *c, PreparedBody &P) {
     }
     up(c->tile + tl0, tb.tiles.data(), n_tiles);
     up(c->halo + hl0, tb.halo.data(), n_halo);
-    const ResidentDesc rdh{-1, (int32_t)ch0, (int32_t)N, (int32_t)n_act, act0};
+    // a resident body's incidence copy: names (4 B per entry, to a 16-byte
+    // boundary), rest lengths and per-incidence constants (8 B each)
+    int64_t cache = ((4 * n_inc + 15) & ~(int64_t)15) + 8 * n_inc * (1 + (P.per_k ? 1 : 0) + (P.per_d ? 1 : 0));
+    if (cache > kResidentCacheMax) cache = 0;
+    const ResidentDesc rdh{-1, (int32_t)ch0, (int32_t)N, (int32_t)n_act, act0, inc0, (int32_t)n_inc, (int32_t)cache};
     if (resident) up(c->rdesc + rs0, &rdh, 1);
     up(c->act + 4 * act0, P.acth.data(), (int64_t)P.acth.size());
     up(c->lc + lc0, P.lch.data(), n_lc);
@@ -1308,6 +1325,7 @@ int32_t commit_body(titan_ctx *c, PreparedBody &P) {
     b.tiled = tiled;
     b.resident = resident;
     b.res = rs0;
+    b.res_cache = resident ? (int32_t)cache : 0;
     b.n_masses = N;
     b.chunk0 = ch0;
     b.nchunks = nch;
@@ -1500,7 +1518,7 @@ int titan_ctx_create(int device, uint64_t hbm_budget_bytes, titan_ctx **out) {
     c = new titan_ctx();
     c->device = device;
     c->budget = hbm_budget_bytes;
-    HIPCHK(set_tiles_lds_limit());
+    HIPCHK(set_tiles_lds_limit(&c->res_lds_max));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&c->up, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&c->ev0));

@@ -16,9 +16,14 @@ size_t tiles_lds_bytes(int ldim);
 hipError_t launch_set_body(const BodyTable &t, const BodyInit &bi, hipStream_t s);
 // Actuation factors for the first step of a stepping call (launch slot a.slot).
 hipError_t launch_actuate(const StepArgs &a, hipStream_t s);
+// LDS of the resident kernel's two state buffers for `rows` masses.
+size_t resident_lds_bytes(int rows);
 hipError_t launch_rollout_resident(const StepArgs &a, bool actuated, int final_slot, hipStream_t s);
 const char *resident_kernel_name(bool actuated);
-hipError_t set_tiles_lds_limit();
+// Raise the dynamic LDS limits of the tiled and resident kernels; *resident_max
+// receives the resident kernel's limit (the device's per-workgroup maximum
+// or what the kernel can use, the smaller).
+hipError_t set_tiles_lds_limit(int *resident_max);
 hipError_t launch_advance_T(double *Tm, const uint32_t *mflags, const int32_t *chunk_body,
                             const int64_t *bn, int64_t *bnsync, int64_t chunk_begin, int64_t chunk_end,
                             int64_t nb, double dt, hipStream_t s);

@@ -339,6 +339,28 @@ struct BodyKD {
     uint32_t kd;
 };
 
+// Where a walk reads incidence data: the arena (global memory), addressed
+// through the kernel arguments or through pointers held by the walk, or a
+// copy in LDS for a resident body (entry indices then relative to the body's
+// first).  Entry offsets are even, so rest lengths and constants load as
+// 16-byte pairs.  Which form the compiler allocates fewer registers for
+// differs by kernel; each kernel uses the one measured best.
+struct ArenaSrc {
+    const StepArgs &a;
+    __device__ const uint32_t *nbr() const { return a.inc_nbr; }
+    __device__ const double *rest() const { return a.inc_rest; }
+    __device__ const double *k() const { return a.inc_k; }
+    __device__ const double *damp() const { return a.inc_damp; }
+};
+struct PtrSrc {
+    const uint32_t *n;
+    const double *r, *kk, *d;
+    __device__ const uint32_t *nbr() const { return n; }
+    __device__ const double *rest() const { return r; }
+    __device__ const double *k() const { return kk; }
+    __device__ const double *damp() const { return d; }
+};
+
 // One group's incidence data in registers: the names, rest lengths (or
 // actuator indices), per-spring constants (bodies that have them) and the
 // actuated rest lengths (actuated only).
@@ -346,24 +368,25 @@ template <bool ACT, int G>
 struct Group {
     Names<G> n;
     double rest[G], k[G], damp[G], f[G];
-    __device__ __forceinline__ void load(const StepArgs &a, int64_t ebase, int lane, int g, const BodyKD &bkd) {
+    template <class Src>
+    __device__ __forceinline__ void load(const Src &src, int64_t ebase, int lane, int g, const BodyKD &bkd) {
         constexpr int P = G / 2;  // 16-byte pairs per group and lane
-        n = Names<G>::load(a.inc_nbr, ebase, lane, g);
+        n = Names<G>::load(src.nbr(), ebase, lane, g);
         const int64_t e2 = ebase / 2 + ((int64_t)g * kChunk + lane) * P;  // double2 index
 #pragma unroll
         for (int h = 0; h < P; ++h) {
-            const double2 rr = reinterpret_cast<const double2 *>(a.inc_rest)[e2 + h];
+            const double2 rr = reinterpret_cast<const double2 *>(src.rest())[e2 + h];
             rest[2 * h] = rr.x;
             rest[2 * h + 1] = rr.y;
             if (bkd.kd & 1u) {
-                const double2 kk = reinterpret_cast<const double2 *>(a.inc_k)[e2 + h];
+                const double2 kk = reinterpret_cast<const double2 *>(src.k())[e2 + h];
                 k[2 * h] = kk.x;
                 k[2 * h + 1] = kk.y;
             } else {
                 k[2 * h] = k[2 * h + 1] = bkd.k;
             }
             if (bkd.kd & 2u) {
-                const double2 dd = reinterpret_cast<const double2 *>(a.inc_damp)[e2 + h];
+                const double2 dd = reinterpret_cast<const double2 *>(src.damp())[e2 + h];
                 damp[2 * h] = dd.x;
                 damp[2 * h + 1] = dd.y;
             } else {
@@ -470,28 +493,31 @@ __device__ __forceinline__ void eval_set(MassRegs &r, const Group<ACT, G> (&set)
 // clamped to it and re-reads a group already loaded).  prime() issues the
 // first set's loads, so that a kernel can start them before it waits for
 // anything else.
-template <bool ACT, int G>
+template <bool ACT, int G, class Src>
 struct SpringWalk {
     static_assert(G == 2 || G == 4, "groups of 2 or 4");
     static constexpr int K = 4 / G;  // groups per register set: four incidences
     Group<ACT, G> A[K], B[K];
+    const Src &src;
     int64_t ebase;
     int ng, lane;
     BodyKD bkd;
 
-    __device__ __forceinline__ void prime(const StepArgs &a, int64_t eb, int n, int ln, const BodyKD &kd) {
+    __device__ explicit SpringWalk(const Src &s) : src(s) {}
+
+    __device__ __forceinline__ void prime(int64_t eb, int n, int ln, const BodyKD &kd) {
         ebase = eb;
         ng = n;
         lane = ln;
         bkd = kd;
         if (ng <= 0) return;
 #pragma unroll
-        for (int j = 0; j < K; ++j) A[j].load(a, ebase, lane, min(j, ng - 1), bkd);
+        for (int j = 0; j < K; ++j) A[j].load(src, ebase, lane, min(j, ng - 1), bkd);
     }
 
     // rc: this step's actuation factors (actuated only)
     template <class Fetch>
-    __device__ __forceinline__ void run(const StepArgs &a, MassRegs &r, int d, const double *rc, Fetch fetch) {
+    __device__ __forceinline__ void run(MassRegs &r, int d, const double *rc, Fetch fetch) {
         r.fx = 0.0;
         r.fy = 0.0;
         r.fz = 0.0;
@@ -502,7 +528,7 @@ struct SpringWalk {
                 for (int j = 0; j < K; ++j) A[j].fetch_f(rc);
             }
 #pragma unroll
-            for (int j = 0; j < K; ++j) B[j].load(a, ebase, lane, min(g + K + j, last), bkd);
+            for (int j = 0; j < K; ++j) B[j].load(src, ebase, lane, min(g + K + j, last), bkd);
             eval_set<ACT, G, K>(r, A, g, d, fetch);
             if (g + K >= ng) break;
             if (ACT) {
@@ -510,7 +536,7 @@ struct SpringWalk {
                 for (int j = 0; j < K; ++j) B[j].fetch_f(rc);
             }
 #pragma unroll
-            for (int j = 0; j < K; ++j) A[j].load(a, ebase, lane, min(g + 2 * K + j, last), bkd);
+            for (int j = 0; j < K; ++j) A[j].load(src, ebase, lane, min(g + 2 * K + j, last), bkd);
             eval_set<ACT, G, K>(r, B, g + K, d, fetch);
         }
     }
@@ -581,8 +607,9 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     const double2 o0 = cur[3 * i], o1 = cur[3 * i + 1], o2 = cur[3 * i + 2];
     const int d = a.deg[i];
     const double m = a.m[i];
-    SpringWalk<ACT, kGroupGather> w;
-    w.prime(a, ebase, ng, lane, bkd);
+    const PtrSrc src{a.inc_nbr, a.inc_rest, a.inc_k, a.inc_damp};
+    SpringWalk<ACT, kGroupGather, PtrSrc> w(src);
+    w.prime(ebase, ng, lane, bkd);
     // the chunk's share of its body's actuation factors for the next step
     // (the body's clock then reads T + dt), while the loads are in flight
     if (ACT && left > 1) actuate_slice(a, a.chunk_act0[chunk], a.chunk_nact[chunk], T + a.hdt, a.ract[s ^ 1], lane, kChunk);
@@ -601,7 +628,7 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     r.vy = o2.x;
     r.vz = o2.y;
     // 48 bytes per mass; the arena caps a buffer below 4 GiB
-    w.run(a, r, d, a.ract[s], [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
+    w.run(r, d, a.ract[s], [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
         const uint32_t ob = o * 48u;
         q0 = ldo(cur, ob);
         q1 = ldo(cur, ob + 16u);
@@ -665,7 +692,8 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     double2 o0 = make_double2(0.0, 0.0), o1 = o0, o2 = o0;
     int d = 0;
     double m = 1.0;
-    SpringWalk<ACT, kGroupTiled> w;
+    const ArenaSrc src{a};
+    SpringWalk<ACT, kGroupTiled, ArenaSrc> w(src);
     if (live) {
         fl = a.mflags[i];
         o0 = cur[3 * i];
@@ -673,7 +701,7 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
         o2 = cur[3 * i + 2];
         d = a.deg[i];
         m = a.m[i];
-        w.prime(a, ebase, ng, t & (kChunk - 1), bkd);
+        w.prime(ebase, ng, t & (kChunk - 1), bkd);
         // the chunk's share of its body's actuation factors for the next step
         if (ACT && left > 1)
             actuate_slice(a, a.chunk_act0[chunk], a.chunk_nact[chunk], T + a.hdt, a.ract[s ^ 1], t & (kChunk - 1), kChunk);
@@ -717,7 +745,7 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     r.vx = o1.y;
     r.vy = o2.x;
     r.vz = o2.y;
-    w.run(a, r, d, a.ract[s], [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
+    w.run(r, d, a.ract[s], [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
         q0 = S[3 * o];
         q1 = S[3 * o + 1];
         q2 = S[3 * o + 2];
@@ -750,26 +778,115 @@ __global__ __launch_bounds__(kBlock) void k_actuate(const StepArgs a) {
 // and writes the other, with one barrier per step, so it never returns to
 // HBM until the last step.  Each wave walks the body's chunks w, w + 4, ...;
 // every mass is computed exactly as the other kernels do (the same spring
-// walk, mass pass and Euler update, T += dt per step), so results are
-// identical.  The body's clock fields are written to launch slot
-// final_slot, where the stepping call ends.
+// walk, mass pass and integrator, T += hdt per launch), so results are
+// identical.  A body whose incidence data fits the launch's cache region
+// copies it to LDS first and walks it there; a wave with one chunk keeps
+// its masses' constants in registers across steps.  The body's clock
+// fields are written to launch slot final_slot, where the stepping call
+// ends.
+struct MassConst {
+    uint32_t fl;
+    int deg;
+    double m;
+    int64_t ebase;
+    int ng;
+};
+
+template <bool ACT, class Src>
+__device__ __forceinline__ void resident_steps(const StepArgs &a, const Src &src, int64_t ebase_off, int b,
+                                               const BodyKD &bkd, int chunk0, int nch, int64_t g0, int64_t K,
+                                               int64_t nst, double &T, int nact, int64_t act0,
+                                               double2 *__restrict__ B0, double2 *__restrict__ B1) {
+    const int t = threadIdx.x, wave = uniform(t >> 6), lane = t & (kChunk - 1);
+    const int64_t n6 = a.ncap;
+    constexpr int kWaves = kResidentBlock / kChunk;
+    auto consts = [&](int c) {
+        MassConst mc;
+        const int64_t i = g0 + (int64_t)c * kChunk + lane;
+        mc.fl = a.mflags[i];
+        mc.deg = a.deg[i];
+        mc.m = a.m[i];
+        mc.ebase = a.chunk_ptr[chunk0 + c] - ebase_off;
+        mc.ng = a.chunk_ng[chunk0 + c];
+        return mc;
+    };
+    auto one_mass = [&](int c, const MassConst &mc, int64_t k, const double2 *__restrict__ cur,
+                        double2 *__restrict__ nxt) {
+        const int q = c * kChunk + lane;
+        const int64_t i = g0 + q;
+        if (mc.fl & kPad) return;
+        const double2 o0 = cur[3 * q], o1 = cur[3 * q + 1], o2 = cur[3 * q + 2];
+        if (mc.fl & kFixed) {  // src/sim.cu:1100-1101
+            nxt[3 * q] = o0;
+            nxt[3 * q + 1] = o1;
+            nxt[3 * q + 2] = o2;
+            return;
+        }
+        MassRegs r;
+        r.px = o0.x;
+        r.py = o0.y;
+        r.pz = o1.x;
+        r.vx = o1.y;
+        r.vy = o2.x;
+        r.vz = o2.y;
+        SpringWalk<ACT, kGroupTiled, Src> w(src);
+        w.prime(mc.ebase, mc.ng, lane, bkd);
+        w.run(r, mc.deg, a.ract[k & 1], [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
+            q0 = cur[3 * o];
+            q1 = cur[3 * o + 1];
+            q2 = cur[3 * o + 2];
+        });
+        double ax, ay, az;
+        const int stage = (int)((nst + k) & 1);
+        const IntegIn in = load_integ(a, i, stage);
+        mass_pass(a, r, i, mc.fl, b, mc.m, stage, in, ax, ay, az);
+        nxt[3 * q] = make_double2(r.px, r.py);
+        nxt[3 * q + 1] = make_double2(r.pz, r.vx);
+        nxt[3 * q + 2] = make_double2(r.vy, r.vz);
+        if (k == K - 1 || a.integ == kIntVerlet) {
+            a.acc[i] = ax;
+            a.acc[n6 + i] = ay;
+            a.acc[2 * n6 + i] = az;
+        }
+    };
+    if (nch <= kWaves) {  // one chunk per wave: its constants stay in registers
+        const bool mine = wave < nch;
+        MassConst mc{};
+        if (mine) mc = consts(wave);
+        for (int64_t k = 0; k < K; ++k) {
+            if (mine) one_mass(wave, mc, k, (k & 1) ? B1 : B0, (k & 1) ? B0 : B1);
+            T = T + a.hdt;  // src/sim.cu:1590 (RK2: :1577,1582)
+            if (ACT && k + 1 < K) actuate_slice(a, act0, nact, T, a.ract[(k + 1) & 1], t, kResidentBlock);
+            __syncthreads();
+        }
+    } else {
+        for (int64_t k = 0; k < K; ++k) {
+            for (int c = wave; c < nch; c += kWaves) one_mass(c, consts(c), k, (k & 1) ? B1 : B0, (k & 1) ? B0 : B1);
+            T = T + a.hdt;
+            if (ACT && k + 1 < K) actuate_slice(a, act0, nact, T, a.ract[(k + 1) & 1], t, kResidentBlock);
+            __syncthreads();
+        }
+    }
+}
+
 template <bool ACT>
 __global__ __launch_bounds__(kResidentBlock) void k_rollout_resident(const StepArgs a, int final_slot) {
-    extern __shared__ double2 S[];  // two buffers of 3 x res_rows pairs
+    extern __shared__ double2 S[];  // two buffers of 3 x res_rows pairs, then the incidence copy
     const int64_t rid = a.res_base + blockIdx.x;
     if (rid >= a.nres) return;
     const ResidentDesc *__restrict__ rd = a.rdesc + rid;
     const int b = rd->body;
     if (b < 0) return;
     const int nmass = rd->nmass, chunk0 = rd->chunk0, nact = rd->nact;
-    const int64_t act0 = rd->act0;
+    const int64_t act0 = rd->act0, inc0 = rd->inc0;
+    const int ninc = rd->ninc, cache = rd->cache;
     const int nch = (nmass + kChunk - 1) / kChunk;
     const BodyKD bkd{a.b_k[b], a.b_d[b], a.b_kd[b]};
     const int s0 = a.slot;
     const int64_t K = a.bleft[s0][b];
     const int64_t nst = a.bn[s0][b];
     double T = a.bT[s0][b];
-    const int t = threadIdx.x, wave = uniform(t >> 6), lane = t & (kChunk - 1);
+    const int t = threadIdx.x;
     if (K <= 0) {  // not stepped in this call: only its clock fields move to final_slot
         if (t == 0) {
             a.bT[final_slot][b] = T;
@@ -793,52 +910,28 @@ __global__ __launch_bounds__(kResidentBlock) void k_rollout_resident(const StepA
             B0[3 * q + 2] = src[3 * i + 2];
         }
     }
-    __syncthreads();
-    for (int64_t k = 0; k < K; ++k) {
-        const double2 *__restrict__ cur = (k & 1) ? B1 : B0;
-        double2 *__restrict__ nxt = (k & 1) ? B0 : B1;
-        for (int c = wave; c < nch; c += kResidentBlock / kChunk) {
-            const int q = c * kChunk + lane;
-            const int64_t i = g0 + q;
-            const uint32_t fl = a.mflags[i];
-            if (fl & kPad) continue;
-            const double2 o0 = cur[3 * q], o1 = cur[3 * q + 1], o2 = cur[3 * q + 2];
-            if (fl & kFixed) {  // src/sim.cu:1100-1101
-                nxt[3 * q] = o0;
-                nxt[3 * q + 1] = o1;
-                nxt[3 * q + 2] = o2;
-                continue;
-            }
-            MassRegs r;
-            r.px = o0.x;
-            r.py = o0.y;
-            r.pz = o1.x;
-            r.vx = o1.y;
-            r.vy = o2.x;
-            r.vz = o2.y;
-            SpringWalk<ACT, kGroupTiled> w;
-            w.prime(a, a.chunk_ptr[chunk0 + c], a.chunk_ng[chunk0 + c], lane, bkd);
-            w.run(a, r, a.deg[i], a.ract[k & 1], [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
-                q0 = cur[3 * o];
-                q1 = cur[3 * o + 1];
-                q2 = cur[3 * o + 2];
-            });
-            double ax, ay, az;
-            const int stage = (int)((nst + k) & 1);
-            const IntegIn in = load_integ(a, i, stage);
-            mass_pass(a, r, i, fl, b, a.m[i], stage, in, ax, ay, az);
-            nxt[3 * q] = make_double2(r.px, r.py);
-            nxt[3 * q + 1] = make_double2(r.pz, r.vx);
-            nxt[3 * q + 2] = make_double2(r.vy, r.vz);
-            if (k == K - 1 || a.integ == kIntVerlet) {
-                a.acc[i] = ax;
-                a.acc[n6 + i] = ay;
-                a.acc[2 * n6 + i] = az;
-            }
+    const bool cached = cache > 0 && cache <= a.res_cache;  // uniform over the block
+    if (cached) {
+        // names (4 B per entry), then from a 16-byte boundary the rest
+        // lengths and, where the body has them, k and damping (8 B each)
+        char *base = reinterpret_cast<char *>(S + 6 * a.res_rows);
+        uint32_t *cn = reinterpret_cast<uint32_t *>(base);
+        double *cr = reinterpret_cast<double *>(base + ((4 * (int64_t)ninc + 15) & ~(int64_t)15));
+        double *ck = cr + ninc;
+        double *cd = ck + ((bkd.kd & 1u) ? ninc : 0);
+        for (int e = t; e < ninc; e += kResidentBlock) {
+            cn[e] = a.inc_nbr[inc0 + e];
+            cr[e] = a.inc_rest[inc0 + e];
+            if (bkd.kd & 1u) ck[e] = a.inc_k[inc0 + e];
+            if (bkd.kd & 2u) cd[e] = a.inc_damp[inc0 + e];
         }
-        T = T + a.hdt;  // src/sim.cu:1590 (RK2: :1577,1582)
-        if (ACT && k + 1 < K) actuate_slice(a, act0, nact, T, a.ract[(k + 1) & 1], t, kResidentBlock);
         __syncthreads();
+        const PtrSrc src{cn, cr, ck, cd};
+        resident_steps<ACT>(a, src, inc0, b, bkd, chunk0, nch, g0, K, nst, T, nact, act0, B0, B1);
+    } else {
+        __syncthreads();
+        const ArenaSrc src{a};
+        resident_steps<ACT>(a, src, 0, b, bkd, chunk0, nch, g0, K, nst, T, nact, act0, B0, B1);
     }
     // the final state to the state buffer of the body's new step parity
     {
@@ -956,7 +1049,7 @@ size_t resident_lds_bytes(int rows) { return (size_t)2 * 48 * rows; }
 
 hipError_t launch_rollout_resident(const StepArgs &a, bool act, int final_slot, hipStream_t s) {
     if (a.nres <= a.res_base) return hipSuccess;
-    const size_t lds = resident_lds_bytes(a.res_rows);
+    const size_t lds = resident_lds_bytes(a.res_rows) + (size_t)a.res_cache;
     const dim3 g((unsigned)(a.nres - a.res_base)), blk(kResidentBlock);
     if (act) hipLaunchKernelGGL((k_rollout_resident<true>), g, blk, lds, s, a, final_slot);
     else hipLaunchKernelGGL((k_rollout_resident<false>), g, blk, lds, s, a, final_slot);
@@ -967,9 +1060,15 @@ const char *resident_kernel_name(bool act) {
     return act ? "k_rollout_resident<actuated>" : "k_rollout_resident<passive>";
 }
 
-hipError_t set_tiles_lds_limit() {
-    const int rb = (int)resident_lds_bytes(kResidentMax);
-    hipError_t er = hipFuncSetAttribute((const void *)k_rollout_resident<false>,
+hipError_t set_tiles_lds_limit(int *resident_max) {
+    int dev = 0, lim = 0;
+    hipError_t er = hipGetDevice(&dev);
+    if (er != hipSuccess) return er;
+    er = hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (er != hipSuccess) return er;
+    const int rb = std::min(lim, (int)resident_lds_bytes(kResidentMax) + kResidentCacheMax);
+    *resident_max = rb;
+    er = hipFuncSetAttribute((const void *)k_rollout_resident<false>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, rb);
     if (er != hipSuccess) return er;
     er = hipFuncSetAttribute((const void *)k_rollout_resident<true>, hipFuncAttributeMaxDynamicSharedMemorySize, rb);

@@ -73,7 +73,14 @@ struct ResidentDesc {
     int32_t nmass;
     int32_t nact;     // the body's actuators: [act0, act0 + nact)
     int64_t act0;
+    int64_t inc0;     // the body's incidence entries: [inc0, inc0 + ninc)
+    int32_t ninc;
+    int32_t cache;    // LDS bytes of a copy of its incidence data; 0: read from the arena
 };
+
+// A resident body whose incidence data fits this many bytes keeps a copy in
+// LDS for all the steps of a launch.
+constexpr int kResidentCacheMax = 64 * 1024;
 
 // The body table and the arena fields a body's activation (joining the
 // launches) and deactivation write, on the step stream.
@@ -198,6 +205,7 @@ struct StepArgs {
     const ResidentDesc *rdesc;
     int64_t res_base, nres;   // launch range: descriptors [res_base, nres)
     int32_t res_rows;         // LDS rows per buffer: the largest resident body
+    int32_t res_cache;        // LDS bytes after the two state buffers for an incidence copy
 };
 
 }  // namespace titan_mi355x
