@@ -384,3 +384,27 @@ def test_per_body_constants_in_one_context_bitexact(layout):
     eng, _ = run_engine(sc, 300, layout=layout)
     orc = run_oracle(sc, 300)
     assert_bitexact(eng, orc)
+
+
+@LAYOUTS
+def test_extern_force_uploaded_after_graph_capture(layout):
+    # the body starts without extern forces; after graph-launched steps an
+    # upload gives some masses one (the engine then creates its extern-force
+    # array, so launches captured before must not be replayed)
+    sc = scenes.readme_lattice(2.45)
+    n = len(sc.bodies[0]["pos"])
+    rng = np.random.default_rng(12)
+    ext = rng.normal(0, 0.5, (n, 3)) * (np.arange(n) % 4 == 0)[:, None]
+    _say(f"case extern force uploaded after 128 graph-launched steps, then 128 more, layout {layout}")
+    with Context() as ctx:
+        (bid,) = sc.load(ctx, layout=layout)
+        ctx.step(128)
+        ctx.upload(bid, ext=ext)
+        ctx.step(128)
+        got = ctx.download(bid)
+    o = to_oracle(sc)
+    o.run(128)
+    o.e = [np.ascontiguousarray(ext[:, c]) for c in range(3)]
+    o.run(128)
+    np.testing.assert_array_equal(got["pos"], o.pos)
+    np.testing.assert_array_equal(got["vel"], o.vel)

@@ -193,6 +193,9 @@ struct titan_ctx {
     hipEvent_t left_ev[2] = {nullptr, nullptr};
     int64_t left_cap = 0;
     int left_k = 0;
+    // pinned staging for body downloads and uploads (grown on demand)
+    double *stage = nullptr;
+    int64_t stage_cap = 0;
     int32_t inject = -1;
 
     std::map<void *, size_t> allocs;
@@ -1373,7 +1376,7 @@ void body_destroy(titan_ctx *c, int32_t id) {
 }
 
 // device order -> caller's order for a (N, K) host array read from K SoA rows
-void unpermute(const BodyH &b, const std::vector<double> &soa, int K, double *out) {
+void unpermute(const BodyH &b, const double *soa, int K, double *out) {
     const int64_t N = b.n_masses;
     for (int64_t q = 0; q < N; ++q) {
         const int64_t i = b.perm.empty() ? q : b.perm[q];
@@ -1381,17 +1384,29 @@ void unpermute(const BodyH &b, const std::vector<double> &soa, int K, double *ou
     }
 }
 
+// A pinned host buffer of at least n doubles: copies between it and the
+// device run at the link's rate, without the driver's pageable staging.
+double *staging(titan_ctx *c, int64_t n) {
+    if (n > c->stage_cap) {
+        if (c->stage) (void)hipHostFree(c->stage);
+        c->stage = nullptr;
+        c->stage_cap = 0;
+        const int64_t cap = std::max<int64_t>(n, 2 * c->stage_cap);
+        HIPCHK(hipHostMalloc((void **)&c->stage, sizeof(double) * cap, hipHostMallocDefault));
+        c->stage_cap = cap;
+    }
+    return c->stage;
+}
+
 void body_download(titan_ctx *c, int32_t id, double *pos, double *vel, double *acc, double *T) {
     BodyH &b = body_of(c, id);
     const int64_t N = b.n_masses, g0 = b.chunk0 * kChunk, n6 = c->ncap;
     const int p = (int)((b.n * launches_per_step(c)) & 1);
-    auto dev = [&](int64_t q) -> int64_t { return q; };
     auto host = [&](int64_t q) -> int64_t { return b.perm.empty() ? q : b.perm[q]; };
-    (void)dev;
     HIPCHK(hipStreamSynchronize(c->stream));
     if (pos || vel) {
-        std::vector<double> st(6 * N);
-        d2h(c, st.data(), c->st + ((int64_t)p * n6 + g0) * 6, 6 * N);
+        double *st = staging(c, 6 * N);
+        d2h(c, st, c->st + ((int64_t)p * n6 + g0) * 6, 6 * N);
         HIPCHK(hipStreamSynchronize(c->stream));
         for (int64_t q = 0; q < N; ++q) {
             const int64_t i = host(q);
@@ -1401,9 +1416,9 @@ void body_download(titan_ctx *c, int32_t id, double *pos, double *vel, double *a
             }
         }
     }
-    std::vector<double> tmp(3 * N);
     auto fetch = [&](const double *base, int K, double *out) {
-        for (int k = 0; k < K; ++k) d2h(c, tmp.data() + k * N, base + k * n6 + g0, N);
+        double *tmp = staging(c, 3 * N);
+        for (int k = 0; k < K; ++k) d2h(c, tmp + k * N, base + k * n6 + g0, N);
         HIPCHK(hipStreamSynchronize(c->stream));
         unpermute(b, tmp, K, out);
     };
@@ -1419,9 +1434,9 @@ void body_upload(titan_ctx *c, int32_t id, const double *pos, const double *vel,
     HIPCHK(hipStreamSynchronize(c->stream));
     auto host = [&](int64_t q) -> int64_t { return b.perm.empty() ? q : b.perm[q]; };
     if (pos || vel) {  // read-modify-write of the 48-byte records
-        std::vector<double> st(6 * N);
+        double *st = staging(c, 6 * N);
         double *base = c->st + ((int64_t)p * n6 + g0) * 6;
-        d2h(c, st.data(), base, 6 * N);
+        d2h(c, st, base, 6 * N);
         HIPCHK(hipStreamSynchronize(c->stream));
         for (int64_t q = 0; q < N; ++q) {
             const int64_t i = host(q);
@@ -1430,16 +1445,16 @@ void body_upload(titan_ctx *c, int32_t id, const double *pos, const double *vel,
                 if (vel) st[6 * q + 3 + k] = vel[3 * i + k];
             }
         }
-        h2d(c, base, st.data(), 6 * N);
+        h2d(c, base, st, 6 * N);
         HIPCHK(hipStreamSynchronize(c->stream));
     }
-    std::vector<double> tmp(3 * N);
     auto put = [&](double *base, int K, const double *in) {
+        double *tmp = staging(c, 3 * N);
         for (int64_t q = 0; q < N; ++q) {
             const int64_t i = host(q);
             for (int k = 0; k < K; ++k) tmp[k * N + q] = in[K * i + k];
         }
-        for (int k = 0; k < K; ++k) h2d(c, base + k * n6 + g0, tmp.data() + k * N, N);
+        for (int k = 0; k < K; ++k) h2d(c, base + k * n6 + g0, tmp + k * N, N);
         HIPCHK(hipStreamSynchronize(c->stream));
     };
     if (m) put(c->m, 1, m);
@@ -1455,9 +1470,10 @@ void body_upload(titan_ctx *c, int32_t id, const double *pos, const double *vel,
                 changed = true;
             }
         }
-        if (!c->ext) {
+        if (!c->ext) {  // launches captured so far have no extern-force array: drop them
             c->ext = dalloc<double>(c, 3 * c->ncap);
             HIPCHK(hipMemsetAsync(c->ext, 0, sizeof(double) * 3 * c->ncap, c->stream));
+            drop_graphs(c);
         }
         put(c->ext, 3, ext);
         if (changed) {
@@ -1561,6 +1577,7 @@ int titan_ctx_destroy(titan_ctx *c) {
         if (c->left_ev[k]) (void)hipEventDestroy(c->left_ev[k]);
         if (c->left_pin[k]) (void)hipHostFree(c->left_pin[k]);
     }
+    if (c->stage) (void)hipHostFree(c->stage);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
