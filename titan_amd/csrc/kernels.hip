@@ -492,11 +492,13 @@ __device__ __forceinline__ void eval_set(MassRegs &r, const Group<ACT, G> (&set)
 // flight per wave without copies (a refill index past the last group is
 // clamped to it and re-reads a group already loaded).  prime() issues the
 // first set's loads, so that a kernel can start them before it waits for
-// anything else.
+// anything else.  K = 1: a set of G incidences.  (Sets of four incidences
+// with G = 2 took the tiled step to 164 VGPRs and 3 waves per SIMD; one
+// group per set keeps it at 110 and 4 waves, measured 5-10% faster.)
 template <bool ACT, int G, class Src>
 struct SpringWalk {
     static_assert(G == 2 || G == 4, "groups of 2 or 4");
-    static constexpr int K = 4 / G;  // groups per register set: four incidences
+    static constexpr int K = 1;  // groups per register set
     Group<ACT, G> A[K], B[K];
     const Src &src;
     int64_t ebase;
