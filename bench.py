@@ -62,7 +62,7 @@ def cpu_threads() -> int:
         return os.cpu_count() or 1
 
 
-def cpu_baseline(scene, budget_s: float = 12.0) -> dict:
+def cpu_baseline(scene, budget_s: float = 15.0) -> dict:
     """The oracle on the host's cores over a bounded number of steps of the
     same workload (state copied fresh from the scene)."""
     from oracle import oracle as orc  # checker / CPU baseline only
