@@ -1928,11 +1928,15 @@ int titan_kernel_name(const titan_ctx *c, char *buf, int64_t len) {
     NEED_CTX(c);
     if (!buf || len <= 0) return TITAN_E_INVALID;
     const bool act = c->has_act;
-    std::string n;
-    if (any_body(c, kModeGather)) n = step_kernel_name(act);
-    if (any_body(c, kModeTiled)) n += std::string(n.empty() ? "" : "+") + step_tiles_kernel_name(act);
-    if (any_body(c, kModeResident)) n += std::string(n.empty() ? "" : "+") + resident_kernel_name(act);
-    std::snprintf(buf, (size_t)len, "%s", n.c_str());
+    const char *parts[3] = {any_body(c, kModeGather) ? step_kernel_name(act) : nullptr,
+                            any_body(c, kModeTiled) ? step_tiles_kernel_name(act) : nullptr,
+                            any_body(c, kModeResident) ? resident_kernel_name(act) : nullptr};
+    int64_t at = 0;
+    buf[0] = 0;
+    for (const char *p : parts) {
+        if (!p || at >= len) continue;
+        at += std::snprintf(buf + at, (size_t)(len - at), "%s%s", at ? "+" : "", p);
+    }
     return TITAN_OK;
 }
 

@@ -139,7 +139,13 @@ void prepare_all(const titan_ctx *ctx, const std::vector<Request *> &rs, unsigne
     };
     const unsigned nt = std::min<unsigned>(threads, (unsigned)rs.size());
     std::vector<std::thread> pool;
-    for (unsigned t = 1; t < nt; ++t) pool.emplace_back(work);
+    for (unsigned t = 1; t < nt; ++t) {
+        try {
+            pool.emplace_back(work);
+        } catch (const std::exception &) {  // no more threads: the rest runs here
+            break;
+        }
+    }
     work();
     for (auto &t : pool) t.join();
 }
@@ -211,7 +217,7 @@ extern "C" __attribute__((visibility("hidden"))) void titan_internal_serve_forge
     g_servers.erase(ctx);
 }
 
-int titan_serve_config(titan_ctx *ctx, int32_t max_running, uint64_t hbm_bytes, int64_t quantum) {
+static int serve_config(titan_ctx *ctx, int32_t max_running, uint64_t hbm_bytes, int64_t quantum) {
     if (!ctx || max_running < 1 || quantum < 1) return TITAN_E_INVALID;
     Server &sv = server_of(ctx);
     sv.max_running = max_running;
@@ -220,7 +226,7 @@ int titan_serve_config(titan_ctx *ctx, int32_t max_running, uint64_t hbm_bytes, 
     return TITAN_OK;
 }
 
-int titan_serve_submit(titan_ctx *ctx, const titan_request *req, int64_t *ticket) {
+static int serve_submit(titan_ctx *ctx, const titan_request *req, int64_t *ticket) {
     if (!ctx || !req || req->horizon < 0 || req->body.n_masses <= 0 || !req->body.pos || !req->body.m)
         return TITAN_E_INVALID;
     Server &sv = server_of(ctx);
@@ -244,7 +250,7 @@ int titan_serve_submit(titan_ctx *ctx, const titan_request *req, int64_t *ticket
     return TITAN_OK;
 }
 
-int titan_serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
+static int serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
     if (!ctx) return TITAN_E_INVALID;
     Server &sv = server_of(ctx);
     std::vector<Request *> fresh;  // admitted during this pump's quantum: they join the next one
@@ -331,7 +337,7 @@ int titan_serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
     return TITAN_OK;
 }
 
-int titan_serve_poll(titan_ctx *ctx, int64_t ticket, int32_t *state, int64_t *steps_done) {
+static int serve_poll(titan_ctx *ctx, int64_t ticket, int32_t *state, int64_t *steps_done) {
     if (!ctx) return TITAN_E_INVALID;
     Server &sv = server_of(ctx);
     auto it = sv.reqs.find(ticket);
@@ -341,7 +347,7 @@ int titan_serve_poll(titan_ctx *ctx, int64_t ticket, int32_t *state, int64_t *st
     return TITAN_OK;
 }
 
-int titan_serve_cancel(titan_ctx *ctx, int64_t ticket) {
+static int serve_cancel(titan_ctx *ctx, int64_t ticket) {
     if (!ctx) return TITAN_E_INVALID;
     Server &sv = server_of(ctx);
     auto it = sv.reqs.find(ticket);
@@ -362,7 +368,7 @@ int titan_serve_cancel(titan_ctx *ctx, int64_t ticket) {
     return TITAN_OK;
 }
 
-int titan_serve_stats(titan_ctx *ctx, titan_serve_report *out) {
+static int serve_stats(titan_ctx *ctx, titan_serve_report *out) {
     if (!ctx || !out) return TITAN_E_INVALID;
     Server &sv = server_of(ctx);
     std::memset(out, 0, sizeof(*out));
@@ -390,7 +396,7 @@ int titan_serve_stats(titan_ctx *ctx, titan_serve_report *out) {
     return TITAN_OK;
 }
 
-int titan_serve_reset_stats(titan_ctx *ctx) {
+static int serve_reset_stats(titan_ctx *ctx) {
     if (!ctx) return TITAN_E_INVALID;
     Server &sv = server_of(ctx);
     for (auto it = sv.reqs.begin(); it != sv.reqs.end();) {
@@ -405,4 +411,47 @@ int titan_serve_reset_stats(titan_ctx *ctx) {
     sv.first_submit = sv.reqs.empty() ? -1 : now_s();
     sv.last_done = -1;
     return TITAN_OK;
+}
+
+// The C ABI: no C++ exception leaves the library (a host allocation that
+// fails is TITAN_E_NOMEM).
+namespace {
+template <class F>
+int guarded(F f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        return TITAN_E_NOMEM;
+    } catch (...) {
+        return TITAN_E_INVALID;
+    }
+}
+}  // namespace
+
+int titan_serve_config(titan_ctx *ctx, int32_t max_running, uint64_t hbm_bytes, int64_t quantum) {
+    return guarded([&] { return serve_config(ctx, max_running, hbm_bytes, quantum); });
+}
+
+int titan_serve_submit(titan_ctx *ctx, const titan_request *req, int64_t *ticket) {
+    return guarded([&] { return serve_submit(ctx, req, ticket); });
+}
+
+int titan_serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
+    return guarded([&] { return serve_pump(ctx, n_running, n_queued); });
+}
+
+int titan_serve_poll(titan_ctx *ctx, int64_t ticket, int32_t *state, int64_t *steps_done) {
+    return guarded([&] { return serve_poll(ctx, ticket, state, steps_done); });
+}
+
+int titan_serve_cancel(titan_ctx *ctx, int64_t ticket) {
+    return guarded([&] { return serve_cancel(ctx, ticket); });
+}
+
+int titan_serve_stats(titan_ctx *ctx, titan_serve_report *out) {
+    return guarded([&] { return serve_stats(ctx, out); });
+}
+
+int titan_serve_reset_stats(titan_ctx *ctx) {
+    return guarded([&] { return serve_reset_stats(ctx); });
 }
