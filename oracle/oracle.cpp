@@ -2,9 +2,9 @@
 //
 // A host C++17 + OpenMP restatement of the reference's step arithmetic,
 // operation by operation, with forces summed per mass in ascending spring
-// index (the order a serial run of the reference's scatter would use).  It
-// is the checker for the HIP path and the CPU baseline timed beside it; the
-// product never links it.  Build: oracle/Makefile (g++ -O2 -fopenmp
+// index (the order a serial run of the reference's scatter would use).  The
+// tests compare the HIP path's results with it and the benchmark times it as
+// the CPU baseline; the product never links it.  Build: oracle/Makefile (g++ -O2 -fopenmp
 // -ffp-contract=off; x86-64 has no FMA at the default -march, so no
 // contraction can occur either way).
 //

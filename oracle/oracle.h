@@ -1,9 +1,9 @@
 /*
  * CPU oracle for the FP64 mass-spring step: TEST INFRASTRUCTURE ONLY.
  *
- * This is a host C++17 restatement of the reference's step arithmetic, used
- * as the checker for the HIP path and as the CPU baseline that the benchmark
- * times beside it.  Nothing in the product (libtitan_mi355x.so, the C++
+ * This is a host C++17 restatement of the reference's step arithmetic: the
+ * tests compare the HIP path's results with it, and the benchmark times it
+ * beside the HIP path as the CPU baseline.  Nothing in the product (libtitan_mi355x.so, the C++
  * facade) links or calls it; only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py load it.
  *

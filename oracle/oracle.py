@@ -1,8 +1,8 @@
 """ctypes view of the CPU oracle (oracle/liboracle.so) — TEST INFRASTRUCTURE ONLY.
 
 Imported only by tests/, __graft_entry__.smoke() and the cpu_baseline leg of
-bench.py, always as the checker or the timed CPU baseline, never as the path
-under test.  The scene is described with the same arrays the engine's
+bench.py, always as what the HIP path's results are compared with or as the
+timed CPU baseline, never as the path under test.  The scene is described with the same arrays the engine's
 titan_body_desc takes (positions and velocities as (N, 3), springs as
 parallel arrays), so one scene can be handed to both.
 """

@@ -1385,7 +1385,7 @@ void unpermute(const BodyH &b, const double *soa, int K, double *out) {
 }
 
 // A pinned host buffer of at least n doubles: copies between it and the
-// device run at the link's rate, without the driver's pageable staging.
+// device run at the link's rate, without the runtime's pageable staging.
 double *staging(titan_ctx *c, int64_t n) {
     if (n > c->stage_cap) {
         if (c->stage) (void)hipHostFree(c->stage);
