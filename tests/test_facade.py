@@ -26,11 +26,11 @@ def test_facade_tests_compile_against_the_headers(tmp_path):
     assert out.exists()
 
 
-def _run(arg: str, timeout: int = 240) -> subprocess.CompletedProcess:
-    if not BIN.exists():
-        raise RuntimeError(f"{BIN} is not built; run __graft_entry__.build()")
-    print(f"facade_tests {arg}", flush=True)
-    return subprocess.run([str(BIN), arg], capture_output=True, text=True, timeout=timeout)
+def _run(arg: str, timeout: int = 240, binary: Path = BIN) -> subprocess.CompletedProcess:
+    if not binary.exists():
+        raise RuntimeError(f"{binary} is not built; run __graft_entry__.build()")
+    print(f"{binary.name} {arg}", flush=True)
+    return subprocess.run([str(binary), arg], capture_output=True, text=True, timeout=timeout)
 
 
 @pytest.mark.gpu
@@ -44,8 +44,23 @@ def test_reference_physics_tests_through_the_facade():
 
 
 @pytest.mark.gpu
-def test_facade_readme_scene_matches_oracle_bitexact():
-    res = _run("readme_parity")
+@pytest.mark.parametrize("name", ["simple", "energy", "external", "external_acc", "multiagent", "control"])
+def test_reference_physics_tests_built_with_rk2(name):
+    # the rotate test is left out: with k = 1e5 and dt = 1e-4 its lattice's
+    # fastest modes have omega dt near 0.7, where the midpoint rule's
+    # amplification 1 + (omega dt)^4 / 8 per step grows them without bound
+    # over its 50,000 steps (the Euler branch is stable there)
+    res = _run(name, binary=build.FACADE_TESTS_RK2_BIN)
+    assert res.returncode == 0, res.stdout[-4000:] + res.stderr[-2000:]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("binary", [BIN, build.FACADE_TESTS_RK2_BIN], ids=["euler", "rk2"])
+def test_facade_readme_scene_matches_oracle_bitexact(binary):
+    # the RK2 program is the same source built with -DRK2, the reference's
+    # build option: Simulation() then steps with RK2
+    rk2 = binary == build.FACADE_TESTS_RK2_BIN
+    res = _run("readme_parity", binary=binary)
     assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
     T = None
     rows = []
@@ -56,11 +71,16 @@ def test_facade_readme_scene_matches_oracle_bitexact():
             rows.append([float.fromhex(x) for x in line.split()[1:]])
     got = np.array(rows)
     sc = scenes.readme_lattice(2.45)
-    o = to_oracle(sc)
+    o = to_oracle(sc, integrator=2 if rk2 else 0)
     # pause(0.0105): the first step boundary with T >= 0.0105 under T += 1e-4
+    # (RK2: T += 0.5 * 1e-4 twice per step)
     n, t = 0, 0.0
     while not (0.0105 <= t):
-        t += 1e-4
+        if rk2:
+            t += 0.5 * 1e-4
+            t += 0.5 * 1e-4
+        else:
+            t += 1e-4
         n += 1
     o.run(n)
     assert T == o.t

@@ -82,14 +82,17 @@ def build_facade(verbose: bool = False) -> Path:
 
 FACADE_TESTS_SRC = ROOT / "tests" / "cpp" / "facade_tests.cpp"
 FACADE_TESTS_BIN = ROOT / "tests" / "cpp" / "facade_tests"
+# the same tests built as a program that passes the reference's RK2 build option
+FACADE_TESTS_RK2_BIN = ROOT / "tests" / "cpp" / "facade_tests_rk2"
 
 
-def build_facade_tests(verbose: bool = False, out: Path | None = None) -> Path:
+def build_facade_tests(verbose: bool = False, out: Path | None = None, defines=()) -> Path:
     """The reference's physics tests restated against the C++ facade."""
     out = out or FACADE_TESTS_BIN
     tmp = out.with_name(out.name + ".tmp")
-    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", f"-I{CPP}", f"-I{INCLUDE}", FACADE_TESTS_SRC,
-          "-o", tmp, f"-L{PKG}", "-ltitan", "-ltitan_mi355x", f"-Wl,-rpath,{PKG}"], verbose)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", *[f"-D{d}" for d in defines], f"-I{CPP}",
+          f"-I{INCLUDE}", FACADE_TESTS_SRC, "-o", tmp, f"-L{PKG}", "-ltitan", "-ltitan_mi355x",
+          f"-Wl,-rpath,{PKG}"], verbose)
     os.replace(tmp, out)
     return out
 
@@ -112,6 +115,7 @@ def build_all(verbose: bool = False) -> None:
     build_facade(verbose)
     if FACADE_TESTS_SRC.exists():
         build_facade_tests(verbose)
+        build_facade_tests(verbose, out=FACADE_TESTS_RK2_BIN, defines=("RK2",))
     if FASTMATH_CHECK_SRC.exists():
         build_fastmath_check(verbose)
     shutil.rmtree(PKG / "_build", ignore_errors=True)

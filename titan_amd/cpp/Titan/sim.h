@@ -141,6 +141,7 @@ class Simulation {
     titan_ctx *ctx = nullptr;
     int32_t body = -1;
     int integrator_ = 0;
+    void advance_clock(double &t) const;
     bool dirty = false;  // the engine body must be rebuilt before the next step
 };
 

@@ -394,9 +394,20 @@ void Simulation::advance(int64_t n) {
     if (dirty) rebuild();
     if (body >= 0) check(ctx, titan_step(ctx, n));
     else {  // no masses: only the clock moves
-        for (int64_t i = 0; i < n; ++i) T += dt;
+        for (int64_t i = 0; i < n; ++i) advance_clock(T);
     }
     if (body >= 0) check(ctx, titan_time(ctx, &T));
+}
+
+// One step of the driver's clock: T += dt (src/sim.cu:1590), or for RK2 two
+// stages of T += 0.5 * dt (:1577,1582), as the engine counts it.
+void Simulation::advance_clock(double &t) const {
+    if (integrator_ == 2) {
+        t += 0.5 * dt;
+        t += 0.5 * dt;
+    } else {
+        t += dt;
+    }
 }
 
 // Steps until the driver loop of src/sim.cu:1503-1590 would hand control
@@ -412,7 +423,7 @@ int64_t Simulation::steps_until_after(double t, bool strict, bool &hit_breakpoin
             return k;
         }
         if (strict ? (TT > t) : (TT >= t)) return k;
-        TT += dt;
+        advance_clock(TT);
         if (++k > ((int64_t)1 << 40)) throw std::runtime_error("requested time is unreachable");
     }
 }
