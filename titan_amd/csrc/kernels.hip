@@ -439,11 +439,11 @@ __device__ __forceinline__ void add_spring(MassRegs &r, const Grp &cur, int q, c
     r.fz = r.fz - (uz * mag + uz * dd);
 }
 
-// Evaluate the K groups of one register set (four incidences, the first
-// being incidence 4 g0 of the lane; the lane has d).  When every lane of the
-// wave has all four and they are in the fast range, the four evaluations
-// are straight-line code that interleaves; otherwise each incidence that
-// exists takes the general path.
+// Evaluate the K groups of one register set (N = G K incidences, the first
+// being incidence G g0 of the lane; the lane has d).  When every lane of the
+// wave has all N and they are in the fast range, the N evaluations are
+// straight-line code that interleaves; otherwise each incidence that exists
+// takes the general path.
 template <bool ACT, int G, int K, class Fetch>
 __device__ __forceinline__ void eval_set(MassRegs &r, const Group<ACT, G> (&set)[K], int g0, int d,
                                          Fetch &fetch) {
