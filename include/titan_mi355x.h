@@ -189,9 +189,11 @@ int titan_get_integrator(const titan_ctx *ctx, int32_t *kind);
  * layout; of the others, those with at least tiled_min_masses masses get the
  * tiled layout and smaller ones the gather layout; tile_rows bounds a tile's
  * LDS rows (masses + halo, 64 .. 2304; fewer rows allow more workgroups per
- * CU, more rows bigger tiles).  Default: resident_max_masses 1024,
- * tiled_min_masses 0, tile_rows 2304.  Applies to bodies created
- * afterwards. */
+ * CU, more rows bigger tiles).  Default: resident_max_masses 0,
+ * tiled_min_masses 0, tile_rows 2304 (every body tiled: a resident launch
+ * sizes every workgroup's LDS for its largest body, which favours one small
+ * scene over a population; titan_autotune measures both).  Applies to
+ * bodies created afterwards. */
 typedef struct titan_schedule {
     int64_t tiled_min_masses;
     int32_t tile_rows;

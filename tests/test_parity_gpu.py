@@ -283,10 +283,10 @@ def test_auto_layout_by_size():
     big = scenes.lattice_body((0.0, 0.0, 0.55), (1.2, 1.2, 1.2), 11, damping=0.1)  # 1331 masses
     with Context() as ctx:
         (a,) = small.load(ctx)
-        b = ctx.add_body(**big)
-        assert ctx.body_layout(a) == LAYOUT_RESIDENT and ctx.body_layout(b) == LAYOUT_TILED
-        ctx.set_schedule(0, 2304, 0)  # no resident bodies
-        assert ctx.body_layout(ctx.add_body(**small.bodies[0])) == LAYOUT_TILED
+        assert ctx.body_layout(a) == LAYOUT_TILED  # the default schedule tiles every body
+        ctx.set_schedule(0, 2304, 1024)  # bodies of at most 1024 masses resident
+        assert ctx.body_layout(ctx.add_body(**small.bodies[0])) == LAYOUT_RESIDENT
+        assert ctx.body_layout(ctx.add_body(**big)) == LAYOUT_TILED
         with pytest.raises(RuntimeError):
             ctx.add_body(**big, layout=LAYOUT_RESIDENT)  # more than 1024 masses
 

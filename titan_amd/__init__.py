@@ -298,7 +298,7 @@ class Context:
 
     # -- schedule --
     def set_schedule(self, tiled_min_masses: int = 0, tile_rows: int = 2304,
-                     resident_max_masses: int = RESIDENT_MAX):
+                     resident_max_masses: int = 0):
         s = Schedule(int(tiled_min_masses), int(tile_rows), int(resident_max_masses))
         self._chk(self._L.titan_set_schedule(self._h, ctypes.byref(s)))
 

@@ -132,7 +132,7 @@ struct titan_ctx {
 
     Ranges chunks, incs, acts, lcs, bids, tiles, halos, rdescs;
     std::vector<BodyH> bodies;
-    titan_schedule sched{0, kTileRows, kResidentMax};
+    titan_schedule sched{0, kTileRows, 0};  // every body tiled unless a schedule says otherwise
 
     // mass arena (ncap = chunks.cap() * 64)
     int64_t ncap = 0;
@@ -1819,7 +1819,7 @@ int titan_autotune(titan_ctx *c, const titan_body_desc *samples, int32_t n_sampl
     const int32_t res[] = {0, kResidentMax};
     const int64_t mins[] = {0, 128, 256, 512, 1024, INT64_MAX};
     const int32_t rows[] = {682, kMaxTileRows};
-    titan_schedule win{0, kTileRows, kResidentMax};
+    titan_schedule win{0, kTileRows, 0};
     double win_rate = -1.0;
     // candidates that lay out every sample alike are timed once
     std::map<std::string, double> seen;
