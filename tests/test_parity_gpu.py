@@ -338,6 +338,79 @@ def test_resident_many_balls_and_actuation():
     assert normwise_rel(eng["vel"], orc.vel) <= TOL_ACT
 
 
+def _small_bodies(rng, act=False):
+    # resident bodies of one to four chunks (27, 64, 125 and 216 masses and a
+    # 200-mass kNN cloud of irregular degree), k and damping shared or per
+    # spring in every combination, jittered and moving, some masses fixed
+    bodies = []
+    for q, (side, pk, pd) in enumerate([(3, False, False), (4, True, False), (5, False, True), (6, True, True)]):
+        b = scenes.lattice_body((3.0 * q, 0.0, 0.9), (1.5, 1.5, 1.5), side, damping=0.1)
+        s, n = len(b["left"]), len(b["pos"])
+        b["pos"] = b["pos"] + rng.normal(0, 1e-3, b["pos"].shape)
+        b["vel"] = rng.normal(0, 0.3, b["pos"].shape)
+        b["k"] = rng.uniform(5e3, 2e4, s) if pk else np.full(s, 1e4 + 500.0 * q)
+        b["damping"] = rng.uniform(0.0, 0.3, s) if pd else np.full(s, 0.05 * q)
+        b["fixed"] = (np.arange(n) % 23 == 5).astype(np.uint8)
+        bodies.append(scenes.actuate(b, 0.3, rng) if act else b)
+    c = scenes.knn_cloud(200, box=1.0, seed=5, z0=0.05)
+    bodies.append(scenes.actuate(c, 0.3, rng) if act else c)
+    return scenes.Scene(bodies=bodies, planes=[(0.0, 0.0, 1.0, 0.0, 0.8, 1.0)], dt=1e-4)
+
+
+def test_resident_small_bodies_bitexact():
+    # few bodies of at most four chunks: the resident launch computes each
+    # spring's term once over the whole workgroup, then sums per mass
+    sc = _small_bodies(np.random.default_rng(21))
+    _say("case resident small bodies (27..216 masses, kNN 200), per-spring k/damping combos, 300 steps")
+    with Context() as ctx:
+        ids = sc.load(ctx, layout=LAYOUT_RESIDENT)
+        for n in (1, 64, 235):
+            ctx.step(n)
+        got = [ctx.download(i, acc=True, T=True) for i in ids]
+    for q, g in enumerate(got):
+        o = to_oracle(scenes.Scene(bodies=[sc.bodies[q]], planes=sc.planes, dt=sc.dt))
+        o.run(300)
+        np.testing.assert_array_equal(g["pos"], o.pos)
+        np.testing.assert_array_equal(g["vel"], o.vel)
+        np.testing.assert_array_equal(g["acc"], o.acc)
+        np.testing.assert_array_equal(g["T"], o.T)
+
+
+def test_resident_small_bodies_actuated():
+    sc = _small_bodies(np.random.default_rng(22), act=True)
+    _say("case resident small bodies, 30% actuated, 150 steps")
+    eng, _ = run_engine(sc, 150, layout=LAYOUT_RESIDENT)
+    ref, _ = run_engine(sc, 150, layout=LAYOUT_GATHER)
+    np.testing.assert_array_equal(eng["pos"], ref["pos"])  # same device sin on both paths
+    np.testing.assert_array_equal(eng["vel"], ref["vel"])
+    orc = run_oracle(sc, 150)
+    assert normwise_rel(eng["pos"], orc.pos) <= TOL_ACT
+    assert normwise_rel(eng["vel"], orc.vel) <= TOL_ACT
+
+
+def test_resident_more_bodies_than_compute_units():
+    # 520 small bodies, more than the device's compute units (256 on MI355X):
+    # the launch then walks each body's incidences per mass instead
+    rng = np.random.default_rng(23)
+    bodies = []
+    for q in range(520):
+        b = scenes.lattice_body((2.0 * (q % 26), 2.0 * (q // 26), 0.6), (1.0, 1.0, 1.0), 3 + q % 3, damping=0.1)
+        b["pos"] = b["pos"] + rng.normal(0, 1e-3, b["pos"].shape)
+        b["vel"] = rng.normal(0, 0.3, b["pos"].shape)
+        bodies.append(b)
+    sc = scenes.Scene(bodies=bodies, planes=[(0.0, 0.0, 1.0, 0.0, 0.8, 1.0)], dt=1e-4)
+    _say("case 520 resident bodies (27..125 masses), 40 steps")
+    with Context() as ctx:
+        ids = sc.load(ctx, layout=LAYOUT_RESIDENT)
+        ctx.step(40)
+        got = [ctx.download(i) for i in ids]
+    for q in range(0, 520, 7):
+        o = to_oracle(scenes.Scene(bodies=[bodies[q]], planes=sc.planes, dt=sc.dt))
+        o.run(40)
+        np.testing.assert_array_equal(got[q]["pos"], o.pos)
+        np.testing.assert_array_equal(got[q]["vel"], o.vel)
+
+
 @LAYOUTS
 def test_many_balls_culled_by_grid_bitexact(layout):
     # 300 balls (above the grid threshold) scattered through and around a

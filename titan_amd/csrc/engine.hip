@@ -102,6 +102,7 @@ struct BodyH {
     bool resident = false;
     int64_t res = -1;  // resident descriptor (resident layout)
     int32_t res_cache = 0;  // LDS bytes of its incidence copy (resident layout; 0: none)
+    int32_t res_scache = 0;  // LDS bytes of its split-form region (0: none)
     int64_t n_masses = 0, chunk0 = 0, nchunks = 0;
     int64_t inc0 = 0, inc_len = 0, act0 = 0, act_len = 0, lc0 = 0, lc_len = 0;
     int64_t tile0 = 0, ntiles = 0, halo0 = 0, nhalo = 0;
@@ -174,6 +175,7 @@ struct titan_ctx {
 
     bool has_act = false;  // some spring is actuated: launch the actuated variants
     int res_lds_max = 0;   // LDS a resident workgroup may have
+    int n_cu = 0;          // compute units of the device
     int slot = 0;
 
     // captured multi-step graphs, valid while `version` is unchanged, keyed
@@ -407,7 +409,7 @@ void ensure_rdescs(titan_ctx *c, int64_t need) {
     const int64_t old = c->rdescs.cap();
     const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 64));
     grow_soa(c, c->rdesc, 1, old, nc);
-    std::vector<ResidentDesc> dead(nc - old, ResidentDesc{-1, 0, 0, 0, 0, 0, 0, 0});
+    std::vector<ResidentDesc> dead(nc - old, ResidentDesc{-1, 0, 0, 0, 0, 0, 0, 0, 0, 0});
     h2d(c, c->rdesc + old, dead.data(), nc - old);
     HIPCHK(hipStreamSynchronize(c->stream));
     c->rdescs.grow(nc);
@@ -462,13 +464,32 @@ int32_t resident_rows(const titan_ctx *c) {
     return h;
 }
 
-// The resident launch's incidence-copy region: the largest copy of a live
-// resident body, when it fits beside the state buffers (else none copies)
-int32_t resident_cache(const titan_ctx *c) {
-    int32_t h = 0;
+// Whether the resident launch uses the split form: when it has no more
+// bodies than the device has compute units, each workgroup has a CU to
+// itself and its step latency is what counts (the split form spreads a
+// small body's spring terms over all four waves; its larger LDS region would
+// otherwise cost workgroups per CU).
+bool resident_split(const titan_ctx *c) {
+    int64_t n = 0;
     for (auto &b : c->bodies)
-        if (b.live && b.resident) h = std::max(h, b.res_cache);
-    if ((int64_t)resident_lds_bytes(resident_rows(c)) + h > c->res_lds_max) return 0;
+        if (b.live && b.resident) ++n;
+    return n <= c->n_cu;
+}
+
+// The resident launch's LDS region after the state buffers: the largest
+// region a live resident body uses that fits beside them (its split-form
+// region when the launch uses that form and it has one, else its incidence
+// copy).  A body whose region is larger reads the arena.
+int32_t resident_cache(const titan_ctx *c, bool split) {
+    const int64_t avail = (int64_t)c->res_lds_max - (int64_t)resident_lds_bytes(resident_rows(c));
+    int32_t h = 0;
+    for (auto &b : c->bodies) {
+        if (!(b.live && b.resident)) continue;
+        int32_t r = 0;
+        if (split && b.res_scache > 0 && b.res_scache <= avail) r = b.res_scache;
+        else if (b.res_cache > 0 && b.res_cache <= avail) r = b.res_cache;
+        h = std::max(h, r);
+    }
     return h;
 }
 
@@ -595,7 +616,8 @@ StepArgs make_args(titan_ctx *c, int slot, const Range &rg) {
     a.res_base = rg.r0;
     a.nres = rg.r1;
     a.res_rows = resident_rows(c);
-    a.res_cache = resident_cache(c);
+    a.res_split = resident_split(c) ? 1 : 0;
+    a.res_cache = resident_cache(c, a.res_split != 0);
     return a;
 }
 
@@ -881,7 +903,47 @@ struct PreparedBody {
     std::vector<int32_t> drag;
     std::vector<LocalC> lch;        // the shared list, or every per-mass list in device order
     std::vector<int64_t> mlc;       // per-mass lists: (offset in lch << 20) | count
+    // split form (resident bodies of at most kSplitMaxChunks chunks): the
+    // spring words then the incidence words, the rest lengths (then k and
+    // damping where per spring), and each spring's local actuator or -1
+    std::vector<uint32_t> spw;
+    std::vector<double> spd;
+    std::vector<int32_t> spa;
+    int64_t nsp = 0, nreal = 0;
 };
+
+// The split form of a resident body (endpoints in device rows): its springs
+// once each and every mass's incidences in ascending spring index, in the
+// order the SELL entries hold them; nothing when the LDS region would not fit.
+void build_split(PreparedBody &P, const SpringsIn &sp, const std::vector<int32_t> &act_local) {
+    const int64_t S = sp.n_springs, npad = P.nch * kChunk;
+    int64_t nsp = 0, nreal = 0;
+    for (int64_t s = 0; s < S; ++s)
+        if (sp.left[s] >= 0 && sp.right[s] >= 0) ++nsp;
+    for (const int32_t dg : P.sb.deg) nreal += dg;
+    if (nreal != 2 * nsp || split_region_bytes(nreal, nsp, P.per_k, P.per_d) > kResidentCacheMax) return;
+    std::vector<int64_t> at(npad + 1, 0);
+    for (int64_t q = 0; q < npad; ++q) at[q + 1] = at[q] + P.sb.deg[q];
+    P.spw.assign(nsp + nreal, 0);
+    P.spd.assign(nsp * (1 + (P.per_k ? 1 : 0) + (P.per_d ? 1 : 0)), 0.0);
+    if (sp.act) P.spa.assign(nsp, -1);
+    int64_t j = 0;
+    for (int64_t s = 0; s < S; ++s) {
+        const int32_t a = sp.left[s], b = sp.right[s];
+        if (a < 0 || b < 0) continue;
+        const bool act = sp.act && act_local[s] >= 0;
+        P.spw[j] = (uint32_t)a | ((uint32_t)b << 16) | (act ? kActBit16 << 16 : 0u);
+        P.spd[j] = sp.rest[s];
+        if (P.per_k) P.spd[nsp + j] = sp.k[s];
+        if (P.per_d) P.spd[nsp * (P.per_k ? 2 : 1) + j] = sp.damping ? sp.damping[s] : 0.0;
+        if (act) P.spa[j] = act_local[s];
+        P.spw[nsp + at[a]++] = (uint32_t)j;  // as build_sell places them: a, then b
+        P.spw[nsp + at[b]++] = (uint32_t)j | kSideB;
+        ++j;
+    }
+    P.nsp = nsp;
+    P.nreal = nreal;
+}
 
 PreparedBody prepare_body(const titan_schedule &sched, const titan_body_desc *d) {
     if (!d) fail(TITAN_E_INVALID, "body_create: null descriptor");
@@ -1059,6 +1121,7 @@ PreparedBody prepare_body(const titan_schedule &sched, const titan_body_desc *d)
         one.tile_of_chunk.assign(nch, 0);
         one.rmax = (int32_t)N;
         build_sell(N, sp, per_k, per_d, &one, 0, P.sb);
+        if (nch <= kSplitMaxChunks) build_split(P, sp, act_local);
     } else {
         build_sell(N, sp, per_k, per_d, nullptr, 0, P.sb);
     }
@@ -1174,6 +1237,8 @@ int32_t commit_body(titan_ctx *c, PreparedBody &P) {
     TileBuild &tb = P.tb;
     const int64_t n_tiles = (int64_t)tb.tiles.size(), n_halo = (int64_t)tb.halo.size();
     const int64_t n_inc = sb.coff[nch];
+    // a resident body's split-form data follows its SELL entries
+    const int64_t n_extra = std::max<int64_t>((int64_t)P.spw.size(), (int64_t)P.spd.size());
 
     // ---- reserve space; on failure release whatever was taken
     int64_t ch0 = -1, inc0 = -1, act0 = -1, lc0 = -1, bid = -1, tl0 = -1, hl0 = -1, rs0 = -1;
@@ -1193,7 +1258,7 @@ int32_t commit_body(titan_ctx *c, PreparedBody &P) {
         ch0 = take(c->chunks, nch, ensure_chunks);
         if ((ch0 + nch) * kChunk > kMaxMassSlots)
             fail(TITAN_E_NOMEM, "body_create: mass arena limit (2^26 slots) reached");
-        inc0 = take(c->incs, n_inc, ensure_incs);
+        inc0 = take(c->incs, n_inc + n_extra, ensure_incs);
         act0 = take(c->acts, n_act, ensure_acts);
         lc0 = take(c->lcs, n_lc, ensure_lcs);
         tl0 = take(c->tiles, n_tiles, ensure_tiles);
@@ -1226,7 +1291,7 @@ int32_t commit_body(titan_ctx *c, PreparedBody &P) {
         }
     } catch (...) {
         if (ch0 >= 0) c->chunks.release(ch0, nch);
-        if (inc0 >= 0) c->incs.release(inc0, n_inc);
+        if (inc0 >= 0) c->incs.release(inc0, n_inc + n_extra);
         if (act0 >= 0) c->acts.release(act0, n_act);
         if (lc0 >= 0) c->lcs.release(lc0, n_lc);
         if (tl0 >= 0) c->tiles.release(tl0, n_tiles);
@@ -1292,13 +1357,25 @@ int32_t commit_body(titan_ctx *c, PreparedBody &P) {
         if (P.per_k) up(c->inc_k + inc0, sb.k.data(), n_inc);
         if (P.per_d) up(c->inc_damp + inc0, sb.damp.data(), n_inc);
     }
+    if (!P.spw.empty()) {
+        for (size_t j = 0; j < P.spa.size(); ++j) {  // actuated: the actuator's arena index
+            if (P.spa[j] < 0) continue;
+            const int64_t ix = act0 + P.spa[j];
+            std::memcpy(&P.spd[j], &ix, sizeof ix);
+        }
+        up(c->inc_nbr + inc0 + n_inc, P.spw.data(), (int64_t)P.spw.size());
+        up(c->inc_rest + inc0 + n_inc, P.spd.data(), (int64_t)P.spd.size());
+    }
     up(c->tile + tl0, tb.tiles.data(), n_tiles);
     up(c->halo + hl0, tb.halo.data(), n_halo);
     // a resident body's incidence copy: names (4 B per entry, to a 16-byte
     // boundary), rest lengths and per-incidence constants (8 B each)
     int64_t cache = ((4 * n_inc + 15) & ~(int64_t)15) + 8 * n_inc * (1 + (P.per_k ? 1 : 0) + (P.per_d ? 1 : 0));
     if (cache > kResidentCacheMax) cache = 0;
-    const ResidentDesc rdh{-1, (int32_t)ch0, (int32_t)N, (int32_t)n_act, act0, inc0, (int32_t)n_inc, (int32_t)cache};
+    // and its split-form region, where it has that form
+    const int64_t scache = P.spw.empty() ? 0 : split_region_bytes(P.nreal, P.nsp, P.per_k, P.per_d);
+    const ResidentDesc rdh{-1, (int32_t)ch0, (int32_t)N, (int32_t)n_act, act0, inc0, (int32_t)n_inc, (int32_t)cache,
+                           (int32_t)scache, (int32_t)P.nsp};
     if (resident) up(c->rdesc + rs0, &rdh, 1);
     up(c->act + 4 * act0, P.acth.data(), (int64_t)P.acth.size());
     up(c->lc + lc0, P.lch.data(), n_lc);
@@ -1329,11 +1406,12 @@ int32_t commit_body(titan_ctx *c, PreparedBody &P) {
     b.resident = resident;
     b.res = rs0;
     b.res_cache = resident ? (int32_t)cache : 0;
+    b.res_scache = resident ? (int32_t)scache : 0;
     b.n_masses = N;
     b.chunk0 = ch0;
     b.nchunks = nch;
     b.inc0 = inc0;
-    b.inc_len = n_inc;
+    b.inc_len = n_inc + n_extra;
     b.act0 = act0;
     b.act_len = n_act;
     b.lc0 = lc0;
@@ -1535,6 +1613,7 @@ int titan_ctx_create(int device, uint64_t hbm_budget_bytes, titan_ctx **out) {
     c->device = device;
     c->budget = hbm_budget_bytes;
     HIPCHK(set_tiles_lds_limit(&c->res_lds_max));
+    HIPCHK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&c->up, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&c->ev0));

@@ -871,6 +871,211 @@ __device__ __forceinline__ void resident_steps(const StepArgs &a, const Src &src
     }
 }
 
+// The terms G of springs x, x + 256, ..., x + (U - 1) 256 of a split-form
+// body (those below nsp), each from its endpoint a's side and formed exactly
+// as the walk forms the term at a; U springs at a time so that their
+// latencies overlap (a lane past the end repeats spring x and stores it once).
+template <bool ACT, int U>
+__device__ __forceinline__ void split_terms(int x, int nsp, const uint32_t *__restrict__ csw,
+                                            const double *__restrict__ cr, const double *__restrict__ kp,
+                                            const double *__restrict__ dp, const BodyKD &bkd,
+                                            const double2 *__restrict__ cur, const double *__restrict__ rc,
+                                            double *__restrict__ gx, double *__restrict__ gy,
+                                            double *__restrict__ gz) {
+    double n2[U], nrm[U], dvx[U], dvy[U], dvz[U], tx[U], ty[U], tz[U], rl[U], kk[U], dmp[U];
+    uint32_t w[U];
+    int xi[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int xu = x + u * kResidentBlock;
+        xi[u] = xu < nsp ? xu : x;
+        w[u] = csw[xi[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        rl[u] = cr[xi[u]];
+        kk[u] = kp[xi[u]];
+        dmp[u] = dp[xi[u]];
+    }
+    bool fast = true;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t ea = w[u] & 0xffffu, eb = (w[u] >> 16) & (0xffffu & ~kActBit16);
+        if (ACT && (w[u] & (kActBit16 << 16))) rl[u] = rc[__double_as_longlong(rl[u])];
+        if (!(bkd.kd & 1u)) kk[u] = bkd.k;
+        if (!(bkd.kd & 2u)) dmp[u] = bkd.d;
+        const double2 s0 = cur[3 * ea], s1 = cur[3 * ea + 1], s2 = cur[3 * ea + 2];
+        const double2 n0 = cur[3 * eb], n1 = cur[3 * eb + 1], n2v = cur[3 * eb + 2];
+        tx[u] = n0.x - s0.x;
+        ty[u] = n0.y - s0.y;
+        tz[u] = n1.x - s1.x;
+        dvx[u] = s1.y - n1.y;
+        dvy[u] = s2.x - n2v.x;
+        dvz[u] = s2.y - n2v.y;
+        n2[u] = tx[u] * tx[u] + ty[u] * ty[u] + tz[u] * tz[u];
+        fast = fast & fast_range(n2[u], tx[u], ty[u], tz[u]);
+    }
+    if (__all(fast)) {  // the same operations as unit_axis's fast branch
+        double y[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            nrm[u] = fast_sqrt(n2[u]);
+            y[u] = fast_recip(nrm[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            tx[u] = fast_div(tx[u], nrm[u], y[u]);
+            ty[u] = fast_div(ty[u], nrm[u], y[u]);
+            tz[u] = fast_div(tz[u], nrm[u], y[u]);
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            double ux, uy, uz;
+            unit_axis(tx[u], ty[u], tz[u], nrm[u], ux, uy, uz);
+            tx[u] = ux;
+            ty[u] = uy;
+            tz[u] = uz;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const double mag = kk[u] * (rl[u] - nrm[u]);
+        const double dd = (dvx[u] * tx[u] + dvy[u] * ty[u] + dvz[u] * tz[u]) * dmp[u];
+        if (x + u * kResidentBlock < nsp) {
+            gx[x + u * kResidentBlock] = tx[u] * mag + tx[u] * dd;
+            gy[x + u * kResidentBlock] = ty[u] * mag + ty[u] * dd;
+            gz[x + u * kResidentBlock] = tz[u] * mag + tz[u] * dd;
+        }
+    }
+}
+
+// v, or -v when `neg` (a flip of the sign bit; x + (-v) is x - v exactly)
+__device__ __forceinline__ double signed_term(double v, bool neg) {
+    return __hiloint2double(__double2hiint(v) ^ (neg ? (int)0x80000000 : 0), __double2loint(v));
+}
+
+// The split form of resident_steps (a body of at most kSplitMaxChunks
+// chunks with its split-form region in LDS).  Each step, all the
+// workgroup's threads compute the terms G of the body's springs, each once;
+// after a barrier each mass's lane adds up its incidences in ascending
+// spring index as the walk does (-G where it is endpoint a, +G where it is
+// b: the walk's term there is -G) and runs the mass pass.  A body of one
+// or two chunks then spreads its spring work over four waves, with half
+// the evaluations of the walk.
+template <bool ACT>
+__device__ __forceinline__ void resident_steps_split(const StepArgs &a, const uint32_t *__restrict__ ciw,
+                                                     const uint32_t *__restrict__ csw, const double *__restrict__ cr,
+                                                     const double *__restrict__ kp, const double *__restrict__ dp,
+                                                     double *__restrict__ gx, double *__restrict__ gy,
+                                                     double *__restrict__ gz, const int *__restrict__ degs,
+                                                     const int *__restrict__ starts, int nsp, int b,
+                                                     const BodyKD &bkd, int nch, int64_t g0, int64_t K, int64_t nst,
+                                                     double &T, int nact, int64_t act0, double2 *__restrict__ B0,
+                                                     double2 *__restrict__ B1) {
+    constexpr int U = 4;
+    const int t = threadIdx.x, wave = uniform(t >> 6), lane = t & (kChunk - 1);
+    const int64_t n6 = a.ncap;
+    const bool mine = wave < nch;
+    const int q = wave * kChunk + lane;
+    const int64_t i = g0 + q;
+    uint32_t fl = kPad;
+    double m = 1.0;
+    int deg = 0, first = 0;
+    if (mine) {
+        fl = a.mflags[i];
+        m = a.m[i];
+        deg = degs[q];
+        first = starts[q];
+    }
+    for (int64_t k = 0; k < K; ++k) {
+        const double2 *__restrict__ cur = (k & 1) ? B1 : B0;
+        double2 *__restrict__ nxt = (k & 1) ? B0 : B1;
+        const double *__restrict__ rc = a.ract[k & 1];
+        int x = t;
+        for (; x + (U - 1) * kResidentBlock < nsp; x += U * kResidentBlock)
+            split_terms<ACT, U>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, gx, gy, gz);
+        for (; x < nsp; x += kResidentBlock) split_terms<ACT, 1>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, gx, gy, gz);
+        __syncthreads();
+        if (mine && !(fl & kPad)) {
+            const double2 o0 = cur[3 * q], o1 = cur[3 * q + 1], o2 = cur[3 * q + 2];
+            if (fl & kFixed) {  // src/sim.cu:1100-1101
+                nxt[3 * q] = o0;
+                nxt[3 * q + 1] = o1;
+                nxt[3 * q + 2] = o2;
+            } else {
+                MassRegs r;
+                r.px = o0.x;
+                r.py = o0.y;
+                r.pz = o1.x;
+                r.vx = o1.y;
+                r.vy = o2.x;
+                r.vz = o2.y;
+                r.fx = 0.0;
+                r.fy = 0.0;
+                r.fz = 0.0;
+#pragma unroll 4
+                for (int j = first; j < first + deg; ++j) {  // ascending spring index, as the walk sums
+                    const uint32_t w = ciw[j];
+                    const int sj = (int)(w & ~kSideB);
+                    const bool neg = !(w & kSideB);
+                    r.fx = r.fx + signed_term(gx[sj], neg);
+                    r.fy = r.fy + signed_term(gy[sj], neg);
+                    r.fz = r.fz + signed_term(gz[sj], neg);
+                }
+                double ax, ay, az;
+                const int stage = (int)((nst + k) & 1);
+                const IntegIn in = load_integ(a, i, stage);
+                mass_pass(a, r, i, fl, b, m, stage, in, ax, ay, az);
+                nxt[3 * q] = make_double2(r.px, r.py);
+                nxt[3 * q + 1] = make_double2(r.pz, r.vx);
+                nxt[3 * q + 2] = make_double2(r.vy, r.vz);
+                if (k == K - 1 || a.integ == kIntVerlet) {
+                    a.acc[i] = ax;
+                    a.acc[n6 + i] = ay;
+                    a.acc[2 * n6 + i] = az;
+                }
+            }
+        }
+        T = T + a.hdt;  // src/sim.cu:1590 (RK2: :1577,1582)
+        if (ACT && k + 1 < K) actuate_slice(a, act0, nact, T, a.ract[(k + 1) & 1], t, kResidentBlock);
+        __syncthreads();
+    }
+}
+
+// Fill a body's split-form region: each row's degree and first incidence
+// (an exclusive scan of the degrees), then copies of the incidence words,
+// the spring words and the springs' rest lengths, k and damping from the
+// body's incidence span.  Returns the incidence count.
+__device__ __forceinline__ int split_fill(const StepArgs &a, char *__restrict__ base, int nch, int64_t g0,
+                                          int64_t e0, int nsp, int nd) {
+    const int t = threadIdx.x, wave = t >> 6, lane = t & (kChunk - 1);
+    int *degs = reinterpret_cast<int *>(base);
+    int *starts = degs + kSplitMaxChunks * kChunk;
+    int *wsum = starts + kSplitMaxChunks * kChunk;
+    const int dg = t < nch * kChunk ? a.deg[g0 + t] : 0;
+    int inc = dg;  // inclusive scan over the wave
+#pragma unroll
+    for (int off = 1; off < kChunk; off <<= 1) {
+        const int v = __shfl_up(inc, off);
+        if (lane >= off) inc += v;
+    }
+    if (lane == kChunk - 1) wsum[wave] = inc;
+    __syncthreads();
+    int pre = 0;
+    for (int w = 0; w < wave; ++w) pre += wsum[w];
+    const int nreal = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    degs[t] = dg;
+    starts[t] = pre + inc - dg;
+    uint32_t *cw = reinterpret_cast<uint32_t *>(base + kSplitHead);  // incidence words, then spring words
+    double *cr = reinterpret_cast<double *>(base + ((kSplitHead + 4 * (int64_t)(nreal + nsp) + 15) & ~(int64_t)15));
+    for (int j = t; j < nsp; j += kResidentBlock) cw[nreal + j] = a.inc_nbr[e0 + j];
+    for (int j = t; j < nreal; j += kResidentBlock) cw[j] = a.inc_nbr[e0 + nsp + j];
+    for (int j = t; j < nd; j += kResidentBlock) cr[j] = a.inc_rest[e0 + j];
+    __syncthreads();
+    return nreal;
+}
+
 template <bool ACT>
 __global__ __launch_bounds__(kResidentBlock) void k_rollout_resident(const StepArgs a, int final_slot) {
     extern __shared__ double2 S[];  // two buffers of 3 x res_rows pairs, then the incidence copy
@@ -912,11 +1117,34 @@ __global__ __launch_bounds__(kResidentBlock) void k_rollout_resident(const StepA
             B0[3 * q + 2] = src[3 * i + 2];
         }
     }
-    const bool cached = cache > 0 && cache <= a.res_cache;  // uniform over the block
-    if (cached) {
+    // uniform over the block: the split form, else a copy of the incidence
+    // data, when the launch's region holds it; else the arena
+    const bool split = a.res_split && rd->scache > 0 && rd->scache <= a.res_cache;
+    const bool cached = !split && cache > 0 && cache <= a.res_cache;
+    char *base = reinterpret_cast<char *>(S + 6 * a.res_rows);
+    if (split) {
+        const int nsp = rd->nsp;
+        const int nd = nsp * (1 + ((bkd.kd & 1u) ? 1 : 0) + ((bkd.kd & 2u) ? 1 : 0));
+        const int nreal = split_fill(a, base, nch, g0, inc0 + ninc, nsp, nd);
+        const int *degs = reinterpret_cast<const int *>(base);
+        const int *starts = degs + kSplitMaxChunks * kChunk;
+        const uint32_t *ciw = reinterpret_cast<const uint32_t *>(base + kSplitHead);
+        const uint32_t *csw = ciw + nreal;
+        const double *cr =
+            reinterpret_cast<const double *>(base + ((kSplitHead + 4 * (int64_t)(nreal + nsp) + 15) & ~(int64_t)15));
+        // per-spring k and damping where the body has them; else a valid
+        // array read in their place, whose values are then replaced
+        const double *ck = cr + nsp;
+        const double *cd = ck + ((bkd.kd & 1u) ? nsp : 0);
+        const double *kp = (bkd.kd & 1u) ? ck : cr;
+        const double *dp = (bkd.kd & 2u) ? cd : cr;
+        double *gx = const_cast<double *>(cr) + nd;
+        double *gy = gx + nsp, *gz = gy + nsp;
+        resident_steps_split<ACT>(a, ciw, csw, cr, kp, dp, gx, gy, gz, degs, starts, nsp, b, bkd, nch, g0, K, nst, T,
+                                  nact, act0, B0, B1);
+    } else if (cached) {
         // names (4 B per entry), then from a 16-byte boundary the rest
         // lengths and, where the body has them, k and damping (8 B each)
-        char *base = reinterpret_cast<char *>(S + 6 * a.res_rows);
         uint32_t *cn = reinterpret_cast<uint32_t *>(base);
         double *cr = reinterpret_cast<double *>(base + ((4 * (int64_t)ninc + 15) & ~(int64_t)15));
         double *ck = cr + ninc;

@@ -76,11 +76,36 @@ struct ResidentDesc {
     int64_t inc0;     // the body's incidence entries: [inc0, inc0 + ninc)
     int32_t ninc;
     int32_t cache;    // LDS bytes of a copy of its incidence data; 0: read from the arena
+    int32_t scache;   // LDS bytes of its split-form region; 0: no split form
+    int32_t nsp;      // split form: the body's springs (its data follows the ninc entries)
 };
 
-// A resident body whose incidence data fits this many bytes keeps a copy in
-// LDS for all the steps of a launch.
-constexpr int kResidentCacheMax = 64 * 1024;
+// A resident body whose incidence copy (or split-form region) fits this many
+// bytes keeps it in LDS for all the steps of a launch.
+constexpr int kResidentCacheMax = 128 * 1024;
+
+// The split form of a resident body of at most kSplitMaxChunks chunks.  The
+// body's springs are listed once each (ascending spring index): a word with
+// the rows of its endpoints a | b << 16 (b's bit 15: actuated), then the rest
+// lengths (or actuator indices), and k and damping where per spring.  Each
+// mass lists its incidences in ascending spring index, as its SELL entries
+// do: a word with the spring's place in that list, bit 31 set when the mass
+// is endpoint b.  This data sits in the body's incidence span after its SELL
+// entries: nsp spring words then the incidence words in inc_nbr, and the
+// rest lengths, k and damping in inc_rest.  A step has all the workgroup's
+// threads compute each spring's term G once, from endpoint a's side (the
+// -G of the walk at a; the walk's term at b is -G exactly, zero signs
+// aside), and then each mass's lane add up its own: -G at a, +G at b.
+// LDS region: the rows' degrees and first incidences (two int32 arrays of
+// kSplitMaxChunks * 64) and four int32 partial counts, the incidence words,
+// the spring words (to a 16-byte boundary), rest lengths, k and damping
+// where per spring, and the three components of G (8 B each).
+constexpr int kSplitMaxChunks = 4;
+constexpr int kSplitHead = 2 * 4 * kSplitMaxChunks * kChunk + 16;
+constexpr uint32_t kSideB = 0x80000000u;
+constexpr int64_t split_region_bytes(int64_t nreal, int64_t nsp, bool per_k, bool per_d) {
+    return ((kSplitHead + 4 * (nreal + nsp) + 15) & ~(int64_t)15) + 8 * nsp * (4 + (per_k ? 1 : 0) + (per_d ? 1 : 0));
+}
 
 // The body table and the arena fields a body's activation (joining the
 // launches) and deactivation write, on the step stream.
@@ -206,6 +231,7 @@ struct StepArgs {
     int64_t res_base, nres;   // launch range: descriptors [res_base, nres)
     int32_t res_rows;         // LDS rows per buffer: the largest resident body
     int32_t res_cache;        // LDS bytes after the two state buffers for an incidence copy
+    int32_t res_split;        // 1: bodies with a split-form region use it
 };
 
 }  // namespace titan_mi355x
