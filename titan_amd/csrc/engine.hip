@@ -904,8 +904,9 @@ struct PreparedBody {
     std::vector<LocalC> lch;        // the shared list, or every per-mass list in device order
     std::vector<int64_t> mlc;       // per-mass lists: (offset in lch << 20) | count
     // split form (resident bodies of at most kSplitMaxChunks chunks): the
-    // spring words then the incidence words, the rest lengths (then k and
-    // damping where per spring), and each spring's local actuator or -1
+    // springs' endpoint words then their incidence words, the rest lengths
+    // (then k and damping where per spring), and each spring's local
+    // actuator or -1
     std::vector<uint32_t> spw;
     std::vector<double> spd;
     std::vector<int32_t> spa;
@@ -913,18 +914,20 @@ struct PreparedBody {
 };
 
 // The split form of a resident body (endpoints in device rows): its springs
-// once each and every mass's incidences in ascending spring index, in the
-// order the SELL entries hold them; nothing when the LDS region would not fit.
+// once each and the numbers of their two incidences, each mass's numbered in
+// the order its SELL entries hold them; nothing when its LDS region would
+// not fit.
 void build_split(PreparedBody &P, const SpringsIn &sp, const std::vector<int32_t> &act_local) {
     const int64_t S = sp.n_springs, npad = P.nch * kChunk;
     int64_t nsp = 0, nreal = 0;
     for (int64_t s = 0; s < S; ++s)
         if (sp.left[s] >= 0 && sp.right[s] >= 0) ++nsp;
     for (const int32_t dg : P.sb.deg) nreal += dg;
-    if (nreal != 2 * nsp || split_region_bytes(nreal, nsp, P.per_k, P.per_d) > kResidentCacheMax) return;
-    std::vector<int64_t> at(npad + 1, 0);
+    if (nreal != 2 * nsp || nreal > 0xffff || split_region_bytes(nreal, nsp, P.per_k, P.per_d) > kResidentCacheMax)
+        return;
+    std::vector<int64_t> at(npad + 1, 0);  // the next incidence number of each mass
     for (int64_t q = 0; q < npad; ++q) at[q + 1] = at[q] + P.sb.deg[q];
-    P.spw.assign(nsp + nreal, 0);
+    P.spw.assign(2 * nsp, 0);
     P.spd.assign(nsp * (1 + (P.per_k ? 1 : 0) + (P.per_d ? 1 : 0)), 0.0);
     if (sp.act) P.spa.assign(nsp, -1);
     int64_t j = 0;
@@ -937,8 +940,9 @@ void build_split(PreparedBody &P, const SpringsIn &sp, const std::vector<int32_t
         if (P.per_k) P.spd[nsp + j] = sp.k[s];
         if (P.per_d) P.spd[nsp * (P.per_k ? 2 : 1) + j] = sp.damping ? sp.damping[s] : 0.0;
         if (act) P.spa[j] = act_local[s];
-        P.spw[nsp + at[a]++] = (uint32_t)j;  // as build_sell places them: a, then b
-        P.spw[nsp + at[b]++] = (uint32_t)j | kSideB;
+        const uint32_t ia = (uint32_t)at[a]++;  // as build_sell places them: a, then b
+        const uint32_t ib = (uint32_t)at[b]++;
+        P.spw[nsp + j] = ia | (ib << 16);
         ++j;
     }
     P.nsp = nsp;

@@ -871,17 +871,23 @@ __device__ __forceinline__ void resident_steps(const StepArgs &a, const Src &src
     }
 }
 
+// v, or -v when `neg` (a flip of the sign bit; x + (-v) is x - v exactly)
+__device__ __forceinline__ double signed_term(double v, bool neg) {
+    return __hiloint2double(__double2hiint(v) ^ (neg ? (int)0x80000000 : 0), __double2loint(v));
+}
+
 // The terms G of springs x, x + 256, ..., x + (U - 1) 256 of a split-form
 // body (those below nsp), each from its endpoint a's side and formed exactly
-// as the walk forms the term at a; U springs at a time so that their
-// latencies overlap (a lane past the end repeats spring x and stores it once).
+// as the walk forms the term at a, stored as -G at a's incidence and +G at
+// b's; U springs at a time so that their latencies overlap (a lane past the
+// end repeats spring x and stores it once).
 template <bool ACT, int U>
 __device__ __forceinline__ void split_terms(int x, int nsp, const uint32_t *__restrict__ csw,
                                             const double *__restrict__ cr, const double *__restrict__ kp,
                                             const double *__restrict__ dp, const BodyKD &bkd,
                                             const double2 *__restrict__ cur, const double *__restrict__ rc,
-                                            double *__restrict__ gx, double *__restrict__ gy,
-                                            double *__restrict__ gz) {
+                                            double *__restrict__ fx, double *__restrict__ fy,
+                                            double *__restrict__ fz) {
     double n2[U], nrm[U], dvx[U], dvy[U], dvz[U], tx[U], ty[U], tz[U], rl[U], kk[U], dmp[U];
     uint32_t w[U];
     int xi[U];
@@ -942,33 +948,33 @@ __device__ __forceinline__ void split_terms(int x, int nsp, const uint32_t *__re
     for (int u = 0; u < U; ++u) {
         const double mag = kk[u] * (rl[u] - nrm[u]);
         const double dd = (dvx[u] * tx[u] + dvy[u] * ty[u] + dvz[u] * tz[u]) * dmp[u];
+        const double gx = tx[u] * mag + tx[u] * dd, gy = ty[u] * mag + ty[u] * dd, gz = tz[u] * mag + tz[u] * dd;
         if (x + u * kResidentBlock < nsp) {
-            gx[x + u * kResidentBlock] = tx[u] * mag + tx[u] * dd;
-            gy[x + u * kResidentBlock] = ty[u] * mag + ty[u] * dd;
-            gz[x + u * kResidentBlock] = tz[u] * mag + tz[u] * dd;
+            const uint32_t iw = csw[nsp + xi[u]];
+            const int ia = (int)(iw & 0xffffu), ib = (int)(iw >> 16);
+            fx[ia] = signed_term(gx, true);
+            fy[ia] = signed_term(gy, true);
+            fz[ia] = signed_term(gz, true);
+            fx[ib] = gx;
+            fy[ib] = gy;
+            fz[ib] = gz;
         }
     }
 }
 
-// v, or -v when `neg` (a flip of the sign bit; x + (-v) is x - v exactly)
-__device__ __forceinline__ double signed_term(double v, bool neg) {
-    return __hiloint2double(__double2hiint(v) ^ (neg ? (int)0x80000000 : 0), __double2loint(v));
-}
-
 // The split form of resident_steps (a body of at most kSplitMaxChunks
 // chunks with its split-form region in LDS).  Each step, all the
-// workgroup's threads compute the terms G of the body's springs, each once;
-// after a barrier each mass's lane adds up its incidences in ascending
-// spring index as the walk does (-G where it is endpoint a, +G where it is
-// b: the walk's term there is -G) and runs the mass pass.  A body of one
-// or two chunks then spreads its spring work over four waves, with half
-// the evaluations of the walk.
+// workgroup's threads compute the terms G of the body's springs, each once,
+// into the incidences of both endpoints; after a barrier each mass's lane
+// adds up its incidences in ascending spring index as the walk does and
+// runs the mass pass.  A body of one or two chunks then spreads its spring
+// work over four waves, with half the evaluations of the walk.
 template <bool ACT>
-__device__ __forceinline__ void resident_steps_split(const StepArgs &a, const uint32_t *__restrict__ ciw,
-                                                     const uint32_t *__restrict__ csw, const double *__restrict__ cr,
-                                                     const double *__restrict__ kp, const double *__restrict__ dp,
-                                                     double *__restrict__ gx, double *__restrict__ gy,
-                                                     double *__restrict__ gz, const int *__restrict__ degs,
+__device__ __forceinline__ void resident_steps_split(const StepArgs &a, const uint32_t *__restrict__ csw,
+                                                     const double *__restrict__ cr, const double *__restrict__ kp,
+                                                     const double *__restrict__ dp, double *__restrict__ fx,
+                                                     double *__restrict__ fy, double *__restrict__ fz,
+                                                     const int *__restrict__ degs,
                                                      const int *__restrict__ starts, int nsp, int b,
                                                      const BodyKD &bkd, int nch, int64_t g0, int64_t K, int64_t nst,
                                                      double &T, int nact, int64_t act0, double2 *__restrict__ B0,
@@ -994,8 +1000,8 @@ __device__ __forceinline__ void resident_steps_split(const StepArgs &a, const ui
         const double *__restrict__ rc = a.ract[k & 1];
         int x = t;
         for (; x + (U - 1) * kResidentBlock < nsp; x += U * kResidentBlock)
-            split_terms<ACT, U>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, gx, gy, gz);
-        for (; x < nsp; x += kResidentBlock) split_terms<ACT, 1>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, gx, gy, gz);
+            split_terms<ACT, U>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
+        for (; x < nsp; x += kResidentBlock) split_terms<ACT, 1>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
         __syncthreads();
         if (mine && !(fl & kPad)) {
             const double2 o0 = cur[3 * q], o1 = cur[3 * q + 1], o2 = cur[3 * q + 2];
@@ -1016,12 +1022,9 @@ __device__ __forceinline__ void resident_steps_split(const StepArgs &a, const ui
                 r.fz = 0.0;
 #pragma unroll 4
                 for (int j = first; j < first + deg; ++j) {  // ascending spring index, as the walk sums
-                    const uint32_t w = ciw[j];
-                    const int sj = (int)(w & ~kSideB);
-                    const bool neg = !(w & kSideB);
-                    r.fx = r.fx + signed_term(gx[sj], neg);
-                    r.fy = r.fy + signed_term(gy[sj], neg);
-                    r.fz = r.fz + signed_term(gz[sj], neg);
+                    r.fx = r.fx + fx[j];
+                    r.fy = r.fy + fy[j];
+                    r.fz = r.fz + fz[j];
                 }
                 double ax, ay, az;
                 const int stage = (int)((nst + k) & 1);
@@ -1044,9 +1047,9 @@ __device__ __forceinline__ void resident_steps_split(const StepArgs &a, const ui
 }
 
 // Fill a body's split-form region: each row's degree and first incidence
-// (an exclusive scan of the degrees), then copies of the incidence words,
-// the spring words and the springs' rest lengths, k and damping from the
-// body's incidence span.  Returns the incidence count.
+// (an exclusive scan of the degrees), then copies of the springs' endpoint
+// and incidence words, rest lengths, k and damping from the body's
+// incidence span.  Returns the incidence count.
 __device__ __forceinline__ int split_fill(const StepArgs &a, char *__restrict__ base, int nch, int64_t g0,
                                           int64_t e0, int nsp, int nd) {
     const int t = threadIdx.x, wave = t >> 6, lane = t & (kChunk - 1);
@@ -1067,10 +1070,9 @@ __device__ __forceinline__ int split_fill(const StepArgs &a, char *__restrict__ 
     const int nreal = wsum[0] + wsum[1] + wsum[2] + wsum[3];
     degs[t] = dg;
     starts[t] = pre + inc - dg;
-    uint32_t *cw = reinterpret_cast<uint32_t *>(base + kSplitHead);  // incidence words, then spring words
-    double *cr = reinterpret_cast<double *>(base + ((kSplitHead + 4 * (int64_t)(nreal + nsp) + 15) & ~(int64_t)15));
-    for (int j = t; j < nsp; j += kResidentBlock) cw[nreal + j] = a.inc_nbr[e0 + j];
-    for (int j = t; j < nreal; j += kResidentBlock) cw[j] = a.inc_nbr[e0 + nsp + j];
+    uint32_t *cw = reinterpret_cast<uint32_t *>(base + kSplitHead);  // endpoint words, then incidence words
+    double *cr = reinterpret_cast<double *>(base + ((kSplitHead + 8 * (int64_t)nsp + 15) & ~(int64_t)15));
+    for (int j = t; j < 2 * nsp; j += kResidentBlock) cw[j] = a.inc_nbr[e0 + j];
     for (int j = t; j < nd; j += kResidentBlock) cr[j] = a.inc_rest[e0 + j];
     __syncthreads();
     return nreal;
@@ -1128,20 +1130,18 @@ __global__ __launch_bounds__(kResidentBlock) void k_rollout_resident(const StepA
         const int nreal = split_fill(a, base, nch, g0, inc0 + ninc, nsp, nd);
         const int *degs = reinterpret_cast<const int *>(base);
         const int *starts = degs + kSplitMaxChunks * kChunk;
-        const uint32_t *ciw = reinterpret_cast<const uint32_t *>(base + kSplitHead);
-        const uint32_t *csw = ciw + nreal;
-        const double *cr =
-            reinterpret_cast<const double *>(base + ((kSplitHead + 4 * (int64_t)(nreal + nsp) + 15) & ~(int64_t)15));
+        const uint32_t *csw = reinterpret_cast<const uint32_t *>(base + kSplitHead);
+        const double *cr = reinterpret_cast<const double *>(base + ((kSplitHead + 8 * (int64_t)nsp + 15) & ~(int64_t)15));
         // per-spring k and damping where the body has them; else a valid
         // array read in their place, whose values are then replaced
         const double *ck = cr + nsp;
         const double *cd = ck + ((bkd.kd & 1u) ? nsp : 0);
         const double *kp = (bkd.kd & 1u) ? ck : cr;
         const double *dp = (bkd.kd & 2u) ? cd : cr;
-        double *gx = const_cast<double *>(cr) + nd;
-        double *gy = gx + nsp, *gz = gy + nsp;
-        resident_steps_split<ACT>(a, ciw, csw, cr, kp, dp, gx, gy, gz, degs, starts, nsp, b, bkd, nch, g0, K, nst, T,
-                                  nact, act0, B0, B1);
+        double *fx = const_cast<double *>(cr) + nd;
+        double *fy = fx + nreal, *fz = fy + nreal;
+        resident_steps_split<ACT>(a, csw, cr, kp, dp, fx, fy, fz, degs, starts, nsp, b, bkd, nch, g0, K, nst, T, nact,
+                                  act0, B0, B1);
     } else if (cached) {
         // names (4 B per entry), then from a 16-byte boundary the rest
         // lengths and, where the body has them, k and damping (8 B each)

@@ -85,26 +85,31 @@ struct ResidentDesc {
 constexpr int kResidentCacheMax = 128 * 1024;
 
 // The split form of a resident body of at most kSplitMaxChunks chunks.  The
-// body's springs are listed once each (ascending spring index): a word with
-// the rows of its endpoints a | b << 16 (b's bit 15: actuated), then the rest
-// lengths (or actuator indices), and k and damping where per spring.  Each
-// mass lists its incidences in ascending spring index, as its SELL entries
-// do: a word with the spring's place in that list, bit 31 set when the mass
-// is endpoint b.  This data sits in the body's incidence span after its SELL
-// entries: nsp spring words then the incidence words in inc_nbr, and the
-// rest lengths, k and damping in inc_rest.  A step has all the workgroup's
-// threads compute each spring's term G once, from endpoint a's side (the
-// -G of the walk at a; the walk's term at b is -G exactly, zero signs
-// aside), and then each mass's lane add up its own: -G at a, +G at b.
+// body's incidences are numbered mass by mass, each mass's in ascending
+// spring index as its SELL entries hold them: incidence j of mass q is
+// first(q) + j, first = the exclusive prefix sum of the degrees.  (So the
+// threads that store one mass's terms write neighbouring LDS words; terms
+// numbered by SELL entry, 64 words apart per mass, cost more in bank
+// conflicts than contiguous sums gain.)  The springs are listed once each,
+// in ascending spring index: a word with the rows of the endpoints
+// a | b << 16 (b's bit 15: actuated), a word with their incidence numbers
+// ia | ib << 16, then the rest lengths (or actuator indices), and k and
+// damping where per spring.  This data sits in the body's incidence span
+// after its SELL entries: the nsp endpoint words and nsp incidence words in
+// inc_nbr, the rest lengths, k and damping in inc_rest.  A step has all the
+// workgroup's threads compute each spring's term G once, from endpoint a's
+// side, and store -G at incidence ia (the walk's term at a is G,
+// subtracted) and +G at ib (the walk's term at b is -G exactly, zero signs
+// aside); each mass's lane then adds up its own incidences in order.
 // LDS region: the rows' degrees and first incidences (two int32 arrays of
-// kSplitMaxChunks * 64) and four int32 partial counts, the incidence words,
-// the spring words (to a 16-byte boundary), rest lengths, k and damping
-// where per spring, and the three components of G (8 B each).
+// kSplitMaxChunks * 64) and four int32 partial counts, the 2 nsp words (to
+// a 16-byte boundary), rest lengths, k and damping where per spring, and
+// the three components of the incidences' terms (8 B each).
 constexpr int kSplitMaxChunks = 4;
 constexpr int kSplitHead = 2 * 4 * kSplitMaxChunks * kChunk + 16;
-constexpr uint32_t kSideB = 0x80000000u;
 constexpr int64_t split_region_bytes(int64_t nreal, int64_t nsp, bool per_k, bool per_d) {
-    return ((kSplitHead + 4 * (nreal + nsp) + 15) & ~(int64_t)15) + 8 * nsp * (4 + (per_k ? 1 : 0) + (per_d ? 1 : 0));
+    return ((kSplitHead + 8 * nsp + 15) & ~(int64_t)15) + 8 * nsp * (1 + (per_k ? 1 : 0) + (per_d ? 1 : 0)) +
+           24 * nreal;
 }
 
 // The body table and the arena fields a body's activation (joining the
