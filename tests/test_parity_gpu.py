@@ -338,6 +338,47 @@ def test_resident_many_balls_and_actuation():
     assert normwise_rel(eng["vel"], orc.vel) <= TOL_ACT
 
 
+@LAYOUTS3
+def test_edge_case_bodies_bitexact(layout):
+    # one free mass; masses without springs; springs with a missing endpoint
+    # (-1: skipped, as the reference skips a null endpoint, src/sim.cu:1052)
+    # and duplicate springs between one pair; a body with every mass fixed
+    rng = np.random.default_rng(31)
+    one = dict(pos=np.array([[0.0, 0.0, 0.3]]), m=np.array([0.2]), vel=np.array([[0.1, -0.2, 0.0]]))
+    loose = dict(pos=rng.uniform(-1.0, 1.0, (10, 3)) + [6.0, 0.0, 1.5], m=np.full(10, 0.1),
+                 vel=rng.normal(0, 0.5, (10, 3)))
+    b = scenes.lattice_body((3.0, 0.0, 0.6), (1.0, 1.0, 1.0), 4, damping=0.1)
+    s = len(b["left"])
+    dup = np.arange(0, s, 17)
+    left, right = b["left"].copy(), b["right"].copy()
+    left[::11] = -1
+    right[5::13] = -1
+    b["left"] = np.concatenate([left, b["left"][dup]])
+    b["right"] = np.concatenate([right, b["right"][dup]])
+    b["k"] = np.concatenate([b["k"], rng.uniform(5e3, 2e4, len(dup))])
+    b["rest"] = np.concatenate([b["rest"], b["rest"][dup] * 1.01])
+    b["damping"] = np.concatenate([b["damping"], np.full(len(dup), 0.2)])
+    b["vel"] = rng.normal(0, 0.3, b["pos"].shape)
+    fixed = scenes.lattice_body((-3.0, 0.0, 0.6), (1.0, 1.0, 1.0), 3, damping=0.1)
+    fixed["fixed"] = np.ones(len(fixed["pos"]), np.uint8)
+    sc = scenes.Scene(bodies=[one, loose, b, fixed], planes=[(0.0, 0.0, 1.0, 0.0, 0.8, 1.0)], dt=1e-4)
+    _say(f"case edge-case bodies (1 mass, no springs, -1 endpoints, duplicates, all fixed), layout {layout}")
+    with Context() as ctx:
+        ids = sc.load(ctx, layout=layout)
+        ctx.step(0)
+        for n in (1, 70, 129):
+            ctx.step(n)
+        got = [ctx.download(i, acc=True, T=True) for i in ids]
+    for q, g in enumerate(got):
+        o = to_oracle(scenes.Scene(bodies=[sc.bodies[q]], planes=sc.planes, dt=sc.dt))
+        o.run(200)
+        np.testing.assert_array_equal(g["pos"], o.pos)
+        np.testing.assert_array_equal(g["vel"], o.vel)
+        np.testing.assert_array_equal(g["acc"], o.acc)
+        np.testing.assert_array_equal(g["T"], o.T)
+    np.testing.assert_array_equal(got[3]["pos"], fixed["pos"])  # fixed masses never move
+
+
 def _small_bodies(rng, act=False):
     # resident bodies of one to four chunks (27, 64, 125 and 216 masses and a
     # 200-mass kNN cloud of irregular degree), k and damping shared or per
