@@ -1078,7 +1078,10 @@ __device__ __forceinline__ int split_fill(const StepArgs &a, char *__restrict__ 
     return nreal;
 }
 
-template <bool ACT>
+// SPLIT: the launch uses the split form where a body has it.  The variant
+// without it is launched otherwise: the split form's interleaved term
+// evaluation needs more registers, which would cost the walk occupancy.
+template <bool ACT, bool SPLIT>
 __global__ __launch_bounds__(kResidentBlock) void k_rollout_resident(const StepArgs a, int final_slot) {
     extern __shared__ double2 S[];  // two buffers of 3 x res_rows pairs, then the incidence copy
     const int64_t rid = a.res_base + blockIdx.x;
@@ -1121,7 +1124,7 @@ __global__ __launch_bounds__(kResidentBlock) void k_rollout_resident(const StepA
     }
     // uniform over the block: the split form, else a copy of the incidence
     // data, when the launch's region holds it; else the arena
-    const bool split = a.res_split && rd->scache > 0 && rd->scache <= a.res_cache;
+    const bool split = SPLIT && rd->scache > 0 && rd->scache <= a.res_cache;
     const bool cached = !split && cache > 0 && cache <= a.res_cache;
     char *base = reinterpret_cast<char *>(S + 6 * a.res_rows);
     if (split) {
@@ -1281,8 +1284,13 @@ hipError_t launch_rollout_resident(const StepArgs &a, bool act, int final_slot, 
     if (a.nres <= a.res_base) return hipSuccess;
     const size_t lds = resident_lds_bytes(a.res_rows) + (size_t)a.res_cache;
     const dim3 g((unsigned)(a.nres - a.res_base)), blk(kResidentBlock);
-    if (act) hipLaunchKernelGGL((k_rollout_resident<true>), g, blk, lds, s, a, final_slot);
-    else hipLaunchKernelGGL((k_rollout_resident<false>), g, blk, lds, s, a, final_slot);
+    if (a.res_split) {
+        if (act) hipLaunchKernelGGL((k_rollout_resident<true, true>), g, blk, lds, s, a, final_slot);
+        else hipLaunchKernelGGL((k_rollout_resident<false, true>), g, blk, lds, s, a, final_slot);
+    } else {
+        if (act) hipLaunchKernelGGL((k_rollout_resident<true, false>), g, blk, lds, s, a, final_slot);
+        else hipLaunchKernelGGL((k_rollout_resident<false, false>), g, blk, lds, s, a, final_slot);
+    }
     return hipGetLastError();
 }
 
@@ -1298,11 +1306,12 @@ hipError_t set_tiles_lds_limit(int *resident_max) {
     if (er != hipSuccess) return er;
     const int rb = std::min(lim, (int)resident_lds_bytes(kResidentMax) + kResidentCacheMax);
     *resident_max = rb;
-    er = hipFuncSetAttribute((const void *)k_rollout_resident<false>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, rb);
-    if (er != hipSuccess) return er;
-    er = hipFuncSetAttribute((const void *)k_rollout_resident<true>, hipFuncAttributeMaxDynamicSharedMemorySize, rb);
-    if (er != hipSuccess) return er;
+    const void *res[4] = {(const void *)k_rollout_resident<false, false>, (const void *)k_rollout_resident<true, false>,
+                          (const void *)k_rollout_resident<false, true>, (const void *)k_rollout_resident<true, true>};
+    for (const void *f : res) {
+        er = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, rb);
+        if (er != hipSuccess) return er;
+    }
     const int bytes = (int)tiles_lds_bytes(kMaxTileRows);
     const hipError_t e =
         hipFuncSetAttribute((const void *)k_step_tiles<false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
