@@ -90,9 +90,12 @@ typedef struct {
     /* device layout: TITAN_LAYOUT_AUTO (the context's schedule decides, see
      * titan_schedule; tiled only when the body's tiles fit the kernel's
      * limits), TITAN_LAYOUT_GATHER, TITAN_LAYOUT_TILED (fails if a tile does
-     * not fit) or TITAN_LAYOUT_RESIDENT (at most 1024 masses: the body is
-     * held whole in one workgroup's LDS and all its steps of a stepping call
-     * run in one launch).  Results are identical either way. */
+     * not fit) or TITAN_LAYOUT_RESIDENT (at most 32767 masses: one
+     * workgroup runs all the body's steps of a stepping call, in one launch
+     * for every resident body; a body of at most 1024 masses is held in LDS
+     * when the launch has at most one body per compute unit, otherwise the
+     * body steps in its own state buffers).  Results are identical either
+     * way. */
     int32_t layout;
 } titan_body_desc;
 
@@ -185,15 +188,14 @@ int titan_get_integrator(const titan_ctx *ctx, int32_t *kind);
 /* ---- schedule and autotuning (no counterpart in the reference, which has
  *      one fixed launch shape per kernel, src/sim.cu:1569-1590) ---- */
 /* How bodies created with TITAN_LAYOUT_AUTO are laid out: bodies with at
- * most resident_max_masses masses (0 .. 1024; 0 = none) get the resident
+ * most resident_max_masses masses (0 .. 32767; 0 = none) get the resident
  * layout; of the others, those with at least tiled_min_masses masses get the
  * tiled layout and smaller ones the gather layout; tile_rows bounds a tile's
  * LDS rows (masses + halo, 64 .. 2304; fewer rows allow more workgroups per
  * CU, more rows bigger tiles).  Default: resident_max_masses 0,
- * tiled_min_masses 0, tile_rows 2304 (every body tiled: a resident launch
- * sizes every workgroup's LDS for its largest body, which favours one small
- * scene over a population; titan_autotune measures both).  Applies to
- * bodies created afterwards. */
+ * tiled_min_masses 0, tile_rows 2304 (every body tiled: one workgroup per
+ * resident body favours a few small scenes, the tiled launch a population;
+ * titan_autotune measures both).  Applies to bodies created afterwards. */
 typedef struct titan_schedule {
     int64_t tiled_min_masses;
     int32_t tile_rows;
@@ -202,14 +204,14 @@ typedef struct titan_schedule {
 int titan_set_schedule(titan_ctx *ctx, const titan_schedule *sched);
 int titan_get_schedule(const titan_ctx *ctx, titan_schedule *sched);
 /* Pick the schedule by measurement: for each candidate (resident_max_masses
- * in {0, 1024}, tiled_min_masses in {0, 128, 256, 512, 1024, never},
- * tile_rows in {682, 2304}; 24 in that order, the last index fastest) a scratch
+ * in {0, 1024, 32767}, tiled_min_masses in {0, 128, 256, 512, 1024, never},
+ * tile_rows in {682, 2304}; 36 in that order, the last index fastest) a scratch
  * context on the same device with this context's dt, gravity, planes and
  * balls holds `replicas` copies of the n_samples sample bodies and is timed
  * over `steps` steps (after a warm-up of the same length).  The fastest in
  * mass-steps/s becomes this context's schedule and is returned in *best
  * (may be NULL); *rate_out (may be NULL) receives each candidate's rate, in
- * candidate order (24 entries; candidates that lay out every sample alike
+ * candidate order (36 entries; candidates that lay out every sample alike
  * are timed once and share the rate). */
 int titan_autotune(titan_ctx *ctx, const titan_body_desc *samples, int32_t n_samples, int32_t replicas,
                    int32_t steps, titan_schedule *best, double *rate_out);

@@ -287,8 +287,47 @@ def test_auto_layout_by_size():
         ctx.set_schedule(0, 2304, 1024)  # bodies of at most 1024 masses resident
         assert ctx.body_layout(ctx.add_body(**small.bodies[0])) == LAYOUT_RESIDENT
         assert ctx.body_layout(ctx.add_body(**big)) == LAYOUT_TILED
+        assert ctx.body_layout(ctx.add_body(**big, layout=LAYOUT_RESIDENT)) == LAYOUT_RESIDENT
+        huge = scenes.lattice_body((0.0, 0.0, 0.55), (1.2, 1.2, 1.2), (33, 33, 31))  # 33759 masses
         with pytest.raises(RuntimeError):
-            ctx.add_body(**big, layout=LAYOUT_RESIDENT)  # more than 1024 masses
+            ctx.add_body(**huge, layout=LAYOUT_RESIDENT)  # more than 32767 masses
+
+
+def test_resident_bodies_in_lds_and_in_the_arena_bitexact():
+    # a launch with few bodies: the 125-mass body steps in LDS (split form),
+    # the 343-mass one in LDS (walk), the 1331- and 2197-mass ones in their
+    # arena state buffers; then the same bodies among 300 others, where
+    # every body steps in the arena
+    rng = np.random.default_rng(12)
+    bodies = []
+    for q, side in enumerate([5, 7, 11, 13]):
+        b = scenes.lattice_body((4.0 * q, 0.0, 0.6), (1.2, 1.2, 1.2), side, damping=0.1)
+        b["pos"] = b["pos"] + rng.normal(0, 1e-3, b["pos"].shape)
+        b["vel"] = rng.normal(0, 0.2, b["pos"].shape)
+        bodies.append(b)
+    sc = scenes.Scene(bodies=bodies, planes=[(0.0, 0.0, 1.0, 0.0, 0.8, 1.0)], dt=1e-4)
+    filler = [scenes.lattice_body((2.0 * (q % 20), 8.0 + 2.0 * (q // 20), 0.6), (1.0, 1.0, 1.0), 3, damping=0.1)
+              for q in range(300)]
+    calls = [1, 64, 85]
+    orc = []
+    for q in range(4):
+        o = to_oracle(scenes.Scene(bodies=[bodies[q]], planes=sc.planes, dt=sc.dt))
+        o.run(sum(calls))
+        orc.append(o)
+    for extra in (0, 300):
+        _say(f"case resident bodies 125..2197 masses with {extra} other bodies, calls {calls}")
+        with Context() as ctx:
+            ids = sc.load(ctx, layout=LAYOUT_RESIDENT)
+            for f in filler[:extra]:
+                ctx.add_body(**f, layout=LAYOUT_RESIDENT)
+            for n in calls:
+                ctx.step(n)
+            got = [ctx.download(i, acc=True, T=True) for i in ids]
+        for q in range(4):
+            np.testing.assert_array_equal(got[q]["pos"], orc[q].pos)
+            np.testing.assert_array_equal(got[q]["vel"], orc[q].vel)
+            np.testing.assert_array_equal(got[q]["acc"], orc[q].acc)
+            np.testing.assert_array_equal(got[q]["T"], orc[q].T)
 
 
 def test_resident_beside_gather_and_tiled_bodies():

@@ -43,7 +43,7 @@ def test_every_schedule_gives_identical_states():
     n = [len(b["pos"]) for b in sc.bodies]
     assert min(n) <= 1024 < max(n)
     for sched in [(10**9, 2304, 0), (256, 682, 0), (512, 2304, 0), (0, 682, 0), (0, 64, 0),
-                  (0, 2304, 1024), (10**9, 682, 400), (0, 2304, 1)]:
+                  (0, 2304, 1024), (10**9, 682, 400), (0, 2304, 1), (0, 2304, 32767)]:
         st, lay = _run(sc, sched)
         assert lay == [_layout(k, sched) for k in n]
         for a, b in zip(st, ref):
@@ -57,7 +57,7 @@ def test_autotune_picks_a_candidate_and_applies_it():
         ctx.set_dt(sc.dt)
         ctx.set_global_acc(sc.g)
         best, rates = ctx.autotune(sc.bodies, replicas=2, steps=4)
-        assert len(rates) == len(AUTOTUNE_CANDIDATES) == 24
+        assert len(rates) == len(AUTOTUNE_CANDIDATES) == 36
         assert all(r > 0 for _, r in rates)
         won = max(rates, key=lambda cr: cr[1])[0]
         want = (2**63 - 1 if won[0] < 0 else won[0], won[1], won[2])
@@ -69,7 +69,7 @@ def test_autotune_picks_a_candidate_and_applies_it():
 
 def test_bad_schedule_rejected():
     with Context() as ctx:
-        for bad in [(0, 10), (0, 5000), (-1, 2304), (0, 2304, 1025), (0, 2304, -1)]:
+        for bad in [(0, 10), (0, 5000), (-1, 2304), (0, 2304, 32768), (0, 2304, -1)]:
             with pytest.raises(RuntimeError):
                 ctx.set_schedule(*bad)
         assert ctx.schedule() == (0, 2304, 0)

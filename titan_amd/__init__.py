@@ -43,7 +43,8 @@ EXPORTS = (
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
 
 LAYOUT_AUTO, LAYOUT_GATHER, LAYOUT_TILED, LAYOUT_RESIDENT = 0, 1, 2, 3
-RESIDENT_MAX = 1024
+RESIDENT_LDS_MAX = 1024   # resident bodies held in LDS (launches with few bodies)
+RESIDENT_MAX = 32767      # the resident layout's largest body
 # integrators: the reference's default Euler branch and its VERLET / RK2 build options
 INTEGRATOR_EULER, INTEGRATOR_VERLET, INTEGRATOR_RK2 = 0, 1, 2
 
@@ -217,8 +218,8 @@ class Schedule(ctypes.Structure):
 
 # titan_autotune's candidates (tiled_min_masses, tile_rows, resident_max_masses),
 # in the order of its rate_out (min -1: never tiled)
-AUTOTUNE_CANDIDATES = [(mn, rw, rs) for rs in (0, RESIDENT_MAX) for mn in (0, 128, 256, 512, 1024, -1)
-                       for rw in (682, 2304)]
+AUTOTUNE_CANDIDATES = [(mn, rw, rs) for rs in (0, RESIDENT_LDS_MAX, RESIDENT_MAX)
+                       for mn in (0, 128, 256, 512, 1024, -1) for rw in (682, 2304)]
 
 
 class Context:

@@ -456,37 +456,44 @@ int32_t tiled_rows(const titan_ctx *c) {
     return h;
 }
 
-// LDS rows per buffer of the resident kernel: the largest live resident body
-int32_t resident_rows(const titan_ctx *c) {
-    int32_t h = 0;
-    for (auto &b : c->bodies)
-        if (b.live && b.resident) h = std::max(h, (int32_t)b.n_masses);
-    return h;
-}
-
-// Whether the resident launch uses the split form: when it has no more
-// bodies than the device has compute units, each workgroup has a CU to
-// itself and its step latency is what counts (the split form spreads a
-// small body's spring terms over all four waves; its larger LDS region would
-// otherwise cost workgroups per CU).
-bool resident_split(const titan_ctx *c) {
+// Whether the resident launch has at most one body per compute unit.  Then
+// each workgroup has a CU to itself and its step latency is what counts:
+// bodies of at most kResidentMax masses step in LDS, in the split form where
+// they have it (it spreads a small body's spring terms over all four waves).
+// With more bodies every body steps in the arena, so that no workgroup
+// holds LDS: LDS sized for the largest body (and a split-form region) would
+// cost workgroups per CU.
+bool resident_few(const titan_ctx *c) {
     int64_t n = 0;
     for (auto &b : c->bodies)
         if (b.live && b.resident) ++n;
     return n <= c->n_cu;
 }
 
+// LDS rows per buffer of the resident kernel: the largest live resident body
+// of at most kResidentMax masses when the launch has few bodies, else none
+int32_t resident_rows(const titan_ctx *c) {
+    if (!resident_few(c)) return 0;
+    int32_t h = 0;
+    for (auto &b : c->bodies)
+        if (b.live && b.resident && b.n_masses <= kResidentMax) h = std::max(h, (int32_t)b.n_masses);
+    return h;
+}
+
 // The resident launch's LDS region after the state buffers: the largest
-// region a live resident body uses that fits beside them (its split-form
-// region when the launch uses that form and it has one, else its incidence
-// copy).  A body whose region is larger reads the arena.
-int32_t resident_cache(const titan_ctx *c, bool split) {
-    const int64_t avail = (int64_t)c->res_lds_max - (int64_t)resident_lds_bytes(resident_rows(c));
+// region a live resident body in LDS uses that fits beside them (its
+// split-form region when it has one, else its incidence copy).  A body
+// whose region is larger reads the arena.  None when the launch has many
+// bodies.
+int32_t resident_cache(const titan_ctx *c, bool few) {
+    if (!few) return 0;
+    const int32_t rows = resident_rows(c);
+    const int64_t avail = (int64_t)c->res_lds_max - (int64_t)resident_lds_bytes(rows);
     int32_t h = 0;
     for (auto &b : c->bodies) {
-        if (!(b.live && b.resident)) continue;
+        if (!(b.live && b.resident) || b.n_masses > rows) continue;
         int32_t r = 0;
-        if (split && b.res_scache > 0 && b.res_scache <= avail) r = b.res_scache;
+        if (b.res_scache > 0 && b.res_scache <= avail) r = b.res_scache;
         else if (b.res_cache > 0 && b.res_cache <= avail) r = b.res_cache;
         h = std::max(h, r);
     }
@@ -616,7 +623,7 @@ StepArgs make_args(titan_ctx *c, int slot, const Range &rg) {
     a.res_base = rg.r0;
     a.nres = rg.r1;
     a.res_rows = resident_rows(c);
-    a.res_split = resident_split(c) ? 1 : 0;
+    a.res_split = resident_few(c) ? 1 : 0;
     a.res_cache = resident_cache(c, a.res_split != 0);
     return a;
 }
@@ -1051,8 +1058,8 @@ PreparedBody prepare_body(const titan_schedule &sched, const titan_body_desc *d)
     TileBuild &tb = P.tb;
     bool tiled = false;
     std::vector<int32_t> nl, nr;  // endpoints in device order
-    if (d->layout == TITAN_LAYOUT_RESIDENT && (N < 1 || N > kResidentMax))
-        fail(TITAN_E_INVALID, "body_create: the resident layout holds 1 to " + std::to_string(kResidentMax) +
+    if (d->layout == TITAN_LAYOUT_RESIDENT && (N < 1 || N > kResidentMaxBody))
+        fail(TITAN_E_INVALID, "body_create: the resident layout holds 1 to " + std::to_string(kResidentMaxBody) +
                                   " masses, not " + std::to_string(N));
     const bool resident = d->layout == TITAN_LAYOUT_RESIDENT ||
                           (d->layout == TITAN_LAYOUT_AUTO && N >= 1 && N <= sched.resident_max_masses);
@@ -1806,8 +1813,9 @@ int titan_set_schedule(titan_ctx *c, const titan_schedule *s) {
     if (s->tile_rows < kChunk || s->tile_rows > kMaxTileRows)
         fail(TITAN_E_INVALID, "set_schedule: tile_rows must be in [64, " + std::to_string(kMaxTileRows) + "]");
     if (s->tiled_min_masses < 0) fail(TITAN_E_INVALID, "set_schedule: negative tiled_min_masses");
-    if (s->resident_max_masses < 0 || s->resident_max_masses > kResidentMax)
-        fail(TITAN_E_INVALID, "set_schedule: resident_max_masses must be in [0, " + std::to_string(kResidentMax) + "]");
+    if (s->resident_max_masses < 0 || s->resident_max_masses > kResidentMaxBody)
+        fail(TITAN_E_INVALID, "set_schedule: resident_max_masses must be in [0, " + std::to_string(kResidentMaxBody) +
+                                  "]");
     c->sched = *s;
     API_END(c)
 }
@@ -1899,7 +1907,7 @@ int titan_autotune(titan_ctx *c, const titan_body_desc *samples, int32_t n_sampl
     API_BEGIN
     if (!samples || n_samples <= 0 || replicas <= 0 || steps <= 0)
         fail(TITAN_E_INVALID, "autotune: need samples, replicas > 0 and steps > 0");
-    const int32_t res[] = {0, kResidentMax};
+    const int32_t res[] = {0, kResidentMax, kResidentMaxBody};
     const int64_t mins[] = {0, 128, 256, 512, 1024, INT64_MAX};
     const int32_t rows[] = {682, kMaxTileRows};
     titan_schedule win{0, kTileRows, 0};

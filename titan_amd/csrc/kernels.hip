@@ -774,11 +774,12 @@ __global__ __launch_bounds__(kBlock) void k_actuate(const StepArgs a) {
                   kChunk);
 }
 
-// Resident rollout: one workgroup per body of at most kResidentMax masses,
-// which runs the body's whole budget of steps for this call in one launch.
-// The body's state lives in LDS, double-buffered: step k reads buffer k & 1
-// and writes the other, with one barrier per step, so it never returns to
-// HBM until the last step.  Each wave walks the body's chunks w, w + 4, ...;
+// Resident rollout: one workgroup per body of at most kResidentMaxBody
+// masses, which runs the body's whole budget of steps for this call in one
+// launch.  The body's state is double-buffered, in LDS when the launch has
+// LDS rows for it (then it never returns to HBM until the last step), else
+// in its two arena state buffers: step k reads buffer k & 1 and writes the
+// other, with one barrier per step.  Each wave walks the body's chunks w, w + 4, ...;
 // every mass is computed exactly as the other kernels do (the same spring
 // walk, mass pass and integrator, T += hdt per launch), so results are
 // identical.  A body whose incidence data fits the launch's cache region
@@ -1078,41 +1079,19 @@ __device__ __forceinline__ int split_fill(const StepArgs &a, char *__restrict__ 
     return nreal;
 }
 
-// SPLIT: the launch uses the split form where a body has it.  The variant
-// without it is launched otherwise: the split form's interleaved term
-// evaluation needs more registers, which would cost the walk occupancy.
+// The steps of a resident body held in the launch's LDS: its state is
+// copied in, stepped in the two LDS buffers (in the split form, with a copy
+// of its incidence data, or reading that from the arena) and copied out to
+// the state buffer of its new step parity.
 template <bool ACT, bool SPLIT>
-__global__ __launch_bounds__(kResidentBlock) void k_rollout_resident(const StepArgs a, int final_slot) {
-    extern __shared__ double2 S[];  // two buffers of 3 x res_rows pairs, then the incidence copy
-    const int64_t rid = a.res_base + blockIdx.x;
-    if (rid >= a.nres) return;
-    const ResidentDesc *__restrict__ rd = a.rdesc + rid;
-    const int b = rd->body;
-    if (b < 0) return;
-    const int nmass = rd->nmass, chunk0 = rd->chunk0, nact = rd->nact;
-    const int64_t act0 = rd->act0, inc0 = rd->inc0;
-    const int ninc = rd->ninc, cache = rd->cache;
-    const int nch = (nmass + kChunk - 1) / kChunk;
-    const BodyKD bkd{a.b_k[b], a.b_d[b], a.b_kd[b]};
-    const int s0 = a.slot;
-    const int64_t K = a.bleft[s0][b];
-    const int64_t nst = a.bn[s0][b];
-    double T = a.bT[s0][b];
+__device__ __forceinline__ void resident_in_lds(const StepArgs &a, double2 *__restrict__ S,
+                                                const ResidentDesc *__restrict__ rd, int b, const BodyKD &bkd,
+                                                int chunk0, int nch, int nmass, int64_t inc0, int ninc, int cache,
+                                                int64_t g0, int64_t K, int64_t nst, double &T, int nact,
+                                                int64_t act0) {
     const int t = threadIdx.x;
-    if (K <= 0) {  // not stepped in this call: only its clock fields move to final_slot
-        if (t == 0) {
-            a.bT[final_slot][b] = T;
-            a.bn[final_slot][b] = nst;
-            a.bleft[final_slot][b] = 0;
-        }
-        return;
-    }
-    const int64_t g0 = (int64_t)chunk0 * kChunk;
     const int64_t n6 = a.ncap;
     double2 *__restrict__ B0 = S, *__restrict__ B1 = S + 3 * a.res_rows;
-    // actuation factors: step k reads ract[k & 1] (this block's actuators
-    // only), written before the barrier that precedes the step
-    if (ACT) actuate_slice(a, act0, nact, T, a.ract[0], t, kResidentBlock);
     {
         const double2 *__restrict__ src = reinterpret_cast<const double2 *>(a.st + (int64_t)(nst & 1) * 6 * n6);
         for (int q = t; q < nmass; q += kResidentBlock) {
@@ -1176,6 +1155,57 @@ __global__ __launch_bounds__(kResidentBlock) void k_rollout_resident(const StepA
             store_nt(&dst[3 * i + 1], fin[3 * q + 1].x, fin[3 * q + 1].y);
             store_nt(&dst[3 * i + 2], fin[3 * q + 2].x, fin[3 * q + 2].y);
         }
+    }
+}
+
+// SPLIT: the launch uses the split form where a body has it.  The variant
+// without it is launched otherwise: the split form's interleaved term
+// evaluation needs more registers, which would cost the walk occupancy.
+template <bool ACT, bool SPLIT>
+__global__ __launch_bounds__(kResidentBlock) void k_rollout_resident(const StepArgs a, int final_slot) {
+    extern __shared__ double2 S[];  // two buffers of 3 x res_rows pairs, then the incidence copy
+    const int64_t rid = a.res_base + blockIdx.x;
+    if (rid >= a.nres) return;
+    const ResidentDesc *__restrict__ rd = a.rdesc + rid;
+    const int b = rd->body;
+    if (b < 0) return;
+    const int nmass = rd->nmass, chunk0 = rd->chunk0, nact = rd->nact;
+    const int64_t act0 = rd->act0, inc0 = rd->inc0;
+    const int ninc = rd->ninc, cache = rd->cache;
+    const int nch = (nmass + kChunk - 1) / kChunk;
+    const BodyKD bkd{a.b_k[b], a.b_d[b], a.b_kd[b]};
+    const int s0 = a.slot;
+    const int64_t K = a.bleft[s0][b];
+    const int64_t nst = a.bn[s0][b];
+    double T = a.bT[s0][b];
+    const int t = threadIdx.x;
+    if (K <= 0) {  // not stepped in this call: only its clock fields move to final_slot
+        if (t == 0) {
+            a.bT[final_slot][b] = T;
+            a.bn[final_slot][b] = nst;
+            a.bleft[final_slot][b] = 0;
+        }
+        return;
+    }
+    const int64_t g0 = (int64_t)chunk0 * kChunk;
+    const int64_t n6 = a.ncap;
+    // actuation factors: step k reads ract[k & 1] (this block's actuators
+    // only), written before the barrier that precedes the step
+    if (ACT) actuate_slice(a, act0, nact, T, a.ract[0], t, kResidentBlock);
+    if (nmass > a.res_rows) {
+        // a body larger than the launch's LDS rows (every body, when the
+        // launch has none) steps in the arena: the two state buffers of its
+        // slots alternate as the LDS buffers do, and the barrier that ends
+        // a step orders its stores before the next step's loads (the
+        // workgroup shares one CU and its L1)
+        double2 *__restrict__ G0 = reinterpret_cast<double2 *>(a.st + (int64_t)(nst & 1) * 6 * n6) + 3 * g0;
+        double2 *__restrict__ G1 = reinterpret_cast<double2 *>(a.st + (int64_t)((nst + 1) & 1) * 6 * n6) + 3 * g0;
+        __syncthreads();
+        const ArenaSrc src{a};
+        resident_steps<ACT>(a, src, 0, b, bkd, chunk0, nch, g0, K, nst, T, nact, act0, G0, G1);
+    } else {
+        resident_in_lds<ACT, SPLIT>(a, S, rd, b, bkd, chunk0, nch, nmass, inc0, ninc, cache, g0, K, nst, T, nact,
+                                    act0);
     }
     if (t == 0) {
         a.bT[final_slot][b] = T;

@@ -59,12 +59,15 @@ struct TileDesc {
     int32_t pad;
 };
 
-// Resident layout (k_rollout_resident): a body of at most kResidentMax
-// masses is held whole in one workgroup's LDS, double-buffered, for every
-// step of a stepping call; its incidences name the other endpoint by its
-// index in the body (uint16, as the tiled layout's LDS rows, with the whole
-// body as one tile and no halo).
+// Resident layout (k_rollout_resident): one workgroup runs every step of a
+// stepping call for one body of at most kResidentMaxBody masses.  Its
+// incidences name the other endpoint by its index in the body (uint16, bit
+// 15 reserved, as the tiled layout's LDS rows, with the whole body as one
+// tile and no halo).  A body of at most kResidentMax masses is held in the
+// workgroup's LDS, double-buffered, when the launch has at most one body
+// per compute unit; otherwise the body steps in its arena state buffers.
 constexpr int kResidentMax = 1024;
+constexpr int kResidentMaxBody = 32767;
 constexpr int kResidentBlock = 256;
 
 struct ResidentDesc {
