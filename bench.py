@@ -86,14 +86,15 @@ def cpu_baseline(scene, budget_s: float = 15.0) -> dict:
             "ms_per_step": 1e3 * dt / n}
 
 
-def bench_batched(n_bodies: int, seed: int, rollout: int, sample: int) -> dict:
+def bench_batched(n_bodies: int, seed: int, rollout: int, sample: int, schedule=None) -> dict:
     """Batched uneven bodies: a robot population (lattices of side U{3..12},
     30% of springs actuated, each body with its own contact plane, dt 1e-4)
     stepped in one launch per step with graph-captured rollouts, against the
     same GPU stepping one body per launch.  The schedule (which bodies are
     tiled, the tile row budget) is picked by the library's autotuner on a
-    sample of the population.  The one-body-per-launch figure is measured on
-    the first `sample` bodies; both runs are checked to give identical states."""
+    sample of the population, unless `schedule` gives it.  The
+    one-body-per-launch figure is measured on the first `sample` bodies (none
+    when 0); both runs are checked to give identical states."""
     import numpy as np
     from titan_amd import Context, scenes
 
@@ -107,10 +108,16 @@ def bench_batched(n_bodies: int, seed: int, rollout: int, sample: int) -> dict:
     with Context() as ctx:
         ctx.set_dt(sc.dt)
         ctx.set_global_acc(sc.g)
-        t0 = time.perf_counter()
-        best, rates = ctx.autotune(tune_set, replicas=8, steps=20)
-        log(f"batched: autotune {time.perf_counter() - t0:.1f} s -> tiled_min_masses {best[0]}, "
-            f"tile_rows {best[1]}, resident_max_masses {best[2]}")
+        if schedule is None:
+            t0 = time.perf_counter()
+            best, rates = ctx.autotune(tune_set, replicas=8, steps=20)
+            log(f"batched: autotune {time.perf_counter() - t0:.1f} s -> tiled_min_masses {best[0]}, "
+                f"tile_rows {best[1]}, resident_max_masses {best[2]}")
+        else:
+            best, rates = tuple(schedule), []
+            ctx.set_schedule(*best)
+            log(f"batched: given schedule tiled_min_masses {best[0]}, tile_rows {best[1]}, "
+                f"resident_max_masses {best[2]}")
         t0 = time.perf_counter()
         ids = [ctx.add_body(**b) for b in sc.bodies]
         ctx.sync()
@@ -123,6 +130,18 @@ def bench_batched(n_bodies: int, seed: int, rollout: int, sample: int) -> dict:
         del ids
     rate_batched = n_m * rollout / (ms_batched * 1e-3)
     log(f"batched: {rollout}-step rollout {ms_batched:.2f} ms, {rate_batched:.3e} mass-steps/s")
+    out = {
+        "workload": f"{n_bodies} lattice robots, side U{{3..12}} (seed {seed}), {n_m} masses, {n_s} springs, "
+                    f"{sc.n_actuated} actuated (a~U(0,0.2), omega~U(1,20), phi~U(0,2pi)), own contact plane "
+                    f"each, dt=1e-4, FP64, {rollout}-step graph-captured rollouts, one launch per step",
+        "value": rate_batched, "unit": "mass-steps/s", "ms_per_rollout": ms_batched,
+        "schedule": {"tiled_min_masses": best[0], "tile_rows": best[1], "resident_max_masses": best[2],
+                     "autotune": [{"tiled_min_masses": c[0], "tile_rows": c[1], "resident_max_masses": c[2],
+                                   "mass_steps_per_s": r}
+                                  for c, r in rates]},
+    }
+    if sample <= 0:
+        return out
 
     # the same schedule, the sample bodies: batched vs one body per launch, same states
     def run(one_per_launch: bool):
@@ -151,14 +170,7 @@ def bench_batched(n_bodies: int, seed: int, rollout: int, sample: int) -> dict:
     log(f"batched: one body per launch on {sample} bodies ({sub_m} masses): {ms_one:.1f} ms, "
         f"{rate_one:.3e} mass-steps/s; identical states: {same}")
     return {
-        "workload": f"{n_bodies} lattice robots, side U{{3..12}} (seed {seed}), {n_m} masses, {n_s} springs, "
-                    f"{sc.n_actuated} actuated (a~U(0,0.2), omega~U(1,20), phi~U(0,2pi)), own contact plane "
-                    f"each, dt=1e-4, FP64, {rollout}-step graph-captured rollouts, one launch per step",
-        "value": rate_batched, "unit": "mass-steps/s", "ms_per_rollout": ms_batched,
-        "schedule": {"tiled_min_masses": best[0], "tile_rows": best[1], "resident_max_masses": best[2],
-                     "autotune": [{"tiled_min_masses": c[0], "tile_rows": c[1], "resident_max_masses": c[2],
-                                   "mass_steps_per_s": r}
-                                  for c, r in rates]},
+        **out,
         "one_body_per_launch": {"value": rate_one, "unit": "mass-steps/s",
                                 "sample": f"first {sample} bodies ({sub_m} masses), {rollout} steps each, "
                                           f"one launch per body per step over its own tiles or chunks"},
@@ -349,6 +361,11 @@ def main() -> int:
     ap.add_argument("--only", choices=["lattice", "readme", "batched", "serving", "irregular"], default=None,
                     help="run one workload (its figures go in the same JSON line)")
     ap.add_argument("--bodies", type=int, default=4096, help="batched workload: robot count")
+    ap.add_argument("--schedule", default=None, metavar="MIN,ROWS,RES",
+                    help="batched workload: this schedule (tiled_min_masses,tile_rows,resident_max_masses) "
+                         "instead of the autotuner's")
+    ap.add_argument("--per-body-sample", type=int, default=512,
+                    help="batched workload: bodies timed one per launch for the comparison (0: none)")
     ap.add_argument("--requests", type=int, default=256, help="serving workload: requests per arrival rate")
     ap.add_argument("--lambdas", default="50,200,1000,2000",
                     help="serving workload: Poisson arrival rates (requests/s), comma-separated")
@@ -387,7 +404,9 @@ def main() -> int:
     if args.only == "readme" or (args.only is None and not args.no_extra):
         extra["readme"] = bench_readme(10000)
     if args.only == "batched" or (args.only is None and not args.no_extra):
-        extra["batched"] = bench_batched(args.bodies, args.seed, rollout=100, sample=min(512, args.bodies))
+        sched = tuple(int(x) for x in args.schedule.split(",")) if args.schedule else None
+        extra["batched"] = bench_batched(args.bodies, args.seed, rollout=100,
+                                         sample=min(args.per_body_sample, args.bodies), schedule=sched)
     if args.only == "serving" or (args.only is None and not args.no_extra):
         lams = tuple(float(x) for x in args.lambdas.split(","))
         extra["serving"] = bench_serving(args.seed, n_req=args.requests, lams=lams,
