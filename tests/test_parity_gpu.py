@@ -18,7 +18,7 @@ from helpers import normwise_rel, to_oracle
 
 pytestmark = pytest.mark.gpu
 LAYOUTS = pytest.mark.parametrize("layout", [LAYOUT_GATHER, LAYOUT_TILED], ids=["gather", "tiled"])
-# bodies of at most 1024 masses also run in the resident layout
+# the small bodies of these cases also run in the resident layout
 LAYOUTS3 = pytest.mark.parametrize("layout", [LAYOUT_GATHER, LAYOUT_TILED, LAYOUT_RESIDENT],
                                    ids=["gather", "tiled", "resident"])
 
