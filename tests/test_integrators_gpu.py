@@ -111,6 +111,36 @@ def test_actuated_springs_within_tolerance_and_alike_across_layouts(integ):
     assert normwise_rel(runs[0]["vel"], orc.vel) <= 1e-10
 
 
+@pytest.mark.parametrize("integ", [INTEGRATOR_EULER, INTEGRATOR_VERLET, INTEGRATOR_RK2],
+                         ids=["euler", "verlet", "rk2"])
+def test_resident_bodies_in_the_arena_bitexact(integ):
+    # more resident bodies than compute units: each steps in its arena state
+    # buffers, under each integrator
+    rng = np.random.default_rng(9)
+    bodies = []
+    for q in range(300):
+        b = scenes.lattice_body((2.0 * (q % 20), 2.0 * (q // 20), 0.6), (1.0, 1.0, 1.0), 3 + q % 4, damping=0.1)
+        b["pos"] = b["pos"] + rng.normal(0, 1e-3, b["pos"].shape)
+        b["vel"] = rng.normal(0, 0.3, b["pos"].shape)
+        bodies.append(b)
+    sc = scenes.Scene(bodies=bodies, planes=[(0.0, 0.0, 1.0, 0.0, 0.8, 1.0)], dt=1e-4)
+    calls = [1, 64, 33]
+    _say(f"case integrator {integ}: 300 resident bodies (27..216 masses), calls {calls}")
+    with Context() as ctx:
+        ctx.set_integrator(integ)
+        ids = sc.load(ctx, layout=LAYOUT_RESIDENT)
+        for n in calls:
+            ctx.step(n)
+        got = [ctx.download(i, acc=True, T=True) for i in ids]
+    for q in range(0, 300, 23):
+        o = to_oracle(scenes.Scene(bodies=[bodies[q]], planes=sc.planes, dt=sc.dt), integrator=integ)
+        o.run(sum(calls))
+        np.testing.assert_array_equal(got[q]["pos"], o.pos)
+        np.testing.assert_array_equal(got[q]["vel"], o.vel)
+        np.testing.assert_array_equal(got[q]["acc"], o.acc)
+        np.testing.assert_array_equal(got[q]["T"], o.T)
+
+
 def test_rk2_budgets_clock_and_breakpoint():
     a = scenes.readme_lattice(2.45)
     c = scenes.lattice_body((-9.0, 0.0, 2.2), (4.0, 4.0, 4.0), (9, 9, 9), damping=0.2)
