@@ -410,7 +410,7 @@ def main() -> int:
     if args.only == "serving" or (args.only is None and not args.no_extra):
         lams = tuple(float(x) for x in args.lambdas.split(","))
         extra["serving"] = bench_serving(args.seed, n_req=args.requests, lams=lams,
-                                         quantum=64, n_base=16)
+                                         quantum=256, n_base=16)
     if args.only == "irregular" or (args.only is None and not args.no_extra):
         extra["irregular"] = bench_irregular(2_000_000, 1000, args.seed, steps=500, warmup=20)
     if args.only in ("readme", "batched", "serving", "irregular"):

@@ -252,7 +252,7 @@ typedef struct titan_serve_report {
 
 /* max_running: requests on the device at once (>= 1); hbm_bytes: serving
  * budget (0: the context's budget, or unlimited); quantum: steps per pump
- * (>= 1).  Defaults: 4096, 0, 64. */
+ * (>= 1).  Defaults: 4096, 0, 256. */
 int titan_serve_config(titan_ctx *ctx, int32_t max_running, uint64_t hbm_bytes, int64_t quantum);
 int titan_serve_submit(titan_ctx *ctx, const titan_request *req, int64_t *ticket);
 /* One round of admit / step / retire; *running and *queued (may be NULL)

@@ -326,7 +326,7 @@ class Context:
                 list(zip(AUTOTUNE_CANDIDATES, list(rates))))
 
     # -- serving --
-    def serve_config(self, max_running: int = 4096, hbm_bytes: int = 0, quantum: int = 64):
+    def serve_config(self, max_running: int = 4096, hbm_bytes: int = 0, quantum: int = 256):
         self._chk(self._L.titan_serve_config(self._h, int(max_running), int(hbm_bytes), int(quantum)))
 
     def serve_submit(self, body: dict, horizon: int, want_result: bool = True) -> int:

@@ -2081,4 +2081,10 @@ int commit_body_device(titan_ctx *c, Prepared *p, int32_t *id) {
 
 void free_prepared(Prepared *p) { delete p; }
 
+int record_step_event(titan_ctx *c, hipEvent_t ev) {
+    API_BEGIN
+    HIPCHK(hipEventRecord(ev, c->stream));
+    API_END(c)
+}
+
 }  // namespace titan_mi355x

@@ -3,6 +3,8 @@
 
 #include <string>
 
+#include <hip/hip_runtime_api.h>
+
 #include "titan_mi355x.h"
 
 namespace titan_mi355x {
@@ -22,5 +24,8 @@ __attribute__((visibility("hidden"))) Prepared *prepare_body_host(const titan_ct
 // on failure returns the error code with the context's last error set.
 __attribute__((visibility("hidden"))) int commit_body_device(titan_ctx *ctx, Prepared *p, int32_t *id);
 __attribute__((visibility("hidden"))) void free_prepared(Prepared *p);
+
+// Records `ev` on the context's step stream, after the work enqueued so far.
+__attribute__((visibility("hidden"))) int record_step_event(titan_ctx *ctx, hipEvent_t ev);
 
 }  // namespace titan_mi355x

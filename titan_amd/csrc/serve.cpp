@@ -155,19 +155,37 @@ unsigned host_threads() {
     return std::max(1u, std::min(16u, hw ? hw : 1u));
 }
 
+// A request whose last steps are enqueued and whose result nobody reads: it
+// completes when the step stream passes its event.
+struct Finishing {
+    Request *r;
+    hipEvent_t ev;
+};
+
 struct Server {
     int32_t max_running = 4096;
     uint64_t hbm_bytes = 0;
-    int64_t quantum = 64;
+    int64_t quantum = 256;
     int64_t next_ticket = 1;
     std::map<int64_t, std::unique_ptr<Request>> reqs;
     std::deque<Request *> queue;
     std::vector<Request *> running;
+    std::deque<Finishing> finishing;
+    std::vector<hipEvent_t> spare;  // events for reuse
     uint64_t running_bytes = 0;
     // statistics
     int64_t submitted = 0, completed = 0, cancelled = 0, failed = 0;
     std::vector<double> latencies;
     double mass_steps = 0, first_submit = -1, last_done = -1;
+
+    // called when the context is destroyed (not from a destructor, which
+    // may run after the HIP runtime's own teardown at process exit)
+    void release_events() {
+        for (auto &f : finishing) (void)hipEventDestroy(f.ev);
+        for (hipEvent_t e : spare) (void)hipEventDestroy(e);
+        finishing.clear();
+        spare.clear();
+    }
 };
 
 std::mutex g_mu;
@@ -196,16 +214,55 @@ void finish(Server &sv, Request *r, int32_t state, double t) {
     }
 }
 
+// Retire a request whose steps are all enqueued.  With a result to read,
+// the download waits for the steps, and the request completes then.
+// Without one, the body is released at once (its slots are reused only
+// after the steps in flight) and the request completes when the host sees
+// the step stream pass an event recorded after its last step, so that its
+// completion time is never earlier than the device's.
 int retire(titan_ctx *ctx, Server &sv, Request *r) {
     if (r->out_pos || r->out_vel) {
         const int rc = titan_body_download(ctx, r->body, r->out_pos, r->out_vel, nullptr, nullptr);
         if (rc != TITAN_OK) return rc;
     }
+    hipEvent_t ev = nullptr;
+    if (!(r->out_pos || r->out_vel)) {
+        if (!sv.spare.empty()) {
+            ev = sv.spare.back();
+            sv.spare.pop_back();
+        } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            return TITAN_E_HIP;
+        }
+        const int rc = titan_mi355x::record_step_event(ctx, ev);
+        if (rc != TITAN_OK) {
+            sv.spare.push_back(ev);
+            return rc;
+        }
+    }
     const int rc = titan_body_destroy(ctx, r->body);
-    if (rc != TITAN_OK) return rc;
+    if (rc != TITAN_OK) {
+        if (ev) sv.spare.push_back(ev);
+        return rc;
+    }
     r->body = -1;
     sv.running_bytes -= r->est_bytes;
-    finish(sv, r, TITAN_REQ_DONE, now_s());
+    if (ev) sv.finishing.push_back({r, ev});
+    else finish(sv, r, TITAN_REQ_DONE, now_s());
+    return TITAN_OK;
+}
+
+// Complete the finishing requests whose events the step stream has passed
+// (all of them, waiting, when `wait`); they pass in the order retired.
+int observe(Server &sv, bool wait) {
+    while (!sv.finishing.empty()) {
+        Finishing f = sv.finishing.front();
+        const hipError_t e = wait ? hipEventSynchronize(f.ev) : hipEventQuery(f.ev);
+        if (e == hipErrorNotReady) return TITAN_OK;
+        if (e != hipSuccess) return TITAN_E_HIP;
+        sv.finishing.pop_front();
+        sv.spare.push_back(f.ev);
+        finish(sv, f.r, TITAN_REQ_DONE, now_s());
+    }
     return TITAN_OK;
 }
 
@@ -214,7 +271,10 @@ int retire(titan_ctx *ctx, Server &sv, Request *r) {
 // Called by titan_ctx_destroy (internal to the library).
 extern "C" __attribute__((visibility("hidden"))) void titan_internal_serve_forget(const titan_ctx *ctx) {
     std::lock_guard<std::mutex> lk(g_mu);
-    g_servers.erase(ctx);
+    auto it = g_servers.find(ctx);
+    if (it == g_servers.end()) return;
+    it->second->release_events();
+    g_servers.erase(it);
 }
 
 static int serve_config(titan_ctx *ctx, int32_t max_running, uint64_t hbm_bytes, int64_t quantum) {
@@ -298,7 +358,8 @@ static int serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
         }
         return TITAN_OK;
     };
-    int rc = TITAN_OK;
+    int rc = observe(sv, false);
+    if (rc != TITAN_OK) return rc;
     if (sv.running.empty()) {  // nothing on the device: a first batch starts it
         rc = admit(host_threads());
         if (rc != TITAN_OK) return rc;
@@ -332,7 +393,10 @@ static int serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
     }
     still.insert(still.end(), fresh.begin(), fresh.end());
     sv.running.swap(still);
-    if (n_running) *n_running = (int32_t)sv.running.size();
+    // nothing left to step: wait for the last finishing requests
+    rc = observe(sv, sv.running.empty() && sv.queue.empty());
+    if (rc != TITAN_OK) return rc;
+    if (n_running) *n_running = (int32_t)(sv.running.size() + sv.finishing.size());
     if (n_queued) *n_queued = (int32_t)sv.queue.size();
     return TITAN_OK;
 }
@@ -355,6 +419,11 @@ static int serve_cancel(titan_ctx *ctx, int64_t ticket) {
     Request *r = it->second.get();
     if (r->state == TITAN_REQ_QUEUED) {
         sv.queue.erase(std::find(sv.queue.begin(), sv.queue.end(), r));
+    } else if (r->state == TITAN_REQ_RUNNING && r->body < 0) {  // finishing: its body is released
+        auto f = std::find_if(sv.finishing.begin(), sv.finishing.end(),
+                              [&](const Finishing &x) { return x.r == r; });
+        sv.spare.push_back(f->ev);
+        sv.finishing.erase(f);
     } else if (r->state == TITAN_REQ_RUNNING) {
         const int rc = titan_body_destroy(ctx, r->body);
         if (rc != TITAN_OK) return rc;
@@ -376,7 +445,7 @@ static int serve_stats(titan_ctx *ctx, titan_serve_report *out) {
     out->completed = sv.completed;
     out->cancelled = sv.cancelled;
     out->failed = sv.failed;
-    out->running = (int64_t)sv.running.size();
+    out->running = (int64_t)(sv.running.size() + sv.finishing.size());
     out->queued = (int64_t)sv.queue.size();
     out->mass_steps = sv.mass_steps;
     if (!sv.latencies.empty()) {
