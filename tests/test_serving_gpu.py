@@ -146,3 +146,29 @@ def test_cancel_and_stats():
         assert 0 < s["p50_latency_s"] <= s["p99_latency_s"] <= s["max_latency_s"]
         ctx.serve_reset_stats()
         assert ctx.serve_stats()["completed"] == 0
+
+
+def test_requests_without_results_complete_after_their_steps():
+    # a request whose result nobody reads is released at once but completes
+    # only when the step stream has run its last steps; cancelling it in
+    # between loses nothing
+    bodies, horizons, planes = _requests()
+    with Context() as ctx:
+        ctx.set_dt(1e-4)
+        ctx.serve_config(quantum=16)
+        t = [ctx.serve_submit(b, h, want_result=False) for b, h in zip(bodies, horizons)]
+        while ctx.serve_pump() != (0, 0):
+            pass
+        s = ctx.serve_stats()
+        assert (s["completed"], s["running"], s["queued"]) == (len(t), 0, 0)
+        assert all(ctx.serve_poll(x)[0] == REQ_DONE for x in t)
+        assert s["mass_steps"] == sum(len(b["pos"]) * h for b, h in zip(bodies, horizons))
+        late = ctx.serve_submit(bodies[0], 5, want_result=False)
+        ctx.serve_pump()  # its five steps are enqueued and it is retired
+        assert ctx.serve_poll(late)[0] in (REQ_RUNNING, REQ_DONE)
+        ctx.serve_cancel(late)
+        assert ctx.serve_poll(late)[0] in (REQ_DONE, REQ_CANCELLED)
+        while ctx.serve_pump() != (0, 0):
+            pass
+        s = ctx.serve_stats()
+        assert s["completed"] + s["cancelled"] == len(t) + 1 and s["running"] == 0
