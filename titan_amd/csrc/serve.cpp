@@ -214,6 +214,16 @@ void finish(Server &sv, Request *r, int32_t state, double t) {
     }
 }
 
+// An event from the spare list, or a new one.
+int take_event(Server &sv, hipEvent_t *ev) {
+    if (!sv.spare.empty()) {
+        *ev = sv.spare.back();
+        sv.spare.pop_back();
+        return TITAN_OK;
+    }
+    return hipEventCreateWithFlags(ev, hipEventDisableTiming) == hipSuccess ? TITAN_OK : TITAN_E_HIP;
+}
+
 // Retire a request whose steps are all enqueued.  With a result to read,
 // the download waits for the steps, and the request completes then.
 // Without one, the body is released at once (its slots are reused only
@@ -227,12 +237,8 @@ int retire(titan_ctx *ctx, Server &sv, Request *r) {
     }
     hipEvent_t ev = nullptr;
     if (!(r->out_pos || r->out_vel)) {
-        if (!sv.spare.empty()) {
-            ev = sv.spare.back();
-            sv.spare.pop_back();
-        } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
-            return TITAN_E_HIP;
-        }
+        const int rt = take_event(sv, &ev);
+        if (rt != TITAN_OK) return rt;
         const int rc = titan_mi355x::record_step_event(ctx, ev);
         if (rc != TITAN_OK) {
             sv.spare.push_back(ev);
@@ -316,9 +322,14 @@ static int serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
     std::vector<Request *> fresh;  // admitted during this pump's quantum: they join the next one
     // admit, in arrival order, while the running set has room: up to `cap`
     // requests, laid out together on the host, then placed one by one
-    auto admit = [&](size_t cap) -> int {
+    // `until`: stop (between batches) once the step stream has passed this
+    // event, so that the device does not wait for admission
+    auto admit = [&](size_t cap, hipEvent_t until) -> int {
         const unsigned nt = host_threads();
+        bool first = true;
         while (!sv.queue.empty() && cap > 0) {
+            if (until && !first && hipEventQuery(until) == hipSuccess) break;
+            first = false;
             std::vector<Request *> pick;
             uint64_t bytes = sv.running_bytes;
             size_t n_run = sv.running.size() + fresh.size();
@@ -361,7 +372,7 @@ static int serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
     int rc = observe(sv, false);
     if (rc != TITAN_OK) return rc;
     if (sv.running.empty()) {  // nothing on the device: a first batch starts it
-        rc = admit(host_threads());
+        rc = admit(host_threads(), nullptr);
         if (rc != TITAN_OK) return rc;
         sv.running.swap(fresh);
     }
@@ -378,8 +389,19 @@ static int serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
         if (rc != TITAN_OK) return rc;
         for (size_t q = 0; q < ids.size(); ++q) sv.running[q]->done += steps[q];
     }
-    // admit more while the device works (their layouts are built on the host meanwhile)
-    rc = admit(SIZE_MAX);
+    // admit more while the device works (their layouts are built on the host
+    // meanwhile), until it has run the steps just enqueued
+    hipEvent_t stepped = nullptr;
+    if (!sv.running.empty()) {
+        rc = take_event(sv, &stepped);
+        if (rc == TITAN_OK) rc = titan_mi355x::record_step_event(ctx, stepped);
+        if (rc != TITAN_OK) {
+            if (stepped) sv.spare.push_back(stepped);
+            return rc;
+        }
+    }
+    rc = admit(SIZE_MAX, stepped);
+    if (stepped) sv.spare.push_back(stepped);
     if (rc != TITAN_OK) return rc;
     // retire the finished ones (their downloads wait for the quantum)
     std::vector<Request *> still;
