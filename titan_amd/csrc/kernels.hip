@@ -1051,6 +1051,7 @@ __device__ __forceinline__ void resident_steps_split(const StepArgs &a, const ui
 // (an exclusive scan of the degrees), then copies of the springs' endpoint
 // and incidence words, rest lengths, k and damping from the body's
 // incidence span.  Returns the incidence count.
+static_assert(kResidentBlock == kSplitMaxChunks * kChunk, "one thread per row of a split-form body");
 __device__ __forceinline__ int split_fill(const StepArgs &a, char *__restrict__ base, int nch, int64_t g0,
                                           int64_t e0, int nsp, int nd) {
     const int t = threadIdx.x, wave = t >> 6, lane = t & (kChunk - 1);
