@@ -229,7 +229,11 @@ int titan_autotune(titan_ctx *ctx, const titan_body_desc *samples, int32_t n_sam
  * retires the finished ones: their final positions and velocities go to the
  * request's out buffers and their device memory is released.  A request's
  * result does not depend on what runs beside it.  Latency is measured from
- * submit to retirement on the host's steady clock. */
+ * submit to completion on the host's steady clock: for a request with out
+ * buffers, when they are written; for one without, when a pump sees that
+ * the device has run its last step (until then it counts as running).
+ * Admission within a pump stops, between batches, once the device has run
+ * the steps that pump enqueued. */
 #define TITAN_REQ_QUEUED 0
 #define TITAN_REQ_RUNNING 1
 #define TITAN_REQ_DONE 2
@@ -247,7 +251,7 @@ typedef struct titan_serve_report {
     int64_t submitted, completed, cancelled, failed, running, queued;
     double p50_latency_s, p99_latency_s, max_latency_s;  /* over completed requests */
     double mass_steps;          /* of completed requests */
-    double mass_steps_per_s;    /* mass_steps / (last retirement - first submit) */
+    double mass_steps_per_s;    /* mass_steps / (last completion - first submit) */
 } titan_serve_report;
 
 /* max_running: requests on the device at once (>= 1); hbm_bytes: serving
