@@ -192,10 +192,12 @@ int titan_get_integrator(const titan_ctx *ctx, int32_t *kind);
  * layout; of the others, those with at least tiled_min_masses masses get the
  * tiled layout and smaller ones the gather layout; tile_rows bounds a tile's
  * LDS rows (masses + halo, 64 .. 2304; fewer rows allow more workgroups per
- * CU, more rows bigger tiles).  Default: resident_max_masses 0,
- * tiled_min_masses 0, tile_rows 2304 (every body tiled: one workgroup per
- * resident body favours a few small scenes, the tiled launch a population;
- * titan_autotune measures both).  Applies to bodies created afterwards. */
+ * CU, more rows bigger tiles).  Default: resident_max_masses 256,
+ * tiled_min_masses 0, tile_rows 2304 (a body of at most 256 masses is
+ * resident: a few small scenes step fastest so, each in one launch per
+ * stepping call with its spring terms spread over a workgroup; every larger
+ * body is tiled; for a population titan_autotune measures the
+ * alternatives).  Applies to bodies created afterwards. */
 typedef struct titan_schedule {
     int64_t tiled_min_masses;
     int32_t tile_rows;

@@ -283,7 +283,8 @@ def test_auto_layout_by_size():
     big = scenes.lattice_body((0.0, 0.0, 0.55), (1.2, 1.2, 1.2), 11, damping=0.1)  # 1331 masses
     with Context() as ctx:
         (a,) = small.load(ctx)
-        assert ctx.body_layout(a) == LAYOUT_TILED  # the default schedule tiles every body
+        assert ctx.body_layout(a) == LAYOUT_RESIDENT  # the default: at most 256 masses resident
+        assert ctx.body_layout(ctx.add_body(**scenes.lattice_body((0, 0, 1), (1, 1, 1), 7))) == LAYOUT_TILED
         ctx.set_schedule(0, 2304, 1024)  # bodies of at most 1024 masses resident
         assert ctx.body_layout(ctx.add_body(**small.bodies[0])) == LAYOUT_RESIDENT
         assert ctx.body_layout(ctx.add_body(**big)) == LAYOUT_TILED

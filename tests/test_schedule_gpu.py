@@ -72,4 +72,4 @@ def test_bad_schedule_rejected():
         for bad in [(0, 10), (0, 5000), (-1, 2304), (0, 2304, 32768), (0, 2304, -1)]:
             with pytest.raises(RuntimeError):
                 ctx.set_schedule(*bad)
-        assert ctx.schedule() == (0, 2304, 0)
+        assert ctx.schedule() == (0, 2304, 256)

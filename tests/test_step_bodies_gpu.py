@@ -54,7 +54,8 @@ def test_step_bodies_own_counts_then_all(layout):
     print(f"case step_bodies layout {layout}: masses {[len(b['pos']) for b in sc.bodies]}", flush=True)
     with Context() as ctx:
         ids = sc.load(ctx, layout=layout)
-        want = MIXED if layout == MIXED else [LAYOUT_TILED] * 3 if layout == 0 else [layout] * 3
+        auto = [LAYOUT_RESIDENT, LAYOUT_RESIDENT, LAYOUT_TILED]  # the default schedule: <= 256 masses resident
+        want = MIXED if layout == MIXED else auto if layout == 0 else [layout] * 3
         assert [ctx.body_layout(i) for i in ids] == want
         # several bodies per launch, uneven budgets; body 1 listed with 0
         ctx.step_bodies([ids[0], ids[1], ids[2]], [7, 0, 13])

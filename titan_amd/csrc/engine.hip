@@ -133,7 +133,9 @@ struct titan_ctx {
 
     Ranges chunks, incs, acts, lcs, bids, tiles, halos, rdescs;
     std::vector<BodyH> bodies;
-    titan_schedule sched{0, kTileRows, 0};  // every body tiled unless a schedule says otherwise
+    // bodies of at most four chunks resident (alone or among few, one launch
+    // per stepping call with the split form), every larger body tiled
+    titan_schedule sched{0, kTileRows, kSplitMaxChunks * kChunk};
 
     // mass arena (ncap = chunks.cap() * 64)
     int64_t ncap = 0;
