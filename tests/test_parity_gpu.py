@@ -214,6 +214,33 @@ def test_upload_download_roundtrip_and_resume():
     np.testing.assert_array_equal(got["vel"], o.vel)
 
 
+@LAYOUTS3
+def test_mass_upload_changes_uniform_masses(layout):
+    # a body whose masses all share m passes it by value to the tiled step;
+    # uploading unequal masses (and later equal ones again) must be seen
+    sc = scenes.readme_lattice(2.45)
+    n = len(sc.bodies[0]["pos"])
+    o = to_oracle(sc)
+    m1 = 0.1 + 0.01 * (np.arange(n) % 7)
+    m2 = np.full(n, 0.25)
+    with Context() as ctx:
+        (bid,) = sc.load(ctx, layout=layout)
+        ctx.step(40)
+        ctx.upload(bid, m=m1)
+        ctx.step(40)
+        ctx.upload(bid, m=m2)
+        ctx.step(40)
+        got = ctx.download(bid, acc=True)
+    o.run(40)
+    o.m = np.array(m1, dtype=np.float64)
+    o.run(40)
+    o.m = np.array(m2, dtype=np.float64)
+    o.run(40)
+    np.testing.assert_array_equal(got["pos"], o.pos)
+    np.testing.assert_array_equal(got["vel"], o.vel)
+    np.testing.assert_array_equal(got["acc"], o.acc)
+
+
 @LAYOUTS
 def test_million_mass_lattice_bitexact_few_steps(layout):
     sc = scenes.big_lattice(100, seed=0)

@@ -100,6 +100,7 @@ struct BodyH {
     bool live = false;
     bool tiled = false;
     bool resident = false;
+    uint8_t kd = 0;    // its b_kd bits
     int64_t res = -1;  // resident descriptor (resident layout)
     int32_t res_cache = 0;  // LDS bytes of its incidence copy (resident layout; 0: none)
     int32_t res_scache = 0;  // LDS bytes of its split-form region (0: none)
@@ -166,6 +167,7 @@ struct titan_ctx {
     int64_t *bn[2] = {nullptr, nullptr}, *bleft[2] = {nullptr, nullptr}, *bnsync = nullptr;
     int32_t *b_chunk0 = nullptr, *b_lc = nullptr, *b_lcn = nullptr;
     double *b_k = nullptr, *b_d = nullptr;  // the body's k / damping when all its springs share it
+    double *b_m = nullptr;                  // the body's mass value when all its masses share it
     uint8_t *b_kd = nullptr;                 // bit 0: k per incidence, bit 1: damping per incidence
     double *planes = nullptr, *balls = nullptr;
     int64_t planes_cap = 0, balls_cap = 0;
@@ -376,6 +378,7 @@ void ensure_bodies(titan_ctx *c, int64_t need) {
     grow_soa(c, c->b_mode, 1, old, nc);
     grow_soa(c, c->b_k, 1, old, nc);
     grow_soa(c, c->b_d, 1, old, nc);
+    grow_soa(c, c->b_m, 1, old, nc);
     grow_soa(c, c->b_kd, 1, old, nc);
     int32_t *q = dalloc<int32_t>(c, nc);
     HIPCHK(hipMemsetAsync(q, 0xff, sizeof(int32_t) * nc, c->stream));
@@ -578,6 +581,7 @@ StepArgs make_args(titan_ctx *c, int slot, const Range &rg) {
     a.inc_k = c->inc_k;
     a.inc_damp = c->inc_damp;
     a.b_k = c->b_k;
+    a.b_m = c->b_m;
     a.b_d = c->b_d;
     a.b_kd = c->b_kd;
     a.act = c->act;
@@ -903,6 +907,8 @@ struct PreparedBody {
     bool per_k = false, per_d = false, any_lc = false, shared = false, tiled = false, resident = false;
     bool any_ext = false, any_drag = false;
     double k0 = 0, d0 = 0, t0 = 0;
+    bool m_same = false;  // every mass of the body has m = m0
+    double m0 = 0;
     std::vector<double> acth;       // 4 per actuator: a, omega, phi, rest
     std::vector<int32_t> perm;      // device order -> caller's order (empty = identity)
     TileBuild tb;                   // tiled layout, body-relative
@@ -1164,6 +1170,10 @@ PreparedBody prepare_body(const titan_schedule &sched, const titan_body_desc *d)
         }
         P.fl[q] = f;
     }
+    // a body whose masses all share m passes it by value to the tiled step
+    P.m0 = d->m[0];
+    P.m_same = true;
+    for (int64_t i = 1; i < N && P.m_same; ++i) P.m_same = same_bits(d->m[i], P.m0);
     if (P.any_ext) {
         P.ext.assign(3 * npad, 0.0);
         for (int64_t q = 0; q < N; ++q)
@@ -1209,6 +1219,7 @@ BodyTable body_table(titan_ctx *c) {
     t.b_mode = c->b_mode;
     t.b_k = c->b_k;
     t.b_d = c->b_d;
+    t.b_m = c->b_m;
     t.b_kd = c->b_kd;
     return t;
 }
@@ -1404,11 +1415,12 @@ int32_t commit_body(titan_ctx *c, PreparedBody &P) {
     bi.res = resident ? rs0 : -1;
     bi.t0 = P.t0;
     bi.k = P.k0;
+    bi.m = P.m0;
     bi.d = P.d0;
     bi.lcb = (int32_t)(P.shared ? lc0 : 0);
     bi.lcn = (int32_t)(P.shared ? n_lc : 0);
     bi.mode = resident ? kModeResident : tiled ? kModeTiled : kModeGather;
-    bi.kd = (uint8_t)((P.per_k ? 1 : 0) | (P.per_d ? 2 : 0));
+    bi.kd = (uint8_t)((P.per_k ? 1 : 0) | (P.per_d ? 2 : 0) | (P.m_same ? kKdUniformM : 0));
     bi.on = 1;
     HIPCHK(launch_set_body(body_table(c), bi, c->stream));
 
@@ -1417,6 +1429,7 @@ int32_t commit_body(titan_ctx *c, PreparedBody &P) {
     b.live = true;
     b.tiled = tiled;
     b.resident = resident;
+    b.kd = bi.kd;
     b.res = rs0;
     b.res_cache = resident ? (int32_t)cache : 0;
     b.res_scache = resident ? (int32_t)scache : 0;
@@ -1548,7 +1561,17 @@ void body_upload(titan_ctx *c, int32_t id, const double *pos, const double *vel,
         for (int k = 0; k < K; ++k) h2d(c, base + k * n6 + g0, tmp + k * N, N);
         HIPCHK(hipStreamSynchronize(c->stream));
     };
-    if (m) put(c->m, 1, m);
+    if (m) {
+        put(c->m, 1, m);
+        // the body's masses may no longer share one value (or may now)
+        bool same = true;
+        for (int64_t i = 1; i < N && same; ++i) same = same_bits(m[i], m[0]);
+        const uint8_t kd = same ? (uint8_t)(b.kd | kKdUniformM) : (uint8_t)(b.kd & ~kKdUniformM);
+        h2d(c, c->b_m + id, &m[0], 1);
+        h2d(c, c->b_kd + id, &kd, 1);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        b.kd = kd;
+    }
     if (T) put(c->Tm, 1, T);
     if (ext) {
         bool changed = false;

@@ -702,7 +702,7 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
         o1 = cur[3 * i + 1];
         o2 = cur[3 * i + 2];
         d = a.deg[i];
-        m = a.m[i];
+        m = (bkd.kd & kKdUniformM) ? a.b_m[b] : a.m[i];  // uniform over the workgroup
         w.prime(ebase, ng, t & (kChunk - 1), bkd);
         // the chunk's share of its body's actuation factors for the next step
         if (ACT && left > 1)
@@ -1297,6 +1297,7 @@ __global__ __launch_bounds__(kBlock) void k_set_body(const BodyTable t, const Bo
         t.b_lcn[b] = bi.lcn;
         t.b_k[b] = bi.k;
         t.b_d[b] = bi.d;
+        t.b_m[b] = bi.m;
         t.b_kd[b] = bi.kd;
     }
     t.b_chunk0[b] = bi.on ? (int32_t)bi.chunk0 : -1;

@@ -125,12 +125,12 @@ struct BodyTable {
     int64_t *bn[2], *bleft[2], *bnsync;
     int32_t *b_chunk0, *b_lc, *b_lcn;
     uint8_t *b_mode;
-    double *b_k, *b_d;
+    double *b_k, *b_d, *b_m;
     uint8_t *b_kd;
 };
 struct BodyInit {
     int64_t chunk0, nchunks, tile0, ntiles, res;  // res < 0: not resident
-    double t0, k, d;
+    double t0, k, d, m;
     int32_t bid, lcb, lcn;
     uint8_t mode, kd, on;  // on: 1 joins the launches, 0 leaves them
 };
@@ -154,6 +154,9 @@ constexpr int kGroupTiled = 2;
 // the actuator's arena index (an int64 in the same 8 bytes), whose actuated
 // rest length the step reads from the factor buffer instead.
 constexpr uint32_t kActBit32 = 0x80000000u;
+// b_kd bit 2: every mass of the body has the same m, which the tiled step
+// reads from the body table instead of per mass
+constexpr uint8_t kKdUniformM = 4u;
 constexpr uint32_t kActBit16 = 0x8000u;
 
 // local constraint entry (a reference CudaContactPlane / CudaBall /
@@ -212,6 +215,7 @@ struct StepArgs {
     // shares them, else read per incidence (b_kd bit 0: k, bit 1: damping)
     const double *b_k, *b_d;
     const uint8_t *b_kd;
+    const double *b_m;        // per body: the mass of every mass of the body, when they are all equal (b_kd bit 2)
     const int32_t *b_lc;      // shared local list: begin
     const int32_t *b_lcn;     // shared local list: count
     const LocalC *lc;
