@@ -206,14 +206,14 @@ typedef struct titan_schedule {
 int titan_set_schedule(titan_ctx *ctx, const titan_schedule *sched);
 int titan_get_schedule(const titan_ctx *ctx, titan_schedule *sched);
 /* Pick the schedule by measurement: for each candidate (resident_max_masses
- * in {0, 1024, 32767}, tiled_min_masses in {0, 128, 256, 512, 1024, never},
- * tile_rows in {682, 2304}; 36 in that order, the last index fastest) a scratch
+ * in {0, 256, 1024, 32767}, tiled_min_masses in {0, 128, 256, 512, 1024,
+ * never}, tile_rows in {682, 2304}; 48 in that order, the last index fastest) a scratch
  * context on the same device with this context's dt, gravity, planes and
  * balls holds `replicas` copies of the n_samples sample bodies and is timed
  * over `steps` steps (after a warm-up of the same length).  The fastest in
  * mass-steps/s becomes this context's schedule and is returned in *best
  * (may be NULL); *rate_out (may be NULL) receives each candidate's rate, in
- * candidate order (36 entries; candidates that lay out every sample alike
+ * candidate order (48 entries; candidates that lay out every sample alike
  * are timed once and share the rate). */
 int titan_autotune(titan_ctx *ctx, const titan_body_desc *samples, int32_t n_samples, int32_t replicas,
                    int32_t steps, titan_schedule *best, double *rate_out);
@@ -283,6 +283,15 @@ int titan_lattice(double cx, double cy, double cz, double dx, double dy, double 
  * successful calls proceed, then one call of titan_step returns
  * TITAN_E_INJECTED without touching the device.  Negative `after` clears it. */
 int titan_debug_inject_failure(titan_ctx *ctx, int32_t after);
+/* The same at another call site: TITAN_INJECT_STEP (titan_step and
+ * titan_step_bodies, as above), TITAN_INJECT_DOWNLOAD (titan_body_download)
+ * or TITAN_INJECT_DESTROY (titan_body_destroy); the failing call changes
+ * nothing.  Used to test the serving loop's recovery from a failed
+ * retirement. */
+#define TITAN_INJECT_STEP 0
+#define TITAN_INJECT_DOWNLOAD 1
+#define TITAN_INJECT_DESTROY 2
+int titan_debug_inject_failure_at(titan_ctx *ctx, int32_t site, int32_t after);
 
 #ifdef __cplusplus
 }

@@ -57,7 +57,7 @@ def test_autotune_picks_a_candidate_and_applies_it():
         ctx.set_dt(sc.dt)
         ctx.set_global_acc(sc.g)
         best, rates = ctx.autotune(sc.bodies, replicas=2, steps=4)
-        assert len(rates) == len(AUTOTUNE_CANDIDATES) == 36
+        assert len(rates) == len(AUTOTUNE_CANDIDATES) == 48
         assert all(r > 0 for _, r in rates)
         won = max(rates, key=lambda cr: cr[1])[0]
         want = (2**63 - 1 if won[0] < 0 else won[0], won[1], won[2])
@@ -73,3 +73,13 @@ def test_bad_schedule_rejected():
             with pytest.raises(RuntimeError):
                 ctx.set_schedule(*bad)
         assert ctx.schedule() == (0, 2304, 256)
+
+
+def test_partial_schedule_keeps_the_other_fields():
+    with Context() as ctx:
+        ctx.set_schedule(tile_rows=682)
+        assert ctx.schedule() == (0, 682, 256)
+        ctx.set_schedule(resident_max_masses=0)
+        assert ctx.schedule() == (0, 682, 0)
+        ctx.set_schedule(512)
+        assert ctx.schedule() == (512, 682, 0)

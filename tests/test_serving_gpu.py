@@ -12,7 +12,8 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from titan_amd import REQ_CANCELLED, REQ_DONE, REQ_QUEUED, REQ_RUNNING, Context, scenes
+from titan_amd import (INJECT_DESTROY, INJECT_DOWNLOAD, REQ_CANCELLED, REQ_DONE, REQ_QUEUED, REQ_RUNNING,
+                       TITAN_E_INJECTED, Context, TitanError, scenes)
 from helpers import normwise_rel, to_oracle
 
 pytestmark = pytest.mark.gpu
@@ -172,3 +173,76 @@ def test_requests_without_results_complete_after_their_steps():
             pass
         s = ctx.serve_stats()
         assert s["completed"] + s["cancelled"] == len(t) + 1 and s["running"] == 0
+
+
+@pytest.mark.parametrize("site", [INJECT_DOWNLOAD, INJECT_DESTROY])
+def test_failed_retirement_loses_no_request(site):
+    # The pump that retires request 1 (37 steps, quantum 16: its third pump)
+    # also admits the four requests that arrived meanwhile; its retirement
+    # fails once (the result's download, or the body's release).  The pump
+    # reports the failure, the new arrivals still run, request 1 is retired by
+    # a later pump, cancelling works, and every result is the one served
+    # without a failure.
+    bodies, horizons, planes = _requests()
+    want, _, _ = _serve(bodies, horizons, planes, quantum=16)
+    print(f"case failed retirement at site {site}", flush=True)
+    with Context() as ctx:
+        ctx.set_dt(1e-4)
+        for p in planes:
+            ctx.add_plane(p[:3], p[3], p[4], p[5])
+        ctx.serve_config(quantum=16)
+        t = {q: ctx.serve_submit(bodies[q], horizons[q]) for q in (0, 1)}
+        ctx.serve_pump()
+        ctx.serve_pump()
+        for q in range(2, len(bodies)):
+            t[q] = ctx.serve_submit(bodies[q], horizons[q])
+        ctx.inject_failure(0, site)
+        with pytest.raises(TitanError) as err:
+            ctx.serve_pump()
+        assert err.value.code == TITAN_E_INJECTED
+        assert ctx.serve_poll(t[1])[0] == REQ_RUNNING
+        assert all(ctx.serve_poll(t[q])[0] == REQ_RUNNING for q in range(2, len(bodies)))
+        ctx.serve_cancel(t[3])
+        ctx.serve_cancel(t[3])  # a second cancel of a finished request changes nothing
+        pumps = 0
+        while any(ctx.serve_poll(x)[0] in (REQ_QUEUED, REQ_RUNNING) for x in t.values()):
+            ctx.serve_pump()
+            pumps += 1
+            assert pumps < 2000
+        assert ctx.serve_poll(t[3])[0] == REQ_CANCELLED
+        for q in t:
+            if q == 3:
+                continue
+            assert ctx.serve_poll(t[q]) == (REQ_DONE, horizons[q])
+            got = ctx.serve_result(t[q])
+            np.testing.assert_array_equal(got[0], want[q][0])
+            np.testing.assert_array_equal(got[1], want[q][1])
+        s = ctx.serve_stats()
+        assert (s["completed"], s["cancelled"], s["running"], s["queued"]) == (len(bodies) - 1, 1, 0, 0)
+
+
+def test_knn_cloud_requests_alone_and_mixed():
+    # the served workload's other half: actuated kNN-12 clouds (here 1500 and
+    # 6000 masses, as scenes.serving_requests builds them), served alone and
+    # mixed with a robot and the README lattice, against the oracle
+    rng = np.random.default_rng(21)
+    clouds = [scenes.actuate(scenes.knn_cloud(n, box=1.0, seed=s, z0=0.1), f, rng)
+              for n, s, f in ((1500, 5, 0.2), (6000, 6, 0.3))]
+    robots = scenes.robot_population(1, side_lo=8, side_hi=8, seed=13)
+    lat = scenes.readme_lattice(2.45)
+    bodies = clouds + [robots.bodies[0], lat.bodies[0]]
+    horizons = [170, 300, 90, 233]
+    planes = lat.planes
+    print(f"case kNN requests {[len(b['pos']) for b in bodies]} horizons {horizons}", flush=True)
+    alone = {q: _serve([bodies[q]], [horizons[q]], planes)[0][0] for q in range(len(bodies))}
+    for kw in (dict(quantum=64), dict(quantum=16, arrive_every=2, order=[3, 1, 2, 0])):
+        mix, _, stats = _serve(bodies, horizons, planes, **kw)
+        assert stats["completed"] == len(bodies)
+        for q in range(len(bodies)):
+            np.testing.assert_array_equal(mix[q][0], alone[q][0])
+            np.testing.assert_array_equal(mix[q][1], alone[q][1])
+    for q in range(len(bodies)):
+        o = to_oracle(scenes.Scene(bodies=[bodies[q]], planes=planes, dt=1e-4))
+        o.run(horizons[q])
+        assert normwise_rel(alone[q][0], o.pos) <= 1e-10
+        assert normwise_rel(alone[q][1], o.vel) <= 1e-10

@@ -37,7 +37,7 @@ EXPORTS = (
     "titan_body_time", "titan_set_schedule", "titan_get_schedule", "titan_autotune",
     "titan_serve_config", "titan_serve_submit", "titan_serve_pump", "titan_serve_poll",
     "titan_serve_cancel", "titan_serve_stats", "titan_serve_reset_stats", "titan_set_integrator",
-    "titan_get_integrator",
+    "titan_get_integrator", "titan_debug_inject_failure_at",
 )
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
@@ -45,8 +45,11 @@ REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
 LAYOUT_AUTO, LAYOUT_GATHER, LAYOUT_TILED, LAYOUT_RESIDENT = 0, 1, 2, 3
 RESIDENT_LDS_MAX = 1024   # resident bodies held in LDS (launches with few bodies)
 RESIDENT_MAX = 32767      # the resident layout's largest body
+RESIDENT_SPLIT_MAX = 256  # resident bodies of at most four chunks have the split form (the default schedule's bound)
 # integrators: the reference's default Euler branch and its VERLET / RK2 build options
 INTEGRATOR_EULER, INTEGRATOR_VERLET, INTEGRATOR_RK2 = 0, 1, 2
+# call sites of titan_debug_inject_failure_at
+INJECT_STEP, INJECT_DOWNLOAD, INJECT_DESTROY = 0, 1, 2
 
 
 class TitanError(RuntimeError):
@@ -108,6 +111,7 @@ def lib() -> ctypes.CDLL:
         "titan_lattice": (c_int, [c_double] * 6 + [c_int] * 3 + [POINTER(c_double), POINTER(c_int32),
                                                                    POINTER(c_int32), POINTER(c_double)]),
         "titan_debug_inject_failure": (c_int, [vp, c_int32]),
+        "titan_debug_inject_failure_at": (c_int, [vp, c_int32, c_int32]),
         "titan_body_layout": (c_int, [vp, c_int32, POINTER(c_int32)]),
         "titan_step_bodies": (c_int, [vp, vp, vp, c_int32]),
         "titan_body_time": (c_int, [vp, c_int32, POINTER(c_double), POINTER(c_int64)]),
@@ -218,7 +222,7 @@ class Schedule(ctypes.Structure):
 
 # titan_autotune's candidates (tiled_min_masses, tile_rows, resident_max_masses),
 # in the order of its rate_out (min -1: never tiled)
-AUTOTUNE_CANDIDATES = [(mn, rw, rs) for rs in (0, RESIDENT_LDS_MAX, RESIDENT_MAX)
+AUTOTUNE_CANDIDATES = [(mn, rw, rs) for rs in (0, RESIDENT_SPLIT_MAX, RESIDENT_LDS_MAX, RESIDENT_MAX)
                        for mn in (0, 128, 256, 512, 1024, -1) for rw in (682, 2304)]
 
 
@@ -298,9 +302,13 @@ class Context:
         return bid.value
 
     # -- schedule --
-    def set_schedule(self, tiled_min_masses: int = 0, tile_rows: int = 2304,
-                     resident_max_masses: int = 0):
-        s = Schedule(int(tiled_min_masses), int(tile_rows), int(resident_max_masses))
+    def set_schedule(self, tiled_min_masses: int | None = None, tile_rows: int | None = None,
+                     resident_max_masses: int | None = None):
+        """Set the schedule for bodies created afterwards; an omitted field
+        keeps the context's current value."""
+        cur = self.schedule()
+        vals = [cur[q] if v is None else int(v) for q, v in enumerate((tiled_min_masses, tile_rows, resident_max_masses))]
+        s = Schedule(*vals)
         self._chk(self._L.titan_set_schedule(self._h, ctypes.byref(s)))
 
     def schedule(self) -> tuple[int, int, int]:
@@ -456,5 +464,6 @@ class Context:
         self._chk(self._L.titan_kernel_name(self._h, buf, 256))
         return buf.value.decode()
 
-    def inject_failure(self, after: int):
-        self._chk(self._L.titan_debug_inject_failure(self._h, int(after)))
+    def inject_failure(self, after: int, site: int = INJECT_STEP):
+        """Make a call at `site` fail with TITAN_E_INJECTED after `after` successful ones."""
+        self._chk(self._L.titan_debug_inject_failure_at(self._h, int(site), int(after)))

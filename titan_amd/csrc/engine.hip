@@ -182,6 +182,13 @@ struct titan_ctx {
     int n_cu = 0;          // compute units of the device
     int slot = 0;
 
+    // whole-range stepping replays captured 64-step graphs; off when the
+    // environment sets TITAN_GRAPHS=0, and by default under the rocprofv3
+    // tracer (which sets ROCPROF_OUTPUT_PATH for the program it runs): a
+    // traced run of several contexts in one process once died in the host
+    // runtime inside hipGraphLaunch, with no fault found in the engine's use
+    // of the graph, so traced runs launch each step directly instead
+    bool use_graphs = true;
     // captured multi-step graphs, valid while `version` is unchanged, keyed
     // by the launch shape they were captured with (most recent last)
     uint64_t version = 1;
@@ -202,7 +209,7 @@ struct titan_ctx {
     // pinned staging for body downloads and uploads (grown on demand)
     double *stage = nullptr;
     int64_t stage_cap = 0;
-    int32_t inject = -1;
+    int32_t inject[3] = {-1, -1, -1};  // by TITAN_INJECT_* site: calls left before a forced failure
 
     std::map<void *, size_t> allocs;
     // destroyed bodies whose slots are released once `ev` (recorded after
@@ -773,13 +780,15 @@ void prepare_launches(titan_ctx *c) {
     }
 }
 
-void maybe_inject(titan_ctx *c) {
-    if (c->inject >= 0) {
-        if (c->inject == 0) {
-            c->inject = -1;
-            fail(TITAN_E_INJECTED, "step: injected failure");
+void maybe_inject(titan_ctx *c, int site) {
+    int32_t &k = c->inject[site];
+    if (k >= 0) {
+        if (k == 0) {
+            k = -1;
+            static const char *what[3] = {"step", "body_download", "body_destroy"};
+            fail(TITAN_E_INJECTED, std::string(what[site]) + ": injected failure");
         }
-        c->inject--;
+        k--;
     }
 }
 
@@ -805,7 +814,7 @@ void run_launches(titan_ctx *c, int64_t n, const Range &rg) {
         const StepArgs a = make_args(c, s, rg);
         HIPCHK(launch_actuate(a, c->stream));
     }
-    if (rg.all && rest >= kGraphSteps) {
+    if (rg.all && rest >= kGraphSteps && c->use_graphs) {
         hipGraphExec_t g = graph_for(c, s);  // an even number of steps: ends on slot s
         for (int64_t r = 0; r < rest / kGraphSteps; ++r) HIPCHK(hipGraphLaunch(g, c->stream));
         rest %= kGraphSteps;
@@ -822,7 +831,7 @@ void run_launches(titan_ctx *c, int64_t n, const Range &rg) {
 
 void do_step(titan_ctx *c, int64_t n) {
     if (n < 0) fail(TITAN_E_INVALID, "step: negative step count");
-    maybe_inject(c);
+    maybe_inject(c, TITAN_INJECT_STEP);
     if (n == 0) return;
     const int64_t nl = n * launches_per_step(c);
     const double h = launch_dt(c);
@@ -855,7 +864,7 @@ void do_step_bodies(titan_ctx *c, const int32_t *ids, const int64_t *steps, int3
         left[ids[k]] = steps[k] * launches_per_step(c);
         n = std::max(n, left[ids[k]]);
     }
-    maybe_inject(c);
+    maybe_inject(c, TITAN_INJECT_STEP);
     if (n == 0) return;
     prepare_launches(c);
     if (count == 1) {
@@ -1463,6 +1472,7 @@ int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
 // slots are released once that kernel has run (harvest_released).
 void body_destroy(titan_ctx *c, int32_t id) {
     BodyH &b = body_of(c, id);
+    maybe_inject(c, TITAN_INJECT_DESTROY);
     BodyInit bi{};
     bi.bid = id;
     bi.chunk0 = b.chunk0;
@@ -1504,6 +1514,7 @@ double *staging(titan_ctx *c, int64_t n) {
 
 void body_download(titan_ctx *c, int32_t id, double *pos, double *vel, double *acc, double *T) {
     BodyH &b = body_of(c, id);
+    maybe_inject(c, TITAN_INJECT_DOWNLOAD);
     const int64_t N = b.n_masses, g0 = b.chunk0 * kChunk, n6 = c->ncap;
     const int p = (int)((b.n * launches_per_step(c)) & 1);
     auto host = [&](int64_t q) -> int64_t { return b.perm.empty() ? q : b.perm[q]; };
@@ -1648,6 +1659,10 @@ int titan_ctx_create(int device, uint64_t hbm_budget_bytes, titan_ctx **out) {
     c = new titan_ctx();
     c->device = device;
     c->budget = hbm_budget_bytes;
+    {
+        const char *g = std::getenv("TITAN_GRAPHS");
+        c->use_graphs = g ? std::strcmp(g, "0") != 0 : std::getenv("ROCPROF_OUTPUT_PATH") == nullptr;
+    }
     HIPCHK(set_tiles_lds_limit(&c->res_lds_max));
     HIPCHK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1932,7 +1947,7 @@ int titan_autotune(titan_ctx *c, const titan_body_desc *samples, int32_t n_sampl
     API_BEGIN
     if (!samples || n_samples <= 0 || replicas <= 0 || steps <= 0)
         fail(TITAN_E_INVALID, "autotune: need samples, replicas > 0 and steps > 0");
-    const int32_t res[] = {0, kResidentMax, kResidentMaxBody};
+    const int32_t res[] = {0, kSplitMaxChunks * kChunk, kResidentMax, kResidentMaxBody};
     const int64_t mins[] = {0, 128, 256, 512, 1024, INT64_MAX};
     const int32_t rows[] = {682, kMaxTileRows};
     titan_schedule win{0, kTileRows, 0};
@@ -2065,9 +2080,12 @@ int titan_body_layout(const titan_ctx *c, int32_t body_id, int32_t *layout) {
     return TITAN_OK;
 }
 
-int titan_debug_inject_failure(titan_ctx *c, int32_t after) {
+int titan_debug_inject_failure(titan_ctx *c, int32_t after) { return titan_debug_inject_failure_at(c, TITAN_INJECT_STEP, after); }
+
+int titan_debug_inject_failure_at(titan_ctx *c, int32_t site, int32_t after) {
     NEED_CTX(c);
-    c->inject = after < 0 ? -1 : after;
+    if (site < TITAN_INJECT_STEP || site > TITAN_INJECT_DESTROY) return TITAN_E_INVALID;
+    c->inject[site] = after < 0 ? -1 : after;
     return TITAN_OK;
 }
 

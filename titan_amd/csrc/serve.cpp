@@ -320,6 +320,24 @@ static int serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
     if (!ctx) return TITAN_E_INVALID;
     Server &sv = server_of(ctx);
     std::vector<Request *> fresh;  // admitted during this pump's quantum: they join the next one
+    // On every exit, the error returns included, the running set becomes the
+    // requests that hold a device body: those admitted by this pump join it
+    // and those retired (body released) leave it, so that a failed pump
+    // loses no request and the next one steps no released body.  `fresh` is
+    // emptied whenever its requests join the running set.
+    struct Reconcile {
+        Server &sv;
+        std::vector<Request *> &fresh;
+        ~Reconcile() {
+            std::vector<Request *> keep;
+            keep.reserve(sv.running.size() + fresh.size());
+            for (const std::vector<Request *> *list : {&sv.running, &fresh})
+                for (Request *r : *list)
+                    if (r->state == TITAN_REQ_RUNNING && r->body >= 0) keep.push_back(r);
+            sv.running.swap(keep);
+            fresh.clear();
+        }
+    } reconcile{sv, fresh};
     // admit, in arrival order, while the running set has room: up to `cap`
     // requests, laid out together on the host, then placed one by one
     // `until`: stop (between batches) once the step stream has passed this
@@ -374,7 +392,7 @@ static int serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
     if (sv.running.empty()) {  // nothing on the device: a first batch starts it
         rc = admit(host_threads(), nullptr);
         if (rc != TITAN_OK) return rc;
-        sv.running.swap(fresh);
+        sv.running.swap(fresh);  // fresh is now the empty old running set
     }
     // step every running request together, each up to the quantum and its
     // horizon; the call returns once the launches are enqueued
@@ -414,6 +432,7 @@ static int serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
         if (rc != TITAN_OK) return rc;
     }
     still.insert(still.end(), fresh.begin(), fresh.end());
+    fresh.clear();
     sv.running.swap(still);
     // nothing left to step: wait for the last finishing requests
     rc = observe(sv, sv.running.empty() && sv.queue.empty());
@@ -440,18 +459,22 @@ static int serve_cancel(titan_ctx *ctx, int64_t ticket) {
     if (it == sv.reqs.end()) return TITAN_E_NOTFOUND;
     Request *r = it->second.get();
     if (r->state == TITAN_REQ_QUEUED) {
-        sv.queue.erase(std::find(sv.queue.begin(), sv.queue.end(), r));
+        auto q = std::find(sv.queue.begin(), sv.queue.end(), r);
+        if (q != sv.queue.end()) sv.queue.erase(q);
     } else if (r->state == TITAN_REQ_RUNNING && r->body < 0) {  // finishing: its body is released
         auto f = std::find_if(sv.finishing.begin(), sv.finishing.end(),
                               [&](const Finishing &x) { return x.r == r; });
-        sv.spare.push_back(f->ev);
-        sv.finishing.erase(f);
+        if (f != sv.finishing.end()) {
+            sv.spare.push_back(f->ev);
+            sv.finishing.erase(f);
+        }
     } else if (r->state == TITAN_REQ_RUNNING) {
         const int rc = titan_body_destroy(ctx, r->body);
         if (rc != TITAN_OK) return rc;
         r->body = -1;
         sv.running_bytes -= r->est_bytes;
-        sv.running.erase(std::find(sv.running.begin(), sv.running.end(), r));
+        auto q = std::find(sv.running.begin(), sv.running.end(), r);
+        if (q != sv.running.end()) sv.running.erase(q);
     } else {
         return TITAN_OK;  // already finished
     }
