@@ -130,6 +130,20 @@ int titan_body_count(const titan_ctx *ctx, int32_t *n);
  * T += dt (src/sim.cu:1590).  Enqueued on the context's stream; returns
  * without waiting unless a later call needs the results. */
 int titan_step(titan_ctx *ctx, int64_t n_steps);
+/* titan_step that allocates nothing (no counterpart in the reference: for
+ * callers that cannot allocate device memory, such as code under stream
+ * capture or a tight serving loop).  Same steps and results as titan_step,
+ * but every step is launched directly (no graph is captured), the ball grid
+ * is not rebuilt (if the balls changed since the last titan_step, every
+ * mass tests every ball, with the same result), and the step kernels take
+ * their scratch buffers (actuation factors of actuated springs; the RK2
+ * stage state) from a static device array of 64 MiB per device that the
+ * library holds until the process ends.  Returns TITAN_E_INVALID, before any
+ * work, when the context needs more than that: 16 B per actuator slot, plus
+ * 48 B per mass slot under RK2.  Every call on one device shares the array,
+ * so calls that overlap on one device (contexts stepped from different
+ * threads) race; the array's contents at entry never matter. */
+int titan_step_noalloc(titan_ctx *ctx, int64_t n_steps);
 /* Step until the first step boundary with T >= t, the breakpoint rule of
  * src/sim.cu:1505; *steps_done (may be NULL) receives the count. */
 int titan_step_until(titan_ctx *ctx, double t, int64_t *steps_done);

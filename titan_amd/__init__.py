@@ -37,7 +37,7 @@ EXPORTS = (
     "titan_body_time", "titan_set_schedule", "titan_get_schedule", "titan_autotune",
     "titan_serve_config", "titan_serve_submit", "titan_serve_pump", "titan_serve_poll",
     "titan_serve_cancel", "titan_serve_stats", "titan_serve_reset_stats", "titan_set_integrator",
-    "titan_get_integrator", "titan_debug_inject_failure_at",
+    "titan_get_integrator", "titan_debug_inject_failure_at", "titan_step_noalloc",
 )
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
@@ -98,6 +98,7 @@ def lib() -> ctypes.CDLL:
         "titan_body_destroy": (c_int, [vp, c_int32]),
         "titan_body_count": (c_int, [vp, POINTER(c_int32)]),
         "titan_step": (c_int, [vp, c_int64]),
+        "titan_step_noalloc": (c_int, [vp, c_int64]),
         "titan_step_until": (c_int, [vp, c_double, POINTER(c_int64)]),
         "titan_time": (c_int, [vp, POINTER(c_double)]),
         "titan_sync": (c_int, [vp]),
@@ -396,6 +397,11 @@ class Context:
     # -- stepping --
     def step(self, n: int):
         self._chk(self._L.titan_step(self._h, int(n)))
+
+    def step_noalloc(self, n: int):
+        """titan_step that allocates nothing (scratch from the library's static
+        device array; no graph capture, no ball-grid rebuild)."""
+        self._chk(self._L.titan_step_noalloc(self._h, int(n)))
 
     def step_bodies(self, ids, steps):
         """Advance each body in `ids` by its own number of steps (the others do not move)."""

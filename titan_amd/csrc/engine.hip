@@ -19,6 +19,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -182,6 +183,13 @@ struct titan_ctx {
     int n_cu = 0;          // compute units of the device
     int slot = 0;
 
+    // titan_step_noalloc: the device's scratch array (its address on this
+    // context's device) and, during such a call, where the step kernels take
+    // their actuation factors and RK2 stage state from (null outside a call)
+    double *scratch = nullptr;
+    double *na_ract[2] = {nullptr, nullptr};
+    double *na_rk = nullptr;
+    bool in_noalloc = false;
     // whole-range stepping replays captured 64-step graphs; off when the
     // environment sets TITAN_GRAPHS=0, and by default under the rocprofv3
     // tracer (which sets ROCPROF_OUTPUT_PATH for the program it runs): a
@@ -223,6 +231,12 @@ struct titan_ctx {
 };
 
 namespace {
+
+// The scratch array of titan_step_noalloc: one per device, held until the
+// process ends.  Host code uses only the address hipGetSymbolAddress gives
+// for it on each device (device_scratch), never the array's name.
+constexpr int64_t kScratchBytes = (int64_t)64 << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / 8];
 
 constexpr int kGraphSteps = 64;  // even: a replay leaves the launch slot unchanged
 constexpr size_t kGraphCache = 8;  // captured graphs kept per context
@@ -594,9 +608,9 @@ StepArgs make_args(titan_ctx *c, int slot, const Range &rg) {
     a.act = c->act;
     a.integ = c->integ;
     a.hdt = launch_dt(c);
-    a.rk = c->rk;
-    a.ract[0] = c->ract[0];
-    a.ract[1] = c->ract[1];
+    a.rk = c->in_noalloc ? c->na_rk : c->rk;
+    a.ract[0] = c->in_noalloc ? c->na_ract[0] : c->ract[0];
+    a.ract[1] = c->in_noalloc ? c->na_ract[1] : c->ract[1];
     a.chunk_act0 = c->chunk_act0;
     a.chunk_nact = c->chunk_nact;
     for (int s = 0; s < 2; ++s) {
@@ -612,8 +626,12 @@ StepArgs make_args(titan_ctx *c, int slot, const Range &rg) {
     a.nplanes = (int32_t)(c->planes_h.size() / 8);
     a.balls = c->balls;
     a.nballs = (int32_t)(c->balls_h.size() / 4);
-    a.bg_start = c->bg_start;
-    a.bg_list = c->bg_list;
+    // a ball grid not rebuilt since the balls changed (possible only in
+    // titan_step_noalloc, which builds nothing): every mass tests every ball,
+    // with the same result
+    const bool grid = !c->ball_grid_dirty;
+    a.bg_start = grid ? c->bg_start : nullptr;
+    a.bg_list = grid ? c->bg_list : nullptr;
     a.bg_x0 = c->bg_x0;
     a.bg_y0 = c->bg_y0;
     a.bg_z0 = c->bg_z0;
@@ -814,7 +832,7 @@ void run_launches(titan_ctx *c, int64_t n, const Range &rg) {
         const StepArgs a = make_args(c, s, rg);
         HIPCHK(launch_actuate(a, c->stream));
     }
-    if (rg.all && rest >= kGraphSteps && c->use_graphs) {
+    if (rg.all && rest >= kGraphSteps && c->use_graphs && !c->in_noalloc) {
         hipGraphExec_t g = graph_for(c, s);  // an even number of steps: ends on slot s
         for (int64_t r = 0; r < rest / kGraphSteps; ++r) HIPCHK(hipGraphLaunch(g, c->stream));
         rest %= kGraphSteps;
@@ -829,7 +847,36 @@ void run_launches(titan_ctx *c, int64_t n, const Range &rg) {
     c->slot = s;
 }
 
-void do_step(titan_ctx *c, int64_t n) {
+// For the length of a titan_step_noalloc call: the step kernels take their
+// scratch buffers (actuation factors, RK2 stage state) from the device's
+// static scratch array, and no graph is captured.
+struct NoallocScope {
+    titan_ctx *c;
+    NoallocScope(titan_ctx *ctx, bool on) : c(on ? ctx : nullptr) {
+        if (!c) return;
+        const int64_t nact = c->has_act ? c->acts.high_water() : 0;
+        const int64_t nrk = c->integ == kIntRK2 ? 6 * c->chunks.high_water() * kChunk : 0;
+        const int64_t need = 8 * (2 * nact + nrk);
+        if (need > kScratchBytes)
+            fail(TITAN_E_INVALID, "step_noalloc: the context needs " + std::to_string(need) +
+                                      " bytes of scratch (8 B x 2 per actuator slot, and 48 B per mass slot "
+                                      "under RK2), more than the " + std::to_string(kScratchBytes) +
+                                      " the library reserves per device");
+        c->na_ract[0] = c->scratch;
+        c->na_ract[1] = c->scratch + nact;
+        c->na_rk = c->scratch + 2 * nact;
+        c->in_noalloc = true;
+    }
+    ~NoallocScope() {
+        if (!c) return;
+        c->in_noalloc = false;
+        c->na_ract[0] = c->na_ract[1] = c->na_rk = nullptr;
+    }
+};
+
+// noalloc: titan_step_noalloc (nothing allocated: no ball grid rebuilt, no
+// graph captured, scratch from the static array).
+void do_step(titan_ctx *c, int64_t n, bool noalloc = false) {
     if (n < 0) fail(TITAN_E_INVALID, "step: negative step count");
     maybe_inject(c, TITAN_INJECT_STEP);
     if (n == 0) return;
@@ -837,7 +884,8 @@ void do_step(titan_ctx *c, int64_t n) {
     const double h = launch_dt(c);
     const int64_t nb = live_body_hw(c);
     if (nb > 0 && c->chunks.high_water() > 0) {
-        prepare_launches(c);
+        const NoallocScope na(c, noalloc);
+        if (!noalloc) prepare_launches(c);
         HIPCHK(launch_set_left(c->bleft[c->slot], c->b_chunk0, nb, nl, c->stream));
         run_launches(c, nl, full_range(c));
     }
@@ -1608,6 +1656,21 @@ void body_upload(titan_ctx *c, int32_t id, const double *pos, const double *vel,
     }
 }
 
+// The address of the scratch array on `device` (current), looked up once per
+// device and kept for the process.
+double *device_scratch(int device) {
+    static std::mutex mu;
+    static std::map<int, double *> addr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) return it->second;
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) fail(TITAN_E_HIP, "hipGetSymbolAddress(scratch) gave a null address");
+    addr[device] = static_cast<double *>(p);
+    return static_cast<double *>(p);
+}
+
 void set_err(const titan_ctx *c, const std::string &m) {
     if (c) const_cast<titan_ctx *>(c)->err = m;
     else g_create_err = m;
@@ -1664,6 +1727,7 @@ int titan_ctx_create(int device, uint64_t hbm_budget_bytes, titan_ctx **out) {
         c->use_graphs = g ? std::strcmp(g, "0") != 0 : std::getenv("ROCPROF_OUTPUT_PATH") == nullptr;
     }
     HIPCHK(set_tiles_lds_limit(&c->res_lds_max));
+    c->scratch = device_scratch(device);
     HIPCHK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&c->up, hipStreamNonBlocking));
@@ -1817,6 +1881,13 @@ int titan_step(titan_ctx *c, int64_t n_steps) {
     NEED_CTX(c);
     API_BEGIN
     do_step(c, n_steps);
+    API_END(c)
+}
+
+int titan_step_noalloc(titan_ctx *c, int64_t n_steps) {
+    NEED_CTX(c);
+    API_BEGIN
+    do_step(c, n_steps, true);
     API_END(c)
 }
 
