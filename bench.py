@@ -86,7 +86,32 @@ def cpu_baseline(scene, budget_s: float = 15.0) -> dict:
             "ms_per_step": 1e3 * dt / n}
 
 
-def bench_batched(n_bodies: int, seed: int, rollout: int, sample: int, schedule=None) -> dict:
+def cpu_population(sc, rollout: int, sub_bodies: int) -> dict:
+    """The oracle on the robot population: the whole population for one
+    rollout on all of the host's threads, and its first `sub_bodies` bodies
+    for one rollout on one thread (mass-steps/s; bodies are independent, so
+    the rate of a sample stands for the population's)."""
+    from oracle import oracle as orc  # CPU baseline only
+    from tests.helpers import to_oracle
+    from titan_amd import scenes
+
+    threads = cpu_threads()
+    out = {}
+    for name, scene, th in (("all_cores", sc, threads),
+                            ("one_core", scenes.Scene(bodies=sc.bodies[:sub_bodies], planes=sc.planes, dt=sc.dt), 1)):
+        o = to_oracle(scene, threads=th)
+        o.run(2)  # warm-up
+        t0 = time.perf_counter()
+        o.run(rollout)
+        dt = time.perf_counter() - t0
+        out[name] = {"value": scene.n_masses * rollout / dt, "unit": "mass-steps/s", "cores": th, "kind": "port",
+                     "sample": f"oracle (C++17/OpenMP, {cpu_model()}), {len(scene.bodies)} bodies "
+                               f"({scene.n_masses} masses), one {rollout}-step rollout, {dt:.1f} s"}
+        log(f"batched CPU {name}: {out[name]['value']:.3e} mass-steps/s ({th} threads, {dt:.1f} s)")
+    return out
+
+
+def bench_batched(n_bodies: int, seed: int, rollout: int, sample: int, schedule=None, cpu: bool = True) -> dict:
     """Batched uneven bodies: a robot population (lattices of side U{3..12},
     30% of springs actuated, each body with its own contact plane, dt 1e-4)
     stepped in one launch per step with graph-captured rollouts, against the
@@ -140,6 +165,8 @@ def bench_batched(n_bodies: int, seed: int, rollout: int, sample: int, schedule=
                                    "mass_steps_per_s": r}
                                   for c, r in rates]},
     }
+    if cpu:
+        out["cpu_baseline"] = cpu_population(sc, rollout, 256)
     if sample <= 0:
         return out
 
@@ -270,7 +297,7 @@ def bench_serving(seed: int, n_req: int, lams, quantum: int, n_base: int) -> dic
     }
 
 
-def bench_readme(steps: int) -> dict:
+def bench_readme(steps: int, layouts=("gather", "tiled", "resident"), cpu: bool = True) -> dict:
     """The README scene (125 masses, 1036 springs, damping 0.1, ground plane
     mu_k 0.8 / mu_s 1.0, dt 1e-4), both starting heights: latency-bound, so
     reported as time per step for each layout (graph-launched, the call's
@@ -285,6 +312,8 @@ def bench_readme(steps: int) -> dict:
         sc = scenes.readme_lattice(cz)
         per = {}
         for name, lay in (("gather", LAYOUT_GATHER), ("tiled", LAYOUT_TILED), ("resident", LAYOUT_RESIDENT)):
+            if name not in layouts:
+                continue
             with Context() as ctx:
                 sc.load(ctx, layout=lay)
                 ctx.step(steps // 10)
@@ -292,14 +321,16 @@ def bench_readme(steps: int) -> dict:
                 ctx.timer_start()
                 ctx.step(steps)
                 per[name] = 1e3 * ctx.timer_stop() / steps
-        o = to_oracle(sc, threads=1)
-        o.run(steps // 10)
-        t0 = time.perf_counter()
-        o.run(steps)
-        cpu = 1e6 * (time.perf_counter() - t0) / steps
+        cpu_us = None
+        if cpu:
+            o = to_oracle(sc, threads=1)
+            o.run(steps // 10)
+            t0 = time.perf_counter()
+            o.run(steps)
+            cpu_us = 1e6 * (time.perf_counter() - t0) / steps
         log(f"readme cz={cz}: us/step " + ", ".join(f"{k} {v:.2f}" for k, v in per.items()) +
-            f"; CPU oracle (1 thread) {cpu:.2f}")
-        out[f"cz{cz:g}"] = {"us_per_step": per, "cpu_us_per_step_1_thread": cpu}
+            f"; CPU oracle (1 thread) {cpu_us}")
+        out[f"cz{cz:g}"] = {"us_per_step": per, "cpu_us_per_step_1_thread": cpu_us}
     return {"workload": f"README lattice 5x5x5 (125 masses, 1036 springs), cz 10 (free fall) and 2.45 "
                         f"(in contact from step 0), dt=1e-4, FP64, {steps} timed steps after {steps // 10}",
             **out}
@@ -370,27 +401,17 @@ def main() -> int:
     ap.add_argument("--lambdas", default="50,200,1000,2000",
                     help="serving workload: Poisson arrival rates (requests/s), comma-separated")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
-                    help="only run a plain device-to-device copy of MIB MiB three times (the "
-                         "reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters)")
+                    help="only run the library's streaming copy and read kernels (k_stream_copy, "
+                         "k_stream_read) over MIB MiB, 3 timed launches each after a warm-up: the "
+                         "known-size kernels for scaling the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--readme-layouts", default="gather,tiled,resident",
+                    help="README workload: the layouts to time (comma-separated)")
     args = ap.parse_args()
 
     if args.copy_calib:
-        import torch  # device memory plumbing for the calibration copy only
+        from titan_amd import measure_hbm
 
-        n = args.copy_calib * (1 << 20) // 8
-        a = torch.ones(n, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        b.copy_(a)
-        e0.record()
-        for _ in range(3):
-            b.copy_(a)
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / 3
-        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "copies": 4,
-                          "copy_ms": ms, "copy_GBps": 2 * n * 8 / ms / 1e6}))
+        print(json.dumps(measure_hbm(0, args.copy_calib << 20, 3)), flush=True)
         return 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -402,11 +423,12 @@ def main() -> int:
 
     extra = {}
     if args.only == "readme" or (args.only is None and not args.no_extra):
-        extra["readme"] = bench_readme(10000)
+        extra["readme"] = bench_readme(10000, layouts=tuple(args.readme_layouts.split(",")), cpu=not args.no_cpu)
     if args.only == "batched" or (args.only is None and not args.no_extra):
         sched = tuple(int(x) for x in args.schedule.split(",")) if args.schedule else None
         extra["batched"] = bench_batched(args.bodies, args.seed, rollout=100,
-                                         sample=min(args.per_body_sample, args.bodies), schedule=sched)
+                                         sample=min(args.per_body_sample, args.bodies), schedule=sched,
+                                         cpu=not args.no_cpu)
     if args.only == "serving" or (args.only is None and not args.no_extra):
         lams = tuple(float(x) for x in args.lambdas.split(","))
         extra["serving"] = bench_serving(args.seed, n_req=args.requests, lams=lams,
@@ -462,9 +484,19 @@ def bench_lattice(args) -> dict:
     traffic_file = ROOT / "profiles" / "traffic_lattice100.json"
     if traffic_file.exists() and args.side == 100:
         try:
-            roofline["traffic"] = json.loads(traffic_file.read_text())["hbm_bytes_per_launch"]
+            tj = json.loads(traffic_file.read_text())
+            roofline["traffic"] = tj["hbm_bytes_per_launch"]
+            roofline["traffic_source"] = tj.get("source", str(traffic_file.relative_to(ROOT)))
         except Exception:
             pass
+    # what this device's HBM sustains for a streaming kernel of the library's
+    # own, measured in the same run: the achievable ceiling under the 8 TB/s peak
+    from titan_amd import measure_hbm
+
+    cal = measure_hbm(0, 1 << 30, 5)
+    roofline["stream_copy_GBps"] = cal["copy_GBps"]
+    roofline["stream_read_GBps"] = cal["read_GBps"]
+    log(f"streaming copy {cal['copy_GBps']:.0f} GB/s, read {cal['read_GBps']:.0f} GB/s (1 GiB buffers)")
 
     # sanity: the lattice must still be finite after the run
     st = ctx.download(bid)

@@ -285,6 +285,15 @@ int titan_serve_stats(titan_ctx *ctx, titan_serve_report *out);
 /* Forget completed/cancelled requests and reset the statistics. */
 int titan_serve_reset_stats(titan_ctx *ctx);
 
+/* HBM calibration (no counterpart in the reference; a measurement aid for
+ * the roofline): on `device`, a streaming copy between two buffers of
+ * `bytes` bytes (16-byte loads, non-temporal 16-byte stores; *copy_gbps
+ * counts the bytes read plus written) and a streaming read of one
+ * (*read_gbps), each timed with HIP events over `iters` launches after a
+ * warm-up.  Allocates and frees its own buffers; errors are reported by
+ * titan_last_error(NULL). */
+int titan_measure_hbm(int device, uint64_t bytes, int32_t iters, double *copy_gbps, double *read_gbps);
+
 /* Host-side lattice generator with the reference's mass and spring order
  * (Lattice::Lattice, src/object.cu:232-293).  pos: 3*nx*ny*nz; left/right/
  * rest: titan_lattice_spring_count(nx, ny, nz) entries each. */

@@ -38,6 +38,7 @@ EXPORTS = (
     "titan_serve_config", "titan_serve_submit", "titan_serve_pump", "titan_serve_poll",
     "titan_serve_cancel", "titan_serve_stats", "titan_serve_reset_stats", "titan_set_integrator",
     "titan_get_integrator", "titan_debug_inject_failure_at", "titan_step_noalloc",
+    "titan_measure_hbm",
 )
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
@@ -99,6 +100,7 @@ def lib() -> ctypes.CDLL:
         "titan_body_count": (c_int, [vp, POINTER(c_int32)]),
         "titan_step": (c_int, [vp, c_int64]),
         "titan_step_noalloc": (c_int, [vp, c_int64]),
+        "titan_measure_hbm": (c_int, [c_int, c_uint64, c_int32, POINTER(c_double), POINTER(c_double)]),
         "titan_step_until": (c_int, [vp, c_double, POINTER(c_int64)]),
         "titan_time": (c_int, [vp, POINTER(c_double)]),
         "titan_sync": (c_int, [vp]),
@@ -135,6 +137,17 @@ def lib() -> ctypes.CDLL:
         fn.argtypes = args
     _lib = L
     return L
+
+
+def measure_hbm(device: int = 0, nbytes: int = 1 << 30, iters: int = 5) -> dict:
+    """The library's streaming copy and read kernels on `device`: GB/s of a
+    copy (bytes read + written) and of a read, over buffers of `nbytes`."""
+    L = lib()
+    cp, rd = c_double(), c_double()
+    rc = L.titan_measure_hbm(int(device), int(nbytes), int(iters), ctypes.byref(cp), ctypes.byref(rd))
+    if rc != 0:
+        raise TitanError(rc, L.titan_last_error(None).decode())
+    return {"copy_GBps": cp.value, "read_GBps": rd.value, "bytes": int(nbytes), "iters": int(iters)}
 
 
 def _ptr(a, ctype):

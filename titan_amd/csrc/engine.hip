@@ -2151,6 +2151,55 @@ int titan_body_layout(const titan_ctx *c, int32_t body_id, int32_t *layout) {
     return TITAN_OK;
 }
 
+int titan_measure_hbm(int device, uint64_t bytes, int32_t iters, double *copy_gbps, double *read_gbps) {
+    if (bytes < 16 || iters < 1) {
+        g_create_err = "measure_hbm: need bytes >= 16 and iters >= 1";
+        return TITAN_E_INVALID;
+    }
+    void *src = nullptr, *dst = nullptr;
+    double *out = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = TITAN_OK;
+    try {
+        HIPCHK(hipSetDevice(device));
+        const int64_t n = (int64_t)(bytes / 16);
+        HIPCHK(hipMalloc(&src, 16 * (size_t)n));
+        HIPCHK(hipMalloc(&dst, 16 * (size_t)n));
+        HIPCHK(hipMalloc((void **)&out, sizeof(double) * 1024 * 256));  // one per thread of the read grid
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipMemsetAsync(src, 0, 16 * (size_t)n, st));
+        float ms = 0;
+        HIPCHK(launch_stream_copy(src, dst, n, st));  // warm-up
+        HIPCHK(hipEventRecord(e0, st));
+        for (int32_t q = 0; q < iters; ++q) HIPCHK(launch_stream_copy(src, dst, n, st));
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (copy_gbps) *copy_gbps = 2.0 * 16.0 * (double)n * iters / ((double)ms * 1e6);
+        HIPCHK(launch_stream_read(src, out, n, st));
+        HIPCHK(hipEventRecord(e0, st));
+        for (int32_t q = 0; q < iters; ++q) HIPCHK(launch_stream_read(src, out, n, st));
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (read_gbps) *read_gbps = 16.0 * (double)n * iters / ((double)ms * 1e6);
+    } catch (const Fail &f) {
+        g_create_err = f.msg;
+        rc = f.code;
+    }
+    if (st) (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    (void)hipFree(out);
+    return rc;
+}
+
 int titan_debug_inject_failure(titan_ctx *c, int32_t after) { return titan_debug_inject_failure_at(c, TITAN_INJECT_STEP, after); }
 
 int titan_debug_inject_failure_at(titan_ctx *c, int32_t site, int32_t after) {

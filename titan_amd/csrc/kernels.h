@@ -30,5 +30,10 @@ hipError_t launch_advance_T(double *Tm, const uint32_t *mflags, const int32_t *c
 hipError_t launch_set_left(int64_t *bleft, const int32_t *b_chunk0, int64_t nb, int64_t n,
                            hipStream_t s);
 hipError_t launch_set_i64(int64_t *p, int64_t v, hipStream_t s);
+// HBM calibration: a streaming copy of n 16-byte words (read + non-temporal
+// write) and a streaming read of n words (summed into out[0..gridsize), so
+// that the loads are not removed).
+hipError_t launch_stream_copy(const void *src, void *dst, int64_t n, hipStream_t s);
+hipError_t launch_stream_read(const void *src, double *out, int64_t n, hipStream_t s);
 
 }  // namespace titan_mi355x

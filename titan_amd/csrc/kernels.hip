@@ -1372,6 +1372,65 @@ hipError_t launch_set_i64(int64_t *p, int64_t v, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Streaming kernels for the HBM calibration: each lane moves two 16-byte
+// words per round (non-temporal loads and stores), a wave's words
+// contiguous, the whole grid sweeping one contiguous window through memory.
+// Measured on MI355X over 1 GiB, the best of grids of 256 to 16384
+// workgroups of 256 threads: 512 for the copy (6.2 TB/s read + written),
+// 1024 for the read (6.9 TB/s); grids of 8192 or more fall to 5.0-5.5.
+constexpr int kCopyBlocks = 512, kReadBlocks = 1024, kStreamThreads = 256, kStreamUnroll = 2;
+__device__ __forceinline__ double2 load_nt(const double2 *p) {
+    const f64x2 v = __builtin_nontemporal_load(reinterpret_cast<const f64x2 *>(p));
+    return make_double2(v.x, v.y);
+}
+
+__global__ __launch_bounds__(kStreamThreads) void k_stream_copy(const double2 *__restrict__ src,
+                                                                double2 *__restrict__ dst, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * kStreamThreads;
+    for (int64_t i = (int64_t)blockIdx.x * kStreamThreads + threadIdx.x; i < n; i += kStreamUnroll * stride) {
+        double2 v[kStreamUnroll];
+#pragma unroll
+        for (int u = 0; u < kStreamUnroll; ++u) {
+            const int64_t j = i + u * stride;
+            v[u] = j < n ? load_nt(&src[j]) : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < kStreamUnroll; ++u) {
+            const int64_t j = i + u * stride;
+            if (j < n) store_nt(&dst[j], v[u].x, v[u].y);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kStreamThreads) void k_stream_read(const double2 *__restrict__ src,
+                                                                double *__restrict__ out, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * kStreamThreads;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kStreamThreads + threadIdx.x; i < n; i += kStreamUnroll * stride) {
+        double2 v[kStreamUnroll];
+#pragma unroll
+        for (int u = 0; u < kStreamUnroll; ++u) {
+            const int64_t j = i + u * stride;
+            v[u] = j < n ? load_nt(&src[j]) : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < kStreamUnroll; ++u) acc += v[u].x + v[u].y;
+    }
+    out[(int64_t)blockIdx.x * kStreamThreads + threadIdx.x] = acc;
+}
+
+hipError_t launch_stream_copy(const void *src, void *dst, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_stream_copy, dim3(kCopyBlocks), dim3(kStreamThreads), 0, s,
+                       static_cast<const double2 *>(src), static_cast<double2 *>(dst), n);
+    return hipGetLastError();
+}
+
+hipError_t launch_stream_read(const void *src, double *out, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_stream_read, dim3(kReadBlocks), dim3(kStreamThreads), 0, s,
+                       static_cast<const double2 *>(src), out, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_set_left(int64_t *bleft, const int32_t *b_chunk0, int64_t nb, int64_t n,
                            hipStream_t s) {
     if (nb > 0) hipLaunchKernelGGL(k_set_left, dim3((nb + 255) / 256), dim3(256), 0, s, bleft, b_chunk0, nb, n);
