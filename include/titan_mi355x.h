@@ -123,6 +123,18 @@ int titan_clear_planes_and_balls(titan_ctx *ctx);
 int titan_body_create(titan_ctx *ctx, const titan_body_desc *desc, int32_t *body_id);
 int titan_body_destroy(titan_ctx *ctx, int32_t body_id);
 int titan_body_count(const titan_ctx *ctx, int32_t *n);
+/* Remove springs and masses of a live body mid-run (replaces
+ * Simulation::deleteSpring / deleteMass after start, src/sim.cu:352-412):
+ * a removed spring exerts no force from the next step on; a removed mass
+ * stops moving (its state stays as it is, the reference no longer updates
+ * it) and every spring touching it is removed, as an invalidated endpoint
+ * makes a spring inert (src/sim.cu:1052).  Indices are the body's own:
+ * masses in titan_body_desc order, springs in its spring order; removing
+ * one twice changes nothing.  Only the incidence lists of the masses
+ * touched are rewritten, on the device; the rest of the layout is kept.
+ * Waits for the steps in flight. */
+int titan_body_remove(titan_ctx *ctx, int32_t body_id, const int64_t *masses, int64_t n_masses,
+                      const int64_t *springs, int64_t n_springs);
 
 /* ---- stepping (replaces the loop body of Simulation::execute,
  *      src/sim.cu:1569-1590) ---- */

@@ -206,6 +206,36 @@ static void readme_parity() {
     sim.stop();
 }
 
+// Deletions mid-run: a 10x10x10 lattice and a 3x3x3 one on the ground plane
+// step 60 steps; then every 37th spring of the first lattice, two of its
+// masses and the whole second lattice (deleteContainer) are deleted, and the
+// scene steps 90 more.  The live masses' final state is printed as hex
+// floats for the oracle comparison.
+static void delete_parity() {
+    titan::Simulation sim;
+    titan::Lattice *l = sim.createLattice(Vec(0, 0, 0.45), Vec(1, 1, 1), 10, 10, 10);
+    for (titan::Spring *s : l->springs) s->addDamping(0.1);
+    titan::Lattice *l2 = sim.createLattice(Vec(3, 0, 0.45), Vec(1, 1, 1), 3, 3, 3);
+    for (titan::Spring *s : l2->springs) s->addDamping(0.1);
+    sim.createPlane(Vec(0, 0, 1), 0, 0.8, 1.0);
+    sim.start();
+    sim.step(60);
+    sim.pause(sim.time());  // the first step boundary with T >= T: no further step
+    std::vector<titan::Spring *> del;
+    for (size_t k = 0; k < l->springs.size(); k += 37) del.push_back(l->springs[k]);
+    for (titan::Spring *s : del) sim.deleteSpring(s);
+    sim.deleteMass(l->masses[123]);
+    sim.deleteMass(l->masses[456]);
+    sim.deleteContainer(l2);
+    sim.resume();
+    sim.step(90);
+    sim.getAll();
+    std::printf("T %a\n", sim.time());
+    for (titan::Mass *m : sim.masses)
+        std::printf("M %a %a %a %a %a %a %a\n", m->pos[0], m->pos[1], m->pos[2], m->vel[0], m->vel[1], m->vel[2], m->T);
+    sim.stop();
+}
+
 // start / breakpoint / step / set-get round trips of the facade itself
 static void control_test() {
     titan::Simulation sim;
@@ -262,12 +292,13 @@ int main(int argc, char **argv) {
         {"multiagent", multiagent_test},
         {"rotate", rotate_test},
         {"readme_parity", readme_parity},
+        {"delete_parity", delete_parity},
         {"control", control_test},
     };
     if (argc >= 2 && std::string(argv[1]) == "all") {  // every test except the parity dump
         int bad = 0;
         for (auto &t : tests) {
-            if (t.first == "readme_parity") continue;
+            if (t.first == "readme_parity" || t.first == "delete_parity") continue;
             failures = 0;
             try {
                 t.second();

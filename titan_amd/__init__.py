@@ -38,7 +38,7 @@ EXPORTS = (
     "titan_serve_config", "titan_serve_submit", "titan_serve_pump", "titan_serve_poll",
     "titan_serve_cancel", "titan_serve_stats", "titan_serve_reset_stats", "titan_set_integrator",
     "titan_get_integrator", "titan_debug_inject_failure_at", "titan_step_noalloc",
-    "titan_measure_hbm",
+    "titan_measure_hbm", "titan_body_remove",
 )
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
@@ -97,6 +97,7 @@ def lib() -> ctypes.CDLL:
         "titan_clear_planes_and_balls": (c_int, [vp]),
         "titan_body_create": (c_int, [vp, POINTER(BodyDesc), POINTER(c_int32)]),
         "titan_body_destroy": (c_int, [vp, c_int32]),
+        "titan_body_remove": (c_int, [vp, c_int32, vp, c_int64, vp, c_int64]),
         "titan_body_count": (c_int, [vp, POINTER(c_int32)]),
         "titan_step": (c_int, [vp, c_int64]),
         "titan_step_noalloc": (c_int, [vp, c_int64]),
@@ -401,6 +402,13 @@ class Context:
     def destroy_body(self, bid: int):
         self._chk(self._L.titan_body_destroy(self._h, bid))
         self._bodies.pop(bid, None)
+
+    def remove(self, bid: int, masses=(), springs=()):
+        """Remove springs and masses of a live body (indices in its creation order)."""
+        m = np.ascontiguousarray(masses, dtype=np.int64)
+        s = np.ascontiguousarray(springs, dtype=np.int64)
+        self._chk(self._L.titan_body_remove(self._h, int(bid), m.ctypes.data_as(c_void_p), int(m.size),
+                                            s.ctypes.data_as(c_void_p), int(s.size)))
 
     def body_count(self) -> int:
         v = c_int32()

@@ -123,6 +123,8 @@ class Simulation {
     void pull_state();             // engine -> device records (pos, vel, acc, T)
     void push_state();             // device records -> engine (pos, vel, m, T, ext)
     void rebuild();                // re-create the engine body from the records
+    // apply deletions to the built body in place; false: it must be rebuilt
+    bool remove_in_place(const std::vector<Mass *> &ms, const std::vector<Spring *> &ss);
     void advance(int64_t n);       // rebuild if needed, then n engine steps
     int64_t steps_until_after(double t, bool strict, bool &hit_breakpoint);
     void check_alive(const char *what);

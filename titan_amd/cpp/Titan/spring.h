@@ -51,6 +51,7 @@ class Spring {
         double k = 0, rest = 0, damping = 0, omega = 0;
         int type = 0;
     } dev;
+    int64_t index = -1;  // position in the engine body's spring list, -1 = not on the device
     friend class Simulation;
 };
 

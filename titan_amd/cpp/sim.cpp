@@ -220,23 +220,41 @@ Spring *Simulation::createSpring(Mass *m1, Mass *m2) { return createSpring(new S
 // Deleted objects stay allocated until the simulation is destroyed: springs
 // may still point at a deleted mass, which then counts as missing and the
 // spring does nothing (src/sim.cu:1052).
+// On a built body, a deletion is applied in place (titan_body_remove): the
+// removed mass stays in the engine body, frozen, and only the incidence
+// lists it touches are rewritten; otherwise the body is rebuilt before the
+// next step.
+bool Simulation::remove_in_place(const std::vector<Mass *> &ms, const std::vector<Spring *> &ss) {
+    if (!STARTED || !ctx || body < 0 || dirty) return false;
+    std::vector<int64_t> mi, si;
+    for (Mass *m : ms)
+        if (m->index >= 0) mi.push_back(m->index);
+    for (Spring *s : ss)
+        if (s->index >= 0) si.push_back(s->index);
+    check(ctx, titan_body_remove(ctx, body, mi.data(), (int64_t)mi.size(), si.data(), (int64_t)si.size()));
+    for (Spring *s : ss) s->index = -1;
+    return true;
+}
+
 void Simulation::deleteMass(Mass *m) {
     check_alive("Cannot modify simulation after the end of the simulation.");
     if (STARTED && RUNNING) throw std::runtime_error("The simulation is running. Stop the simulation to make changes.");
-    if (STARTED && m->index >= 0) pull_state();
+    const bool in_place = remove_in_place({m}, {});
+    if (STARTED && !in_place && m->index >= 0) pull_state();
     erase_one(masses, m);
     m->valid = false;
     retired_masses.push_back(m);
-    if (STARTED) dirty = true;
+    if (STARTED && !in_place) dirty = true;
 }
 
 void Simulation::deleteSpring(Spring *s) {
     check_alive("Cannot modify simulation after the end of the simulation.");
     if (STARTED && RUNNING) throw std::runtime_error("The simulation is running. Stop the simulation to make changes.");
-    if (STARTED) pull_state();
+    const bool in_place = remove_in_place({}, {s});
+    if (STARTED && !in_place) pull_state();
     erase_one(springs, s);
     retired_springs.push_back(s);
-    if (STARTED) dirty = true;
+    if (STARTED && !in_place) dirty = true;
 }
 
 // CUDA_MASS(Mass&) (src/mass.cu:57-74): m, T, pos, vel, extern_force and the
@@ -348,7 +366,9 @@ void Simulation::rebuild() {
     }
     std::vector<int32_t> L(S), R(S), type(S);
     std::vector<double> k(S), rest(S), damping(S), omega(S);
+    for (Spring *sp : retired_springs) sp->index = -1;
     for (int64_t s = 0; s < S; ++s) {
+        springs[s]->index = s;
         const Spring::Record &r = springs[s]->dev;
         // a missing or deleted endpoint makes the spring inert (src/sim.cu:1052)
         L[s] = (r.left && r.left->valid && r.left->index >= 0) ? (int32_t)r.left->index : -1;
@@ -596,7 +616,8 @@ Container *Simulation::createContainer() {
 void Simulation::deleteContainer(Container *c) {
     check_alive("Cannot modify simulation after the end of the simulation.");
     if (RUNNING) throw std::runtime_error("The simulation is running. Stop the simulation to make changes.");
-    if (STARTED) pull_state();
+    const bool in_place = remove_in_place(c->masses, c->springs);
+    if (STARTED && !in_place) pull_state();
     for (Mass *m : c->masses) {
         erase_one(masses, m);
         m->valid = false;
@@ -610,7 +631,7 @@ void Simulation::deleteContainer(Container *c) {
     c->masses.clear();
     c->springs.clear();
     delete c;
-    if (STARTED) dirty = true;
+    if (STARTED && !in_place) dirty = true;
 }
 
 Cube *Simulation::createCube(const Vec &center, double side_length) {

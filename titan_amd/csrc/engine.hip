@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -113,6 +114,19 @@ struct BodyH {
     int64_t n = 0;  // steps taken (parity selects the state buffer)
     std::vector<uint32_t> flags;   // device order
     std::vector<int32_t> perm;     // device order -> caller's order (empty = identity)
+    // topology, for removals (titan_body_remove): each spring's endpoints in
+    // device order (-1: none) in the caller's spring order, removed springs,
+    // each chunk's first incidence entry (body-relative, nch + 1 entries),
+    // and the first mass of each chunk's tile (the LDS row base of the
+    // tiled and resident layouts)
+    std::vector<int32_t> sl, sr;
+    std::vector<uint8_t> sdead;
+    std::vector<int64_t> coff;
+    std::vector<int32_t> row0;
+    // per-mass spring lists in ascending spring index, built on the first
+    // removal (mptr: N + 1 offsets into mspr)
+    std::vector<int64_t> mptr;
+    std::vector<int32_t> mspr;
 };
 
 }  // namespace
@@ -983,6 +997,7 @@ struct PreparedBody {
     std::vector<double> spd;
     std::vector<int32_t> spa;
     int64_t nsp = 0, nreal = 0;
+    std::vector<int32_t> sl, sr;    // spring endpoints in device order (-1: none)
 };
 
 // The split form of a resident body (endpoints in device rows): its springs
@@ -1255,6 +1270,8 @@ PreparedBody prepare_body(const titan_schedule &sched, const titan_body_desc *d)
         }
         P.n_lc = (int64_t)P.lch.size();
     }
+    P.sl.swap(nl);
+    P.sr.swap(nr);
     return P;
 }
 
@@ -1508,6 +1525,13 @@ int32_t commit_body(titan_ctx *c, PreparedBody &P) {
     b.n = 0;
     b.flags.assign(P.fl.begin(), P.fl.begin() + N);
     b.perm = std::move(P.perm);
+    b.sl = std::move(P.sl);
+    b.sr = std::move(P.sr);
+    b.coff = sb.coff;
+    b.row0.assign(nch, 0);
+    if (tiled)
+        for (int64_t ch = 0; ch < nch; ++ch)
+            b.row0[ch] = (int32_t)((int64_t)tb.tiles[tb.tile_of_chunk[ch]].chunk0 - ch0) * kChunk;
     return (int32_t)bid;
 }
 
@@ -1535,6 +1559,164 @@ void body_destroy(titan_ctx *c, int32_t id) {
     HIPCHK(hipEventRecord(ev, c->stream));
     c->pending.push_back({b, id, ev});
     b = BodyH{};
+}
+
+// Remove springs and masses of a live body (Simulation::deleteSpring /
+// deleteMass after start, src/sim.cu:352-412).  A removed spring leaves the
+// incidence lists of both its endpoints; a removed mass becomes fixed (the
+// reference stops updating it: it leaves d_masses) and every spring
+// touching it is removed, as an invalidated endpoint makes a spring inert
+// (src/sim.cu:1052).  Only the chunks holding a touched mass are rewritten:
+// each touched lane's remaining incidences move up in order, the freed
+// positions become padding, and its degree shrinks; group counts, tiles and
+// halos are kept (a halo row no spring names any more is loaded but unused),
+// so the body's layout, captured graphs and results of untouched masses are
+// as before.  A resident body stops using its split form (its walk gives the
+// same results).
+void body_remove(titan_ctx *c, int32_t id, const int64_t *masses, int64_t n_m, const int64_t *springs,
+                 int64_t n_s) {
+    BodyH &b = body_of(c, id);
+    const int64_t N = b.n_masses, S = (int64_t)b.sl.size();
+    if (n_m < 0 || n_s < 0 || (n_m > 0 && !masses) || (n_s > 0 && !springs))
+        fail(TITAN_E_INVALID, "body_remove: bad arguments");
+    for (int64_t q = 0; q < n_m; ++q)
+        if (masses[q] < 0 || masses[q] >= N) fail(TITAN_E_INVALID, "body_remove: mass index out of range");
+    for (int64_t q = 0; q < n_s; ++q)
+        if (springs[q] < 0 || springs[q] >= S) fail(TITAN_E_INVALID, "body_remove: spring index out of range");
+    if (b.sdead.empty()) b.sdead.assign(S, 0);
+    if (b.mptr.empty()) {  // per-mass lists, in the order build_sell lists them
+        b.mptr.assign(N + 1, 0);
+        for (int64_t k = 0; k < S; ++k) {
+            if (b.sl[k] < 0 || b.sr[k] < 0) continue;
+            b.mptr[b.sr[k] + 1]++;
+            b.mptr[b.sl[k] + 1]++;
+        }
+        for (int64_t q = 0; q < N; ++q) b.mptr[q + 1] += b.mptr[q];
+        b.mspr.assign(b.mptr[N], 0);
+        std::vector<int64_t> at(b.mptr.begin(), b.mptr.end() - 1);
+        for (int64_t k = 0; k < S; ++k) {
+            if (b.sl[k] < 0 || b.sr[k] < 0) continue;
+            b.mspr[at[b.sr[k]]++] = (int32_t)k;  // the right end first (a self-spring is listed twice)
+            b.mspr[at[b.sl[k]]++] = (int32_t)k;
+        }
+    }
+    std::vector<int32_t> dev_of(N);  // caller's mass index -> device order
+    for (int64_t q = 0; q < N; ++q) dev_of[b.perm.empty() ? q : b.perm[q]] = (int32_t)q;
+    // springs removed now, and the masses whose lists change
+    std::vector<int32_t> gone;
+    std::vector<uint8_t> fix(N, 0);
+    for (int64_t q = 0; q < n_m; ++q) {
+        const int32_t m = dev_of[masses[q]];
+        fix[m] = 1;
+        for (int64_t e = b.mptr[m]; e < b.mptr[m + 1]; ++e) gone.push_back(b.mspr[e]);
+    }
+    for (int64_t q = 0; q < n_s; ++q) gone.push_back((int32_t)springs[q]);
+    std::sort(gone.begin(), gone.end());
+    gone.erase(std::unique(gone.begin(), gone.end()), gone.end());
+    std::vector<uint8_t> now(S, 0);
+    std::vector<int32_t> touched;
+    for (int32_t k : gone) {
+        if (b.sdead[k] || b.sl[k] < 0 || b.sr[k] < 0) continue;
+        now[k] = 1;
+        touched.push_back(b.sl[k]);
+        touched.push_back(b.sr[k]);
+    }
+    std::vector<int32_t> fixed_now;
+    for (int64_t m = 0; m < N; ++m)
+        if (fix[m] && !(b.flags[m] & kFixed)) fixed_now.push_back((int32_t)m);
+    std::sort(touched.begin(), touched.end());
+    touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
+    if (touched.empty() && fixed_now.empty()) return;
+
+    HIPCHK(hipStreamSynchronize(c->stream));  // no launch in flight reads what is rewritten
+    const bool packed = b.tiled || b.resident;
+    const int G = packed ? kGroupTiled : kGroupGather;
+    const int64_t g0 = b.chunk0 * kChunk;
+    const bool pk = (b.kd & 1u) != 0, pd = (b.kd & 2u) != 0;
+    size_t t = 0;
+    while (t < touched.size()) {
+        const int64_t ch = touched[t] / kChunk, e0 = b.coff[ch], len = b.coff[ch + 1] - e0;
+        std::vector<uint32_t> nbr(len);
+        std::vector<double> rest(len), kk(pk ? len : 0), dd(pd ? len : 0);
+        std::vector<int32_t> deg(kChunk);
+        d2h(c, nbr.data(), c->inc_nbr + b.inc0 + e0, len);
+        d2h(c, rest.data(), c->inc_rest + b.inc0 + e0, len);
+        if (pk) d2h(c, kk.data(), c->inc_k + b.inc0 + e0, len);
+        if (pd) d2h(c, dd.data(), c->inc_damp + b.inc0 + e0, len);
+        d2h(c, deg.data(), c->deg + g0 + ch * kChunk, kChunk);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        // the name of entry e (relative to the chunk), read and written
+        auto name = [&](int64_t e) -> uint32_t {
+            if (!packed) return nbr[e];
+            return (nbr[e / 2] >> (16 * (int)(e % 2))) & 0xffffu;
+        };
+        auto set_name = [&](int64_t e, uint32_t v) {
+            if (!packed) {
+                nbr[e] = v;
+                return;
+            }
+            uint32_t &w = nbr[e / 2];
+            const int sh = 16 * (int)(e % 2);
+            w = (w & ~(0xffffu << sh)) | ((v & 0xffffu) << sh);
+        };
+        auto entry = [&](int lane, int64_t j) -> int64_t { return ((j / G) * kChunk + lane) * G + j % G; };
+        for (; t < touched.size() && touched[t] / kChunk == ch; ++t) {
+            const int32_t m = touched[t];
+            const int lane = (int)(m % kChunk);
+            struct E {
+                uint32_t n;
+                double r, k, d;
+            };
+            std::vector<E> keep;
+            int64_t j = 0;  // position in the list as it stands on the device
+            for (int64_t e = b.mptr[m]; e < b.mptr[m + 1]; ++e) {
+                const int32_t k = b.mspr[e];
+                if (b.sdead[k]) continue;
+                if (!now[k]) {
+                    const int64_t x = entry(lane, j);
+                    keep.push_back({name(x), rest[x], pk ? kk[x] : 0.0, pd ? dd[x] : 0.0});
+                }
+                ++j;
+            }
+            if (j != deg[lane]) fail(TITAN_E_HIP, "body_remove: incidence list out of step with the device");
+            // padding names the mass itself (its row, or its arena slot)
+            const uint32_t self = packed ? (uint32_t)(m - b.row0[ch]) : (uint32_t)(g0 + m);
+            for (int64_t q = 0; q < j; ++q) {
+                const int64_t x = entry(lane, q);
+                if (q < (int64_t)keep.size()) {
+                    set_name(x, keep[q].n);
+                    rest[x] = keep[q].r;
+                    if (pk) kk[x] = keep[q].k;
+                    if (pd) dd[x] = keep[q].d;
+                } else {
+                    set_name(x, self);
+                    rest[x] = 1.0;
+                    if (pk) kk[x] = 0.0;
+                    if (pd) dd[x] = 0.0;
+                }
+            }
+            deg[lane] = (int32_t)keep.size();
+        }
+        h2d(c, c->inc_nbr + b.inc0 + e0, nbr.data(), len);
+        h2d(c, c->inc_rest + b.inc0 + e0, rest.data(), len);
+        if (pk) h2d(c, c->inc_k + b.inc0 + e0, kk.data(), len);
+        if (pd) h2d(c, c->inc_damp + b.inc0 + e0, dd.data(), len);
+        h2d(c, c->deg + g0 + ch * kChunk, deg.data(), kChunk);
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    for (int64_t k = 0; k < S; ++k)
+        if (now[k]) b.sdead[k] = 1;
+    for (int32_t m : fixed_now) {
+        b.flags[m] |= kFixed;
+        h2d(c, c->mflags + g0 + m, &b.flags[m], 1);
+    }
+    if (b.resident && b.res_scache > 0) {  // the split form lists every spring: walk instead
+        b.res_scache = 0;
+        const int32_t zero = 0;
+        h2d(c, reinterpret_cast<int32_t *>(reinterpret_cast<char *>(c->rdesc + b.res) + offsetof(ResidentDesc, scache)),
+            &zero, 1);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
 }
 
 // device order -> caller's order for a (N, K) host array read from K SoA rows
@@ -1865,6 +2047,14 @@ int titan_body_destroy(titan_ctx *c, int32_t body_id) {
     NEED_CTX(c);
     API_BEGIN
     body_destroy(c, body_id);
+    API_END(c)
+}
+
+int titan_body_remove(titan_ctx *c, int32_t body_id, const int64_t *masses, int64_t n_masses,
+                      const int64_t *springs, int64_t n_springs) {
+    NEED_CTX(c);
+    API_BEGIN
+    body_remove(c, body_id, masses, n_masses, springs, n_springs);
     API_END(c)
 }
 
