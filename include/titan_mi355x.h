@@ -323,6 +323,30 @@ int titan_debug_inject_failure(titan_ctx *ctx, int32_t after);
  * or TITAN_INJECT_DESTROY (titan_body_destroy); the failing call changes
  * nothing.  Used to test the serving loop's recovery from a failed
  * retirement. */
+/* A body's device layout, for tests that compare two layouts byte for byte
+ * (no counterpart in the reference): item `what` copied to out (at most cap
+ * bytes; *len = its full size), arena offsets made relative to the body's
+ * own ranges: TITAN_DEBUG_NAMES (uint32 words of its incidence span),
+ * _REST, _K, _DAMPING (double per entry; empty when shared), _DEG, _MFLAGS
+ * (per mass slot), _STATE (48 B per slot, buffer 0), _MASS, _TILES
+ * (32-byte tile records: body, first chunk, masses, halo size, halo
+ * offset, first-tile flag; body zeroed), _HALO (int32, body
+ * slots), _CHUNKS (4 int64 per chunk: first entry, groups, first actuator,
+ * actuators) and _PERM (int32 per mass: device order -> caller's order). */
+#define TITAN_DEBUG_NAMES 0
+#define TITAN_DEBUG_REST 1
+#define TITAN_DEBUG_K 2
+#define TITAN_DEBUG_DAMPING 3
+#define TITAN_DEBUG_DEG 4
+#define TITAN_DEBUG_MFLAGS 5
+#define TITAN_DEBUG_STATE 6
+#define TITAN_DEBUG_MASS 7
+#define TITAN_DEBUG_TILES 8
+#define TITAN_DEBUG_HALO 9
+#define TITAN_DEBUG_CHUNKS 10
+#define TITAN_DEBUG_PERM 11
+int titan_debug_body_bytes(titan_ctx *ctx, int32_t body_id, int32_t what, void *out, int64_t cap, int64_t *len);
+
 #define TITAN_INJECT_STEP 0
 #define TITAN_INJECT_DOWNLOAD 1
 #define TITAN_INJECT_DESTROY 2

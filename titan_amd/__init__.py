@@ -38,7 +38,7 @@ EXPORTS = (
     "titan_serve_config", "titan_serve_submit", "titan_serve_pump", "titan_serve_poll",
     "titan_serve_cancel", "titan_serve_stats", "titan_serve_reset_stats", "titan_set_integrator",
     "titan_get_integrator", "titan_debug_inject_failure_at", "titan_step_noalloc",
-    "titan_measure_hbm", "titan_body_remove",
+    "titan_measure_hbm", "titan_body_remove", "titan_debug_body_bytes",
 )
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
@@ -98,6 +98,7 @@ def lib() -> ctypes.CDLL:
         "titan_body_create": (c_int, [vp, POINTER(BodyDesc), POINTER(c_int32)]),
         "titan_body_destroy": (c_int, [vp, c_int32]),
         "titan_body_remove": (c_int, [vp, c_int32, vp, c_int64, vp, c_int64]),
+        "titan_debug_body_bytes": (c_int, [vp, c_int32, c_int32, vp, c_int64, POINTER(c_int64)]),
         "titan_body_count": (c_int, [vp, POINTER(c_int32)]),
         "titan_step": (c_int, [vp, c_int64]),
         "titan_step_noalloc": (c_int, [vp, c_int64]),
@@ -490,6 +491,17 @@ class Context:
         buf = ctypes.create_string_buffer(256)
         self._chk(self._L.titan_kernel_name(self._h, buf, 256))
         return buf.value.decode()
+
+    DEBUG_ITEMS = ("names", "rest", "k", "damping", "deg", "mflags", "state", "mass", "tiles", "halo", "chunks", "perm")
+
+    def layout_bytes(self, bid: int, item: str) -> bytes:
+        """One item of a body's device layout (titan_debug_body_bytes), as raw bytes."""
+        what = self.DEBUG_ITEMS.index(item)
+        n = c_int64()
+        self._chk(self._L.titan_debug_body_bytes(self._h, int(bid), what, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(max(1, n.value))
+        self._chk(self._L.titan_debug_body_bytes(self._h, int(bid), what, buf, n.value, ctypes.byref(n)))
+        return buf.raw[:n.value]
 
     def inject_failure(self, after: int, site: int = INJECT_STEP):
         """Make a call at `site` fail with TITAN_E_INJECTED after `after` successful ones."""

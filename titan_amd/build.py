@@ -34,8 +34,8 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-command-line-argument",
 ]
 
-ENGINE_SOURCES = [CSRC / "kernels.hip", CSRC / "engine.hip", CSRC / "tiles.cpp", CSRC / "lattice.cpp",
-                  CSRC / "serve.cpp"]
+ENGINE_SOURCES = [CSRC / "kernels.hip", CSRC / "engine.hip", CSRC / "builder.hip", CSRC / "tiles.cpp",
+                  CSRC / "lattice.cpp", CSRC / "serve.cpp"]
 FACADE_SOURCES = [CPP / "sim.cpp"]
 
 LIB = PKG / "libtitan_mi355x.so"

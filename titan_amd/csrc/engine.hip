@@ -25,6 +25,7 @@
 #include <string>
 #include <vector>
 
+#include "builder.h"
 #include "engine_internal.h"
 #include "kernels.h"
 #include "layout.h"
@@ -211,6 +212,10 @@ struct titan_ctx {
     // runtime inside hipGraphLaunch, with no fault found in the engine's use
     // of the graph, so traced runs launch each step directly instead
     bool use_graphs = true;
+    // tiled bodies of at least this many masses are laid out on the device
+    // (TITAN_DEVICE_BUILD: 0 never, a number: that threshold); its scratch
+    DevBuilder *builder = nullptr;
+    int64_t device_min = kDeviceBuildMin;
     // captured multi-step graphs, valid while `version` is unchanged, keyed
     // by the launch shape they were captured with (most recent last)
     uint64_t version = 1;
@@ -998,6 +1003,13 @@ struct PreparedBody {
     std::vector<int32_t> spa;
     int64_t nsp = 0, nreal = 0;
     std::vector<int32_t> sl, sr;    // spring endpoints in device order (-1: none)
+    // built on the device at commit (DevBuilder): the caller's descriptor
+    // (read during the commit), the schedule it was prepared under, each
+    // spring's body-local actuator (empty: none)
+    bool device = false;
+    const titan_body_desc *desc = nullptr;
+    titan_schedule sched{};
+    std::vector<int32_t> act_local;
 };
 
 // The split form of a resident body (endpoints in device rows): its springs
@@ -1036,7 +1048,9 @@ void build_split(PreparedBody &P, const SpringsIn &sp, const std::vector<int32_t
     P.nreal = nreal;
 }
 
-PreparedBody prepare_body(const titan_schedule &sched, const titan_body_desc *d) {
+// device_min: a tiled body of at least this many masses is laid out on the
+// device at commit (0: never)
+PreparedBody prepare_body(const titan_schedule &sched, const titan_body_desc *d, int64_t device_min) {
     if (!d) fail(TITAN_E_INVALID, "body_create: null descriptor");
     PreparedBody P;
     const int64_t N = d->n_masses, S = d->n_springs;
@@ -1145,6 +1159,24 @@ PreparedBody prepare_body(const titan_schedule &sched, const titan_body_desc *d)
                           (d->layout == TITAN_LAYOUT_AUTO && N >= 1 && N <= sched.resident_max_masses);
     const bool want_tiles = !resident && (d->layout == TITAN_LAYOUT_TILED ||
                                           (d->layout == TITAN_LAYOUT_AUTO && N >= sched.tiled_min_masses));
+    if (want_tiles && N > kTileMasses && device_min > 0 && N >= device_min && !(any_lc && !shared)) {
+        // the layout is built on the device at commit (builder.hip); here only
+        // what the host keeps: flags summaries, the shared local list
+        P.device = true;
+        P.desc = d;
+        P.sched = sched;
+        if (n_act > 0) P.act_local.swap(act_local);
+        for (int64_t i = 0; i < N && d->ext && !P.any_ext; ++i)
+            P.any_ext = d->ext[3 * i] != 0 || d->ext[3 * i + 1] != 0 || d->ext[3 * i + 2] != 0;
+        for (int64_t i = 0; i < N && d->drag && !P.any_drag; ++i) P.any_drag = d->drag[i] != 0;
+        P.m0 = d->m[0];
+        P.m_same = true;
+        for (int64_t i = 1; i < N && P.m_same; ++i) P.m_same = same_bits(d->m[i], P.m0);
+        if (any_lc) P.lch = lists[0];
+        P.n_lc = (int64_t)P.lch.size();
+        P.tiled = true;
+        return P;
+    }
     if (want_tiles && N > kTileMasses) perm = morton_order(d->pos, N);
     {
         std::vector<int32_t> inv;
@@ -1322,50 +1354,46 @@ void harvest_released(titan_ctx *c, bool wait) {
     c->pending.swap(keep);
 }
 
-// Place a prepared body on the device.  Its arrays are copied on the upload
-// stream into free arena space, which the launches in flight do not read;
-// the body then joins the launches through a small kernel on the step
-// stream, after the launches enqueued before it.  Nothing here waits for
-// the step stream (only growing an arena does).
-int32_t commit_body(titan_ctx *c, PreparedBody &P) {
-    harvest_released(c, false);
-    const int64_t N = P.N, nch = P.nch, npad = P.npad, n_act = P.n_act, n_lc = P.n_lc;
-    const bool tiled = P.tiled, resident = P.resident;
-    SellBuild &sb = P.sb;
-    TileBuild &tb = P.tb;
-    const int64_t n_tiles = (int64_t)tb.tiles.size(), n_halo = (int64_t)tb.halo.size();
-    const int64_t n_inc = sb.coff[nch];
-    // a resident body's split-form data follows its SELL entries
-    const int64_t n_extra = std::max<int64_t>((int64_t)P.spw.size(), (int64_t)P.spd.size());
+// Arena ranges of one body, and what a body needs reserved.
+struct Slots {
+    int64_t ch0 = -1, inc0 = -1, act0 = -1, lc0 = -1, tl0 = -1, hl0 = -1, rs0 = -1, bid = -1;
+};
+struct SlotNeeds {
+    int64_t nch, n_inc, n_act, n_lc, n_tiles, n_halo;
+    bool resident;
+};
 
-    // ---- reserve space; on failure release whatever was taken
-    int64_t ch0 = -1, inc0 = -1, act0 = -1, lc0 = -1, bid = -1, tl0 = -1, hl0 = -1, rs0 = -1;
-    auto take = [&](Ranges &r, int64_t len, void (*grow)(titan_ctx *, int64_t)) -> int64_t {
-        int64_t at = r.alloc(len);
+// Reserve a body's arena ranges, growing arenas as needed (slots of
+// destroyed bodies are reclaimed first), and create the side arrays it
+// needs; on failure release whatever was taken.
+Slots reserve_slots(titan_ctx *c, const SlotNeeds &n, const PreparedBody &P) {
+    Slots r;
+    auto take = [&](Ranges &rg, int64_t len, void (*grow)(titan_ctx *, int64_t)) -> int64_t {
+        int64_t at = rg.alloc(len);
         if (at < 0 && !c->pending.empty()) {  // slots of destroyed bodies first
             harvest_released(c, true);
-            at = r.alloc(len);
+            at = rg.alloc(len);
         }
         if (at < 0) {
-            grow(c, r.high_water() + len);
-            at = r.alloc(len);
+            grow(c, rg.high_water() + len);
+            at = rg.alloc(len);
         }
         return at;
     };
     try {
-        ch0 = take(c->chunks, nch, ensure_chunks);
-        if ((ch0 + nch) * kChunk > kMaxMassSlots)
+        r.ch0 = take(c->chunks, n.nch, ensure_chunks);
+        if ((r.ch0 + n.nch) * kChunk > kMaxMassSlots)
             fail(TITAN_E_NOMEM, "body_create: mass arena limit (2^26 slots) reached");
-        inc0 = take(c->incs, n_inc + n_extra, ensure_incs);
-        act0 = take(c->acts, n_act, ensure_acts);
-        lc0 = take(c->lcs, n_lc, ensure_lcs);
-        tl0 = take(c->tiles, n_tiles, ensure_tiles);
-        hl0 = take(c->halos, n_halo, ensure_halos);
-        if (resident) rs0 = take(c->rdescs, 1, ensure_rdescs);
-        bid = take(c->bids, 1, ensure_bodies);
+        r.inc0 = take(c->incs, n.n_inc, ensure_incs);
+        r.act0 = take(c->acts, n.n_act, ensure_acts);
+        r.lc0 = take(c->lcs, n.n_lc, ensure_lcs);
+        r.tl0 = take(c->tiles, n.n_tiles, ensure_tiles);
+        r.hl0 = take(c->halos, n.n_halo, ensure_halos);
+        if (n.resident) r.rs0 = take(c->rdescs, 1, ensure_rdescs);
+        r.bid = take(c->bids, 1, ensure_bodies);
         if (P.per_k) make_inc_array(c, c->inc_k);
         if (P.per_d) make_inc_array(c, c->inc_damp);
-        if (n_act > 0) make_act(c);
+        if (n.n_act > 0) make_act(c);
         // side arrays created on first use, zeroed before any upload into them
         bool made = false;
         if (P.any_ext && !c->ext) {
@@ -1388,16 +1416,46 @@ int32_t commit_body(titan_ctx *c, PreparedBody &P) {
             drop_graphs(c);
         }
     } catch (...) {
-        if (ch0 >= 0) c->chunks.release(ch0, nch);
-        if (inc0 >= 0) c->incs.release(inc0, n_inc + n_extra);
-        if (act0 >= 0) c->acts.release(act0, n_act);
-        if (lc0 >= 0) c->lcs.release(lc0, n_lc);
-        if (tl0 >= 0) c->tiles.release(tl0, n_tiles);
-        if (hl0 >= 0) c->halos.release(hl0, n_halo);
-        if (rs0 >= 0) c->rdescs.release(rs0, 1);
-        if (bid >= 0) c->bids.release(bid, 1);
+        if (r.ch0 >= 0) c->chunks.release(r.ch0, n.nch);
+        if (r.inc0 >= 0) c->incs.release(r.inc0, n.n_inc);
+        if (r.act0 >= 0) c->acts.release(r.act0, n.n_act);
+        if (r.lc0 >= 0) c->lcs.release(r.lc0, n.n_lc);
+        if (r.tl0 >= 0) c->tiles.release(r.tl0, n.n_tiles);
+        if (r.hl0 >= 0) c->halos.release(r.hl0, n.n_halo);
+        if (r.rs0 >= 0) c->rdescs.release(r.rs0, 1);
+        if (r.bid >= 0) c->bids.release(r.bid, 1);
         throw;
     }
+    return r;
+}
+
+// Place a prepared body on the device.  Its arrays are copied on the upload
+// stream into free arena space, which the launches in flight do not read;
+// the body then joins the launches through a small kernel on the step
+// stream, after the launches enqueued before it.  Nothing here waits for
+// the step stream (only growing an arena does).
+int32_t commit_device_built(titan_ctx *c, PreparedBody &P);
+
+int32_t commit_body(titan_ctx *c, PreparedBody &P) {
+    if (P.device) {
+        const int32_t id = commit_device_built(c, P);
+        if (id >= 0) return id;
+        P = prepare_body(P.sched, P.desc, 0);  // a tile outgrew the row budget: the host's greedy decides
+    }
+    harvest_released(c, false);
+    const int64_t N = P.N, nch = P.nch, npad = P.npad, n_act = P.n_act, n_lc = P.n_lc;
+    const bool tiled = P.tiled, resident = P.resident;
+    SellBuild &sb = P.sb;
+    TileBuild &tb = P.tb;
+    const int64_t n_tiles = (int64_t)tb.tiles.size(), n_halo = (int64_t)tb.halo.size();
+    const int64_t n_inc = sb.coff[nch];
+    // a resident body's split-form data follows its SELL entries
+    const int64_t n_extra = std::max<int64_t>((int64_t)P.spw.size(), (int64_t)P.spd.size());
+
+    // ---- reserve space (released again on failure)
+    const Slots sl = reserve_slots(c, {nch, n_inc + n_extra, n_act, n_lc, n_tiles, n_halo, resident}, P);
+    const int64_t ch0 = sl.ch0, inc0 = sl.inc0, act0 = sl.act0, lc0 = sl.lc0, tl0 = sl.tl0, hl0 = sl.hl0,
+                  rs0 = sl.rs0, bid = sl.bid;
     const int64_t g0 = ch0 * kChunk;
 
     // ---- arena addresses: gather slots, actuated incidences, tiles, local lists
@@ -1535,9 +1593,143 @@ int32_t commit_body(titan_ctx *c, PreparedBody &P) {
     return (int32_t)bid;
 }
 
+// Place a body built on the device (DevBuilder): its layout is computed by
+// the builder's first phase, the arena ranges reserved for the sizes it
+// found, and its second phase writes the body into them; the body then joins
+// the launches as a host-built one does.  Returns -1 (nothing reserved) when
+// a tile would not fit the row budget, for the host build to decide.
+int32_t commit_device_built(titan_ctx *c, PreparedBody &P) {
+    harvest_released(c, false);
+    const titan_body_desc *d = P.desc;
+    if (!c->builder) c->builder = new DevBuilder;
+    DevBuildIn in;
+    in.N = P.N;
+    in.S = P.S;
+    in.pos = d->pos;
+    in.vel = d->vel;
+    in.m = d->m;
+    in.T = d->T;
+    in.ext = P.any_ext ? d->ext : nullptr;
+    in.fixed = d->fixed;
+    in.drag = P.any_drag ? d->drag : nullptr;
+    in.left = d->left;
+    in.right = d->right;
+    in.rest = d->rest;
+    in.k = P.per_k ? d->k : nullptr;
+    in.damping = P.per_d ? d->damping : nullptr;
+    in.act_local = P.act_local.empty() ? nullptr : P.act_local.data();
+    in.per_k = P.per_k;
+    in.per_d = P.per_d;
+    in.any_ext = P.any_ext;
+    in.any_drag = P.any_drag;
+    in.tile_rows = P.sched.tile_rows;
+    DevBuildSizes sz;
+    HIPCHK(c->builder->plan(in, c->up, &sz));
+    if (!sz.fits) return -1;
+    const int64_t N = P.N, nch = sz.nch, n_act = P.n_act, n_lc = P.n_lc;
+    const Slots sl = reserve_slots(c, {nch, sz.n_inc, n_act, n_lc, sz.ntiles, sz.n_halo, false}, P);
+    BodyH b;
+    b.perm.resize(N);
+    b.flags.resize(N);
+    b.sl.resize(P.S);
+    b.sr.resize(P.S);
+    b.coff.resize(nch + 1);
+    DevBuildOut o;
+    o.ncap = c->ncap;
+    o.st = c->st;
+    o.m = c->m;
+    o.Tm = c->Tm;
+    o.acc = c->acc;
+    o.ext = c->ext;
+    o.mflags = c->mflags;
+    o.deg = c->deg;
+    o.drag = c->drag;
+    o.mlc = c->mlc;
+    o.chunk_ptr = c->chunk_ptr;
+    o.chunk_act0 = c->chunk_act0;
+    o.chunk_ng = c->chunk_ng;
+    o.chunk_nact = c->chunk_nact;
+    o.inc_nbr = c->inc_nbr;
+    o.inc_rest = c->inc_rest;
+    o.inc_k = P.per_k ? c->inc_k : nullptr;
+    o.inc_damp = P.per_d ? c->inc_damp : nullptr;
+    o.tiles = c->tile;
+    o.halo = c->halo;
+    o.ch0 = sl.ch0;
+    o.inc0 = sl.inc0;
+    o.tl0 = sl.tl0;
+    o.hl0 = sl.hl0;
+    o.act0 = sl.act0;
+    o.n_act = n_act;
+    const DevBuildMirror mr{b.perm.data(), b.flags.data(), b.sl.data(), b.sr.data(), b.coff.data()};
+    hipError_t e = c->builder->write(o, mr, c->up);
+    if (e == hipSuccess && !P.acth.empty())
+        e = hipMemcpyAsync(c->act + 4 * sl.act0, P.acth.data(), sizeof(double) * P.acth.size(), hipMemcpyHostToDevice,
+                           c->up);
+    if (e == hipSuccess && n_lc > 0)
+        e = hipMemcpyAsync(c->lc + sl.lc0, P.lch.data(), sizeof(LocalC) * n_lc, hipMemcpyHostToDevice, c->up);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->up);
+    if (e != hipSuccess) {
+        c->chunks.release(sl.ch0, nch);
+        c->incs.release(sl.inc0, sz.n_inc);
+        c->acts.release(sl.act0, n_act);
+        c->lcs.release(sl.lc0, n_lc);
+        c->tiles.release(sl.tl0, sz.ntiles);
+        c->halos.release(sl.hl0, sz.n_halo);
+        c->bids.release(sl.bid, 1);
+        fail(TITAN_E_HIP, std::string("body_create (device build): ") + hipGetErrorString(e));
+    }
+    // ---- join the launches
+    BodyInit bi{};
+    bi.bid = (int32_t)sl.bid;
+    bi.chunk0 = sl.ch0;
+    bi.nchunks = nch;
+    bi.tile0 = sl.tl0;
+    bi.ntiles = sz.ntiles;
+    bi.res = -1;
+    bi.t0 = P.t0;
+    bi.k = P.k0;
+    bi.m = P.m0;
+    bi.d = P.d0;
+    bi.lcb = (int32_t)(P.shared ? sl.lc0 : 0);
+    bi.lcn = (int32_t)(P.shared ? n_lc : 0);
+    bi.mode = kModeTiled;
+    bi.kd = (uint8_t)((P.per_k ? 1 : 0) | (P.per_d ? 2 : 0) | (P.m_same ? kKdUniformM : 0));
+    bi.on = 1;
+    HIPCHK(launch_set_body(body_table(c), bi, c->stream));
+    BodyH &h = c->bodies[sl.bid];
+    h = std::move(b);
+    h.live = true;
+    h.tiled = true;
+    h.resident = false;
+    h.kd = bi.kd;
+    h.n_masses = N;
+    h.chunk0 = sl.ch0;
+    h.nchunks = nch;
+    h.inc0 = sl.inc0;
+    h.inc_len = sz.n_inc;
+    h.act0 = sl.act0;
+    h.act_len = n_act;
+    h.lc0 = sl.lc0;
+    h.lc_len = n_lc;
+    h.tile0 = sl.tl0;
+    h.ntiles = sz.ntiles;
+    h.halo0 = sl.hl0;
+    h.nhalo = sz.n_halo;
+    h.rows = sz.rmax;
+    h.T = P.t0;
+    h.n = 0;
+    h.row0.assign(nch, 0);
+    for (int64_t ch = 0; ch < nch; ++ch) h.row0[ch] = (int32_t)(ch / (kTileMasses / kChunk)) * kTileMasses;
+    return (int32_t)sl.bid;
+}
+
 int32_t body_create(titan_ctx *c, const titan_body_desc *d) {
-    PreparedBody P = prepare_body(c->sched, d);
-    return commit_body(c, P);
+    PreparedBody P = prepare_body(c->sched, d, c->device_min);
+    const int32_t id = commit_body(c, P);
+    // a large build's scratch is not kept
+    if (c->builder && c->builder->pool_bytes() > ((size_t)256 << 20)) c->builder->release();
+    return id;
 }
 
 // The body leaves the launches through a kernel on the step stream; its
@@ -1907,6 +2099,8 @@ int titan_ctx_create(int device, uint64_t hbm_budget_bytes, titan_ctx **out) {
     {
         const char *g = std::getenv("TITAN_GRAPHS");
         c->use_graphs = g ? std::strcmp(g, "0") != 0 : std::getenv("ROCPROF_OUTPUT_PATH") == nullptr;
+        const char *db = std::getenv("TITAN_DEVICE_BUILD");
+        if (db) c->device_min = std::max<int64_t>(0, std::atoll(db));
     }
     HIPCHK(set_tiles_lds_limit(&c->res_lds_max));
     c->scratch = device_scratch(device);
@@ -1949,6 +2143,7 @@ int titan_ctx_destroy(titan_ctx *c) {
     for (auto &p : c->pending) (void)hipEventDestroy(p.ev);
     for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
     for (auto &a : c->allocs) (void)hipFree(a.first);
+    delete c->builder;
     for (int k = 0; k < 2; ++k) {
         if (c->left_ev[k]) (void)hipEventDestroy(c->left_ev[k]);
         if (c->left_pin[k]) (void)hipHostFree(c->left_pin[k]);
@@ -2390,6 +2585,75 @@ int titan_measure_hbm(int device, uint64_t bytes, int32_t iters, double *copy_gb
     return rc;
 }
 
+int titan_debug_body_bytes(titan_ctx *c, int32_t body_id, int32_t what, void *out, int64_t cap, int64_t *len) {
+    NEED_CTX(c);
+    API_BEGIN
+    const BodyH &b = body_of(c, body_id);
+    const int64_t g0 = b.chunk0 * kChunk, npad = b.nchunks * kChunk;
+    const int64_t ninc = b.coff.empty() ? 0 : b.coff.back();
+    std::vector<char> buf;
+    auto dev = [&](const void *src, size_t bytes) {
+        buf.resize(bytes);
+        if (bytes) HIPCHK(hipMemcpy(buf.data(), src, bytes, hipMemcpyDeviceToHost));
+    };
+    HIPCHK(hipStreamSynchronize(c->stream));
+    switch (what) {
+        case TITAN_DEBUG_NAMES: dev(c->inc_nbr + b.inc0, 4 * (size_t)ninc); break;
+        case TITAN_DEBUG_REST: dev(c->inc_rest + b.inc0, 8 * (size_t)ninc); break;
+        case TITAN_DEBUG_K: dev(b.kd & 1u ? c->inc_k + b.inc0 : nullptr, b.kd & 1u ? 8 * (size_t)ninc : 0); break;
+        case TITAN_DEBUG_DAMPING: dev(b.kd & 2u ? c->inc_damp + b.inc0 : nullptr, b.kd & 2u ? 8 * (size_t)ninc : 0); break;
+        case TITAN_DEBUG_DEG: dev(c->deg + g0, 4 * (size_t)npad); break;
+        case TITAN_DEBUG_MFLAGS: dev(c->mflags + g0, 4 * (size_t)npad); break;
+        case TITAN_DEBUG_STATE: dev(c->st + 6 * g0, 48 * (size_t)npad); break;
+        case TITAN_DEBUG_MASS: dev(c->m + g0, 8 * (size_t)npad); break;
+        case TITAN_DEBUG_TILES: {
+            dev(c->tile + b.tile0, sizeof(TileDesc) * (size_t)b.ntiles);
+            TileDesc *t = reinterpret_cast<TileDesc *>(buf.data());
+            for (int64_t q = 0; q < b.ntiles; ++q) {  // relative to the body's ranges
+                t[q].body = 0;
+                t[q].chunk0 -= (int32_t)b.chunk0;
+                t[q].halo_off -= b.halo0;
+            }
+            break;
+        }
+        case TITAN_DEBUG_HALO: {
+            dev(c->halo + b.halo0, 4 * (size_t)b.nhalo);
+            int32_t *h = reinterpret_cast<int32_t *>(buf.data());
+            for (int64_t q = 0; q < b.nhalo; ++q) h[q] -= (int32_t)g0;
+            break;
+        }
+        case TITAN_DEBUG_CHUNKS: {  // chunk_ptr - inc0, chunk_ng, chunk_act0 - act0, chunk_nact: 4 int64 per chunk
+            std::vector<int64_t> p(b.nchunks), a(b.nchunks);
+            std::vector<int32_t> ng(b.nchunks), na(b.nchunks);
+            HIPCHK(hipMemcpy(p.data(), c->chunk_ptr + b.chunk0, 8 * b.nchunks, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(a.data(), c->chunk_act0 + b.chunk0, 8 * b.nchunks, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(ng.data(), c->chunk_ng + b.chunk0, 4 * b.nchunks, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(na.data(), c->chunk_nact + b.chunk0, 4 * b.nchunks, hipMemcpyDeviceToHost));
+            buf.resize(32 * (size_t)b.nchunks);
+            int64_t *o = reinterpret_cast<int64_t *>(buf.data());
+            for (int64_t q = 0; q < b.nchunks; ++q) {
+                o[4 * q] = p[q] - b.inc0;
+                o[4 * q + 1] = ng[q];
+                o[4 * q + 2] = a[q] - b.act0;
+                o[4 * q + 3] = na[q];
+            }
+            break;
+        }
+        case TITAN_DEBUG_PERM: {
+            buf.resize(4 * (size_t)b.n_masses);
+            for (int64_t q = 0; q < b.n_masses; ++q) {
+                const int32_t v = b.perm.empty() ? (int32_t)q : b.perm[q];
+                std::memcpy(buf.data() + 4 * q, &v, 4);
+            }
+            break;
+        }
+        default: fail(TITAN_E_INVALID, "debug_body_bytes: unknown item");
+    }
+    if (len) *len = (int64_t)buf.size();
+    if (out && cap > 0) std::memcpy(out, buf.data(), (size_t)std::min<int64_t>(cap, (int64_t)buf.size()));
+    API_END(c)
+}
+
 int titan_debug_inject_failure(titan_ctx *c, int32_t after) { return titan_debug_inject_failure_at(c, TITAN_INJECT_STEP, after); }
 
 int titan_debug_inject_failure_at(titan_ctx *c, int32_t site, int32_t after) {
@@ -2410,7 +2674,7 @@ struct Prepared {
 Prepared *prepare_body_host(const titan_ctx *c, const titan_body_desc *d, int *rc, std::string *err) {
     try {
         std::unique_ptr<Prepared> p(new Prepared);
-        p->body = prepare_body(c->sched, d);
+        p->body = prepare_body(c->sched, d, c->device_min);
         *rc = TITAN_OK;
         return p.release();
     } catch (const Fail &f) {

@@ -47,15 +47,19 @@ class Scene:
                 n += int(np.count_nonzero(np.asarray(t) >= 2))
         return n
 
-    def load(self, ctx: Context, layout=0) -> list[int]:
-        """Set the scene's globals on ctx and create its bodies; `layout` is one
-        layout for every body or a list with one per body."""
+    def load_globals(self, ctx: Context):
+        """Set the scene's time step, gravity, planes and balls on ctx."""
         ctx.set_dt(self.dt)
         ctx.set_global_acc(self.g)
         for p in self.planes:
             ctx.add_plane(p[:3], p[3], p[4], p[5])
         for b in self.balls:
             ctx.add_ball(b[:3], b[3])
+
+    def load(self, ctx: Context, layout=0) -> list[int]:
+        """Set the scene's globals on ctx and create its bodies; `layout` is one
+        layout for every body or a list with one per body."""
+        self.load_globals(ctx)
         lays = list(layout) if isinstance(layout, (list, tuple)) else [layout] * len(self.bodies)
         return [ctx.add_body(**b, layout=l) for b, l in zip(self.bodies, lays)]
 
