@@ -547,6 +547,11 @@ int32_t resident_cache(const titan_ctx *c, bool few) {
 
 uint8_t mode_of(const BodyH &b) { return b.resident ? kModeResident : b.tiled ? kModeTiled : kModeGather; }
 
+struct Range;
+// Whether every resident body a launch covers steps in LDS in the split form
+// (then the launch uses k_rollout_split).
+bool all_split(const titan_ctx *c, const Range &rg, const StepArgs &a);
+
 bool any_body(const titan_ctx *c, uint8_t mode) {
     for (auto &b : c->bodies)
         if (b.live && mode_of(b) == mode) return true;
@@ -598,6 +603,16 @@ Range body_range(const BodyH &b) {
     r.resident = b.resident;
     r.all = false;
     return r;
+}
+
+bool all_split(const titan_ctx *c, const Range &rg, const StepArgs &a) {
+    bool any = false;
+    for (const auto &b : c->bodies) {
+        if (!(b.live && b.resident) || b.res < rg.r0 || b.res >= rg.r1) continue;
+        if (!(b.res_scache > 0 && b.res_scache <= a.res_cache && b.n_masses <= a.res_rows)) return false;
+        any = true;
+    }
+    return any;
 }
 
 StepArgs make_args(titan_ctx *c, int slot, const Range &rg) {
@@ -675,6 +690,7 @@ StepArgs make_args(titan_ctx *c, int slot, const Range &rg) {
     a.res_rows = resident_rows(c);
     a.res_split = resident_few(c) ? 1 : 0;
     a.res_cache = resident_cache(c, a.res_split != 0);
+    if (a.res_split && all_split(c, rg, a)) a.res_split = 2;
     return a;
 }
 
@@ -2518,6 +2534,10 @@ int titan_kernel_name(const titan_ctx *c, char *buf, int64_t len) {
     const char *parts[3] = {any_body(c, kModeGather) ? step_kernel_name(act) : nullptr,
                             any_body(c, kModeTiled) ? step_tiles_kernel_name(act) : nullptr,
                             any_body(c, kModeResident) ? resident_kernel_name(act) : nullptr};
+    if (parts[2]) {
+        const StepArgs a = make_args(const_cast<titan_ctx *>(c), c->slot, full_range(c));
+        if (a.res_split == 2) parts[2] = split_kernel_name(act);
+    }
     int64_t at = 0;
     buf[0] = 0;
     for (const char *p : parts) {

@@ -20,6 +20,8 @@ hipError_t launch_actuate(const StepArgs &a, hipStream_t s);
 size_t resident_lds_bytes(int rows);
 hipError_t launch_rollout_resident(const StepArgs &a, bool actuated, int final_slot, hipStream_t s);
 const char *resident_kernel_name(bool actuated);
+// the resident launch whose bodies are all split-form bodies in LDS
+const char *split_kernel_name(bool actuated);
 // Raise the dynamic LDS limits of the tiled and resident kernels; *resident_max
 // receives the resident kernel's limit (the device's per-workgroup maximum
 // or what the kernel can use, the smaller).

@@ -951,12 +951,12 @@ __device__ __forceinline__ double signed_term(double v, bool neg) {
     return __hiloint2double(__double2hiint(v) ^ (neg ? (int)0x80000000 : 0), __double2loint(v));
 }
 
-// The terms G of springs x, x + 256, ..., x + (U - 1) 256 of a split-form
+// The terms G of springs x, x + B, ..., x + (U - 1) B (B = BLOCK) of a split-form
 // body (those below nsp), each from its endpoint a's side and formed exactly
 // as the walk forms the term at a, stored as -G at a's incidence and +G at
 // b's; U springs at a time so that their latencies overlap (a lane past the
 // end repeats spring x and stores it once).
-template <bool ACT, int U>
+template <bool ACT, int U, int BLOCK = kResidentBlock>
 __device__ __forceinline__ void split_terms(int x, int nsp, const uint32_t *__restrict__ csw,
                                             const double *__restrict__ cr, const double *__restrict__ kp,
                                             const double *__restrict__ dp, const BodyKD &bkd,
@@ -968,7 +968,7 @@ __device__ __forceinline__ void split_terms(int x, int nsp, const uint32_t *__re
     int xi[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int xu = x + u * kResidentBlock;
+        const int xu = x + u * BLOCK;
         xi[u] = xu < nsp ? xu : x;
         w[u] = csw[xi[u]];
     }
@@ -1024,7 +1024,7 @@ __device__ __forceinline__ void split_terms(int x, int nsp, const uint32_t *__re
         const double mag = kk[u] * (rl[u] - nrm[u]);
         const double dd = (dvx[u] * tx[u] + dvy[u] * ty[u] + dvz[u] * tz[u]) * dmp[u];
         const double gx = tx[u] * mag + tx[u] * dd, gy = ty[u] * mag + ty[u] * dd, gz = tz[u] * mag + tz[u] * dd;
-        if (x + u * kResidentBlock < nsp) {
+        if (x + u * BLOCK < nsp) {
             const uint32_t iw = csw[nsp + xi[u]];
             const int ia = (int)(iw & 0xffffu), ib = (int)(iw >> 16);
             fx[ia] = signed_term(gx, true);
@@ -1044,7 +1044,7 @@ __device__ __forceinline__ void split_terms(int x, int nsp, const uint32_t *__re
 // adds up its incidences in ascending spring index as the walk does and
 // runs the mass pass.  A body of one or two chunks then spreads its spring
 // work over four waves, with half the evaluations of the walk.
-template <bool ACT>
+template <bool ACT, int BLOCK = kResidentBlock>
 __device__ __forceinline__ void resident_steps_split(const StepArgs &a, const uint32_t *__restrict__ csw,
                                                      const double *__restrict__ cr, const double *__restrict__ kp,
                                                      const double *__restrict__ dp, double *__restrict__ fx,
@@ -1073,10 +1073,21 @@ __device__ __forceinline__ void resident_steps_split(const StepArgs &a, const ui
         const double2 *__restrict__ cur = (k & 1) ? B1 : B0;
         double2 *__restrict__ nxt = (k & 1) ? B0 : B1;
         const double *__restrict__ rc = a.ract[k & 1];
-        int x = t;
-        for (; x + (U - 1) * kResidentBlock < nsp; x += U * kResidentBlock)
-            split_terms<ACT, U>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
-        for (; x < nsp; x += kResidentBlock) split_terms<ACT, 1>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
+        if (BLOCK == kResidentBlock) {
+            int x = t;
+            for (; x + (U - 1) * BLOCK < nsp; x += U * BLOCK)
+                split_terms<ACT, U, BLOCK>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
+            for (; x < nsp; x += BLOCK) split_terms<ACT, 1, BLOCK>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
+        } else if (nsp <= BLOCK) {  // one spring per thread
+            if (t < nsp) split_terms<ACT, 1, BLOCK>(t, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
+        } else if (nsp <= 2 * BLOCK) {  // two or three per thread, interleaved, every wave alike
+            split_terms<ACT, 2, BLOCK>(t, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
+        } else if (nsp <= 3 * BLOCK) {
+            split_terms<ACT, 3, BLOCK>(t, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
+        } else {
+            for (int x = t; x < nsp; x += 2 * BLOCK)
+                split_terms<ACT, 2, BLOCK>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
+        }
         __syncthreads();
         if (mine && !(fl & kPad)) {
             const double2 o0 = cur[3 * q], o1 = cur[3 * q + 1], o2 = cur[3 * q + 2];
@@ -1116,7 +1127,7 @@ __device__ __forceinline__ void resident_steps_split(const StepArgs &a, const ui
             }
         }
         T = T + a.hdt;  // src/sim.cu:1590 (RK2: :1577,1582)
-        if (ACT && k + 1 < K) actuate_slice(a, act0, nact, T, a.ract[(k + 1) & 1], t, kResidentBlock);
+        if (ACT && k + 1 < K) actuate_slice(a, act0, nact, T, a.ract[(k + 1) & 1], t, BLOCK);
         __syncthreads();
     }
 }
@@ -1125,33 +1136,77 @@ __device__ __forceinline__ void resident_steps_split(const StepArgs &a, const ui
 // (an exclusive scan of the degrees), then copies of the springs' endpoint
 // and incidence words, rest lengths, k and damping from the body's
 // incidence span.  Returns the incidence count.
+// The first kSplitMaxChunks waves hold one row per thread; a larger
+// workgroup's other threads only help with the copies.
 static_assert(kResidentBlock == kSplitMaxChunks * kChunk, "one thread per row of a split-form body");
+template <int BLOCK = kResidentBlock>
 __device__ __forceinline__ int split_fill(const StepArgs &a, char *__restrict__ base, int nch, int64_t g0,
                                           int64_t e0, int nsp, int nd) {
     const int t = threadIdx.x, wave = t >> 6, lane = t & (kChunk - 1);
     int *degs = reinterpret_cast<int *>(base);
     int *starts = degs + kSplitMaxChunks * kChunk;
     int *wsum = starts + kSplitMaxChunks * kChunk;
-    const int dg = t < nch * kChunk ? a.deg[g0 + t] : 0;
+    const bool row = wave < kSplitMaxChunks;
+    const int dg = (row && t < nch * kChunk) ? a.deg[g0 + t] : 0;
     int inc = dg;  // inclusive scan over the wave
 #pragma unroll
     for (int off = 1; off < kChunk; off <<= 1) {
         const int v = __shfl_up(inc, off);
         if (lane >= off) inc += v;
     }
-    if (lane == kChunk - 1) wsum[wave] = inc;
+    if (row && lane == kChunk - 1) wsum[wave] = inc;
     __syncthreads();
     int pre = 0;
-    for (int w = 0; w < wave; ++w) pre += wsum[w];
+    for (int w = 0; w < wave && w < kSplitMaxChunks; ++w) pre += wsum[w];
     const int nreal = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    degs[t] = dg;
-    starts[t] = pre + inc - dg;
+    if (row) {
+        degs[t] = dg;
+        starts[t] = pre + inc - dg;
+    }
     uint32_t *cw = reinterpret_cast<uint32_t *>(base + kSplitHead);  // endpoint words, then incidence words
     double *cr = reinterpret_cast<double *>(base + ((kSplitHead + 8 * (int64_t)nsp + 15) & ~(int64_t)15));
-    for (int j = t; j < 2 * nsp; j += kResidentBlock) cw[j] = a.inc_nbr[e0 + j];
-    for (int j = t; j < nd; j += kResidentBlock) cr[j] = a.inc_rest[e0 + j];
+    for (int j = t; j < 2 * nsp; j += BLOCK) cw[j] = a.inc_nbr[e0 + j];
+    for (int j = t; j < nd; j += BLOCK) cr[j] = a.inc_rest[e0 + j];
     __syncthreads();
     return nreal;
+}
+
+// A split-form body held in LDS, stepped by a workgroup of BLOCK threads:
+// its state copied in, the split-form region filled, K steps, the state
+// copied out to the buffer of its new step parity.
+template <bool ACT, int BLOCK>
+__device__ __forceinline__ void split_in_lds(const StepArgs &a, double2 *__restrict__ S,
+                                             const ResidentDesc *__restrict__ rd, int b, const BodyKD &bkd, int nch,
+                                             int nmass, int64_t inc0, int ninc, int64_t g0, int64_t K, int64_t nst,
+                                             double &T, int nact, int64_t act0) {
+    const int t = threadIdx.x;
+    const int64_t n6 = a.ncap;
+    double2 *__restrict__ B0 = S, *__restrict__ B1 = S + 3 * a.res_rows;
+    {
+        const double2 *__restrict__ src = reinterpret_cast<const double2 *>(a.st + (int64_t)(nst & 1) * 6 * n6);
+        for (int q = t; q < 3 * nmass; q += BLOCK) B0[q] = src[3 * g0 + q];
+    }
+    char *base = reinterpret_cast<char *>(S + 6 * a.res_rows);
+    const int nsp = rd->nsp;
+    const int nd = nsp * (1 + ((bkd.kd & 1u) ? 1 : 0) + ((bkd.kd & 2u) ? 1 : 0));
+    const int nreal = split_fill<BLOCK>(a, base, nch, g0, inc0 + ninc, nsp, nd);
+    const int *degs = reinterpret_cast<const int *>(base);
+    const int *starts = degs + kSplitMaxChunks * kChunk;
+    const uint32_t *csw = reinterpret_cast<const uint32_t *>(base + kSplitHead);
+    const double *cr = reinterpret_cast<const double *>(base + ((kSplitHead + 8 * (int64_t)nsp + 15) & ~(int64_t)15));
+    // per-spring k and damping where the body has them; else a valid array
+    // read in their place, whose values are then replaced
+    const double *ck = cr + nsp;
+    const double *cd = ck + ((bkd.kd & 1u) ? nsp : 0);
+    const double *kp = (bkd.kd & 1u) ? ck : cr;
+    const double *dp = (bkd.kd & 2u) ? cd : cr;
+    double *fx = const_cast<double *>(cr) + nd;
+    double *fy = fx + nreal, *fz = fy + nreal;
+    resident_steps_split<ACT, BLOCK>(a, csw, cr, kp, dp, fx, fy, fz, degs, starts, nsp, b, bkd, nch, g0, K, nst, T,
+                                     nact, act0, B0, B1);
+    const double2 *__restrict__ fin = (K & 1) ? B1 : B0;
+    double2 *__restrict__ dst = reinterpret_cast<double2 *>(a.st + (int64_t)((nst + K) & 1) * 6 * n6);
+    for (int q = t; q < 3 * nmass; q += BLOCK) store_nt(&dst[3 * g0 + q], fin[q].x, fin[q].y);
 }
 
 // The steps of a resident body held in the launch's LDS: its state is
@@ -1289,6 +1344,40 @@ __global__ __launch_bounds__(kResidentBlock) void k_rollout_resident(const StepA
     }
 }
 
+// The resident launch when every body in it is in LDS in the split form:
+// workgroups of kSplitBlock threads, so that a body's spring terms are spread
+// over four times as many threads (the README scene's 1,036 springs: one per
+// thread), with the same arithmetic and order as k_rollout_resident.
+template <bool ACT>
+__global__ __launch_bounds__(kSplitBlock) void k_rollout_split(const StepArgs a, int final_slot) {
+    extern __shared__ double2 S[];  // two buffers of 3 x res_rows pairs, then the split-form region
+    const int64_t rid = a.res_base + blockIdx.x;
+    if (rid >= a.nres) return;
+    const ResidentDesc *__restrict__ rd = a.rdesc + rid;
+    const int b = rd->body;
+    if (b < 0) return;
+    const int nmass = rd->nmass, chunk0 = rd->chunk0, nact = rd->nact;
+    const int64_t act0 = rd->act0, inc0 = rd->inc0;
+    const int ninc = rd->ninc;
+    const int nch = (nmass + kChunk - 1) / kChunk;
+    const BodyKD bkd{a.b_k[b], a.b_d[b], a.b_kd[b]};
+    const int s0 = a.slot;
+    const int64_t K = a.bleft[s0][b];
+    const int64_t nst = a.bn[s0][b];
+    double T = a.bT[s0][b];
+    const int t = threadIdx.x;
+    if (K > 0) {
+        if (ACT) actuate_slice(a, act0, nact, T, a.ract[0], t, kSplitBlock);
+        split_in_lds<ACT, kSplitBlock>(a, S, rd, b, bkd, nch, nmass, inc0, ninc, (int64_t)chunk0 * kChunk, K, nst, T,
+                                       nact, act0);
+    }
+    if (t == 0) {
+        a.bT[final_slot][b] = T;
+        a.bn[final_slot][b] = nst + (K > 0 ? K : 0);
+        a.bleft[final_slot][b] = 0;
+    }
+}
+
 // Per-mass local time: T += dt once per step the body took since the last
 // sync (src/sim.cu:1165), for every non-fixed mass.
 __global__ void k_advance_T(double *Tm, const uint32_t *mflags, const int32_t *chunk_body,
@@ -1390,7 +1479,10 @@ hipError_t launch_rollout_resident(const StepArgs &a, bool act, int final_slot, 
     if (a.nres <= a.res_base) return hipSuccess;
     const size_t lds = resident_lds_bytes(a.res_rows) + (size_t)a.res_cache;
     const dim3 g((unsigned)(a.nres - a.res_base)), blk(kResidentBlock);
-    if (a.res_split) {
+    if (a.res_split == 2) {
+        if (act) hipLaunchKernelGGL((k_rollout_split<true>), g, dim3(kSplitBlock), lds, s, a, final_slot);
+        else hipLaunchKernelGGL((k_rollout_split<false>), g, dim3(kSplitBlock), lds, s, a, final_slot);
+    } else if (a.res_split) {
         if (act) hipLaunchKernelGGL((k_rollout_resident<true, true>), g, blk, lds, s, a, final_slot);
         else hipLaunchKernelGGL((k_rollout_resident<false, true>), g, blk, lds, s, a, final_slot);
     } else {
@@ -1404,6 +1496,8 @@ const char *resident_kernel_name(bool act) {
     return act ? "k_rollout_resident<actuated>" : "k_rollout_resident<passive>";
 }
 
+const char *split_kernel_name(bool act) { return act ? "k_rollout_split<actuated>" : "k_rollout_split<passive>"; }
+
 hipError_t set_tiles_lds_limit(int *resident_max) {
     int dev = 0, lim = 0;
     hipError_t er = hipGetDevice(&dev);
@@ -1412,8 +1506,9 @@ hipError_t set_tiles_lds_limit(int *resident_max) {
     if (er != hipSuccess) return er;
     const int rb = std::min(lim, (int)resident_lds_bytes(kResidentMax) + kResidentCacheMax);
     *resident_max = rb;
-    const void *res[4] = {(const void *)k_rollout_resident<false, false>, (const void *)k_rollout_resident<true, false>,
-                          (const void *)k_rollout_resident<false, true>, (const void *)k_rollout_resident<true, true>};
+    const void *res[6] = {(const void *)k_rollout_resident<false, false>, (const void *)k_rollout_resident<true, false>,
+                          (const void *)k_rollout_resident<false, true>, (const void *)k_rollout_resident<true, true>,
+                          (const void *)k_rollout_split<false>, (const void *)k_rollout_split<true>};
     for (const void *f : res) {
         er = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, rb);
         if (er != hipSuccess) return er;

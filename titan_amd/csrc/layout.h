@@ -69,6 +69,12 @@ struct TileDesc {
 constexpr int kResidentMax = 1024;
 constexpr int kResidentMaxBody = 32767;
 constexpr int kResidentBlock = 256;
+// Workgroup of the resident launch whose bodies are all in the split form:
+// two threads per row of the largest split-form body (256 masses), so that
+// a spring term pass takes one to three terms per thread.  At 1024 threads
+// the register budget (128 VGPRs) spills; 512 threads keep everything in
+// registers.
+constexpr int kSplitBlock = 512;
 
 struct ResidentDesc {
     int32_t body;     // -1: free
@@ -243,7 +249,7 @@ struct StepArgs {
     int64_t res_base, nres;   // launch range: descriptors [res_base, nres)
     int32_t res_rows;         // LDS rows per buffer: the largest resident body
     int32_t res_cache;        // LDS bytes after the two state buffers for an incidence copy
-    int32_t res_split;        // 1: bodies with a split-form region use it
+    int32_t res_split;        // 1: bodies with a split-form region use it; 2: every body does (k_rollout_split)
 };
 
 }  // namespace titan_mi355x
