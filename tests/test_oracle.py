@@ -213,6 +213,32 @@ def test_momentum_invariant_random_graph():
     assert np.max(np.abs(p1 - p0)) < 1e-12
 
 
+
+def test_norm_and_dot_on_a_general_direction():
+    # the Vec ops under the spring law (include/Titan/vec.h:109-159): norm of
+    # (3, 4, 12) is 13 exactly, the unit vector has three unequal components
+    # and dot(vL - vR, u) = (3 - 8 + 6) / 13 mixes all three, so a dropped or
+    # swapped component, dot(a, a) for dot(a, b) or a reversed difference
+    # changes the force
+    k, rest, c = 10.0, 11.0, 0.5
+    s = orc.Scene(np.array([[0.0, 0, 0], [3.0, 4.0, 12.0]]), [1.0, 1.0],
+                  vel=np.array([[1.0, -2.0, 0.5], [0.0, 0.0, 0.0]]), left=[0], right=[1], k=[k],
+                  rest=[rest], damping=[c], g=(0, 0, 0), dt=1e-3)
+    F = s.spring_forces()[0]
+    u = [Fraction(3, 13), Fraction(4, 13), Fraction(12, 13)]
+    dotv = Fraction(1, 13)
+    exact = [Fraction(k) * (Fraction(rest) - 13) * ui + dotv * Fraction(c) * ui for ui in u]
+    for got, want in zip(F, exact):
+        assert got == pytest.approx(float(want), rel=2e-16, abs=0)
+    # the damping term alone (at rest length) and its sign
+    s2 = orc.Scene(np.array([[0.0, 0, 0], [3.0, 4.0, 12.0]]), [1.0, 1.0],
+                   vel=np.array([[1.0, -2.0, 0.5], [0.0, 0.0, 0.0]]), left=[0], right=[1], k=[k],
+                   rest=[13.0], damping=[c], g=(0, 0, 0), dt=1e-3)
+    F2 = s2.spring_forces()[0]
+    for got, ui in zip(F2, u):
+        assert got == pytest.approx(float(dotv * Fraction(c) * ui), rel=2e-16, abs=0)
+        assert got > 0
+
 # ------------------------------------------------------ scatter == gather order
 
 def test_gather_equals_serial_scatter_bit_for_bit():
@@ -307,6 +333,24 @@ def test_reference_simple_test_energy_within_one_percent():
         assert abs(avg - e0) <= 0.01 * e0
     assert contact_samples >= 1  # the check does cover a sample inside a contact
 
+
+
+def test_energy_two_springs_hand_computed():
+    # orc_energy (test/testutil/utils.h:12-30): sum of g z m, m |v|^2 / 2 and
+    # k (|R - L| - rest)^2 / 2 over live springs; every term below is exact
+    # in binary, so the sum is compared for equality.  Spring 0: |(3, 4, 12)|
+    # = 13, rest 10, k 8 -> 36; spring 1: length 2, rest 1, k 6 -> 3; spring 2
+    # has a missing endpoint and adds nothing.  Kinetic: 2 * 9 / 2 + 1 * 49 / 2
+    # = 33.5; gravity (g = 10): 10 (1 * 2 + 13 * 0.5 + 15 * 1) = 235.
+    pos = np.array([[0.0, 0.0, 1.0], [3.0, 4.0, 13.0], [3.0, 4.0, 15.0]])
+    vel = np.array([[1.0, 2.0, 2.0], [0.0, 0.0, 0.0], [2.0, 3.0, 6.0]])
+    s = orc.Scene(pos, [2.0, 0.5, 1.0], vel=vel, left=[0, 1, -1], right=[1, 2, 0], k=[8.0, 6.0, 100.0],
+                  rest=[10.0, 1.0, 0.5], g=(0, 0, 0), dt=1e-3)
+    assert s.energy(10.0) == 36.0 + 3.0 + 33.5 + 235.0
+    # swapping a spring's endpoints leaves its energy unchanged; k enters once
+    s2 = orc.Scene(pos, [2.0, 0.5, 1.0], vel=vel, left=[1, 2, -1], right=[0, 1, 0], k=[16.0, 6.0, 100.0],
+                   rest=[10.0, 1.0, 0.5], g=(0, 0, 0), dt=1e-3)
+    assert s2.energy(10.0) == s.energy(10.0) + 36.0
 
 # ----------------------------------------------------------------------- ball
 
