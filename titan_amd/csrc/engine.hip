@@ -195,6 +195,7 @@ struct titan_ctx {
 
     bool has_act = false;  // some spring is actuated: launch the actuated variants
     int res_lds_max = 0;   // LDS a resident workgroup may have
+    int split_lds_max = 0;  // dynamic LDS a split-form resident workgroup may have
     int n_cu = 0;          // compute units of the device
     int slot = 0;
 
@@ -606,6 +607,11 @@ Range body_range(const BodyH &b) {
 }
 
 bool all_split(const titan_ctx *c, const Range &rg, const StepArgs &a) {
+    // the split launch copies the planes into its static LDS, which its
+    // dynamic region must leave room for
+    if (a.nplanes > kSplitPlanes ||
+        resident_lds_bytes(a.res_rows) + (int64_t)a.res_cache > (int64_t)c->split_lds_max)
+        return false;
     bool any = false;
     for (const auto &b : c->bodies) {
         if (!(b.live && b.resident) || b.res < rg.r0 || b.res >= rg.r1) continue;
@@ -1017,7 +1023,7 @@ struct PreparedBody {
     std::vector<uint32_t> spw;
     std::vector<double> spd;
     std::vector<int32_t> spa;
-    int64_t nsp = 0, nreal = 0;
+    int64_t nsp = 0, nslots = 0;
     std::vector<int32_t> sl, sr;    // spring endpoints in device order (-1: none)
     // built on the device at commit (DevBuilder): the caller's descriptor
     // (read during the commit), the schedule it was prepared under, each
@@ -1029,19 +1035,23 @@ struct PreparedBody {
 };
 
 // The split form of a resident body (endpoints in device rows): its springs
-// once each and the numbers of their two incidences, each mass's numbered in
-// the order its SELL entries hold them; nothing when its LDS region would
+// once each and the slots of their two incidences' terms (incidence j of
+// row q: j * N + q, each mass's incidences in the order its SELL entries
+// hold them); nothing when the slots exceed 16 bits or its LDS region would
 // not fit.
 void build_split(PreparedBody &P, const SpringsIn &sp, const std::vector<int32_t> &act_local) {
-    const int64_t S = sp.n_springs, npad = P.nch * kChunk;
-    int64_t nsp = 0, nreal = 0;
+    const int64_t S = sp.n_springs, N = P.N;
+    int64_t nsp = 0, nreal = 0, maxdeg = 0;
     for (int64_t s = 0; s < S; ++s)
         if (sp.left[s] >= 0 && sp.right[s] >= 0) ++nsp;
-    for (const int32_t dg : P.sb.deg) nreal += dg;
-    if (nreal != 2 * nsp || nreal > 0xffff || split_region_bytes(nreal, nsp, P.per_k, P.per_d) > kResidentCacheMax)
+    for (const int32_t dg : P.sb.deg) {
+        nreal += dg;
+        maxdeg = std::max<int64_t>(maxdeg, dg);
+    }
+    const int64_t nslots = maxdeg * N;
+    if (nreal != 2 * nsp || nslots > 0x10000 || split_region_bytes(nslots, nsp, P.per_k, P.per_d) > kResidentCacheMax)
         return;
-    std::vector<int64_t> at(npad + 1, 0);  // the next incidence number of each mass
-    for (int64_t q = 0; q < npad; ++q) at[q + 1] = at[q] + P.sb.deg[q];
+    std::vector<int64_t> at(N, 0);  // the next incidence of each mass
     P.spw.assign(2 * nsp, 0);
     P.spd.assign(nsp * (1 + (P.per_k ? 1 : 0) + (P.per_d ? 1 : 0)), 0.0);
     if (sp.act) P.spa.assign(nsp, -1);
@@ -1055,13 +1065,13 @@ void build_split(PreparedBody &P, const SpringsIn &sp, const std::vector<int32_t
         if (P.per_k) P.spd[nsp + j] = sp.k[s];
         if (P.per_d) P.spd[nsp * (P.per_k ? 2 : 1) + j] = sp.damping ? sp.damping[s] : 0.0;
         if (act) P.spa[j] = act_local[s];
-        const uint32_t ia = (uint32_t)at[a]++;  // as build_sell places them: a, then b
-        const uint32_t ib = (uint32_t)at[b]++;
-        P.spw[nsp + j] = ia | (ib << 16);
+        const uint32_t sa = (uint32_t)(at[a]++ * N + a);  // as build_sell places them: a, then b
+        const uint32_t sb = (uint32_t)(at[b]++ * N + b);
+        P.spw[nsp + j] = sa | (sb << 16);
         ++j;
     }
     P.nsp = nsp;
-    P.nreal = nreal;
+    P.nslots = nslots;
 }
 
 // device_min: a tiled body of at least this many masses is laid out on the
@@ -1545,7 +1555,7 @@ int32_t commit_body(titan_ctx *c, PreparedBody &P) {
     int64_t cache = ((4 * n_inc + 15) & ~(int64_t)15) + 8 * n_inc * (1 + (P.per_k ? 1 : 0) + (P.per_d ? 1 : 0));
     if (cache > kResidentCacheMax) cache = 0;
     // and its split-form region, where it has that form
-    const int64_t scache = P.spw.empty() ? 0 : split_region_bytes(P.nreal, P.nsp, P.per_k, P.per_d);
+    const int64_t scache = P.spw.empty() ? 0 : split_region_bytes(P.nslots, P.nsp, P.per_k, P.per_d);
     const ResidentDesc rdh{-1, (int32_t)ch0, (int32_t)N, (int32_t)n_act, act0, inc0, (int32_t)n_inc, (int32_t)cache,
                            (int32_t)scache, (int32_t)P.nsp};
     if (resident) up(c->rdesc + rs0, &rdh, 1);
@@ -2118,7 +2128,7 @@ int titan_ctx_create(int device, uint64_t hbm_budget_bytes, titan_ctx **out) {
         const char *db = std::getenv("TITAN_DEVICE_BUILD");
         if (db) c->device_min = std::max<int64_t>(0, std::atoll(db));
     }
-    HIPCHK(set_tiles_lds_limit(&c->res_lds_max));
+    HIPCHK(set_tiles_lds_limit(&c->res_lds_max, &c->split_lds_max));
     c->scratch = device_scratch(device);
     HIPCHK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));

@@ -24,8 +24,9 @@ const char *resident_kernel_name(bool actuated);
 const char *split_kernel_name(bool actuated);
 // Raise the dynamic LDS limits of the tiled and resident kernels; *resident_max
 // receives the resident kernel's limit (the device's per-workgroup maximum
-// or what the kernel can use, the smaller).
-hipError_t set_tiles_lds_limit(int *resident_max);
+// or what the kernel can use, the smaller), *split_max the split-form
+// resident kernel's (less its static LDS).
+hipError_t set_tiles_lds_limit(int *resident_max, int *split_max);
 hipError_t launch_advance_T(double *Tm, const uint32_t *mflags, const int32_t *chunk_body,
                             const int64_t *bn, int64_t *bnsync, int64_t chunk_begin, int64_t chunk_end,
                             int64_t nb, double dt, hipStream_t s);

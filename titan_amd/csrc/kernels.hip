@@ -184,9 +184,18 @@ __device__ __forceinline__ IntegIn load_integ(const StepArgs &a, int64_t i, int 
 // Everything of massForcesAndUpdate after the spring sum (src/sim.cu:1104-1167),
 // applied to registers, with the context's integrator.  Returns the
 // acceleration.
+// Where the mass pass reads what is the same for every mass of a body and
+// every step of a launch: the global planes (8 doubles each: the launch's
+// array, or a copy in LDS) and the body's shared local-constraint list.
+struct MassEnv {
+    const double *__restrict__ planes;
+    int64_t lb, ln;
+};
+__device__ __forceinline__ MassEnv mass_env(const StepArgs &a, int b) { return {a.planes, a.b_lc[b], a.b_lcn[b]}; }
+
 __device__ __forceinline__ void mass_pass(const StepArgs &a, MassRegs &r, int64_t i, uint32_t fl,
-                                          int b, double m, int stage, const IntegIn &in, double &ax,
-                                          double &ay, double &az) {
+                                          const MassEnv &env, double m, int stage, const IntegIn &in,
+                                          double &ax, double &ay, double &az) {
     r.fx = r.fx + a.gx * m;                                    // :1104
     r.fy = r.fy + a.gy * m;
     r.fz = r.fz + a.gz * m;
@@ -196,7 +205,7 @@ __device__ __forceinline__ void mass_pass(const StepArgs &a, MassRegs &r, int64_
         r.fz = r.fz + a.ext[2 * a.ncap + i];
     }
     for (int q = 0; q < a.nplanes; ++q) {                      // :1109-1111
-        const double *P = a.planes + 8 * q;
+        const double *P = env.planes + 8 * q;
         contact_plane(r, P[0], P[1], P[2], P[3], P[4], P[5]);
     }
     if (a.bg_start) {                                          // :1113-1115
@@ -226,8 +235,8 @@ __device__ __forceinline__ void mass_pass(const StepArgs &a, MassRegs &r, int64_
         lb = v >> 20;
         ln = v & 0xFFFFF;
     } else {
-        lb = a.b_lc[b];
-        ln = a.b_lcn[b];
+        lb = env.lb;
+        ln = env.ln;
     }
     for (int64_t q = 0; q < ln; ++q) apply_local(r, a.lc[lb + q]);
     if (fl & kDrag) {                                          // :1135-1138
@@ -581,7 +590,11 @@ struct SpringWalk3 {
     }
 
     // rc: this step's actuation factors (actuated only).  A refill index
-    // past the last group is clamped to it.
+    // past the last group is clamped to it.  The loop has one exit: a group
+    // past the last is skipped by a uniform branch rather than a break, so
+    // that the compiler's wait at the loop head leaves the two refills
+    // issued in the previous round in flight (with the breaks it waited for
+    // every load there).
     template <class Fetch>
     __device__ __forceinline__ void run(MassRegs &r, int d, const double *rc, Fetch fetch) {
         r.fx = 0.0;
@@ -589,17 +602,16 @@ struct SpringWalk3 {
         r.fz = 0.0;
         const int last = ng - 1;
         for (int g = 0; g < ng; g += 3) {
+            const bool b_on = g + 1 < ng, c_on = g + 2 < ng;  // uniform over the wave
             if (ACT) A[0].fetch_f(rc);
             C[0].load(src, ebase, lane, min(g + 2, last), bkd);
             eval_set<ACT, G, 1>(r, A, g, d, fetch);
-            if (g + 1 >= ng) break;
-            if (ACT) B[0].fetch_f(rc);
+            if (ACT && b_on) B[0].fetch_f(rc);
             A[0].load(src, ebase, lane, min(g + 3, last), bkd);
-            eval_set<ACT, G, 1>(r, B, g + 1, d, fetch);
-            if (g + 2 >= ng) break;
-            if (ACT) C[0].fetch_f(rc);
+            if (b_on) eval_set<ACT, G, 1>(r, B, g + 1, d, fetch);
+            if (ACT && c_on) C[0].fetch_f(rc);
             B[0].load(src, ebase, lane, min(g + 4, last), bkd);
-            eval_set<ACT, G, 1>(r, C, g + 2, d, fetch);
+            if (c_on) eval_set<ACT, G, 1>(r, C, g + 2, d, fetch);
         }
     }
 };
@@ -700,7 +712,7 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
     const int stage = (int)(nst & 1);
     const IntegIn in = load_integ(a, i, stage);
     double ax, ay, az;
-    mass_pass(a, r, i, fl, b, m, stage, in, ax, ay, az);
+    mass_pass(a, r, i, fl, mass_env(a, b), m, stage, in, ax, ay, az);
     store_mass(a, nxt, i, r, ax, ay, az, left == 1);
 }
 
@@ -790,7 +802,7 @@ __device__ __forceinline__ void tiles_body(const StepArgs &a, double2 *__restric
     const int stage = (int)(nst & 1);
     const IntegIn in = load_integ(a, i, stage);  // after the walk (see the gather step)
     double ax, ay, az;
-    mass_pass(a, r, i, fl, b, m, stage, in, ax, ay, az);
+    mass_pass(a, r, i, fl, mass_env(a, b), m, stage, in, ax, ay, az);
     store_mass(a, nxt, i, r, ax, ay, az, left == 1);
 }
 
@@ -877,6 +889,7 @@ __device__ __forceinline__ void resident_steps(const StepArgs &a, const Src &src
     const int t = threadIdx.x, wave = uniform(t >> 6), lane = t & (kChunk - 1);
     const int64_t n6 = a.ncap;
     constexpr int kWaves = kResidentBlock / kChunk;
+    const MassEnv env = mass_env(a, b);
     auto consts = [&](int c) {
         MassConst mc;
         const int64_t i = g0 + (int64_t)c * kChunk + lane;
@@ -916,7 +929,7 @@ __device__ __forceinline__ void resident_steps(const StepArgs &a, const Src &src
         double ax, ay, az;
         const int stage = (int)((nst + k) & 1);
         const IntegIn in = load_integ(a, i, stage);
-        mass_pass(a, r, i, mc.fl, b, mc.m, stage, in, ax, ay, az);
+        mass_pass(a, r, i, mc.fl, env, mc.m, stage, in, ax, ay, az);
         nxt[3 * q] = make_double2(r.px, r.py);
         nxt[3 * q + 1] = make_double2(r.pz, r.vx);
         nxt[3 * q + 2] = make_double2(r.vy, r.vz);
@@ -953,9 +966,9 @@ __device__ __forceinline__ double signed_term(double v, bool neg) {
 
 // The terms G of springs x, x + B, ..., x + (U - 1) B (B = BLOCK) of a split-form
 // body (those below nsp), each from its endpoint a's side and formed exactly
-// as the walk forms the term at a, stored as -G at a's incidence and +G at
-// b's; U springs at a time so that their latencies overlap (a lane past the
-// end repeats spring x and stores it once).
+// as the walk forms the term at a, stored as -G in the slot of a's incidence
+// and +G in b's; U springs at a time so that their latencies overlap (a lane
+// past the end repeats spring x and stores it once).
 template <bool ACT, int U, int BLOCK = kResidentBlock>
 __device__ __forceinline__ void split_terms(int x, int nsp, const uint32_t *__restrict__ csw,
                                             const double *__restrict__ cr, const double *__restrict__ kp,
@@ -1049,25 +1062,24 @@ __device__ __forceinline__ void resident_steps_split(const StepArgs &a, const ui
                                                      const double *__restrict__ cr, const double *__restrict__ kp,
                                                      const double *__restrict__ dp, double *__restrict__ fx,
                                                      double *__restrict__ fy, double *__restrict__ fz,
-                                                     const int *__restrict__ degs,
-                                                     const int *__restrict__ starts, int nsp, int b,
-                                                     const BodyKD &bkd, int nch, int64_t g0, int64_t K, int64_t nst,
-                                                     double &T, int nact, int64_t act0, double2 *__restrict__ B0,
-                                                     double2 *__restrict__ B1) {
+                                                     int maxdeg, int nsp, const MassEnv &env,
+                                                     const BodyKD &bkd, int nch, int nmass, int64_t g0, int64_t K,
+                                                     int64_t nst, double &T, int nact, int64_t act0,
+                                                     double2 *__restrict__ B0, double2 *__restrict__ B1) {
     constexpr int U = 4;
     const int t = threadIdx.x, wave = uniform(t >> 6), lane = t & (kChunk - 1);
     const int64_t n6 = a.ncap;
-    const bool mine = wave < nch;
-    const int q = wave * kChunk + lane;
+    // the rows go to the first four waves (one per SIMD), 16, 32 or 64 lanes
+    // each, so that every SIMD runs a share of the sums and mass passes
+    const int L = nmass > 2 * kChunk ? kChunk : nmass > kChunk ? kChunk / 2 : kChunk / 4;
+    const int q = wave * L + lane;
+    const bool mine = wave < kSplitMaxChunks && lane < L && q < nmass;
     const int64_t i = g0 + q;
     uint32_t fl = kPad;
     double m = 1.0;
-    int deg = 0, first = 0;
     if (mine) {
         fl = a.mflags[i];
         m = a.m[i];
-        deg = degs[q];
-        first = starts[q];
     }
     for (int64_t k = 0; k < K; ++k) {
         const double2 *__restrict__ cur = (k & 1) ? B1 : B0;
@@ -1078,15 +1090,16 @@ __device__ __forceinline__ void resident_steps_split(const StepArgs &a, const ui
             for (; x + (U - 1) * BLOCK < nsp; x += U * BLOCK)
                 split_terms<ACT, U, BLOCK>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
             for (; x < nsp; x += BLOCK) split_terms<ACT, 1, BLOCK>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
-        } else if (nsp <= BLOCK) {  // one spring per thread
-            if (t < nsp) split_terms<ACT, 1, BLOCK>(t, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
-        } else if (nsp <= 2 * BLOCK) {  // two or three per thread, interleaved, every wave alike
-            split_terms<ACT, 2, BLOCK>(t, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
-        } else if (nsp <= 3 * BLOCK) {
-            split_terms<ACT, 3, BLOCK>(t, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
         } else {
-            for (int x = t; x < nsp; x += 2 * BLOCK)
+            // one spring per thread and pass, four waves per SIMD hiding the
+            // chain's latency; a wave with a lane that has a second spring
+            // evaluates both interleaved (every x < BLOCK < nsp there)
+            int x = t;
+            if (__any(x + BLOCK < nsp)) {
                 split_terms<ACT, 2, BLOCK>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
+                x += 2 * BLOCK;
+            }
+            for (; x < nsp; x += BLOCK) split_terms<ACT, 1, BLOCK>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
         }
         __syncthreads();
         if (mine && !(fl & kPad)) {
@@ -1106,16 +1119,39 @@ __device__ __forceinline__ void resident_steps_split(const StepArgs &a, const ui
                 r.fx = 0.0;
                 r.fy = 0.0;
                 r.fz = 0.0;
-#pragma unroll 4
-                for (int j = first; j < first + deg; ++j) {  // ascending spring index, as the walk sums
-                    r.fx = r.fx + fx[j];
-                    r.fy = r.fy + fy[j];
-                    r.fz = r.fz + fz[j];
+                // ascending spring index, as the walk sums: the term of
+                // incidence j of row q is in slot j * nmass + q, so that a
+                // wave's reads of one j are consecutive words, and the
+                // slots past a row's degree hold +0 (x + 0 is x, zero signs
+                // aside), so that every lane runs the body's largest degree
+                // without masks; eight incidences' terms are read before
+                // their adds, so that one wait covers them
+                const double *__restrict__ qx = fx + q, *__restrict__ qy = fy + q, *__restrict__ qz = fz + q;
+                int j = 0;
+                for (; j + 8 <= maxdeg; j += 8) {
+                    double vx[8], vy[8], vz[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        vx[u] = qx[(j + u) * nmass];
+                        vy[u] = qy[(j + u) * nmass];
+                        vz[u] = qz[(j + u) * nmass];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        r.fx = r.fx + vx[u];
+                        r.fy = r.fy + vy[u];
+                        r.fz = r.fz + vz[u];
+                    }
+                }
+                for (; j < maxdeg; ++j) {
+                    r.fx = r.fx + qx[j * nmass];
+                    r.fy = r.fy + qy[j * nmass];
+                    r.fz = r.fz + qz[j * nmass];
                 }
                 double ax, ay, az;
                 const int stage = (int)((nst + k) & 1);
                 const IntegIn in = load_integ(a, i, stage);
-                mass_pass(a, r, i, fl, b, m, stage, in, ax, ay, az);
+                mass_pass(a, r, i, fl, env, m, stage, in, ax, ay, az);
                 nxt[3 * q] = make_double2(r.px, r.py);
                 nxt[3 * q + 1] = make_double2(r.pz, r.vx);
                 nxt[3 * q + 2] = make_double2(r.vy, r.vz);
@@ -1132,43 +1168,40 @@ __device__ __forceinline__ void resident_steps_split(const StepArgs &a, const ui
     }
 }
 
-// Fill a body's split-form region: each row's degree and first incidence
-// (an exclusive scan of the degrees), then copies of the springs' endpoint
-// and incidence words, rest lengths, k and damping from the body's
-// incidence span.  Returns the incidence count.
+// Fill a body's split-form region: the body's largest degree, copies of the springs' endpoint and slot words, rest lengths, k
+// and damping from the body's incidence span, and +0 in the term slots past
+// each row's degree (terms at fx, fy, fz: at most the returned largest
+// degree x nmass slots each).
 // The first kSplitMaxChunks waves hold one row per thread; a larger
 // workgroup's other threads only help with the copies.
 static_assert(kResidentBlock == kSplitMaxChunks * kChunk, "one thread per row of a split-form body");
 template <int BLOCK = kResidentBlock>
-__device__ __forceinline__ int split_fill(const StepArgs &a, char *__restrict__ base, int nch, int64_t g0,
-                                          int64_t e0, int nsp, int nd) {
+__device__ __forceinline__ int split_fill(const StepArgs &a, char *__restrict__ base, int nch, int nmass,
+                                          int64_t g0, int64_t e0, int nsp, int nd) {
     const int t = threadIdx.x, wave = t >> 6, lane = t & (kChunk - 1);
-    int *degs = reinterpret_cast<int *>(base);
-    int *starts = degs + kSplitMaxChunks * kChunk;
-    int *wsum = starts + kSplitMaxChunks * kChunk;
+    int *wmax = reinterpret_cast<int *>(base);
     const bool row = wave < kSplitMaxChunks;
     const int dg = (row && t < nch * kChunk) ? a.deg[g0 + t] : 0;
-    int inc = dg;  // inclusive scan over the wave
+    int mx = dg;  // maximum over the wave
 #pragma unroll
-    for (int off = 1; off < kChunk; off <<= 1) {
-        const int v = __shfl_up(inc, off);
-        if (lane >= off) inc += v;
-    }
-    if (row && lane == kChunk - 1) wsum[wave] = inc;
-    __syncthreads();
-    int pre = 0;
-    for (int w = 0; w < wave && w < kSplitMaxChunks; ++w) pre += wsum[w];
-    const int nreal = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    if (row) {
-        degs[t] = dg;
-        starts[t] = pre + inc - dg;
-    }
-    uint32_t *cw = reinterpret_cast<uint32_t *>(base + kSplitHead);  // endpoint words, then incidence words
+    for (int off = 1; off < kChunk; off <<= 1) mx = max(mx, __shfl_xor(mx, off));
+    if (row && lane == 0) wmax[wave] = mx;
+    uint32_t *cw = reinterpret_cast<uint32_t *>(base + kSplitHead);  // endpoint words, then slot words
     double *cr = reinterpret_cast<double *>(base + ((kSplitHead + 8 * (int64_t)nsp + 15) & ~(int64_t)15));
     for (int j = t; j < 2 * nsp; j += BLOCK) cw[j] = a.inc_nbr[e0 + j];
     for (int j = t; j < nd; j += BLOCK) cr[j] = a.inc_rest[e0 + j];
     __syncthreads();
-    return nreal;
+    const int maxdeg = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+    const int ns = maxdeg * nmass;
+    double *fx = cr + nd;
+    if (t < nmass) {
+        for (int j = dg; j < maxdeg; ++j) {
+            fx[j * nmass + t] = 0.0;
+            fx[ns + j * nmass + t] = 0.0;
+            fx[2 * ns + j * nmass + t] = 0.0;
+        }
+    }
+    return maxdeg;
 }
 
 // A split-form body held in LDS, stepped by a workgroup of BLOCK threads:
@@ -1176,9 +1209,9 @@ __device__ __forceinline__ int split_fill(const StepArgs &a, char *__restrict__ 
 // copied out to the buffer of its new step parity.
 template <bool ACT, int BLOCK>
 __device__ __forceinline__ void split_in_lds(const StepArgs &a, double2 *__restrict__ S,
-                                             const ResidentDesc *__restrict__ rd, int b, const BodyKD &bkd, int nch,
-                                             int nmass, int64_t inc0, int ninc, int64_t g0, int64_t K, int64_t nst,
-                                             double &T, int nact, int64_t act0) {
+                                             const ResidentDesc *__restrict__ rd, const MassEnv &env,
+                                             const BodyKD &bkd, int nch, int nmass, int64_t inc0, int ninc, int64_t g0,
+                                             int64_t K, int64_t nst, double &T, int nact, int64_t act0) {
     const int t = threadIdx.x;
     const int64_t n6 = a.ncap;
     double2 *__restrict__ B0 = S, *__restrict__ B1 = S + 3 * a.res_rows;
@@ -1189,9 +1222,8 @@ __device__ __forceinline__ void split_in_lds(const StepArgs &a, double2 *__restr
     char *base = reinterpret_cast<char *>(S + 6 * a.res_rows);
     const int nsp = rd->nsp;
     const int nd = nsp * (1 + ((bkd.kd & 1u) ? 1 : 0) + ((bkd.kd & 2u) ? 1 : 0));
-    const int nreal = split_fill<BLOCK>(a, base, nch, g0, inc0 + ninc, nsp, nd);
-    const int *degs = reinterpret_cast<const int *>(base);
-    const int *starts = degs + kSplitMaxChunks * kChunk;
+    const int maxdeg = split_fill<BLOCK>(a, base, nch, nmass, g0, inc0 + ninc, nsp, nd);
+    const int nslots = maxdeg * nmass;
     const uint32_t *csw = reinterpret_cast<const uint32_t *>(base + kSplitHead);
     const double *cr = reinterpret_cast<const double *>(base + ((kSplitHead + 8 * (int64_t)nsp + 15) & ~(int64_t)15));
     // per-spring k and damping where the body has them; else a valid array
@@ -1201,9 +1233,9 @@ __device__ __forceinline__ void split_in_lds(const StepArgs &a, double2 *__restr
     const double *kp = (bkd.kd & 1u) ? ck : cr;
     const double *dp = (bkd.kd & 2u) ? cd : cr;
     double *fx = const_cast<double *>(cr) + nd;
-    double *fy = fx + nreal, *fz = fy + nreal;
-    resident_steps_split<ACT, BLOCK>(a, csw, cr, kp, dp, fx, fy, fz, degs, starts, nsp, b, bkd, nch, g0, K, nst, T,
-                                     nact, act0, B0, B1);
+    double *fy = fx + nslots, *fz = fy + nslots;
+    resident_steps_split<ACT, BLOCK>(a, csw, cr, kp, dp, fx, fy, fz, maxdeg, nsp, env, bkd, nch, nmass, g0, K, nst,
+                                     T, nact, act0, B0, B1);
     const double2 *__restrict__ fin = (K & 1) ? B1 : B0;
     double2 *__restrict__ dst = reinterpret_cast<double2 *>(a.st + (int64_t)((nst + K) & 1) * 6 * n6);
     for (int q = t; q < 3 * nmass; q += BLOCK) store_nt(&dst[3 * g0 + q], fin[q].x, fin[q].y);
@@ -1239,9 +1271,8 @@ __device__ __forceinline__ void resident_in_lds(const StepArgs &a, double2 *__re
     if (split) {
         const int nsp = rd->nsp;
         const int nd = nsp * (1 + ((bkd.kd & 1u) ? 1 : 0) + ((bkd.kd & 2u) ? 1 : 0));
-        const int nreal = split_fill(a, base, nch, g0, inc0 + ninc, nsp, nd);
-        const int *degs = reinterpret_cast<const int *>(base);
-        const int *starts = degs + kSplitMaxChunks * kChunk;
+        const int maxdeg = split_fill(a, base, nch, nmass, g0, inc0 + ninc, nsp, nd);
+        const int nslots = maxdeg * nmass;
         const uint32_t *csw = reinterpret_cast<const uint32_t *>(base + kSplitHead);
         const double *cr = reinterpret_cast<const double *>(base + ((kSplitHead + 8 * (int64_t)nsp + 15) & ~(int64_t)15));
         // per-spring k and damping where the body has them; else a valid
@@ -1251,9 +1282,9 @@ __device__ __forceinline__ void resident_in_lds(const StepArgs &a, double2 *__re
         const double *kp = (bkd.kd & 1u) ? ck : cr;
         const double *dp = (bkd.kd & 2u) ? cd : cr;
         double *fx = const_cast<double *>(cr) + nd;
-        double *fy = fx + nreal, *fz = fy + nreal;
-        resident_steps_split<ACT>(a, csw, cr, kp, dp, fx, fy, fz, degs, starts, nsp, b, bkd, nch, g0, K, nst, T, nact,
-                                  act0, B0, B1);
+        double *fy = fx + nslots, *fz = fy + nslots;
+        resident_steps_split<ACT>(a, csw, cr, kp, dp, fx, fy, fz, maxdeg, nsp, mass_env(a, b), bkd, nch, nmass, g0,
+                                  K, nst, T, nact, act0, B0, B1);
     } else if (cached) {
         // names (4 B per entry), then from a 16-byte boundary the rest
         // lengths and, where the body has them, k and damping (8 B each)
@@ -1351,6 +1382,7 @@ __global__ __launch_bounds__(kResidentBlock) void k_rollout_resident(const StepA
 template <bool ACT>
 __global__ __launch_bounds__(kSplitBlock) void k_rollout_split(const StepArgs a, int final_slot) {
     extern __shared__ double2 S[];  // two buffers of 3 x res_rows pairs, then the split-form region
+    __shared__ double planes[8 * kSplitPlanes];  // the global planes (the launch has at most kSplitPlanes)
     const int64_t rid = a.res_base + blockIdx.x;
     if (rid >= a.nres) return;
     const ResidentDesc *__restrict__ rd = a.rdesc + rid;
@@ -1368,8 +1400,10 @@ __global__ __launch_bounds__(kSplitBlock) void k_rollout_split(const StepArgs a,
     const int t = threadIdx.x;
     if (K > 0) {
         if (ACT) actuate_slice(a, act0, nact, T, a.ract[0], t, kSplitBlock);
-        split_in_lds<ACT, kSplitBlock>(a, S, rd, b, bkd, nch, nmass, inc0, ninc, (int64_t)chunk0 * kChunk, K, nst, T,
-                                       nact, act0);
+        for (int j = t; j < 8 * a.nplanes; j += kSplitBlock) planes[j] = a.planes[j];  // visible after split_fill's barriers
+        const MassEnv env{planes, a.b_lc[b], a.b_lcn[b]};
+        split_in_lds<ACT, kSplitBlock>(a, S, rd, env, bkd, nch, nmass, inc0, ninc, (int64_t)chunk0 * kChunk, K, nst,
+                                       T, nact, act0);
     }
     if (t == 0) {
         a.bT[final_slot][b] = T;
@@ -1498,7 +1532,7 @@ const char *resident_kernel_name(bool act) {
 
 const char *split_kernel_name(bool act) { return act ? "k_rollout_split<actuated>" : "k_rollout_split<passive>"; }
 
-hipError_t set_tiles_lds_limit(int *resident_max) {
+hipError_t set_tiles_lds_limit(int *resident_max, int *split_max) {
     int dev = 0, lim = 0;
     hipError_t er = hipGetDevice(&dev);
     if (er != hipSuccess) return er;
@@ -1506,11 +1540,17 @@ hipError_t set_tiles_lds_limit(int *resident_max) {
     if (er != hipSuccess) return er;
     const int rb = std::min(lim, (int)resident_lds_bytes(kResidentMax) + kResidentCacheMax);
     *resident_max = rb;
-    const void *res[6] = {(const void *)k_rollout_resident<false, false>, (const void *)k_rollout_resident<true, false>,
-                          (const void *)k_rollout_resident<false, true>, (const void *)k_rollout_resident<true, true>,
-                          (const void *)k_rollout_split<false>, (const void *)k_rollout_split<true>};
+    const void *res[4] = {(const void *)k_rollout_resident<false, false>, (const void *)k_rollout_resident<true, false>,
+                          (const void *)k_rollout_resident<false, true>, (const void *)k_rollout_resident<true, true>};
     for (const void *f : res) {
         er = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, rb);
+        if (er != hipSuccess) return er;
+    }
+    // the split launch's own LDS (its planes) comes out of the same budget
+    *split_max = std::min(rb, lim - kSplitStaticLds);
+    const void *spl[2] = {(const void *)k_rollout_split<false>, (const void *)k_rollout_split<true>};
+    for (const void *f : spl) {
+        er = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, *split_max);
         if (er != hipSuccess) return er;
     }
     const int bytes = (int)tiles_lds_bytes(kMaxTileRows);

@@ -70,11 +70,16 @@ constexpr int kResidentMax = 1024;
 constexpr int kResidentMaxBody = 32767;
 constexpr int kResidentBlock = 256;
 // Workgroup of the resident launch whose bodies are all in the split form:
-// two threads per row of the largest split-form body (256 masses), so that
-// a spring term pass takes one to three terms per thread.  At 1024 threads
-// the register budget (128 VGPRs) spills; 512 threads keep everything in
-// registers.
-constexpr int kSplitBlock = 512;
+// four threads per row of the largest split-form body (256 masses), each
+// evaluating one spring term per pass (the README scene's 1,036 springs:
+// one pass, and a second for 12 threads), with four waves per SIMD to hide
+// the latency of a term's dependent FP64 chain.  It copies the global
+// planes (at most kSplitPlanes; a launch with more uses the 256-thread
+// resident kernel) into static LDS, which the dynamic region leaves room
+// for.
+constexpr int kSplitBlock = 1024;
+constexpr int kSplitPlanes = 16;
+constexpr int kSplitStaticLds = 8 * 8 * kSplitPlanes;
 
 struct ResidentDesc {
     int32_t body;     // -1: free
@@ -93,32 +98,30 @@ struct ResidentDesc {
 // bytes keeps it in LDS for all the steps of a launch.
 constexpr int kResidentCacheMax = 128 * 1024;
 
-// The split form of a resident body of at most kSplitMaxChunks chunks.  The
-// body's incidences are numbered mass by mass, each mass's in ascending
-// spring index as its SELL entries hold them: incidence j of mass q is
-// first(q) + j, first = the exclusive prefix sum of the degrees.  (So the
-// threads that store one mass's terms write neighbouring LDS words; terms
-// numbered by SELL entry, 64 words apart per mass, cost more in bank
-// conflicts than contiguous sums gain.)  The springs are listed once each,
-// in ascending spring index: a word with the rows of the endpoints
-// a | b << 16 (b's bit 15: actuated), a word with their incidence numbers
-// ia | ib << 16, then the rest lengths (or actuator indices), and k and
-// damping where per spring.  This data sits in the body's incidence span
-// after its SELL entries: the nsp endpoint words and nsp incidence words in
-// inc_nbr, the rest lengths, k and damping in inc_rest.  A step has all the
-// workgroup's threads compute each spring's term G once, from endpoint a's
-// side, and store -G at incidence ia (the walk's term at a is G,
-// subtracted) and +G at ib (the walk's term at b is -G exactly, zero signs
-// aside); each mass's lane then adds up its own incidences in order.
-// LDS region: the rows' degrees and first incidences (two int32 arrays of
-// kSplitMaxChunks * 64) and four int32 partial counts, the 2 nsp words (to
-// a 16-byte boundary), rest lengths, k and damping where per spring, and
-// the three components of the incidences' terms (8 B each).
+// The split form of a resident body of at most kSplitMaxChunks chunks.  Each
+// mass's incidences are in ascending spring index as its SELL entries hold
+// them; the term of incidence j of row q has the slot j * nmass + q, so that
+// a wave's reads of every mass's j-th term are consecutive words (a mass's
+// terms kept together cost the per-mass sums many-way bank conflicts).  The
+// springs are listed once each, in ascending spring index: a word with the
+// rows of the endpoints a | b << 16 (b's bit 15: actuated), a word with
+// their incidences' slots sa | sb << 16, then the rest lengths (or actuator
+// indices), and k and damping where per spring.  This data sits in the
+// body's incidence span after its SELL entries: the nsp endpoint words and
+// nsp slot words in inc_nbr, the rest lengths, k and damping in inc_rest.
+// A step has the workgroup's threads compute each spring's term G once,
+// from endpoint a's side, and store -G in slot sa (the walk's term at a is
+// G, subtracted) and +G in sb (the walk's term at b is -G exactly, zero
+// signs aside); each mass's lane then adds up its own slots in order.
+// LDS region: four int32 partial maxima of the degrees, the 2 nsp words,
+// from a 16-byte boundary the rest lengths, k and damping where per spring,
+// and the three components of the terms, largest degree x nmass slots each
+// (8 B).
 constexpr int kSplitMaxChunks = 4;
-constexpr int kSplitHead = 2 * 4 * kSplitMaxChunks * kChunk + 16;
-constexpr int64_t split_region_bytes(int64_t nreal, int64_t nsp, bool per_k, bool per_d) {
+constexpr int kSplitHead = 16;
+constexpr int64_t split_region_bytes(int64_t nslots, int64_t nsp, bool per_k, bool per_d) {
     return ((kSplitHead + 8 * nsp + 15) & ~(int64_t)15) + 8 * nsp * (1 + (per_k ? 1 : 0) + (per_d ? 1 : 0)) +
-           24 * nreal;
+           24 * nslots;
 }
 
 // The body table and the arena fields a body's activation (joining the
