@@ -589,3 +589,63 @@ def test_extern_force_uploaded_after_graph_capture(layout):
     o.run(128)
     np.testing.assert_array_equal(got["pos"], o.pos)
     np.testing.assert_array_equal(got["vel"], o.vel)
+
+
+def test_split_form_lean_and_general_mass_pass():
+    # The split-form launch takes the lean mass pass (Euler, no balls, no
+    # extern force, drag or local constraints) when every mass allows it.
+    # Uploading extern forces mid-run must switch a graph-launched body to
+    # the general pass and back; drag, a local plane, a ball or another
+    # integrator keep it general.  Each against the oracle, bit for bit.
+    sc = scenes.readme_lattice(2.45)
+    n = sc.n_masses
+    ext = np.zeros((n, 3))
+    ext[::9] = np.random.default_rng(4).normal(0.0, 0.5, (len(ext[::9]), 3))
+    _say("case split form: extern force uploaded mid-run and removed again (graph launches)")
+    o = to_oracle(sc)
+    with Context() as ctx:
+        (bid,) = sc.load(ctx)
+        assert ctx.body_layout(bid) == LAYOUT_RESIDENT
+        assert "k_rollout_split" in ctx.kernel_name()
+        ctx.step(70)
+        o.run(70)
+        ctx.upload(bid, ext=ext)
+        o.e = [np.ascontiguousarray(ext[:, c]) for c in range(3)]
+        ctx.step(70)
+        o.run(70)
+        got = ctx.download(bid)
+        np.testing.assert_array_equal(got["pos"], o.pos)
+        np.testing.assert_array_equal(got["vel"], o.vel)
+        ctx.upload(bid, ext=np.zeros((n, 3)))
+        o.e = None
+        ctx.step(70)
+        o.run(70)
+        got = ctx.download(bid)
+        np.testing.assert_array_equal(got["pos"], o.pos)
+        np.testing.assert_array_equal(got["vel"], o.vel)
+    b = dict(sc.bodies[0])
+    b["drag"] = (np.arange(n) % 11 == 0).astype(np.int32)
+    b["lc_ptr"] = np.arange(n + 1, dtype=np.int64)
+    b["lc_type"] = np.full(n, 1, np.int32)
+    b["lc_vec"] = np.tile([0.0, 0.0, 1.0], (n, 1))
+    b["lc_num"] = np.full(n, -0.01)
+    cases = [("drag and a local plane", scenes.Scene(bodies=[b], planes=sc.planes, dt=sc.dt), [], 0),
+             ("a ball", sc, [((0.0, 0.0, 2.0), 0.3)], 0),
+             ("RK2", sc, [], 2)]
+    for name, s2, balls, integ in cases:
+        _say(f"case split form with {name}: 150 steps")
+        o = to_oracle(s2, integrator=integ)
+        for c, r in balls:
+            o.balls = np.ascontiguousarray(np.array([[*c, r]], dtype=np.float64))
+        with Context() as ctx:
+            if integ:
+                ctx.set_integrator(integ)
+            for c, r in balls:
+                ctx.add_ball(c, r)
+            (bid,) = s2.load(ctx)
+            assert "k_rollout_split" in ctx.kernel_name()
+            ctx.step(150)
+            o.run(150)
+            got = ctx.download(bid)
+        np.testing.assert_array_equal(got["pos"], o.pos)
+        np.testing.assert_array_equal(got["vel"], o.vel)

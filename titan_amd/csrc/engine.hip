@@ -551,7 +551,7 @@ uint8_t mode_of(const BodyH &b) { return b.resident ? kModeResident : b.tiled ? 
 struct Range;
 // Whether every resident body a launch covers steps in LDS in the split form
 // (then the launch uses k_rollout_split).
-bool all_split(const titan_ctx *c, const Range &rg, const StepArgs &a);
+int32_t all_split(const titan_ctx *c, const Range &rg, const StepArgs &a);
 
 bool any_body(const titan_ctx *c, uint8_t mode) {
     for (auto &b : c->bodies)
@@ -606,19 +606,25 @@ Range body_range(const BodyH &b) {
     return r;
 }
 
-bool all_split(const titan_ctx *c, const Range &rg, const StepArgs &a) {
+int32_t all_split(const titan_ctx *c, const Range &rg, const StepArgs &a) {
     // the split launch copies the planes into its static LDS, which its
     // dynamic region must leave room for
     if (a.nplanes > kSplitPlanes ||
         resident_lds_bytes(a.res_rows) + (int64_t)a.res_cache > (int64_t)c->split_lds_max)
-        return false;
+        return 1;
     bool any = false;
+    // the lean mass pass: Euler, no balls, no extern force, drag or local
+    // constraints on any mass of the launch's bodies
+    bool lean = a.integ == kIntEuler && a.nballs == 0;
     for (const auto &b : c->bodies) {
         if (!(b.live && b.resident) || b.res < rg.r0 || b.res >= rg.r1) continue;
-        if (!(b.res_scache > 0 && b.res_scache <= a.res_cache && b.n_masses <= a.res_rows)) return false;
+        if (!(b.res_scache > 0 && b.res_scache <= a.res_cache && b.n_masses <= a.res_rows)) return 1;
         any = true;
+        if (lean && b.lc_len > 0) lean = false;
+        for (size_t q = 0; lean && q < b.flags.size(); ++q)
+            if (b.flags[q] & (kExt | kDrag | kLocalLC)) lean = false;
     }
-    return any;
+    return any ? (lean ? 3 : 2) : 1;
 }
 
 StepArgs make_args(titan_ctx *c, int slot, const Range &rg) {
@@ -696,7 +702,7 @@ StepArgs make_args(titan_ctx *c, int slot, const Range &rg) {
     a.res_rows = resident_rows(c);
     a.res_split = resident_few(c) ? 1 : 0;
     a.res_cache = resident_cache(c, a.res_split != 0);
-    if (a.res_split && all_split(c, rg, a)) a.res_split = 2;
+    if (a.res_split) a.res_split = all_split(c, rg, a);
     return a;
 }
 
@@ -2052,6 +2058,7 @@ void body_upload(titan_ctx *c, int32_t id, const double *pos, const double *vel,
         if (changed) {
             h2d(c, c->mflags + g0, b.flags.data(), N);
             HIPCHK(hipStreamSynchronize(c->stream));
+            drop_graphs(c);  // a captured split-form launch may have chosen the lean mass pass
         }
     }
 }
@@ -2546,7 +2553,7 @@ int titan_kernel_name(const titan_ctx *c, char *buf, int64_t len) {
                             any_body(c, kModeResident) ? resident_kernel_name(act) : nullptr};
     if (parts[2]) {
         const StepArgs a = make_args(const_cast<titan_ctx *>(c), c->slot, full_range(c));
-        if (a.res_split == 2) parts[2] = split_kernel_name(act);
+        if (a.res_split >= 2) parts[2] = split_kernel_name(act);
     }
     int64_t at = 0;
     buf[0] = 0;

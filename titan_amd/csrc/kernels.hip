@@ -290,6 +290,31 @@ __device__ __forceinline__ void mass_pass(const StepArgs &a, MassRegs &r, int64_
     }
 }
 
+// mass_pass for a mass without extern force, drag or local constraints, in a
+// launch without balls, under symplectic Euler: the same operations in the
+// same order (src/sim.cu:1104-1111, :1143, :1161-1167), with none of the
+// branches the other cases need.
+__device__ __forceinline__ void mass_pass_lean(const StepArgs &a, MassRegs &r, const MassEnv &env, double m,
+                                               double &ax, double &ay, double &az) {
+    r.fx = r.fx + a.gx * m;
+    r.fy = r.fy + a.gy * m;
+    r.fz = r.fz + a.gz * m;
+    for (int q = 0; q < a.nplanes; ++q) {
+        const double *P = env.planes + 8 * q;
+        contact_plane(r, P[0], P[1], P[2], P[3], P[4], P[5]);
+    }
+    const double dt = a.dt;
+    ax = r.fx / m;
+    ay = r.fy / m;
+    az = r.fz / m;
+    r.vx = r.vx + ax * dt;
+    r.vy = r.vy + ay * dt;
+    r.vz = r.vz + az * dt;
+    r.px = r.px + r.vx * dt;
+    r.py = r.py + r.vy * dt;
+    r.pz = r.pz + r.vz * dt;
+}
+
 }  // namespace
 
 // Spring forces on one mass, src/sim.cu:1055-1073, summed in ascending
@@ -1057,7 +1082,7 @@ __device__ __forceinline__ void split_terms(int x, int nsp, const uint32_t *__re
 // adds up its incidences in ascending spring index as the walk does and
 // runs the mass pass.  A body of one or two chunks then spreads its spring
 // work over four waves, with half the evaluations of the walk.
-template <bool ACT, int BLOCK = kResidentBlock>
+template <bool ACT, int BLOCK = kResidentBlock, bool LEAN = false>
 __device__ __forceinline__ void resident_steps_split(const StepArgs &a, const uint32_t *__restrict__ csw,
                                                      const double *__restrict__ cr, const double *__restrict__ kp,
                                                      const double *__restrict__ dp, double *__restrict__ fx,
@@ -1149,13 +1174,17 @@ __device__ __forceinline__ void resident_steps_split(const StepArgs &a, const ui
                     r.fz = r.fz + qz[j * nmass];
                 }
                 double ax, ay, az;
-                const int stage = (int)((nst + k) & 1);
-                const IntegIn in = load_integ(a, i, stage);
-                mass_pass(a, r, i, fl, env, m, stage, in, ax, ay, az);
+                if (LEAN) {
+                    mass_pass_lean(a, r, env, m, ax, ay, az);
+                } else {
+                    const int stage = (int)((nst + k) & 1);
+                    const IntegIn in = load_integ(a, i, stage);
+                    mass_pass(a, r, i, fl, env, m, stage, in, ax, ay, az);
+                }
                 nxt[3 * q] = make_double2(r.px, r.py);
                 nxt[3 * q + 1] = make_double2(r.pz, r.vx);
                 nxt[3 * q + 2] = make_double2(r.vy, r.vz);
-                if (k == K - 1 || a.integ == kIntVerlet) {
+                if (k == K - 1 || (!LEAN && a.integ == kIntVerlet)) {
                     a.acc[i] = ax;
                     a.acc[n6 + i] = ay;
                     a.acc[2 * n6 + i] = az;
@@ -1207,7 +1236,7 @@ __device__ __forceinline__ int split_fill(const StepArgs &a, char *__restrict__ 
 // A split-form body held in LDS, stepped by a workgroup of BLOCK threads:
 // its state copied in, the split-form region filled, K steps, the state
 // copied out to the buffer of its new step parity.
-template <bool ACT, int BLOCK>
+template <bool ACT, int BLOCK, bool LEAN>
 __device__ __forceinline__ void split_in_lds(const StepArgs &a, double2 *__restrict__ S,
                                              const ResidentDesc *__restrict__ rd, const MassEnv &env,
                                              const BodyKD &bkd, int nch, int nmass, int64_t inc0, int ninc, int64_t g0,
@@ -1234,8 +1263,8 @@ __device__ __forceinline__ void split_in_lds(const StepArgs &a, double2 *__restr
     const double *dp = (bkd.kd & 2u) ? cd : cr;
     double *fx = const_cast<double *>(cr) + nd;
     double *fy = fx + nslots, *fz = fy + nslots;
-    resident_steps_split<ACT, BLOCK>(a, csw, cr, kp, dp, fx, fy, fz, maxdeg, nsp, env, bkd, nch, nmass, g0, K, nst,
-                                     T, nact, act0, B0, B1);
+    resident_steps_split<ACT, BLOCK, LEAN>(a, csw, cr, kp, dp, fx, fy, fz, maxdeg, nsp, env, bkd, nch, nmass, g0, K,
+                                           nst, T, nact, act0, B0, B1);
     const double2 *__restrict__ fin = (K & 1) ? B1 : B0;
     double2 *__restrict__ dst = reinterpret_cast<double2 *>(a.st + (int64_t)((nst + K) & 1) * 6 * n6);
     for (int q = t; q < 3 * nmass; q += BLOCK) store_nt(&dst[3 * g0 + q], fin[q].x, fin[q].y);
@@ -1379,7 +1408,9 @@ __global__ __launch_bounds__(kResidentBlock) void k_rollout_resident(const StepA
 // workgroups of kSplitBlock threads, so that a body's spring terms are spread
 // over four times as many threads (the README scene's 1,036 springs: one per
 // thread), with the same arithmetic and order as k_rollout_resident.
-template <bool ACT>
+// LEAN: every mass of the launch's bodies takes mass_pass_lean's case (the
+// launch checks this on the host).
+template <bool ACT, bool LEAN>
 __global__ __launch_bounds__(kSplitBlock) void k_rollout_split(const StepArgs a, int final_slot) {
     extern __shared__ double2 S[];  // two buffers of 3 x res_rows pairs, then the split-form region
     __shared__ double planes[8 * kSplitPlanes];  // the global planes (the launch has at most kSplitPlanes)
@@ -1402,8 +1433,8 @@ __global__ __launch_bounds__(kSplitBlock) void k_rollout_split(const StepArgs a,
         if (ACT) actuate_slice(a, act0, nact, T, a.ract[0], t, kSplitBlock);
         for (int j = t; j < 8 * a.nplanes; j += kSplitBlock) planes[j] = a.planes[j];  // visible after split_fill's barriers
         const MassEnv env{planes, a.b_lc[b], a.b_lcn[b]};
-        split_in_lds<ACT, kSplitBlock>(a, S, rd, env, bkd, nch, nmass, inc0, ninc, (int64_t)chunk0 * kChunk, K, nst,
-                                       T, nact, act0);
+        split_in_lds<ACT, kSplitBlock, LEAN>(a, S, rd, env, bkd, nch, nmass, inc0, ninc, (int64_t)chunk0 * kChunk, K,
+                                             nst, T, nact, act0);
     }
     if (t == 0) {
         a.bT[final_slot][b] = T;
@@ -1513,9 +1544,15 @@ hipError_t launch_rollout_resident(const StepArgs &a, bool act, int final_slot, 
     if (a.nres <= a.res_base) return hipSuccess;
     const size_t lds = resident_lds_bytes(a.res_rows) + (size_t)a.res_cache;
     const dim3 g((unsigned)(a.nres - a.res_base)), blk(kResidentBlock);
-    if (a.res_split == 2) {
-        if (act) hipLaunchKernelGGL((k_rollout_split<true>), g, dim3(kSplitBlock), lds, s, a, final_slot);
-        else hipLaunchKernelGGL((k_rollout_split<false>), g, dim3(kSplitBlock), lds, s, a, final_slot);
+    if (a.res_split >= 2) {
+        const dim3 sb(kSplitBlock);
+        if (a.res_split == 3) {
+            if (act) hipLaunchKernelGGL((k_rollout_split<true, true>), g, sb, lds, s, a, final_slot);
+            else hipLaunchKernelGGL((k_rollout_split<false, true>), g, sb, lds, s, a, final_slot);
+        } else {
+            if (act) hipLaunchKernelGGL((k_rollout_split<true, false>), g, sb, lds, s, a, final_slot);
+            else hipLaunchKernelGGL((k_rollout_split<false, false>), g, sb, lds, s, a, final_slot);
+        }
     } else if (a.res_split) {
         if (act) hipLaunchKernelGGL((k_rollout_resident<true, true>), g, blk, lds, s, a, final_slot);
         else hipLaunchKernelGGL((k_rollout_resident<false, true>), g, blk, lds, s, a, final_slot);
@@ -1548,7 +1585,8 @@ hipError_t set_tiles_lds_limit(int *resident_max, int *split_max) {
     }
     // the split launch's own LDS (its planes) comes out of the same budget
     *split_max = std::min(rb, lim - kSplitStaticLds);
-    const void *spl[2] = {(const void *)k_rollout_split<false>, (const void *)k_rollout_split<true>};
+    const void *spl[4] = {(const void *)k_rollout_split<false, false>, (const void *)k_rollout_split<true, false>,
+                          (const void *)k_rollout_split<false, true>, (const void *)k_rollout_split<true, true>};
     for (const void *f : spl) {
         er = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, *split_max);
         if (er != hipSuccess) return er;

@@ -252,7 +252,8 @@ struct StepArgs {
     int64_t res_base, nres;   // launch range: descriptors [res_base, nres)
     int32_t res_rows;         // LDS rows per buffer: the largest resident body
     int32_t res_cache;        // LDS bytes after the two state buffers for an incidence copy
-    int32_t res_split;        // 1: bodies with a split-form region use it; 2: every body does (k_rollout_split)
+    int32_t res_split;        // 1: bodies with a split-form region use it; 2: every body does (k_rollout_split);
+                              // 3: as 2, and every mass takes the lean mass pass
 };
 
 }  // namespace titan_mi355x
