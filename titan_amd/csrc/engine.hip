@@ -1043,8 +1043,9 @@ struct PreparedBody {
 // The split form of a resident body (endpoints in device rows): its springs
 // once each and the slots of their two incidences' terms (incidence j of
 // row q: j * N + q, each mass's incidences in the order its SELL entries
-// hold them); nothing when the slots exceed 16 bits or its LDS region would
-// not fit.
+// hold them; the LDS region has the largest degree rounded up to a multiple
+// of 4 slots per row); nothing when the slots exceed 16 bits or its LDS
+// region would not fit.
 void build_split(PreparedBody &P, const SpringsIn &sp, const std::vector<int32_t> &act_local) {
     const int64_t S = sp.n_springs, N = P.N;
     int64_t nsp = 0, nreal = 0, maxdeg = 0;
@@ -1054,7 +1055,7 @@ void build_split(PreparedBody &P, const SpringsIn &sp, const std::vector<int32_t
         nreal += dg;
         maxdeg = std::max<int64_t>(maxdeg, dg);
     }
-    const int64_t nslots = maxdeg * N;
+    const int64_t nslots = ((maxdeg + 3) & ~(int64_t)3) * N;  // the kernel sums whole fours of slots
     if (nreal != 2 * nsp || nslots > 0x10000 || split_region_bytes(nslots, nsp, P.per_k, P.per_d) > kResidentCacheMax)
         return;
     std::vector<int64_t> at(N, 0);  // the next incidence of each mass

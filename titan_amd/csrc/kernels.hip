@@ -984,6 +984,47 @@ __device__ __forceinline__ void resident_steps(const StepArgs &a, const Src &src
     }
 }
 
+// One force component of a row of a split-form body: its incidences' terms
+// (slots j * nmass + the row, +0 past the row's degree) added in ascending
+// spring index, as the walk sums.  nd (the body's largest degree rounded up
+// to a multiple of 4) terms, all read before the first add when nd <= 32,
+// so that one wait covers them; else eight at a time.
+template <int N>
+__device__ __forceinline__ double row_sum_n(const double *__restrict__ f, int nmass) {
+    double v[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u) v[u] = f[u * nmass];
+    double sum = 0.0;
+#pragma unroll
+    for (int u = 0; u < N; ++u) sum = sum + v[u];
+    return sum;
+}
+__device__ __forceinline__ double row_sum(const double *__restrict__ f, int nmass, int nd) {
+    switch (nd) {
+        case 4: return row_sum_n<4>(f, nmass);
+        case 8: return row_sum_n<8>(f, nmass);
+        case 12: return row_sum_n<12>(f, nmass);
+        case 16: return row_sum_n<16>(f, nmass);
+        case 20: return row_sum_n<20>(f, nmass);
+        case 24: return row_sum_n<24>(f, nmass);
+        case 28: return row_sum_n<28>(f, nmass);
+        case 32: return row_sum_n<32>(f, nmass);
+        default: break;
+    }
+    const int maxdeg = nd;
+    double sum = 0.0;
+    int j = 0;
+    for (; j + 8 <= maxdeg; j += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = f[(j + u) * nmass];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) sum = sum + v[u];
+    }
+    for (; j < maxdeg; ++j) sum = sum + f[j * nmass];
+    return sum;
+}
+
 // v, or -v when `neg` (a flip of the sign bit; x + (-v) is x - v exactly)
 __device__ __forceinline__ double signed_term(double v, bool neg) {
     return __hiloint2double(__double2hiint(v) ^ (neg ? (int)0x80000000 : 0), __double2loint(v));
@@ -994,7 +1035,9 @@ __device__ __forceinline__ double signed_term(double v, bool neg) {
 // as the walk forms the term at a, stored as -G in the slot of a's incidence
 // and +G in b's; U springs at a time so that their latencies overlap (a lane
 // past the end repeats spring x and stores it once).
-template <bool ACT, int U, int BLOCK = kResidentBlock>
+// KD: where k and damping come from (as the walk's Group: bit 0 k per
+// spring, bit 1 damping per spring, kKdRuntime from bkd.kd).
+template <bool ACT, int U, int BLOCK = kResidentBlock, int KD = kKdRuntime>
 __device__ __forceinline__ void split_terms(int x, int nsp, const uint32_t *__restrict__ csw,
                                             const double *__restrict__ cr, const double *__restrict__ kp,
                                             const double *__restrict__ dp, const BodyKD &bkd,
@@ -1002,27 +1045,35 @@ __device__ __forceinline__ void split_terms(int x, int nsp, const uint32_t *__re
                                             double *__restrict__ fx, double *__restrict__ fy,
                                             double *__restrict__ fz) {
     double n2[U], nrm[U], dvx[U], dvy[U], dvz[U], tx[U], ty[U], tz[U], rl[U], kk[U], dmp[U];
-    uint32_t w[U];
+    uint32_t w[U], iw[U];
     int xi[U];
+    const bool pk = KD == kKdRuntime ? (bkd.kd & 1u) != 0 : (KD & 1) != 0;
+    const bool pd = KD == kKdRuntime ? (bkd.kd & 2u) != 0 : (KD & 2) != 0;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int xu = x + u * BLOCK;
         xi[u] = xu < nsp ? xu : x;
         w[u] = csw[xi[u]];
+        iw[u] = csw[nsp + xi[u]];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         rl[u] = cr[xi[u]];
-        kk[u] = kp[xi[u]];
-        dmp[u] = dp[xi[u]];
+        if (KD == kKdRuntime) {  // read in any case (kp, dp are valid arrays), then replaced
+            kk[u] = kp[xi[u]];
+            dmp[u] = dp[xi[u]];
+        } else {
+            kk[u] = pk ? kp[xi[u]] : bkd.k;
+            dmp[u] = pd ? dp[xi[u]] : bkd.d;
+        }
     }
     bool fast = true;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const uint32_t ea = w[u] & 0xffffu, eb = (w[u] >> 16) & (0xffffu & ~kActBit16);
         if (ACT && (w[u] & (kActBit16 << 16))) rl[u] = rc[__double_as_longlong(rl[u])];
-        if (!(bkd.kd & 1u)) kk[u] = bkd.k;
-        if (!(bkd.kd & 2u)) dmp[u] = bkd.d;
+        if (KD == kKdRuntime && !pk) kk[u] = bkd.k;
+        if (KD == kKdRuntime && !pd) dmp[u] = bkd.d;
         const double2 s0 = cur[3 * ea], s1 = cur[3 * ea + 1], s2 = cur[3 * ea + 2];
         const double2 n0 = cur[3 * eb], n1 = cur[3 * eb + 1], n2v = cur[3 * eb + 2];
         tx[u] = n0.x - s0.x;
@@ -1063,8 +1114,7 @@ __device__ __forceinline__ void split_terms(int x, int nsp, const uint32_t *__re
         const double dd = (dvx[u] * tx[u] + dvy[u] * ty[u] + dvz[u] * tz[u]) * dmp[u];
         const double gx = tx[u] * mag + tx[u] * dd, gy = ty[u] * mag + ty[u] * dd, gz = tz[u] * mag + tz[u] * dd;
         if (x + u * BLOCK < nsp) {
-            const uint32_t iw = csw[nsp + xi[u]];
-            const int ia = (int)(iw & 0xffffu), ib = (int)(iw >> 16);
+            const int ia = (int)(iw[u] & 0xffffu), ib = (int)(iw[u] >> 16);
             fx[ia] = signed_term(gx, true);
             fy[ia] = signed_term(gy, true);
             fz[ia] = signed_term(gz, true);
@@ -1073,6 +1123,24 @@ __device__ __forceinline__ void split_terms(int x, int nsp, const uint32_t *__re
             fz[ib] = gz;
         }
     }
+}
+
+// Every spring's term once, thread t taking springs t, t + BLOCK, ...: one
+// spring per thread and pass, four waves per SIMD hiding the chain's
+// latency; a wave with a lane that has a second spring evaluates both
+// interleaved (every x < BLOCK < nsp there).
+template <bool ACT, int BLOCK, int KD>
+__device__ __forceinline__ void terms_pass(int t, int nsp, const uint32_t *__restrict__ csw,
+                                           const double *__restrict__ cr, const double *__restrict__ kp,
+                                           const double *__restrict__ dp, const BodyKD &bkd,
+                                           const double2 *__restrict__ cur, const double *__restrict__ rc,
+                                           double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz) {
+    int x = t;
+    if (__any(x + BLOCK < nsp)) {
+        split_terms<ACT, 2, BLOCK, KD>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
+        x += 2 * BLOCK;
+    }
+    for (; x < nsp; x += BLOCK) split_terms<ACT, 1, BLOCK, KD>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
 }
 
 // The split form of resident_steps (a body of at most kSplitMaxChunks
@@ -1087,11 +1155,17 @@ __device__ __forceinline__ void resident_steps_split(const StepArgs &a, const ui
                                                      const double *__restrict__ cr, const double *__restrict__ kp,
                                                      const double *__restrict__ dp, double *__restrict__ fx,
                                                      double *__restrict__ fy, double *__restrict__ fz,
-                                                     int maxdeg, int nsp, const MassEnv &env,
+                                                     double *__restrict__ sums, int maxdeg, int nsp,
+                                                     const MassEnv &env,
                                                      const BodyKD &bkd, int nch, int nmass, int64_t g0, int64_t K,
                                                      int64_t nst, double &T, int nact, int64_t act0,
                                                      double2 *__restrict__ B0, double2 *__restrict__ B1) {
     constexpr int U = 4;
+    // a workgroup with a lane for each component of each row sums the
+    // components on lanes of their own, after which the mass passes read
+    // them (one more barrier per step, a third of the dependent adds per
+    // lane)
+    constexpr bool SPREAD = BLOCK >= 3 * kSplitMaxChunks * kChunk;
     const int t = threadIdx.x, wave = uniform(t >> 6), lane = t & (kChunk - 1);
     const int64_t n6 = a.ncap;
     // the rows go to the first four waves (one per SIMD), 16, 32 or 64 lanes
@@ -1116,17 +1190,20 @@ __device__ __forceinline__ void resident_steps_split(const StepArgs &a, const ui
                 split_terms<ACT, U, BLOCK>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
             for (; x < nsp; x += BLOCK) split_terms<ACT, 1, BLOCK>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
         } else {
-            // one spring per thread and pass, four waves per SIMD hiding the
-            // chain's latency; a wave with a lane that has a second spring
-            // evaluates both interleaved (every x < BLOCK < nsp there)
-            int x = t;
-            if (__any(x + BLOCK < nsp)) {
-                split_terms<ACT, 2, BLOCK>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
-                x += 2 * BLOCK;
+            switch (bkd.kd & 3u) {  // uniform: the body's constant mode fixed at compile time
+                case 0: terms_pass<ACT, BLOCK, 0>(t, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz); break;
+                case 1: terms_pass<ACT, BLOCK, 1>(t, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz); break;
+                case 2: terms_pass<ACT, BLOCK, 2>(t, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz); break;
+                default: terms_pass<ACT, BLOCK, 3>(t, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz); break;
             }
-            for (; x < nsp; x += BLOCK) split_terms<ACT, 1, BLOCK>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
         }
         __syncthreads();
+        if (SPREAD) {  // each component's sums on a lane of its own, into `sums`
+            const int P = (nmass + kChunk - 1) & ~(kChunk - 1);
+            const int c = uniform(t / P), qq = t - c * P;
+            if (c < 3 && qq < nmass) sums[c * P + qq] = row_sum((c == 0 ? fx : c == 1 ? fy : fz) + qq, nmass, maxdeg);
+            __syncthreads();
+        }
         if (mine && !(fl & kPad)) {
             const double2 o0 = cur[3 * q], o1 = cur[3 * q + 1], o2 = cur[3 * q + 2];
             if (fl & kFixed) {  // src/sim.cu:1100-1101
@@ -1141,37 +1218,15 @@ __device__ __forceinline__ void resident_steps_split(const StepArgs &a, const ui
                 r.vx = o1.y;
                 r.vy = o2.x;
                 r.vz = o2.y;
-                r.fx = 0.0;
-                r.fy = 0.0;
-                r.fz = 0.0;
-                // ascending spring index, as the walk sums: the term of
-                // incidence j of row q is in slot j * nmass + q, so that a
-                // wave's reads of one j are consecutive words, and the
-                // slots past a row's degree hold +0 (x + 0 is x, zero signs
-                // aside), so that every lane runs the body's largest degree
-                // without masks; eight incidences' terms are read before
-                // their adds, so that one wait covers them
-                const double *__restrict__ qx = fx + q, *__restrict__ qy = fy + q, *__restrict__ qz = fz + q;
-                int j = 0;
-                for (; j + 8 <= maxdeg; j += 8) {
-                    double vx[8], vy[8], vz[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        vx[u] = qx[(j + u) * nmass];
-                        vy[u] = qy[(j + u) * nmass];
-                        vz[u] = qz[(j + u) * nmass];
-                    }
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        r.fx = r.fx + vx[u];
-                        r.fy = r.fy + vy[u];
-                        r.fz = r.fz + vz[u];
-                    }
-                }
-                for (; j < maxdeg; ++j) {
-                    r.fx = r.fx + qx[j * nmass];
-                    r.fy = r.fy + qy[j * nmass];
-                    r.fz = r.fz + qz[j * nmass];
+                if (SPREAD) {
+                    const int P = (nmass + kChunk - 1) & ~(kChunk - 1);
+                    r.fx = sums[q];
+                    r.fy = sums[P + q];
+                    r.fz = sums[2 * P + q];
+                } else {
+                    r.fx = row_sum(fx + q, nmass, maxdeg);
+                    r.fy = row_sum(fy + q, nmass, maxdeg);
+                    r.fz = row_sum(fz + q, nmass, maxdeg);
                 }
                 double ax, ay, az;
                 if (LEAN) {
@@ -1197,10 +1252,11 @@ __device__ __forceinline__ void resident_steps_split(const StepArgs &a, const ui
     }
 }
 
-// Fill a body's split-form region: the body's largest degree, copies of the springs' endpoint and slot words, rest lengths, k
+// Fill a body's split-form region: the body's largest degree rounded up to a
+// multiple of 4 (returned), copies of the springs' endpoint and slot words, rest lengths, k
 // and damping from the body's incidence span, and +0 in the term slots past
-// each row's degree (terms at fx, fy, fz: at most the returned largest
-// degree x nmass slots each).
+// each row's degree up to the returned value (terms at fx, fy, fz: that
+// many x nmass slots each).
 // The first kSplitMaxChunks waves hold one row per thread; a larger
 // workgroup's other threads only help with the copies.
 static_assert(kResidentBlock == kSplitMaxChunks * kChunk, "one thread per row of a split-form body");
@@ -1220,7 +1276,7 @@ __device__ __forceinline__ int split_fill(const StepArgs &a, char *__restrict__ 
     for (int j = t; j < 2 * nsp; j += BLOCK) cw[j] = a.inc_nbr[e0 + j];
     for (int j = t; j < nd; j += BLOCK) cr[j] = a.inc_rest[e0 + j];
     __syncthreads();
-    const int maxdeg = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+    const int maxdeg = (max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3])) + 3) & ~3;
     const int ns = maxdeg * nmass;
     double *fx = cr + nd;
     if (t < nmass) {
@@ -1237,7 +1293,7 @@ __device__ __forceinline__ int split_fill(const StepArgs &a, char *__restrict__ 
 // its state copied in, the split-form region filled, K steps, the state
 // copied out to the buffer of its new step parity.
 template <bool ACT, int BLOCK, bool LEAN>
-__device__ __forceinline__ void split_in_lds(const StepArgs &a, double2 *__restrict__ S,
+__device__ __forceinline__ void split_in_lds(const StepArgs &a, double2 *__restrict__ S, double *__restrict__ sums,
                                              const ResidentDesc *__restrict__ rd, const MassEnv &env,
                                              const BodyKD &bkd, int nch, int nmass, int64_t inc0, int ninc, int64_t g0,
                                              int64_t K, int64_t nst, double &T, int nact, int64_t act0) {
@@ -1263,8 +1319,8 @@ __device__ __forceinline__ void split_in_lds(const StepArgs &a, double2 *__restr
     const double *dp = (bkd.kd & 2u) ? cd : cr;
     double *fx = const_cast<double *>(cr) + nd;
     double *fy = fx + nslots, *fz = fy + nslots;
-    resident_steps_split<ACT, BLOCK, LEAN>(a, csw, cr, kp, dp, fx, fy, fz, maxdeg, nsp, env, bkd, nch, nmass, g0, K,
-                                           nst, T, nact, act0, B0, B1);
+    resident_steps_split<ACT, BLOCK, LEAN>(a, csw, cr, kp, dp, fx, fy, fz, sums, maxdeg, nsp, env, bkd, nch, nmass,
+                                           g0, K, nst, T, nact, act0, B0, B1);
     const double2 *__restrict__ fin = (K & 1) ? B1 : B0;
     double2 *__restrict__ dst = reinterpret_cast<double2 *>(a.st + (int64_t)((nst + K) & 1) * 6 * n6);
     for (int q = t; q < 3 * nmass; q += BLOCK) store_nt(&dst[3 * g0 + q], fin[q].x, fin[q].y);
@@ -1312,8 +1368,8 @@ __device__ __forceinline__ void resident_in_lds(const StepArgs &a, double2 *__re
         const double *dp = (bkd.kd & 2u) ? cd : cr;
         double *fx = const_cast<double *>(cr) + nd;
         double *fy = fx + nslots, *fz = fy + nslots;
-        resident_steps_split<ACT>(a, csw, cr, kp, dp, fx, fy, fz, maxdeg, nsp, mass_env(a, b), bkd, nch, nmass, g0,
-                                  K, nst, T, nact, act0, B0, B1);
+        resident_steps_split<ACT>(a, csw, cr, kp, dp, fx, fy, fz, nullptr, maxdeg, nsp, mass_env(a, b), bkd, nch,
+                                  nmass, g0, K, nst, T, nact, act0, B0, B1);
     } else if (cached) {
         // names (4 B per entry), then from a 16-byte boundary the rest
         // lengths and, where the body has them, k and damping (8 B each)
@@ -1414,6 +1470,7 @@ template <bool ACT, bool LEAN>
 __global__ __launch_bounds__(kSplitBlock) void k_rollout_split(const StepArgs a, int final_slot) {
     extern __shared__ double2 S[];  // two buffers of 3 x res_rows pairs, then the split-form region
     __shared__ double planes[8 * kSplitPlanes];  // the global planes (the launch has at most kSplitPlanes)
+    __shared__ double sums[3 * kSplitMaxChunks * kChunk];  // each row's force components
     const int64_t rid = a.res_base + blockIdx.x;
     if (rid >= a.nres) return;
     const ResidentDesc *__restrict__ rd = a.rdesc + rid;
@@ -1433,8 +1490,8 @@ __global__ __launch_bounds__(kSplitBlock) void k_rollout_split(const StepArgs a,
         if (ACT) actuate_slice(a, act0, nact, T, a.ract[0], t, kSplitBlock);
         for (int j = t; j < 8 * a.nplanes; j += kSplitBlock) planes[j] = a.planes[j];  // visible after split_fill's barriers
         const MassEnv env{planes, a.b_lc[b], a.b_lcn[b]};
-        split_in_lds<ACT, kSplitBlock, LEAN>(a, S, rd, env, bkd, nch, nmass, inc0, ninc, (int64_t)chunk0 * kChunk, K,
-                                             nst, T, nact, act0);
+        split_in_lds<ACT, kSplitBlock, LEAN>(a, S, sums, rd, env, bkd, nch, nmass, inc0, ninc, (int64_t)chunk0 * kChunk,
+                                             K, nst, T, nact, act0);
     }
     if (t == 0) {
         a.bT[final_slot][b] = T;

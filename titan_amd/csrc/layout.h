@@ -75,11 +75,12 @@ constexpr int kResidentBlock = 256;
 // one pass, and a second for 12 threads), with four waves per SIMD to hide
 // the latency of a term's dependent FP64 chain.  It copies the global
 // planes (at most kSplitPlanes; a launch with more uses the 256-thread
-// resident kernel) into static LDS, which the dynamic region leaves room
-// for.
+// resident kernel) into static LDS, where it also keeps each row's three
+// force components; the dynamic region leaves room for both.
+constexpr int kSplitMaxChunks = 4;  // a split-form body has at most this many chunks
 constexpr int kSplitBlock = 1024;
 constexpr int kSplitPlanes = 16;
-constexpr int kSplitStaticLds = 8 * 8 * kSplitPlanes;
+constexpr int kSplitStaticLds = 8 * (8 * kSplitPlanes + 3 * kSplitMaxChunks * kChunk);
 
 struct ResidentDesc {
     int32_t body;     // -1: free
@@ -115,9 +116,8 @@ constexpr int kResidentCacheMax = 128 * 1024;
 // signs aside); each mass's lane then adds up its own slots in order.
 // LDS region: four int32 partial maxima of the degrees, the 2 nsp words,
 // from a 16-byte boundary the rest lengths, k and damping where per spring,
-// and the three components of the terms, largest degree x nmass slots each
-// (8 B).
-constexpr int kSplitMaxChunks = 4;
+// and the three components of the terms, (largest degree rounded up to a
+// multiple of 4) x nmass slots each (8 B).
 constexpr int kSplitHead = 16;
 constexpr int64_t split_region_bytes(int64_t nslots, int64_t nsp, bool per_k, bool per_d) {
     return ((kSplitHead + 8 * nsp + 15) & ~(int64_t)15) + 8 * nsp * (1 + (per_k ? 1 : 0) + (per_d ? 1 : 0)) +
