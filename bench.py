@@ -284,6 +284,17 @@ def bench_serving(seed: int, n_req: int, lams, quantum: int, n_base: int) -> dic
     best = max(x["mass_steps_per_s"] for x in per_lam)
     log(f"serving: one request at a time, one launch per step: {base_rate:.3e} mass-steps/s "
         f"({n_base} requests, {ms:.0f} ms)")
+    # the same requests all present from the start, stepped together to their
+    # horizons (no arrivals, admissions or pumps): the batched rate of this mix
+    with Context() as ctx:
+        setup(ctx, sched)
+        ids = [ctx.add_body(**b) for b, _ in reqs]
+        ctx.sync()
+        ctx.timer_start()
+        ctx.step_bodies(ids, [h for _, h in reqs])
+        ms_all = ctx.timer_stop()
+    batched_rate = total / (ms_all * 1e-3)
+    log(f"serving: the same requests batched from the start: {batched_rate:.3e} mass-steps/s ({ms_all:.0f} ms)")
     return {
         "workload": f"{n_req} rollout requests per lambda (seed {seed}): lattices side U{{3..40}} or kNN-12 "
                     f"clouds of 1e2..1e5 masses, 10-30% actuated, horizon U{{100..10000}} steps, ground "
@@ -293,6 +304,9 @@ def bench_serving(seed: int, n_req: int, lams, quantum: int, n_base: int) -> dic
         "one_request_at_a_time": {"value": base_rate, "unit": "mass-steps/s",
                                   "sample": f"first {n_base} requests, one launch per step each"},
         "speedup_vs_one_request_at_a_time": best / base_rate,
+        "same_requests_batched": {"value": batched_rate, "unit": "mass-steps/s",
+                                  "sample": f"all {n_req} requests present at once, stepped to their horizons"},
+        "served_over_batched": best / batched_rate,
         "schedule": {"tiled_min_masses": sched[0], "tile_rows": sched[1], "resident_max_masses": sched[2]},
     }
 

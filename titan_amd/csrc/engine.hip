@@ -2709,10 +2709,10 @@ struct Prepared {
     PreparedBody body;
 };
 
-Prepared *prepare_body_host(const titan_ctx *c, const titan_body_desc *d, int *rc, std::string *err) {
+Prepared *prepare_body_sched(const titan_schedule &sched, const titan_body_desc *d, int *rc, std::string *err) {
     try {
         std::unique_ptr<Prepared> p(new Prepared);
-        p->body = prepare_body(c->sched, d, c->device_min);
+        p->body = prepare_body(sched, d, 0);
         *rc = TITAN_OK;
         return p.release();
     } catch (const Fail &f) {

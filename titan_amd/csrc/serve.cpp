@@ -4,16 +4,19 @@
 // each pump admits queued requests in arrival order while they fit the
 // running-set limits, steps all running requests together through
 // titan_step_bodies (one launch per step covers every body; each body stops
-// at its own horizon), and retires the finished ones.  Admission lays out
-// several bodies at once on short-lived host threads, joined before the
-// pump returns, while the device runs the quantum just enqueued; placing a
-// body on the device does not wait for the launches in flight.  The engine
-// keeps every body's arithmetic independent of the others, so a request's
-// result is the same alone and in any mix.
+// at its own horizon), and retires the finished ones.  Layouts are built
+// on the host by background worker threads from the moment a request is
+// submitted (in submission order, under the schedule current at its
+// submission), so that admission mostly places bodies already laid out,
+// while the device runs the quantum just enqueued; placing a body on the
+// device does not wait for the launches in flight.  The engine keeps every
+// body's arithmetic independent of the others, so a request's result is the
+// same alone and in any mix.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -56,6 +59,10 @@ struct Request {
     titan_mi355x::Prepared *prep = nullptr;  // laid out on the host, not yet placed
     int prep_rc = TITAN_OK;
     std::string prep_err;
+    // layout by a worker: 0 none yet, 1 in progress, 2 done (prep / prep_rc
+    // set); guarded by the server's prep_mu
+    int prep_state = 0;
+    titan_schedule sched{};  // the context's schedule when submitted
 
     ~Request() { titan_mi355x::free_prepared(prep); }
     std::vector<double> pos, vel, m, T, ext, k, rest, damping, omega, act_a, act_phi, lc_vec, lc_num;
@@ -127,29 +134,6 @@ uint64_t estimate_bytes(int64_t n, int64_t s) {
     return (uint64_t)(256 * n + 96 * s + (1 << 16));
 }
 
-// Lay out the bodies of `rs` on up to `threads` host threads.
-void prepare_all(const titan_ctx *ctx, const std::vector<Request *> &rs, unsigned threads) {
-    std::atomic<size_t> next{0};
-    auto work = [&]() {
-        for (size_t q; (q = next.fetch_add(1)) < rs.size();) {
-            Request *r = rs[q];
-            if (r->prep) continue;
-            r->prep = titan_mi355x::prepare_body_host(ctx, &r->desc, &r->prep_rc, &r->prep_err);
-        }
-    };
-    const unsigned nt = std::min<unsigned>(threads, (unsigned)rs.size());
-    std::vector<std::thread> pool;
-    for (unsigned t = 1; t < nt; ++t) {
-        try {
-            pool.emplace_back(work);
-        } catch (const std::exception &) {  // no more threads: the rest runs here
-            break;
-        }
-    }
-    work();
-    for (auto &t : pool) t.join();
-}
-
 unsigned host_threads() {
     const unsigned hw = std::thread::hardware_concurrency();
     return std::max(1u, std::min(16u, hw ? hw : 1u));
@@ -163,6 +147,15 @@ struct Finishing {
 };
 
 struct Server {
+    // background layouts: requests waiting for one, in submission order, and
+    // the worker threads taking them (started with the first submission,
+    // stopped when the context is destroyed)
+    std::mutex prep_mu;
+    std::condition_variable prep_cv;
+    std::deque<Request *> todo;
+    std::vector<std::thread> workers;
+    bool stop = false;
+
     int32_t max_running = 4096;
     uint64_t hbm_bytes = 0;
     int64_t quantum = 256;
@@ -177,6 +170,83 @@ struct Server {
     int64_t submitted = 0, completed = 0, cancelled = 0, failed = 0;
     std::vector<double> latencies;
     double mass_steps = 0, first_submit = -1, last_done = -1;
+
+    void worker() {
+        for (;;) {
+            Request *r;
+            {
+                std::unique_lock<std::mutex> lk(prep_mu);
+                prep_cv.wait(lk, [&] { return stop || !todo.empty(); });
+                if (stop) return;
+                r = todo.front();
+                todo.pop_front();
+                r->prep_state = 1;
+            }
+            int rc = TITAN_OK;
+            std::string err;
+            titan_mi355x::Prepared *p = titan_mi355x::prepare_body_sched(r->sched, &r->desc, &rc, &err);
+            {
+                std::lock_guard<std::mutex> lk(prep_mu);
+                r->prep = p;
+                r->prep_rc = rc;
+                r->prep_err.swap(err);
+                r->prep_state = 2;
+            }
+            prep_cv.notify_all();
+        }
+    }
+    void start_workers(unsigned n) {
+        for (unsigned t = 0; t < n; ++t) {
+            try {
+                workers.emplace_back([this] { worker(); });
+            } catch (const std::exception &) {  // no more threads: admission lays out the rest itself
+                break;
+            }
+        }
+    }
+    void stop_workers() {
+        {
+            std::lock_guard<std::mutex> lk(prep_mu);
+            stop = true;
+        }
+        prep_cv.notify_all();
+        for (auto &t : workers) t.join();
+        workers.clear();
+    }
+    ~Server() { stop_workers(); }  // a context never destroyed: its workers end with the process
+    // Take r out of the workers' hands: unclaimed, it leaves the to-do list;
+    // claimed, wait for its layout.  Afterwards no worker touches r.
+    void settle(Request *r) {
+        std::unique_lock<std::mutex> lk(prep_mu);
+        if (r->prep_state == 0) {
+            auto q = std::find(todo.begin(), todo.end(), r);
+            if (q != todo.end()) todo.erase(q);
+            return;
+        }
+        prep_cv.wait(lk, [&] { return r->prep_state == 2; });
+    }
+
+    // Wait for r's layout (the workers take the to-do list in order, and r
+    // is among its oldest); with no worker thread, lay it out here.
+    void wait_layout(Request *r) {
+        std::unique_lock<std::mutex> lk(prep_mu);
+        if (workers.empty() && r->prep_state == 0) {
+            auto q = std::find(todo.begin(), todo.end(), r);
+            if (q != todo.end()) todo.erase(q);
+            r->prep_state = 1;
+            lk.unlock();
+            int rc = TITAN_OK;
+            std::string err;
+            titan_mi355x::Prepared *p = titan_mi355x::prepare_body_sched(r->sched, &r->desc, &rc, &err);
+            lk.lock();
+            r->prep = p;
+            r->prep_rc = rc;
+            r->prep_err.swap(err);
+            r->prep_state = 2;
+            return;
+        }
+        prep_cv.wait(lk, [&] { return r->prep_state == 2; });
+    }
 
     // called when the context is destroyed (not from a destructor, which
     // may run after the HIP runtime's own teardown at process exit)
@@ -276,11 +346,16 @@ int observe(Server &sv, bool wait) {
 
 // Called by titan_ctx_destroy (internal to the library).
 extern "C" __attribute__((visibility("hidden"))) void titan_internal_serve_forget(const titan_ctx *ctx) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    auto it = g_servers.find(ctx);
-    if (it == g_servers.end()) return;
-    it->second->release_events();
-    g_servers.erase(it);
+    std::unique_ptr<Server> sv;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_servers.find(ctx);
+        if (it == g_servers.end()) return;
+        sv = std::move(it->second);
+        g_servers.erase(it);
+    }
+    sv->stop_workers();  // they read requests' arrays, which go with the server
+    sv->release_events();
 }
 
 static int serve_config(titan_ctx *ctx, int32_t max_running, uint64_t hbm_bytes, int64_t quantum) {
@@ -308,11 +383,19 @@ static int serve_submit(titan_ctx *ctx, const titan_request *req, int64_t *ticke
     r->out_vel = req->out_vel;
     r->t_submit = now_s();
     r->est_bytes = estimate_bytes(r->n, r->s);
+    if (titan_get_schedule(ctx, &r->sched) != TITAN_OK) return TITAN_E_INVALID;
     if (sv.first_submit < 0) sv.first_submit = r->t_submit;
     sv.submitted++;
-    sv.queue.push_back(r.get());
+    Request *rp = r.get();
+    sv.queue.push_back(rp);
     if (ticket) *ticket = r->ticket;
     sv.reqs[r->ticket] = std::move(r);
+    {
+        std::lock_guard<std::mutex> lk(sv.prep_mu);
+        sv.todo.push_back(rp);
+    }
+    if (sv.workers.empty()) sv.start_workers(std::max(1u, host_threads() - 1));
+    sv.prep_cv.notify_one();
     return TITAN_OK;
 }
 
@@ -339,9 +422,10 @@ static int serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
         }
     } reconcile{sv, fresh};
     // admit, in arrival order, while the running set has room: up to `cap`
-    // requests, laid out together on the host, then placed one by one
-    // `until`: stop (between batches) once the step stream has passed this
-    // event, so that the device does not wait for admission
+    // requests, in batches of the queue's head requests that fit, each laid
+    // out by the workers (waited for) and then placed one by one; `until`:
+    // stop (between batches) once the step stream has passed this event, so
+    // that the device does not wait for admission
     auto admit = [&](size_t cap, hipEvent_t until) -> int {
         const unsigned nt = host_threads();
         bool first = true;
@@ -360,7 +444,7 @@ static int serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
                 bytes += r->est_bytes;
             }
             if (pick.empty()) break;
-            prepare_all(ctx, pick, nt);
+            for (Request *r : pick) sv.wait_layout(r);
             for (Request *r : pick) {
                 const bool busy = !sv.running.empty() || !fresh.empty();
                 int32_t id = -1;
@@ -459,6 +543,7 @@ static int serve_cancel(titan_ctx *ctx, int64_t ticket) {
     if (it == sv.reqs.end()) return TITAN_E_NOTFOUND;
     Request *r = it->second.get();
     if (r->state == TITAN_REQ_QUEUED) {
+        sv.settle(r);  // no worker may still read its arrays
         auto q = std::find(sv.queue.begin(), sv.queue.end(), r);
         if (q != sv.queue.end()) sv.queue.erase(q);
     } else if (r->state == TITAN_REQ_RUNNING && r->body < 0) {  // finishing: its body is released

@@ -56,56 +56,95 @@ std::vector<int32_t> morton_order(const double *pos, int64_t n) {
     return perm;
 }
 
-bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb, int tile_rows) {
+namespace {
+
+// Each mass's incident springs in ascending spring index, as CSR: for every
+// live spring s (ascending), its right end lists (L, s), then its left end
+// lists (R, s), so that a self-spring appears twice, right end first.
+struct Incidence {
+    std::vector<int64_t> row;    // n + 1 offsets
+    std::vector<int32_t> other;  // the other endpoint
+    std::vector<int64_t> spring;
+};
+
+void incidence_lists(int64_t n, const SpringsIn &sp, Incidence &inc, bool with_springs) {
     const int64_t S = sp.n_springs;
-    // other endpoints per mass (CSR), to collect each tile's halo
-    std::vector<int64_t> row(n + 1, 0);
+    inc.row.assign(n + 1, 0);
     for (int64_t s = 0; s < S; ++s) {
         if (sp.left[s] < 0 || sp.right[s] < 0) continue;
-        row[sp.right[s] + 1]++;
-        row[sp.left[s] + 1]++;
+        inc.row[sp.right[s] + 1]++;
+        inc.row[sp.left[s] + 1]++;
     }
-    for (int64_t i = 0; i < n; ++i) row[i + 1] += row[i];
-    std::vector<int32_t> other(row[n]);
-    {
-        std::vector<int64_t> fill(row.begin(), row.end() - 1);
-        for (int64_t s = 0; s < S; ++s) {
-            const int32_t L = sp.left[s], R = sp.right[s];
-            if (L < 0 || R < 0) continue;
-            other[fill[R]++] = L;
-            other[fill[L]++] = R;
+    for (int64_t i = 0; i < n; ++i) inc.row[i + 1] += inc.row[i];
+    inc.other.resize(inc.row[n]);
+    if (with_springs) inc.spring.resize(inc.row[n]);
+    std::vector<int64_t> fill(inc.row.begin(), inc.row.end() - 1);
+    for (int64_t s = 0; s < S; ++s) {
+        const int32_t L = sp.left[s], R = sp.right[s];
+        if (L < 0 || R < 0) continue;
+        const int64_t a = fill[R]++;
+        inc.other[a] = L;
+        const int64_t b = fill[L]++;
+        inc.other[b] = R;
+        if (with_springs) {
+            inc.spring[a] = s;
+            inc.spring[b] = s;
         }
     }
+}
+
+}  // namespace
+
+bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb, int tile_rows) {
+    Incidence inc;
+    incidence_lists(n, sp, inc, false);
+    const std::vector<int64_t> &row = inc.row;
+    const std::vector<int32_t> &other = inc.other;
     tb.tiles.clear();
     tb.halo.clear();
     tb.hmax = 0;
     tb.rmax = 0;
-    std::vector<int32_t> halo, best;
+    std::vector<int32_t> best;
     std::vector<int64_t> stamp(n, -1);
     int64_t mark = 0;
-    auto halo_of = [&](int64_t m0, int64_t m1, std::vector<int32_t> &h) {
-        h.clear();
-        ++mark;
+    for (int64_t m0 = 0; m0 < n;) {
+        // grow the tile a chunk at a time (up to 4, at least one) while its
+        // masses and halo fit tile_rows, counting the halo incrementally: a
+        // joining mass leaves it, the joining masses' outside neighbours
+        // enter it
+        const int64_t mk = ++mark;
+        int64_t cnt = 0, e_prev = m0, m1 = m0;
+        for (int k = 1; k <= kTileMasses / kChunk; ++k) {
+            if (k > 1 && m0 + (int64_t)(k - 1) * kChunk >= n) break;
+            const int64_t e = std::min(n, m0 + (int64_t)k * kChunk);
+            for (int64_t m = e_prev; m < e; ++m)
+                if (stamp[m] == mk) {
+                    stamp[m] = -1;
+                    --cnt;
+                }
+            for (int64_t m = e_prev; m < e; ++m)
+                for (int64_t j = row[m]; j < row[m + 1]; ++j) {
+                    const int32_t o = other[j];
+                    if ((o < m0 || o >= e) && stamp[o] != mk) {
+                        stamp[o] = mk;
+                        ++cnt;
+                    }
+                }
+            if (k > 1 && e - m0 + cnt > tile_rows) break;
+            m1 = e;
+            e_prev = e;
+        }
+        // the chosen range's halo, ascending
+        const int64_t mh = ++mark;
+        best.clear();
         for (int64_t j = row[m0]; j < row[m1]; ++j) {
             const int32_t o = other[j];
-            if ((o < m0 || o >= m1) && stamp[o] != mark) {
-                stamp[o] = mark;
-                h.push_back(o);
+            if ((o < m0 || o >= m1) && stamp[o] != mh) {
+                stamp[o] = mh;
+                best.push_back(o);
             }
         }
-        std::sort(h.begin(), h.end());
-    };
-    for (int64_t m0 = 0; m0 < n;) {
-        // the most chunks (up to 4) whose rows fit tile_rows; at least one
-        int64_t m1 = std::min(n, m0 + kChunk);
-        halo_of(m0, m1, best);
-        for (int k = 2; k <= kTileMasses / kChunk && m0 + (int64_t)(k - 1) * kChunk < n; ++k) {
-            const int64_t e = std::min(n, m0 + (int64_t)k * kChunk);
-            halo_of(m0, e, halo);
-            if (e - m0 + (int64_t)halo.size() > tile_rows) break;
-            m1 = e;
-            best.swap(halo);
-        }
+        std::sort(best.begin(), best.end());
         const int64_t rows = m1 - m0 + (int64_t)best.size();
         if (rows > kMaxTileRows) {
             tb.why = "a chunk and its halo need more than " + std::to_string(kMaxTileRows) + " LDS rows";
@@ -135,15 +174,12 @@ bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb, int tile_rows) {
 
 void build_sell(int64_t n, const SpringsIn &sp, bool per_k, bool per_d, const TileBuild *tb, int64_t g0,
                 SellBuild &out) {
-    const int64_t S = sp.n_springs;
     const int64_t nch = (n + kChunk - 1) / kChunk;
     const int G = tb ? kGroupTiled : kGroupGather;
+    Incidence inc;
+    incidence_lists(n, sp, inc, true);
     out.deg.assign(nch * kChunk, 0);
-    for (int64_t s = 0; s < S; ++s) {
-        if (sp.left[s] < 0 || sp.right[s] < 0) continue;
-        out.deg[sp.left[s]]++;
-        out.deg[sp.right[s]]++;
-    }
+    for (int64_t m = 0; m < n; ++m) out.deg[m] = (int32_t)(inc.row[m + 1] - inc.row[m]);
     out.coff.assign(nch + 1, 0);
     for (int64_t c = 0; c < nch; ++c) {
         int32_t w = 0;
@@ -151,30 +187,6 @@ void build_sell(int64_t n, const SpringsIn &sp, bool per_k, bool per_d, const Ti
         out.coff[c + 1] = out.coff[c] + (int64_t)((w + G - 1) / G) * G * kChunk;
     }
     const int64_t ne = out.coff[nch];
-    // the LDS row of mass `o` in the tile of mass `me`
-    auto row_of = [&](int64_t me, int64_t o) -> uint32_t {
-        const TileDesc &d = tb->tiles[tb->tile_of_chunk[me / kChunk]];
-        const int64_t m0 = (int64_t)d.chunk0 * kChunk;
-        if (o >= m0 && o < m0 + d.nmass) return (uint32_t)(o - m0);
-        const int32_t *h = tb->halo.data() + d.halo_off;
-        return (uint32_t)(d.nmass + (std::lower_bound(h, h + d.nhalo, (int32_t)o) - h));
-    };
-    // entry e of a chunk holds incidence j of lane l: e = ((j / G) * 64 + l) * G + j % G
-    auto entry = [&](int64_t me, int64_t j) -> int64_t {
-        return out.coff[me / kChunk] + ((j / G) * kChunk + me % kChunk) * G + j % G;
-    };
-    // gather: one uint32 arena slot per entry; tiled: the LDS rows of a
-    // group's two entries packed in the uint32 at (entry - coff) / 2
-    auto put_nbr = [&](int64_t me, int64_t e, int64_t o) {
-        if (!tb) {
-            out.nbr[e] = (uint32_t)(g0 + o);
-            return;
-        }
-        const int64_t c0 = out.coff[me / kChunk], rel = e - c0;
-        uint32_t &w = out.nbr[c0 + rel / 2];
-        const int sh = 16 * (int)(rel % 2);
-        w = (w & ~(0xffffu << sh)) | (row_of(me, o) << sh);
-    };
     out.nbr.assign(ne, 0);
     out.rest.assign(ne, 1.0);
     out.k.clear();
@@ -183,27 +195,52 @@ void build_sell(int64_t n, const SpringsIn &sp, bool per_k, bool per_d, const Ti
     if (per_k) out.k.assign(ne, 0.0);
     if (per_d) out.damp.assign(ne, 0.0);
     if (sp.act) out.act.assign(ne, -1);
-    // padding entries: the mass itself (tiled: row 0 for slots past the body)
-    for (int64_t c = 0; c < nch; ++c)
-        for (int64_t e = out.coff[c]; e < out.coff[c + 1]; ++e) {
-            const int64_t me = c * kChunk + ((e - out.coff[c]) / G) % kChunk;
-            if (tb && me >= n) put_nbr(me, e, (int64_t)tb->tiles[tb->tile_of_chunk[c]].chunk0 * kChunk);
-            else put_nbr(me, e, me);
+    // tiled: the LDS row of every mass the current tile names (its masses,
+    // then its halo), set when the walk enters the tile
+    std::vector<uint32_t> rowmap(tb ? n : 0);
+    int32_t cur_tile = -1;
+    for (int64_t c = 0; c < nch; ++c) {
+        const int64_t c0 = out.coff[c];
+        const int64_t W = (out.coff[c + 1] - c0) / kChunk;  // entries per lane
+        int64_t m0 = 0;
+        if (tb) {
+            const int32_t q = tb->tile_of_chunk[c];
+            const TileDesc &d = tb->tiles[q];
+            m0 = (int64_t)d.chunk0 * kChunk;
+            if (q != cur_tile) {
+                cur_tile = q;
+                for (int32_t r = 0; r < d.nmass; ++r) rowmap[m0 + r] = (uint32_t)r;
+                const int32_t *h = tb->halo.data() + d.halo_off;
+                for (int32_t r = 0; r < d.nhalo; ++r) rowmap[h[r]] = (uint32_t)(d.nmass + r);
+            }
         }
-    std::vector<int32_t> fill(nch * kChunk, 0);
-    auto place = [&](int64_t me, int64_t o, int64_t s) {
-        const int64_t e = entry(me, fill[me]++);
-        put_nbr(me, e, o);
-        out.rest[e] = sp.rest[s];
-        if (per_k) out.k[e] = sp.k[s];
-        if (per_d) out.damp[e] = sp.damping ? sp.damping[s] : 0.0;
-        if (sp.act) out.act[e] = sp.act[s];
-    };
-    for (int64_t s = 0; s < S; ++s) {
-        const int32_t L = sp.left[s], R = sp.right[s];
-        if (L < 0 || R < 0) continue;
-        place(R, L, s);  // a self-spring is listed for its right end first
-        place(L, R, s);
+        for (int l = 0; l < kChunk; ++l) {
+            const int64_t me = c * kChunk + l;
+            const int64_t d = me < n ? inc.row[me + 1] - inc.row[me] : 0;
+            const int64_t i0 = me < n ? inc.row[me] : 0;
+            // padding entries name the mass itself (tiled: row 0 past the body)
+            const uint32_t pad = tb ? (uint32_t)(me < n ? me - m0 : 0) : (uint32_t)(g0 + me);
+            // entry of incidence j: c0 + ((j / G) * 64 + l) * G + j % G
+            for (int64_t j = 0; j < W; ++j) {
+                const int64_t e = c0 + ((j / G) * kChunk + l) * G + j % G;
+                uint32_t name = pad;
+                if (j < d) {
+                    const int32_t o = inc.other[i0 + j];
+                    const int64_t s = inc.spring[i0 + j];
+                    name = tb ? rowmap[o] : (uint32_t)(g0 + o);
+                    out.rest[e] = sp.rest[s];
+                    if (per_k) out.k[e] = sp.k[s];
+                    if (per_d) out.damp[e] = sp.damping ? sp.damping[s] : 0.0;
+                    if (sp.act) out.act[e] = sp.act[s];
+                }
+                if (tb) {  // the LDS rows of a group's two entries packed at c0 + (e - c0) / 2
+                    const int64_t rel = e - c0;
+                    out.nbr[c0 + rel / 2] |= name << (16 * (int)(rel % 2));
+                } else {
+                    out.nbr[e] = name;
+                }
+            }
+        }
     }
 }
 
