@@ -158,7 +158,9 @@ def bench_batched(n_bodies: int, seed: int, rollout: int, sample: int, schedule=
     out = {
         "workload": f"{n_bodies} lattice robots, side U{{3..12}} (seed {seed}), {n_m} masses, {n_s} springs, "
                     f"{sc.n_actuated} actuated (a~U(0,0.2), omega~U(1,20), phi~U(0,2pi)), own contact plane "
-                    f"each, dt=1e-4, FP64, {rollout}-step graph-captured rollouts, one launch per step",
+                    f"each, dt=1e-4, FP64, {rollout}-step graph-captured rollouts, one launch per step (the "
+                    f"resident layout, one launch per rollout, is among the autotuner's candidates and is "
+                    f"slower here: one workgroup per body walks its chunks in turn every step)",
         "value": rate_batched, "unit": "mass-steps/s", "ms_per_rollout": ms_batched,
         "schedule": {"tiled_min_masses": best[0], "tile_rows": best[1], "resident_max_masses": best[2],
                      "autotune": [{"tiled_min_masses": c[0], "tile_rows": c[1], "resident_max_masses": c[2],
