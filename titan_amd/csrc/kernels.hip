@@ -586,23 +586,31 @@ struct SpringWalk {
     }
 };
 
-// The walk of the tiled step: three register sets of one group each, so
-// that while one group is evaluated the next two groups' loads are in
+// The walk of the tiled step: D register sets of one group each in a ring,
+// so that while one group is evaluated the next D - 1 groups' loads are in
 // flight, with the body's constant mode KD fixed at compile time (each
 // group issues the same loads, and the waits the compiler places allow the
-// two later groups to stay outstanding).  On the 1M lattice, the 2M kNN
-// cloud and the robot population this was 1%, 4% and 6% faster than two
-// register sets (measured on one box, alternating).
-template <bool ACT, int KD, class Src>
-struct SpringWalk3 {
+// later groups to stay outstanding).  D = 3 on the 1M lattice, the 2M kNN
+// cloud and the robot population was 1%, 4% and 6% faster than two
+// register sets (measured on one box, alternating).  Deeper rings cost
+// registers: D = 4 and 5 take the passive step from 122 to 134 and 148
+// VGPRs (three waves per SIMD instead of four) and were 5-15% slower on all
+// three workloads; D = 4 held to four waves spills and was no faster.
+#ifndef TITAN_WALK_DEPTH
+#define TITAN_WALK_DEPTH 3
+#endif
+template <bool ACT, int KD, class Src, int D = TITAN_WALK_DEPTH>
+struct SpringWalkD {
+    static_assert(D >= 2 && D <= 8, "ring of 2 to 8 register sets");
     static constexpr int G = kGroupTiled;
-    Group<ACT, G, KD> A[1], B[1], C[1];
+    typedef Group<ACT, G, KD> Grp;
+    Grp R[D];
     const Src &src;
     int64_t ebase;
     int ng, lane;
     BodyKD bkd;
 
-    __device__ explicit SpringWalk3(const Src &s) : src(s) {}
+    __device__ explicit SpringWalkD(const Src &s) : src(s) {}
 
     __device__ __forceinline__ void prime(int64_t eb, int n, int ln, const BodyKD &kd) {
         ebase = eb;
@@ -610,33 +618,30 @@ struct SpringWalk3 {
         lane = ln;
         bkd = kd;
         if (ng <= 0) return;
-        A[0].load(src, ebase, lane, 0, bkd);
-        B[0].load(src, ebase, lane, min(1, ng - 1), bkd);
+#pragma unroll
+        for (int j = 0; j < D - 1; ++j) R[j].load(src, ebase, lane, min(j, ng - 1), bkd);
     }
 
     // rc: this step's actuation factors (actuated only).  A refill index
     // past the last group is clamped to it.  The loop has one exit: a group
     // past the last is skipped by a uniform branch rather than a break, so
-    // that the compiler's wait at the loop head leaves the two refills
-    // issued in the previous round in flight (with the breaks it waited for
-    // every load there).
+    // that the compiler's wait at the loop head leaves the refills issued in
+    // the previous round in flight (with breaks it waited for every load
+    // there).
     template <class Fetch>
     __device__ __forceinline__ void run(MassRegs &r, int d, const double *rc, Fetch fetch) {
         r.fx = 0.0;
         r.fy = 0.0;
         r.fz = 0.0;
         const int last = ng - 1;
-        for (int g = 0; g < ng; g += 3) {
-            const bool b_on = g + 1 < ng, c_on = g + 2 < ng;  // uniform over the wave
-            if (ACT) A[0].fetch_f(rc);
-            C[0].load(src, ebase, lane, min(g + 2, last), bkd);
-            eval_set<ACT, G, 1>(r, A, g, d, fetch);
-            if (ACT && b_on) B[0].fetch_f(rc);
-            A[0].load(src, ebase, lane, min(g + 3, last), bkd);
-            if (b_on) eval_set<ACT, G, 1>(r, B, g + 1, d, fetch);
-            if (ACT && c_on) C[0].fetch_f(rc);
-            B[0].load(src, ebase, lane, min(g + 4, last), bkd);
-            if (c_on) eval_set<ACT, G, 1>(r, C, g + 2, d, fetch);
+        for (int g = 0; g < ng; g += D) {
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                const bool on = g + j < ng;  // uniform over the wave
+                if (ACT && on) R[j].fetch_f(rc);
+                R[(j + D - 1) % D].load(src, ebase, lane, min(g + j + D - 1, last), bkd);
+                if (on) eval_set<ACT, G, 1>(r, *reinterpret_cast<const Grp(*)[1]>(&R[j]), g + j, d, fetch);
+            }
         }
     }
 };
@@ -767,7 +772,7 @@ __device__ __forceinline__ void tiles_body(const StepArgs &a, double2 *__restric
     int d = 0;
     double m = 1.0;
     const ArenaSrc src{a};
-    SpringWalk3<ACT, KD, ArenaSrc> w(src);
+    SpringWalkD<ACT, KD, ArenaSrc> w(src);
     if (live) {
         fl = a.mflags[i];
         o0 = cur[3 * i];
