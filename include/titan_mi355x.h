@@ -106,6 +106,15 @@ int titan_ctx_destroy(titan_ctx *ctx);
 /* Message of the last failure on ctx (or of titan_ctx_create if ctx is
  * NULL); never NULL. */
 const char *titan_last_error(const titan_ctx *ctx);
+/* Grow the context's arenas once, now, to hold n_masses masses and
+ * n_springs springs (n_actuated of them actuated) in n_bodies bodies, so that
+ * creating bodies up to that scale grows nothing.  No counterpart in the
+ * reference, whose every mass and spring is an allocation of its own: this
+ * is for servers, which reserve their HBM at startup rather than grow it
+ * under load (a growth waits for the steps in flight).  A capacity hint:
+ * bodies beyond it still grow the arenas on demand.  Counts toward the HBM
+ * budget (TITAN_E_NOMEM if it would exceed it; nothing is grown then). */
+int titan_reserve(titan_ctx *ctx, int64_t n_masses, int64_t n_springs, int64_t n_actuated, int32_t n_bodies);
 
 /* ---- globals (src/sim.cu:797-807,2081-2087,1998-2040) ---- */
 int titan_set_dt(titan_ctx *ctx, double dt);                       /* setTimeStep */
@@ -280,6 +289,7 @@ typedef struct titan_serve_report {
     double p50_latency_s, p99_latency_s, max_latency_s;  /* over completed requests */
     double mass_steps;          /* of completed requests */
     double mass_steps_per_s;    /* mass_steps / (last completion - first submit) */
+    double device_idle_s;       /* step stream idle between consecutive quanta (device clock) */
 } titan_serve_report;
 
 /* max_running: requests on the device at once (>= 1); hbm_bytes: serving

@@ -93,3 +93,28 @@ def test_step_bodies_rejects_bad_input():
             ctx.step_bodies([99], [1])
         ctx.step_bodies([ids[0]], [2])
         _check(ctx, ids[0], sc, 0, 2)
+
+
+@pytest.mark.parametrize("layout", [LAYOUT_TILED, LAYOUT_GATHER])
+def test_reserve_then_create_grows_nothing(layout):
+    # titan_reserve grows the arenas once; bodies within the reservation are
+    # then created without allocating (the device bytes stay put) and step
+    # exactly as in a context that grew on demand
+    sc = _bodies()
+    masses = sum(len(b["pos"]) for b in sc.bodies)
+    springs = sum(len(b["left"]) for b in sc.bodies)
+    print(f"case reserve {masses} masses {springs} springs layout {layout}", flush=True)
+    with Context() as ctx:
+        sc.load_globals(ctx)
+        ctx.reserve(masses, springs, 0, len(sc.bodies))
+        before = ctx.mem_used()
+        ids = [ctx.add_body(**b, layout=layout) for b in sc.bodies]
+        assert ctx.mem_used() == before
+        ctx.step(70)
+        for i, bid in enumerate(ids):
+            _check(ctx, bid, sc, i, 70)
+    with Context() as ctx:
+        with pytest.raises(Exception):
+            ctx.reserve(-1, 0, 0, 1)
+        with pytest.raises(Exception):
+            ctx.reserve(10, 5, 6, 1)  # more actuated springs than springs

@@ -38,7 +38,7 @@ EXPORTS = (
     "titan_serve_config", "titan_serve_submit", "titan_serve_pump", "titan_serve_poll",
     "titan_serve_cancel", "titan_serve_stats", "titan_serve_reset_stats", "titan_set_integrator",
     "titan_get_integrator", "titan_debug_inject_failure_at", "titan_step_noalloc",
-    "titan_measure_hbm", "titan_body_remove", "titan_debug_body_bytes",
+    "titan_measure_hbm", "titan_body_remove", "titan_debug_body_bytes", "titan_reserve",
 )
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED = range(5)
@@ -121,6 +121,7 @@ def lib() -> ctypes.CDLL:
         "titan_step_bodies": (c_int, [vp, vp, vp, c_int32]),
         "titan_body_time": (c_int, [vp, c_int32, POINTER(c_double), POINTER(c_int64)]),
         "titan_set_schedule": (c_int, [vp, vp]),
+        "titan_reserve": (c_int, [vp, c_int64, c_int64, c_int64, c_int32]),
         "titan_set_integrator": (c_int, [vp, c_int32]),
         "titan_get_integrator": (c_int, [vp, POINTER(c_int32)]),
         "titan_get_schedule": (c_int, [vp, vp]),
@@ -229,7 +230,7 @@ class ServeReport(ctypes.Structure):
     _fields_ = [("submitted", c_int64), ("completed", c_int64), ("cancelled", c_int64), ("failed", c_int64),
                 ("running", c_int64), ("queued", c_int64), ("p50_latency_s", c_double),
                 ("p99_latency_s", c_double), ("max_latency_s", c_double), ("mass_steps", c_double),
-                ("mass_steps_per_s", c_double)]
+                ("mass_steps_per_s", c_double), ("device_idle_s", c_double)]
 
 
 class Schedule(ctypes.Structure):
@@ -476,6 +477,11 @@ class Context:
         v = c_float()
         self._chk(self._L.titan_timer_stop(self._h, ctypes.byref(v)))
         return v.value
+
+    def reserve(self, masses: int, springs: int, actuated: int = 0, bodies: int = 1):
+        """Grow the arenas once to hold `masses` masses and `springs` springs
+        (`actuated` of them actuated) in `bodies` bodies (titan_reserve)."""
+        self._chk(self._L.titan_reserve(self._h, int(masses), int(springs), int(actuated), int(bodies)))
 
     def mem_used(self) -> int:
         v = c_uint64()

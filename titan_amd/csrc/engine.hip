@@ -2336,6 +2336,34 @@ int titan_time(const titan_ctx *c, double *T) {
     return TITAN_OK;
 }
 
+int titan_reserve(titan_ctx *c, int64_t n_masses, int64_t n_springs, int64_t n_actuated, int32_t n_bodies) {
+    NEED_CTX(c);
+    API_BEGIN
+    if (n_masses < 0 || n_springs < 0 || n_actuated < 0 || n_actuated > n_springs || n_bodies < 0)
+        fail(TITAN_E_INVALID, "reserve: negative count, or more actuated springs than springs");
+    if (n_masses == 0 && n_bodies == 0) return TITAN_OK;
+    const int64_t nb = std::max<int64_t>(n_bodies, 1);
+    // every body starts on a chunk boundary; a lane's incidence list is
+    // padded to its chunk's group count (half as many entries again, plus
+    // eight per mass slot, covers the padding of every layout on lattices
+    // and kNN clouds) and a tile's halo lists up to about two outside masses
+    // per tile mass
+    const int64_t chunks = (n_masses + kChunk - 1) / kChunk + nb;
+    const int64_t incs = 3 * n_springs + chunks * kChunk * 8;
+    const int64_t tiles = (chunks * kChunk + kTileMasses - 1) / kTileMasses + nb;
+    // the whole reservation must fit the budget before anything grows
+    check_growth(c, (size_t)chunks * kChunk * 180 + (size_t)incs * (4 + 8) + (size_t)n_actuated * 48 +
+                        (size_t)tiles * sizeof(TileDesc) + (size_t)n_masses * 8 + (size_t)nb * 128);
+    ensure_chunks(c, chunks);
+    ensure_incs(c, incs);
+    if (n_actuated > 0) ensure_acts(c, n_actuated);
+    ensure_tiles(c, tiles);
+    ensure_halos(c, 2 * n_masses);
+    ensure_rdescs(c, nb);
+    ensure_bodies(c, nb);
+    API_END(c)
+}
+
 int titan_set_schedule(titan_ctx *c, const titan_schedule *s) {
     NEED_CTX(c);
     API_BEGIN

@@ -165,11 +165,23 @@ struct Server {
     std::vector<Request *> running;
     std::deque<Finishing> finishing;
     std::vector<hipEvent_t> spare;  // events for reuse
+    // events recorded after the two latest quanta (q_prev, then q_last): a
+    // pump enqueues its quantum only once q_prev has passed, so that at
+    // most two quanta are in the step stream and the next is enqueued
+    // while the last one runs
+    hipEvent_t q_prev = nullptr, q_last = nullptr;
     uint64_t running_bytes = 0;
     // statistics
     int64_t submitted = 0, completed = 0, cancelled = 0, failed = 0;
     std::vector<double> latencies;
     double mass_steps = 0, first_submit = -1, last_done = -1;
+    // device idle time between quanta: timed events recorded on the step
+    // stream after each quantum (last_end) and before the next one; a pair
+    // (previous end, next begin) is read once the begin has passed
+    std::vector<hipEvent_t> tspare;
+    hipEvent_t last_end = nullptr;
+    std::deque<std::pair<hipEvent_t, hipEvent_t>> gaps;
+    double idle_s = 0;
 
     void worker() {
         for (;;) {
@@ -253,8 +265,43 @@ struct Server {
     void release_events() {
         for (auto &f : finishing) (void)hipEventDestroy(f.ev);
         for (hipEvent_t e : spare) (void)hipEventDestroy(e);
+        for (auto &g : gaps) {
+            (void)hipEventDestroy(g.first);
+            (void)hipEventDestroy(g.second);
+        }
+        if (last_end) (void)hipEventDestroy(last_end);
+        if (q_prev) (void)hipEventDestroy(q_prev);
+        if (q_last) (void)hipEventDestroy(q_last);
+        q_prev = q_last = nullptr;
+        for (hipEvent_t e : tspare) (void)hipEventDestroy(e);
         finishing.clear();
         spare.clear();
+        gaps.clear();
+        last_end = nullptr;
+        tspare.clear();
+    }
+
+    // A timed event from the spare list, or a new one.
+    int take_timed(hipEvent_t *ev) {
+        if (!tspare.empty()) {
+            *ev = tspare.back();
+            tspare.pop_back();
+            return TITAN_OK;
+        }
+        return hipEventCreate(ev) == hipSuccess ? TITAN_OK : TITAN_E_HIP;
+    }
+
+    // Add the gaps whose second event the device has passed (in order; never
+    // waits).
+    void read_gaps() {
+        while (!gaps.empty() && hipEventQuery(gaps.front().second) == hipSuccess) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, gaps.front().first, gaps.front().second) == hipSuccess && ms > 0.0f)
+                idle_s += 1e-3 * ms;
+            tspare.push_back(gaps.front().first);
+            tspare.push_back(gaps.front().second);
+            gaps.pop_front();
+        }
     }
 };
 
@@ -478,6 +525,13 @@ static int serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
         if (rc != TITAN_OK) return rc;
         sv.running.swap(fresh);  // fresh is now the empty old running set
     }
+    // at most two quanta in the step stream: wait for the one before the
+    // last (its successor keeps the device busy meanwhile)
+    if (!sv.running.empty() && sv.q_prev) {
+        if (hipEventSynchronize(sv.q_prev) != hipSuccess) return TITAN_E_HIP;
+        sv.spare.push_back(sv.q_prev);
+        sv.q_prev = nullptr;
+    }
     // step every running request together, each up to the quantum and its
     // horizon; the call returns once the launches are enqueued
     if (!sv.running.empty()) {
@@ -487,23 +541,44 @@ static int serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
             ids.push_back(r->body);
             steps.push_back(std::min(sv.quantum, r->horizon - r->done));
         }
+        sv.read_gaps();
+        hipEvent_t tb = nullptr, te = nullptr;  // the step stream's idle time since the last quantum
+        if (sv.last_end && sv.take_timed(&tb) == TITAN_OK) {
+            if (titan_mi355x::record_step_event(ctx, tb) == TITAN_OK) {
+                sv.gaps.push_back({sv.last_end, tb});
+                sv.last_end = nullptr;
+            } else {
+                sv.tspare.push_back(tb);
+            }
+        }
         rc = titan_step_bodies(ctx, ids.data(), steps.data(), (int32_t)ids.size());
         if (rc != TITAN_OK) return rc;
         for (size_t q = 0; q < ids.size(); ++q) sv.running[q]->done += steps[q];
+        if (sv.take_timed(&te) == TITAN_OK) {
+            if (titan_mi355x::record_step_event(ctx, te) == TITAN_OK) {
+                if (sv.last_end) sv.tspare.push_back(sv.last_end);
+                sv.last_end = te;
+            } else {
+                sv.tspare.push_back(te);
+            }
+        }
     }
     // admit more while the device works (their layouts are built on the host
-    // meanwhile), until it has run the steps just enqueued
-    hipEvent_t stepped = nullptr;
+    // meanwhile), until it has run the quantum before the one just enqueued,
+    // so that the next pump enqueues its quantum while this one runs
     if (!sv.running.empty()) {
+        hipEvent_t stepped = nullptr;
         rc = take_event(sv, &stepped);
         if (rc == TITAN_OK) rc = titan_mi355x::record_step_event(ctx, stepped);
         if (rc != TITAN_OK) {
             if (stepped) sv.spare.push_back(stepped);
             return rc;
         }
+        if (sv.q_prev) sv.spare.push_back(sv.q_prev);
+        sv.q_prev = sv.q_last;
+        sv.q_last = stepped;
     }
-    rc = admit(SIZE_MAX, stepped);
-    if (stepped) sv.spare.push_back(stepped);
+    rc = admit(SIZE_MAX, sv.running.empty() ? nullptr : (sv.q_prev ? sv.q_prev : sv.q_last));
     if (rc != TITAN_OK) return rc;
     // retire the finished ones (their downloads wait for the quantum)
     std::vector<Request *> still;
@@ -592,6 +667,8 @@ static int serve_stats(titan_ctx *ctx, titan_serve_report *out) {
     }
     if (sv.last_done > sv.first_submit && sv.first_submit >= 0)
         out->mass_steps_per_s = sv.mass_steps / (sv.last_done - sv.first_submit);
+    sv.read_gaps();
+    out->device_idle_s = sv.idle_s;
     return TITAN_OK;
 }
 
@@ -609,6 +686,8 @@ static int serve_reset_stats(titan_ctx *ctx) {
     sv.mass_steps = 0;
     sv.first_submit = sv.reqs.empty() ? -1 : now_s();
     sv.last_done = -1;
+    sv.read_gaps();
+    sv.idle_s = 0;
     return TITAN_OK;
 }
 
