@@ -11,7 +11,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from titan_amd import LAYOUT_GATHER, LAYOUT_RESIDENT, LAYOUT_TILED, Context, scenes
+from titan_amd import LAYOUT_GATHER, LAYOUT_RESIDENT, LAYOUT_TILED, TITAN_E_NOMEM, Context, TitanError, scenes
 from helpers import to_oracle
 
 pytestmark = pytest.mark.gpu
@@ -118,3 +118,12 @@ def test_reserve_then_create_grows_nothing(layout):
             ctx.reserve(-1, 0, 0, 1)
         with pytest.raises(Exception):
             ctx.reserve(10, 5, 6, 1)  # more actuated springs than springs
+    # beyond the HBM budget: refused, and nothing grows
+    with Context(hbm_budget=64 << 20) as ctx:
+        before = ctx.mem_used()
+        with pytest.raises(TitanError) as err:
+            ctx.reserve(4_000_000, 30_000_000, 1_000_000, 100)
+        assert err.value.code == TITAN_E_NOMEM
+        assert ctx.mem_used() == before
+        ctx.reserve(masses, springs, 0, len(sc.bodies))  # a reservation that fits
+        ids = [ctx.add_body(**b, layout=layout) for b in sc.bodies]

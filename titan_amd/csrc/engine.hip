@@ -330,18 +330,29 @@ void drop_graphs(titan_ctx *c) {
     c->version++;
 }
 
+// The capacity an arena of capacity `old` grows to when it must hold `need`
+// entries (at least doubling, at least `floor`), and the device bytes of the
+// arrays each arena entry carries: what the growth checks count.
+int64_t grown_cap(int64_t old, int64_t need, int64_t floor) {
+    return old >= need ? old : std::max<int64_t>(need, std::max<int64_t>(2 * old, floor));
+}
+int64_t max_chunk_cap() { return kMaxMassSlots / kChunk; }  // the kernels' 32-bit state byte offsets
+size_t chunk_bytes(const titan_ctx *c) {  // a chunk: 64 mass slots and the chunk table
+    return (size_t)kChunk * (96 + 8 + 8 + 24 + 4 + 4 + (c->ext ? 24 : 0) + (c->drag ? 4 : 0) + (c->mlc ? 8 : 0) +
+                             (c->integ == kIntRK2 ? 48 : 0)) +
+           (4 + 8 + 4 + 8 + 4);
+}
+size_t inc_bytes(const titan_ctx *c) { return 4 + 8 + (c->inc_k ? 8 : 0) + (c->inc_damp ? 8 : 0); }
+size_t act_bytes(const titan_ctx *c) { return 4 * 8 + (c->ract[0] ? 2 * 8 : 0); }
+size_t body_bytes() { return 2 * (8 + 8 + 8) + 8 + 4 + 4 + 1 + 4 + 8 + 8 + 1; }
+
 void ensure_chunks(titan_ctx *c, int64_t need) {
     if (c->chunks.cap() >= need) return;
     const int64_t old = c->chunks.cap();
-    int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 16));
-    // the kernels address a state buffer with a 32-bit byte offset
-    const int64_t max_chunks = kMaxMassSlots / kChunk;
-    if (need > max_chunks) fail(TITAN_E_NOMEM, "mass arena limit: at most 2^26 mass slots");
-    nc = std::min(nc, max_chunks);
+    if (need > max_chunk_cap()) fail(TITAN_E_NOMEM, "mass arena limit: at most 2^26 mass slots");
+    const int64_t nc = std::min(grown_cap(old, need, 16), max_chunk_cap());
     const int64_t on = old * kChunk, nn = nc * kChunk;
-    check_growth(c, (size_t)nn * (96 + 8 + 8 + 24 + 4 + 4 + (c->ext ? 24 : 0) + (c->drag ? 4 : 0) +
-                                  (c->mlc ? 8 : 0) + (c->integ == kIntRK2 ? 48 : 0)) +
-                        (size_t)nc * (4 + 8 + 4 + 8 + 4));
+    check_growth(c, (size_t)nc * chunk_bytes(c));
     grow_soa(c, c->st, 2, 6 * on, 6 * nn);
     if (c->integ == kIntRK2) grow_soa(c, c->rk, 1, 6 * on, 6 * nn);
     grow_soa(c, c->m, 1, on, nn);
@@ -371,8 +382,8 @@ void ensure_chunks(titan_ctx *c, int64_t need) {
 void ensure_incs(titan_ctx *c, int64_t need) {
     if (c->incs.cap() >= need) return;
     const int64_t old = c->incs.cap();
-    const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 4096));
-    check_growth(c, (size_t)nc * (4 + 8 + (c->inc_k ? 8 : 0) + (c->inc_damp ? 8 : 0)));
+    const int64_t nc = grown_cap(old, need, 4096);
+    check_growth(c, (size_t)nc * inc_bytes(c));
     grow_soa(c, c->inc_nbr, 1, old, nc);
     grow_soa(c, c->inc_rest, 1, old, nc);
     if (c->inc_k) grow_soa(c, c->inc_k, 1, old, nc);
@@ -384,7 +395,8 @@ void ensure_incs(titan_ctx *c, int64_t need) {
 void ensure_acts(titan_ctx *c, int64_t need) {
     if (c->acts.cap() >= need) return;
     const int64_t old = c->acts.cap();
-    const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 1024));
+    const int64_t nc = grown_cap(old, need, 1024);
+    check_growth(c, (size_t)nc * act_bytes(c));
     grow_soa(c, c->act, 1, 4 * old, 4 * nc);
     if (c->ract[0]) {
         grow_soa(c, c->ract[0], 1, old, nc);
@@ -406,8 +418,8 @@ void ensure_lcs(titan_ctx *c, int64_t need) {
 void ensure_bodies(titan_ctx *c, int64_t need) {
     if (c->bcap >= need) return;
     const int64_t old = c->bcap;
-    const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 64));
-    check_growth(c, (size_t)nc * (2 * (8 + 8 + 8) + 8 + 4 + 4 + 1 + 4 + 8 + 8 + 1));
+    const int64_t nc = grown_cap(old, need, 64);
+    check_growth(c, (size_t)nc * body_bytes());
     for (int s = 0; s < 2; ++s) {
         grow_soa(c, c->bT[s], 1, old, nc);
         grow_soa(c, c->bn[s], 1, old, nc);
@@ -436,7 +448,8 @@ void ensure_bodies(titan_ctx *c, int64_t need) {
 void ensure_tiles(titan_ctx *c, int64_t need) {
     if (c->tiles.cap() >= need) return;
     const int64_t old = c->tiles.cap();
-    const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 64));
+    const int64_t nc = grown_cap(old, need, 64);
+    check_growth(c, (size_t)nc * sizeof(TileDesc));
     grow_soa(c, c->tile, 1, old, nc);
     // launches cover the whole arena: unused tiles are marked free
     std::vector<TileDesc> dead(nc - old);
@@ -453,7 +466,8 @@ void ensure_tiles(titan_ctx *c, int64_t need) {
 void ensure_rdescs(titan_ctx *c, int64_t need) {
     if (c->rdescs.cap() >= need) return;
     const int64_t old = c->rdescs.cap();
-    const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 64));
+    const int64_t nc = grown_cap(old, need, 64);
+    check_growth(c, (size_t)nc * sizeof(ResidentDesc));
     grow_soa(c, c->rdesc, 1, old, nc);
     std::vector<ResidentDesc> dead(nc - old, ResidentDesc{-1, 0, 0, 0, 0, 0, 0, 0, 0, 0});
     h2d(c, c->rdesc + old, dead.data(), nc - old);
@@ -465,7 +479,8 @@ void ensure_rdescs(titan_ctx *c, int64_t need) {
 void ensure_halos(titan_ctx *c, int64_t need) {
     if (c->halos.cap() >= need) return;
     const int64_t old = c->halos.cap();
-    const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * old, 1024));
+    const int64_t nc = grown_cap(old, need, 1024);
+    check_growth(c, (size_t)nc * sizeof(int32_t));
     grow_soa(c, c->halo, 1, old, nc);
     c->halos.grow(nc);
     drop_graphs(c);
@@ -2351,9 +2366,22 @@ int titan_reserve(titan_ctx *c, int64_t n_masses, int64_t n_springs, int64_t n_a
     const int64_t chunks = (n_masses + kChunk - 1) / kChunk + nb;
     const int64_t incs = 3 * n_springs + chunks * kChunk * 8;
     const int64_t tiles = (chunks * kChunk + kTileMasses - 1) / kTileMasses + nb;
-    // the whole reservation must fit the budget before anything grows
-    check_growth(c, (size_t)chunks * kChunk * 180 + (size_t)incs * (4 + 8) + (size_t)n_actuated * 48 +
-                        (size_t)tiles * sizeof(TileDesc) + (size_t)n_masses * 8 + (size_t)nb * 128);
+    if (chunks > max_chunk_cap()) fail(TITAN_E_NOMEM, "reserve: mass arena limit: at most 2^26 mass slots");
+    // the whole reservation must fit the budget before anything grows: the
+    // bytes of every arena that grows, by the rule each growth applies
+    auto grows = [](int64_t old, int64_t need, int64_t floor, size_t per) -> size_t {
+        const int64_t nc = grown_cap(old, need, floor);
+        return nc > old ? (size_t)nc * per : 0;
+    };
+    const int64_t old_ch = c->chunks.cap();
+    size_t bytes = old_ch >= chunks ? 0 : (size_t)std::min(grown_cap(old_ch, chunks, 16), max_chunk_cap()) * chunk_bytes(c);
+    bytes += grows(c->incs.cap(), incs, 4096, inc_bytes(c));
+    if (n_actuated > 0) bytes += grows(c->acts.cap(), n_actuated, 1024, act_bytes(c));
+    bytes += grows(c->tiles.cap(), tiles, 64, sizeof(TileDesc));
+    bytes += grows(c->halos.cap(), 2 * n_masses, 1024, sizeof(int32_t));
+    bytes += grows(c->rdescs.cap(), nb, 64, sizeof(ResidentDesc));
+    bytes += grows(c->bcap, nb, 64, body_bytes());
+    check_growth(c, bytes);
     ensure_chunks(c, chunks);
     ensure_incs(c, incs);
     if (n_actuated > 0) ensure_acts(c, n_actuated);
