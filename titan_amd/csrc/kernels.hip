@@ -36,6 +36,18 @@ __device__ __forceinline__ int64_t xcd_block(int64_t w, int64_t G) {
     return x * q + (x < r ? x : r) + k;
 }
 
+// The order a step launch walks its tiles or chunks in: forward on even
+// launch slots, backward on odd ones, so that each launch starts on the
+// incidence data and states the previous launch read last, which the
+// device's 256 MB memory-side cache still holds (the tiles of a launch are
+// independent, so the order changes no result).  Measured against walking
+// forward every launch: 1M lattice 86.9 against 91.6 us per step, 2M kNN
+// cloud 176.9 against 187.5 us, 4096 robots 303.4 against 316.6 us.
+__device__ __forceinline__ int64_t step_block(const StepArgs &a) {
+    const int64_t w = xcd_block(blockIdx.x, gridDim.x);
+    return (a.slot & 1) ? (int64_t)gridDim.x - 1 - w : w;
+}
+
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Load base[byte_off / sizeof(T)] with a 32-bit byte offset from a
@@ -684,7 +696,7 @@ __device__ __forceinline__ void store_mass(const StepArgs &a, double2 *__restric
 // current state buffer (global memory).  ACT: some springs are actuated.
 template <bool ACT>
 __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
-    const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t blk = step_block(a);
     const int64_t chunk = a.chunk_base + blk * kWavesPerBlock + uniform(threadIdx.x >> 6);
     if (chunk >= a.nchunks) return;
     const int lane = threadIdx.x & (kChunk - 1);
@@ -839,7 +851,7 @@ __device__ __forceinline__ void tiles_body(const StepArgs &a, double2 *__restric
 template <bool ACT>
 __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     extern __shared__ double2 S[];  // 3 per LDS row
-    const int64_t tile = a.tile_base + xcd_block(blockIdx.x, gridDim.x);
+    const int64_t tile = a.tile_base + step_block(a);
     if (tile >= a.ntiles) return;
     const int t = threadIdx.x;
     // uniform fields first: read before this kernel's first store, they
