@@ -339,10 +339,12 @@ int titan_debug_inject_failure(titan_ctx *ctx, int32_t after);
  * own ranges: TITAN_DEBUG_NAMES (uint32 words of its incidence span),
  * _REST, _K, _DAMPING (double per entry; empty when shared), _DEG, _MFLAGS
  * (per mass slot), _STATE (48 B per slot, buffer 0), _MASS, _TILES
- * (32-byte tile records: body, first chunk, masses, halo size, halo
- * offset, first-tile flag; body zeroed), _HALO (int32, body
- * slots), _CHUNKS (4 int64 per chunk: first entry, groups, first actuator,
- * actuators) and _PERM (int32 per mass: device order -> caller's order). */
+ * (40-byte tile records: body, first chunk, masses, halo size, halo
+ * offset, first-tile flag, actuators, first actuator record; body zeroed),
+ * _HALO (int32, body slots), _CHUNKS (4 int64 per chunk: first entry,
+ * groups, first actuator, actuators), _PERM (int32 per mass: device order
+ * -> caller's order) and _TACT (the tiles' actuator records, 4 doubles
+ * each: a, omega, phi, rest). */
 #define TITAN_DEBUG_NAMES 0
 #define TITAN_DEBUG_REST 1
 #define TITAN_DEBUG_K 2
@@ -355,6 +357,7 @@ int titan_debug_inject_failure(titan_ctx *ctx, int32_t after);
 #define TITAN_DEBUG_HALO 9
 #define TITAN_DEBUG_CHUNKS 10
 #define TITAN_DEBUG_PERM 11
+#define TITAN_DEBUG_TACT 12
 int titan_debug_body_bytes(titan_ctx *ctx, int32_t body_id, int32_t what, void *out, int64_t cap, int64_t *len);
 
 #define TITAN_INJECT_STEP 0

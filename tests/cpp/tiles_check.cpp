@@ -1,7 +1,8 @@
 // Host check of the layout builders (titan_amd/csrc/tiles.cpp): for random
-// graphs and a Morton-ordered lattice, every tile's halo and every mass's
-// incidence list (other endpoint, spring parameters, ascending spring
-// order, padding) in both layouts are compared with brute force.
+// graphs and a Morton-ordered lattice, every tile's halo and actuator list
+// and every mass's incidence list (other endpoint, spring parameters,
+// actuator, ascending spring order, padding) in both layouts are compared
+// with brute force.
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -75,7 +76,17 @@ static void check_sell(int64_t n, const std::vector<int32_t> &L, const std::vect
                 CHECK(decode(m, name(c, e)) == want[m][j].first);
                 CHECK(sb.rest[e] == rest[s]);
                 if (kd) CHECK(sb.k[e] == k[s] && sb.damp[e] == damp[s]);
-                if (!act.empty()) CHECK(sb.act[e] == act[s]);
+                if (!act.empty() && tb && tb->acts) {  // the spring's position in its tile's list
+                    const TileDesc &d = tb->tiles[tb->tile_of_chunk[c]];
+                    if (act[s] < 0) {
+                        CHECK(sb.act[e] == -1);
+                    } else {
+                        CHECK(sb.act[e] >= 0 && sb.act[e] < d.nact);
+                        if (sb.act[e] >= 0 && sb.act[e] < d.nact) CHECK(tb->tact[d.act_off + sb.act[e]] == s);
+                    }
+                } else if (!act.empty()) {
+                    CHECK(sb.act[e] == act[s]);
+                }
             }
         }
     }
@@ -115,26 +126,35 @@ static void check(int64_t n, const std::vector<int32_t> &L, const std::vector<in
         const int64_t m0 = (int64_t)d.chunk0 * kChunk, m1 = m0 + d.nmass;
         CHECK(m0 == next && m1 <= n && d.nmass > 0 && d.nmass <= kTileMasses && d.first == (t == 0));
         CHECK(m1 == n || d.nmass % kChunk == 0);
+        CHECK(d.act_off == (t == 0 ? 0 : tb.tiles[t - 1].act_off + tb.tiles[t - 1].nact));
         next = m1;
         std::set<int32_t> halo;
+        std::vector<int64_t> acts;  // ascending
         for (int64_t s = 0; s < S; ++s) {
             if (L[s] < 0 || R[s] < 0) continue;
             const bool in = (L[s] >= m0 && L[s] < m1) || (R[s] >= m0 && R[s] < m1);
             if (!in) continue;
             for (int32_t e : {L[s], R[s]})
                 if (e < m0 || e >= m1) halo.insert(e);
+            if (sp.act && sp.act[s] >= 0) acts.push_back(s);
         }
         CHECK(d.nhalo == (int32_t)halo.size());
+        CHECK(tb.acts == (sp.act != nullptr));
+        CHECK(d.nact == (int32_t)acts.size());
+        if (d.nact == (int32_t)acts.size())
+            for (int32_t q = 0; q < d.nact; ++q) CHECK(tb.tact[d.act_off + q] == acts[q]);
+        const int64_t r = d.nmass + d.nhalo + tile_act_rows(d.nact);
         // within the row budget, unless the tile is a single chunk
-        CHECK(d.nmass + d.nhalo <= rows || d.nmass <= kChunk);
-        CHECK(d.nmass + d.nhalo <= kMaxTileRows);
+        CHECK(r <= rows || d.nmass <= kChunk);
+        CHECK(r <= kMaxTileRows);
         hmax = std::max<int32_t>(hmax, d.nhalo);
-        rmax = std::max<int32_t>(rmax, d.nmass + d.nhalo);
+        rmax = std::max<int32_t>(rmax, (int32_t)r);
         std::vector<int32_t> hv(halo.begin(), halo.end());
         for (int32_t h = 0; h < d.nhalo; ++h) CHECK(tb.halo[d.halo_off + h] == hv[h]);
         for (int64_t q = m0; q < m1; ++q) CHECK(tb.tile_of_chunk[q / kChunk] == (int32_t)t);
     }
     CHECK(next == n);
+    CHECK((int64_t)tb.tact.size() == (tb.tiles.empty() ? 0 : tb.tiles.back().act_off + tb.tiles.back().nact));
     CHECK(tb.rmax == rmax);
     CHECK(tb.hmax == hmax);
     };

@@ -498,7 +498,8 @@ class Context:
         self._chk(self._L.titan_kernel_name(self._h, buf, 256))
         return buf.value.decode()
 
-    DEBUG_ITEMS = ("names", "rest", "k", "damping", "deg", "mflags", "state", "mass", "tiles", "halo", "chunks", "perm")
+    DEBUG_ITEMS = ("names", "rest", "k", "damping", "deg", "mflags", "state", "mass", "tiles", "halo", "chunks", "perm",
+                   "tact")
 
     def layout_bytes(self, bid: int, item: str) -> bytes:
         """One item of a body's device layout (titan_debug_body_bytes), as raw bytes."""

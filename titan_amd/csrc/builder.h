@@ -45,13 +45,16 @@ struct DevBuildOut {
     double *inc_rest = nullptr, *inc_k = nullptr, *inc_damp = nullptr;
     TileDesc *tiles = nullptr;
     int32_t *halo = nullptr;
-    int64_t ch0 = 0, inc0 = 0, tl0 = 0, hl0 = 0, act0 = 0, n_act = 0;
+    const double *act = nullptr;  // the actuator records, already written at act0
+    double *tact = nullptr;       // the tiles' copies of them go to [ta0, ta0 + n_tact)
+    int64_t ch0 = 0, inc0 = 0, tl0 = 0, hl0 = 0, act0 = 0, n_act = 0, ta0 = 0;
 };
 
 // The sizes a build needs reserved, known after its first phase.
 struct DevBuildSizes {
     int64_t n_inc = 0, n_halo = 0, ntiles = 0, nch = 0;
-    int32_t rmax = 0;       // largest tile rows (masses + halo)
+    int64_t n_tact = 0;     // the tiles' actuators (entries of the tile actuator arena)
+    int32_t rmax = 0;       // largest tile rows (masses + halo + actuator rows)
     bool fits = true;       // false: a tile exceeds tile_rows, or a degree exceeds 65535 (build on the host)
 };
 

@@ -4,16 +4,19 @@
 //   * morton_order: codes from the positions' bounding box, sorted stably
 //     (ties keep the caller's order);
 //   * build_tiles: tiles of four whole chunks (256 masses) whenever every
-//     such tile's masses and halo fit the row budget (rows only grow with a
-//     tile's chunks, so the host's greedy then takes four chunks too); a
-//     halo is the ascending set of outside endpoints of the tile's springs;
+//     such tile's masses, halo and actuator rows fit the row budget (rows
+//     only grow with a tile's chunks, so the host's greedy then takes four
+//     chunks too); a halo is the ascending set of outside endpoints of the
+//     tile's springs, and a tile's actuators the ascending set of actuated
+//     springs with an endpoint in it, each with a copy of its record;
 //   * degree_sort_tiles: masses of a tile by descending degree, stably;
 //   * build_sell: each mass's incidences in ascending spring index (the
 //     right end of a spring first, so a self-spring is listed twice), at
 //     entry coff[c] + ((j / 2) * 64 + lane) * 2 + j % 2, padding naming the
 //     mass itself (row 0 for slots past the body) with rest 1 and zero
 //     constants, and an actuated incidence marked by bit 15 of its row with
-//     its actuator's arena index in the rest slot (mark_actuated).
+//     its spring's position in its tile's actuator list in the rest slot
+//     (mark_actuated).
 #include "builder.h"
 
 #include <hip/hip_runtime.h>
@@ -176,10 +179,48 @@ __global__ __launch_bounds__(kBT) void k_halo_pairs(const int32_t *__restrict__ 
     hk[p + 1] = ((uint64_t)(y / kTileMasses) << 32) | (uint32_t)x;
 }
 
-// each tile's halo range in the sorted unique pairs, its rows, the largest
+// tile actuator pairs (tile << 32 | spring): one per tile an actuated
+// spring has an endpoint in, the slots of absent pairs kNone64 (sorted last)
+constexpr uint64_t kNone64 = ~(uint64_t)0;
+__global__ __launch_bounds__(kBT) void k_act_pairs(const int32_t *__restrict__ fl, const int32_t *__restrict__ fr,
+                                                   const int32_t *__restrict__ act, int64_t S,
+                                                   uint64_t *__restrict__ ak) {
+    const int64_t s = (int64_t)blockIdx.x * kBT + threadIdx.x;
+    if (s >= S) return;
+    const int32_t x = fl[s], y = fr[s];
+    uint64_t p = kNone64, q = kNone64;
+    if (act[s] >= 0 && x >= 0 && y >= 0) {
+        const uint64_t tx = (uint64_t)(x / kTileMasses), ty = (uint64_t)(y / kTileMasses);
+        p = (tx << 32) | (uint64_t)s;
+        if (ty != tx) q = (ty << 32) | (uint64_t)s;
+    }
+    ak[2 * s] = p;
+    ak[2 * s + 1] = q;
+}
+
+// the first of n sorted keys that is at least key
+__device__ __forceinline__ int64_t lower_bound64(const uint64_t *__restrict__ v, int64_t n, uint64_t key) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) / 2;
+        if (v[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// each tile's first actuator pair (ntiles + 1 offsets: the last is the count)
+__global__ __launch_bounds__(kBT) void k_tile_acts(const uint64_t *__restrict__ aks, int64_t n, int64_t ntiles,
+                                                   int64_t *__restrict__ aoff) {
+    const int64_t t = (int64_t)blockIdx.x * kBT + threadIdx.x;
+    if (t <= ntiles) aoff[t] = lower_bound64(aks, n, (uint64_t)t << 32);
+}
+
+// each tile's halo range in the sorted unique pairs, its rows (with its
+// actuator rows: aoff, or null when the body has none), the largest
 __global__ __launch_bounds__(kBT) void k_tile_halos(const uint64_t *__restrict__ hu, const uint64_t *__restrict__ nu,
-                                                    int64_t ntiles, int64_t N, int64_t *__restrict__ hoff,
-                                                    int32_t *__restrict__ rmax) {
+                                                    const int64_t *__restrict__ aoff, int64_t ntiles, int64_t N,
+                                                    int64_t *__restrict__ hoff, int32_t *__restrict__ rmax) {
     const int64_t t = (int64_t)blockIdx.x * kBT + threadIdx.x;
     if (t > ntiles) return;
     const int64_t n = (int64_t)*nu;
@@ -200,7 +241,8 @@ __global__ __launch_bounds__(kBT) void k_tile_halos(const uint64_t *__restrict__
             else hi2 = mid;
         }
         const int64_t nm = min((int64_t)kTileMasses, N - t * kTileMasses);
-        atomicMax(rmax, (int32_t)(nm + (lo2 - lo)));
+        const int64_t na = aoff ? aoff[t + 1] - aoff[t] : 0;
+        atomicMax(rmax, (int32_t)(nm + (lo2 - lo) + tile_act_rows(na)));
     }
 }
 
@@ -272,8 +314,8 @@ __global__ __launch_bounds__(kBT) void k_items(const int32_t *__restrict__ fl, c
 struct PlaceArgs {
     const uint32_t *key, *val;
     const int32_t *fl, *fr;
-    const int64_t *start, *coff, *hoff;
-    const uint64_t *hu;
+    const int64_t *start, *coff, *hoff, *aoff;
+    const uint64_t *hu, *aks;
     const double *rest, *k, *damp;
     const int32_t *act;
     int64_t nvalid, N;
@@ -310,8 +352,9 @@ __global__ __launch_bounds__(kBT) void k_place(PlaceArgs p, DevBuildOut o) {
     const int32_t a = p.act ? p.act[s] : -1;
     if (a >= 0) row |= kActBit16;
     reinterpret_cast<uint16_t *>(o.inc_nbr)[2 * (o.inc0 + p.coff[c]) + rel] = (uint16_t)row;
-    if (a >= 0) {
-        const int64_t ix = o.act0 + a;
+    if (a >= 0) {  // the spring's position in the tile's actuator list
+        const int64_t a0 = p.aoff[t];
+        const int64_t ix = lower_bound64(p.aks + a0, p.aoff[t + 1] - a0, ((uint64_t)t << 32) | (uint64_t)s);
         o.inc_rest[o.inc0 + e] = __longlong_as_double(ix);
     } else {
         o.inc_rest[o.inc0 + e] = p.rest[s];
@@ -371,14 +414,23 @@ __global__ __launch_bounds__(kBT) void k_chunks(const int64_t *__restrict__ coff
     if (c >= nch) return;
     o.chunk_ptr[o.ch0 + c] = o.inc0 + coff[c];
     o.chunk_ng[o.ch0 + c] = (int32_t)((coff[c + 1] - coff[c]) / (kGroupTiled * kChunk));
-    // each chunk an equal share of the body's actuators
-    const int64_t per = (o.n_act + nch - 1) / nch;
-    o.chunk_act0[o.ch0 + c] = o.act0 + c * per;
-    o.chunk_nact[o.ch0 + c] = (int32_t)max((int64_t)0, min(per, o.n_act - c * per));
+    // a tiled body's tiles compute their actuators: no chunk share
+    o.chunk_act0[o.ch0 + c] = o.act0;
+    o.chunk_nact[o.ch0 + c] = 0;
+}
+
+// the tiles' copies of their actuators' records
+__global__ __launch_bounds__(kBT) void k_tact(const uint64_t *__restrict__ aks, const int32_t *__restrict__ act, int64_t n,
+                                              DevBuildOut o) {
+    const int64_t x = (int64_t)blockIdx.x * kBT + threadIdx.x;
+    if (x >= n) return;
+    const int64_t a = act[(uint32_t)aks[x]];
+    for (int k = 0; k < 4; ++k) o.tact[4 * (o.ta0 + x) + k] = o.act[4 * (o.act0 + a) + k];
 }
 
 __global__ __launch_bounds__(kBT) void k_tiles(const int64_t *__restrict__ hoff, const uint64_t *__restrict__ hu,
-                                               int64_t ntiles, int64_t N, DevBuildOut o) {
+                                               const int64_t *__restrict__ aoff, int64_t ntiles, int64_t N,
+                                               DevBuildOut o) {
     const int64_t x = (int64_t)blockIdx.x * kBT + threadIdx.x;
     const int64_t nh = hoff[ntiles];
     if (x < ntiles) {
@@ -389,7 +441,8 @@ __global__ __launch_bounds__(kBT) void k_tiles(const int64_t *__restrict__ hoff,
         d.nhalo = (int32_t)(hoff[x + 1] - hoff[x]);
         d.halo_off = o.hl0 + hoff[x];
         d.first = x == 0 ? 1 : 0;
-        d.pad = 0;
+        d.nact = aoff ? (int32_t)(aoff[x + 1] - aoff[x]) : 0;
+        d.act_off = o.ta0 + (aoff ? aoff[x] : 0);
         o.tiles[o.tl0 + x] = d;
     }
     if (x < nh) o.halo[o.hl0 + x] = (int32_t)(o.ch0 * kChunk + (int64_t)(uint32_t)hu[x]);
@@ -404,10 +457,10 @@ struct DevBuilder::Bufs {
     int32_t *drag, *L, *R, *act;
     // orders
     double *part, *box;
-    uint64_t *code, *code2, *key2, *key2s, *hk, *hks, *hu, *nu;
+    uint64_t *code, *code2, *key2, *key2s, *hk, *hks, *hu, *nu, *ak, *aks;
     uint32_t *idx, *perm_m, *val2, *order, *cross, *at, *ik, *iks, *iv, *ivs;
     int32_t *inv_m, *deg_m, *perm, *inv, *deg, *fl, *fr, *maxdeg, *rmax;
-    int64_t *start, *hoff, *ent, *coff;
+    int64_t *start, *hoff, *aoff, *ent, *coff;
     uint32_t *flags;
     void *tmp;
     size_t tmp_bytes;
@@ -510,6 +563,11 @@ hipError_t DevBuilder::plan(const DevBuildIn &in, hipStream_t s, DevBuildSizes *
     add(b.hu, 2 * S);
     add(b.nu, 1);
     add(b.hoff, ntiles + 1);
+    if (in.act_local) {
+        add(b.ak, 2 * S);
+        add(b.aks, 2 * S);
+        add(b.aoff, ntiles + 1);
+    }
     add(b.rmax, 1);
     add(b.ent, nch + 1);
     add(b.coff, nch + 1);
@@ -608,8 +666,18 @@ hipError_t DevBuilder::plan(const DevBuildIn &in, hipStream_t s, DevBuildSizes *
         tmpb = b.tmp_bytes;
         BCHK(rocprim::unique(b.tmp, tmpb, b.hks, b.hu, b.nu, (size_t)(2 * ncross), rocprim::equal_to<uint64_t>(), s));
     }
+    // each tile's actuators: the sorted tile actuator pairs
+    if (b.act && S > 0) {
+        hipLaunchKernelGGL(k_act_pairs, dim3(blocks_for(S)), dim3(kBT), 0, s, b.fl, b.fr, b.act, S, b.ak);
+        tmpb = b.tmp_bytes;
+        BCHK(rocprim::radix_sort_keys(b.tmp, tmpb, b.ak, b.aks, (size_t)(2 * S), 0, 64, s));
+        hipLaunchKernelGGL(k_tile_acts, dim3(blocks_for(ntiles + 1)), dim3(kBT), 0, s, b.aks, 2 * S, ntiles, b.aoff);
+    } else if (b.act) {
+        BCHK(hipMemsetAsync(b.aoff, 0, sizeof(int64_t) * (ntiles + 1), s));
+    }
     BCHK(hipMemsetAsync(b.rmax, 0, sizeof(int32_t), s));
-    hipLaunchKernelGGL(k_tile_halos, dim3(blocks_for(ntiles + 1)), dim3(kBT), 0, s, b.hu, b.nu, ntiles, N, b.hoff, b.rmax);
+    hipLaunchKernelGGL(k_tile_halos, dim3(blocks_for(ntiles + 1)), dim3(kBT), 0, s, b.hu, b.nu, b.aoff, ntiles, N,
+                       b.hoff, b.rmax);
     // entries per chunk and their offsets
     hipLaunchKernelGGL(k_chunk_entries, dim3(blocks_for(nch)), dim3(kBT), 0, s, b.deg, N, nch, b.ent);
     tmpb = b.tmp_bytes;
@@ -618,15 +686,17 @@ hipError_t DevBuilder::plan(const DevBuildIn &in, hipStream_t s, DevBuildSizes *
     tmpb = b.tmp_bytes;
     BCHK(rocprim::exclusive_scan(b.tmp, tmpb, b.deg, b.start, (int64_t)0, (size_t)(N + 1), rocprim::plus<int64_t>(), s));
     BCHK(hipGetLastError());
-    int64_t nhalo = 0, ninc = 0;
+    int64_t nhalo = 0, ninc = 0, ntact = 0;
     int32_t rmax = 0, maxdeg = 0;
     BCHK(hipMemcpyAsync(&nhalo, b.nu, 8, hipMemcpyDeviceToHost, s));
+    if (b.aoff) BCHK(hipMemcpyAsync(&ntact, b.aoff + ntiles, 8, hipMemcpyDeviceToHost, s));
     BCHK(hipMemcpyAsync(&ninc, b.coff + nch, 8, hipMemcpyDeviceToHost, s));
     BCHK(hipMemcpyAsync(&rmax, b.rmax, 4, hipMemcpyDeviceToHost, s));
     BCHK(hipMemcpyAsync(&maxdeg, b.maxdeg, 4, hipMemcpyDeviceToHost, s));
     BCHK(hipStreamSynchronize(s));
     sz_.n_halo = nhalo;
     sz_.n_inc = ninc;
+    sz_.n_tact = ntact;
     sz_.rmax = rmax;
     sz_.fits = rmax <= in.tile_rows && maxdeg <= 65535;
     *sizes = sz_;
@@ -648,13 +718,17 @@ hipError_t DevBuilder::write(const DevBuildOut &o, const DevBuildMirror &mr, hip
         BCHK(rocprim::radix_sort_pairs(b.tmp, tmpb, b.ik, b.iks, b.iv, b.ivs, (size_t)(2 * S), 0, 32, s));
         BCHK(hipMemcpyAsync(&nvalid, b.start + N, 8, hipMemcpyDeviceToHost, s));
         BCHK(hipStreamSynchronize(s));
-        PlaceArgs p{b.iks, b.ivs, b.fl, b.fr, b.start, b.coff, b.hoff, b.hu, b.rest, b.k, b.damp, b.act, nvalid, N, pk, pd};
+        PlaceArgs p{b.iks,  b.ivs, b.fl,   b.fr,  b.start, b.coff,  b.hoff, b.aoff, b.hu,
+                    b.aks,  b.rest, b.k,   b.damp, b.act, nvalid, N,      pk,     pd};
         if (nvalid > 0) hipLaunchKernelGGL(k_place, dim3(blocks_for(nvalid)), dim3(kBT), 0, s, p, o);
     }
     MassArgs ma{b.perm, b.deg, b.pos, b.vel, b.m, b.T, b.ext, b.fixed, b.drag, N, npad};
     hipLaunchKernelGGL(k_masses, dim3(blocks_for(npad)), dim3(kBT), 0, s, ma, o, b.flags);
     hipLaunchKernelGGL(k_chunks, dim3(blocks_for(nch)), dim3(kBT), 0, s, b.coff, nch, o);
-    hipLaunchKernelGGL(k_tiles, dim3(blocks_for(std::max(ntiles, sz_.n_halo))), dim3(kBT), 0, s, b.hoff, b.hu, ntiles, N, o);
+    hipLaunchKernelGGL(k_tiles, dim3(blocks_for(std::max(ntiles, sz_.n_halo))), dim3(kBT), 0, s, b.hoff, b.hu, b.aoff,
+                       ntiles, N, o);
+    if (sz_.n_tact > 0)
+        hipLaunchKernelGGL(k_tact, dim3(blocks_for(sz_.n_tact)), dim3(kBT), 0, s, b.aks, b.act, sz_.n_tact, o);
     BCHK(hipGetLastError());
     // host mirror
     BCHK(hipMemcpyAsync(mr.perm, b.perm, 4 * (size_t)N, hipMemcpyDeviceToHost, s));

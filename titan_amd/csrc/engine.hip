@@ -109,7 +109,7 @@ struct BodyH {
     int32_t res_scache = 0;  // LDS bytes of its split-form region (0: none)
     int64_t n_masses = 0, chunk0 = 0, nchunks = 0;
     int64_t inc0 = 0, inc_len = 0, act0 = 0, act_len = 0, lc0 = 0, lc_len = 0;
-    int64_t tile0 = 0, ntiles = 0, halo0 = 0, nhalo = 0;
+    int64_t tile0 = 0, ntiles = 0, halo0 = 0, nhalo = 0, tact0 = 0, tact_len = 0;
     int32_t rows = 0;  // largest LDS row count of a tile (tiled layout)
     double T = 0;   // host mirror of the body clock
     int64_t n = 0;  // steps taken (parity selects the state buffer)
@@ -148,7 +148,7 @@ struct titan_ctx {
     double T = 0.0;                      // Simulation::T
     std::vector<double> planes_h, balls_h;
 
-    Ranges chunks, incs, acts, lcs, bids, tiles, halos, rdescs;
+    Ranges chunks, incs, acts, lcs, bids, tiles, halos, tacts, rdescs;
     std::vector<BodyH> bodies;
     // bodies of at most four chunks resident (alone or among few, one launch
     // per stepping call with the split form), every larger body tiled
@@ -174,6 +174,7 @@ struct titan_ctx {
     // tiled layout arenas
     TileDesc *tile = nullptr;
     int32_t *halo = nullptr;
+    double *tact = nullptr;  // 4 per tile actuator: a copy of its actuator's record
     // resident layout: one descriptor per body
     ResidentDesc *rdesc = nullptr;
     uint8_t *b_mode = nullptr;  // per body: kModeGather, kModeTiled, kModeResident
@@ -493,6 +494,16 @@ void ensure_halos(titan_ctx *c, int64_t need) {
     drop_graphs(c);
 }
 
+void ensure_tacts(titan_ctx *c, int64_t need) {
+    if (c->tacts.cap() >= need) return;
+    const int64_t old = c->tacts.cap();
+    const int64_t nc = grown_cap(old, need, 1024);
+    check_growth(c, (size_t)nc * 4 * sizeof(double));
+    grow_soa(c, c->tact, 1, 4 * old, 4 * nc);
+    c->tacts.grow(nc);
+    drop_graphs(c);
+}
+
 // Create the per-incidence spring constant (or damping) array the first time
 // a body needs one.  At least one entry, so that it exists (and grows with
 // the arena) even while the arena is empty.
@@ -674,6 +685,7 @@ StepArgs make_args(titan_ctx *c, int slot, const Range &rg) {
     a.b_d = c->b_d;
     a.b_kd = c->b_kd;
     a.act = c->act;
+    a.tact = c->tact;
     a.integ = c->integ;
     a.hdt = launch_dt(c);
     a.rk = c->in_noalloc ? c->na_rk : c->rk;
@@ -1359,6 +1371,7 @@ PreparedBody prepare_body(const titan_schedule &sched, const titan_body_desc *d,
         }
         P.n_lc = (int64_t)P.lch.size();
     }
+    if (tiled && n_act > 0) P.act_local.swap(act_local);  // the tiles' actuator records come from it
     P.sl.swap(nl);
     P.sr.swap(nr);
     return P;
@@ -1404,6 +1417,7 @@ void harvest_released(titan_ctx *c, bool wait) {
         c->lcs.release(b.lc0, b.lc_len);
         c->tiles.release(b.tile0, b.ntiles);
         c->halos.release(b.halo0, b.nhalo);
+        c->tacts.release(b.tact0, b.tact_len);
         if (b.resident) c->rdescs.release(b.res, 1);
         c->bids.release(p.id, 1);
         (void)hipEventDestroy(p.ev);
@@ -1413,10 +1427,10 @@ void harvest_released(titan_ctx *c, bool wait) {
 
 // Arena ranges of one body, and what a body needs reserved.
 struct Slots {
-    int64_t ch0 = -1, inc0 = -1, act0 = -1, lc0 = -1, tl0 = -1, hl0 = -1, rs0 = -1, bid = -1;
+    int64_t ch0 = -1, inc0 = -1, act0 = -1, lc0 = -1, tl0 = -1, hl0 = -1, ta0 = -1, rs0 = -1, bid = -1;
 };
 struct SlotNeeds {
-    int64_t nch, n_inc, n_act, n_lc, n_tiles, n_halo;
+    int64_t nch, n_inc, n_act, n_lc, n_tiles, n_halo, n_tact;
     bool resident;
 };
 
@@ -1446,6 +1460,7 @@ Slots reserve_slots(titan_ctx *c, const SlotNeeds &n, const PreparedBody &P) {
         r.lc0 = take(c->lcs, n.n_lc, ensure_lcs);
         r.tl0 = take(c->tiles, n.n_tiles, ensure_tiles);
         r.hl0 = take(c->halos, n.n_halo, ensure_halos);
+        r.ta0 = take(c->tacts, n.n_tact, ensure_tacts);
         if (n.resident) r.rs0 = take(c->rdescs, 1, ensure_rdescs);
         r.bid = take(c->bids, 1, ensure_bodies);
         if (P.per_k) make_inc_array(c, c->inc_k);
@@ -1479,6 +1494,7 @@ Slots reserve_slots(titan_ctx *c, const SlotNeeds &n, const PreparedBody &P) {
         if (r.lc0 >= 0) c->lcs.release(r.lc0, n.n_lc);
         if (r.tl0 >= 0) c->tiles.release(r.tl0, n.n_tiles);
         if (r.hl0 >= 0) c->halos.release(r.hl0, n.n_halo);
+        if (r.ta0 >= 0) c->tacts.release(r.ta0, n.n_tact);
         if (r.rs0 >= 0) c->rdescs.release(r.rs0, 1);
         if (r.bid >= 0) c->bids.release(r.bid, 1);
         throw;
@@ -1504,33 +1520,43 @@ int32_t commit_body(titan_ctx *c, PreparedBody &P) {
     const bool tiled = P.tiled, resident = P.resident;
     SellBuild &sb = P.sb;
     TileBuild &tb = P.tb;
-    const int64_t n_tiles = (int64_t)tb.tiles.size(), n_halo = (int64_t)tb.halo.size();
+    const int64_t n_tiles = (int64_t)tb.tiles.size(), n_halo = (int64_t)tb.halo.size(),
+                  n_tact = (int64_t)tb.tact.size();
     const int64_t n_inc = sb.coff[nch];
     // a resident body's split-form data follows its SELL entries
     const int64_t n_extra = std::max<int64_t>((int64_t)P.spw.size(), (int64_t)P.spd.size());
 
     // ---- reserve space (released again on failure)
-    const Slots sl = reserve_slots(c, {nch, n_inc + n_extra, n_act, n_lc, n_tiles, n_halo, resident}, P);
+    const Slots sl = reserve_slots(c, {nch, n_inc + n_extra, n_act, n_lc, n_tiles, n_halo, n_tact, resident}, P);
     const int64_t ch0 = sl.ch0, inc0 = sl.inc0, act0 = sl.act0, lc0 = sl.lc0, tl0 = sl.tl0, hl0 = sl.hl0,
-                  rs0 = sl.rs0, bid = sl.bid;
+                  ta0 = sl.ta0, rs0 = sl.rs0, bid = sl.bid;
     const int64_t g0 = ch0 * kChunk;
 
-    // ---- arena addresses: gather slots, actuated incidences, tiles, local lists
+    // ---- arena addresses: gather slots, actuated incidences (a tiled
+    // body's name their tile's list positions), tiles, local lists
     if (!tiled && !resident)
         for (auto &x : sb.nbr) x += (uint32_t)g0;
-    mark_actuated(sb, tiled || resident, act0);
+    mark_actuated(sb, tiled || resident, tiled ? 0 : act0);
     for (auto &t : tb.tiles) {
         t.body = -1;  // set by the activation kernel
         t.chunk0 += (int32_t)ch0;
         t.halo_off += hl0;
+        t.act_off += ta0;
+    }
+    // the tiles' copies of their actuators' records
+    std::vector<double> trec(4 * (size_t)n_tact);
+    for (int64_t x = 0; x < n_tact; ++x) {
+        const int32_t a = P.act_local[tb.tact[x]];
+        for (int k = 0; k < 4; ++k) trec[4 * x + k] = P.acth[4 * (size_t)a + k];
     }
     for (auto &h : tb.halo) h += (int32_t)g0;
     for (auto &v : P.mlc)
         if (v) v += lc0 << 20;
     std::vector<int64_t> cptr(nch), ca0(nch, 0);
     std::vector<int32_t> cng(nch), cna(nch, 0);
-    {  // each chunk takes an equal share of the body's actuators
-        const int64_t per = nch > 0 ? (n_act + nch - 1) / nch : 0;
+    {  // each chunk takes an equal share of the body's actuators (a tiled
+       // body's tiles compute theirs: no share)
+        const int64_t per = nch > 0 && !tiled ? (n_act + nch - 1) / nch : 0;
         for (int64_t ch = 0; ch < nch; ++ch) {
             ca0[ch] = act0 + ch * per;
             cna[ch] = (int32_t)std::max<int64_t>(0, std::min(per, n_act - ch * per));
@@ -1581,6 +1607,7 @@ int32_t commit_body(titan_ctx *c, PreparedBody &P) {
     }
     up(c->tile + tl0, tb.tiles.data(), n_tiles);
     up(c->halo + hl0, tb.halo.data(), n_halo);
+    up(c->tact + 4 * ta0, trec.data(), 4 * n_tact);
     // a resident body's incidence copy: names (4 B per entry, to a 16-byte
     // boundary), rest lengths and per-incidence constants (8 B each)
     int64_t cache = ((4 * n_inc + 15) & ~(int64_t)15) + 8 * n_inc * (1 + (P.per_k ? 1 : 0) + (P.per_d ? 1 : 0));
@@ -1635,6 +1662,8 @@ int32_t commit_body(titan_ctx *c, PreparedBody &P) {
     b.ntiles = n_tiles;
     b.halo0 = hl0;
     b.nhalo = n_halo;
+    b.tact0 = ta0;
+    b.tact_len = n_tact;
     b.rows = tb.rmax;
     b.T = P.t0;
     b.n = 0;
@@ -1684,7 +1713,7 @@ int32_t commit_device_built(titan_ctx *c, PreparedBody &P) {
     HIPCHK(c->builder->plan(in, c->up, &sz));
     if (!sz.fits) return -1;
     const int64_t N = P.N, nch = sz.nch, n_act = P.n_act, n_lc = P.n_lc;
-    const Slots sl = reserve_slots(c, {nch, sz.n_inc, n_act, n_lc, sz.ntiles, sz.n_halo, false}, P);
+    const Slots sl = reserve_slots(c, {nch, sz.n_inc, n_act, n_lc, sz.ntiles, sz.n_halo, sz.n_tact, false}, P);
     BodyH b;
     b.perm.resize(N);
     b.flags.resize(N);
@@ -1712,17 +1741,22 @@ int32_t commit_device_built(titan_ctx *c, PreparedBody &P) {
     o.inc_damp = P.per_d ? c->inc_damp : nullptr;
     o.tiles = c->tile;
     o.halo = c->halo;
+    o.act = c->act;
+    o.tact = c->tact;
     o.ch0 = sl.ch0;
     o.inc0 = sl.inc0;
     o.tl0 = sl.tl0;
     o.hl0 = sl.hl0;
     o.act0 = sl.act0;
+    o.ta0 = sl.ta0;
     o.n_act = n_act;
     const DevBuildMirror mr{b.perm.data(), b.flags.data(), b.sl.data(), b.sr.data(), b.coff.data()};
-    hipError_t e = c->builder->write(o, mr, c->up);
-    if (e == hipSuccess && !P.acth.empty())
+    // the actuator records first: the tiles' copies are taken from them
+    hipError_t e = hipSuccess;
+    if (!P.acth.empty())
         e = hipMemcpyAsync(c->act + 4 * sl.act0, P.acth.data(), sizeof(double) * P.acth.size(), hipMemcpyHostToDevice,
                            c->up);
+    if (e == hipSuccess) e = c->builder->write(o, mr, c->up);
     if (e == hipSuccess && n_lc > 0)
         e = hipMemcpyAsync(c->lc + sl.lc0, P.lch.data(), sizeof(LocalC) * n_lc, hipMemcpyHostToDevice, c->up);
     if (e == hipSuccess) e = hipStreamSynchronize(c->up);
@@ -1733,6 +1767,7 @@ int32_t commit_device_built(titan_ctx *c, PreparedBody &P) {
         c->lcs.release(sl.lc0, n_lc);
         c->tiles.release(sl.tl0, sz.ntiles);
         c->halos.release(sl.hl0, sz.n_halo);
+        c->tacts.release(sl.ta0, sz.n_tact);
         c->bids.release(sl.bid, 1);
         fail(TITAN_E_HIP, std::string("body_create (device build): ") + hipGetErrorString(e));
     }
@@ -1773,6 +1808,8 @@ int32_t commit_device_built(titan_ctx *c, PreparedBody &P) {
     h.ntiles = sz.ntiles;
     h.halo0 = sl.hl0;
     h.nhalo = sz.n_halo;
+    h.tact0 = sl.ta0;
+    h.tact_len = sz.n_tact;
     h.rows = sz.rmax;
     h.T = P.t0;
     h.n = 0;
@@ -2385,7 +2422,10 @@ int titan_reserve(titan_ctx *c, int64_t n_masses, int64_t n_springs, int64_t n_a
     const int64_t old_ch = c->chunks.cap();
     size_t bytes = old_ch >= chunks ? 0 : (size_t)std::min(grown_cap(old_ch, chunks, 16), max_chunk_cap()) * chunk_bytes(c);
     bytes += grows(c->incs.cap(), incs, 4096, inc_bytes(c));
-    if (n_actuated > 0) bytes += grows(c->acts.cap(), n_actuated, 1024, act_bytes(c));
+    if (n_actuated > 0) {
+        bytes += grows(c->acts.cap(), n_actuated, 1024, act_bytes(c));
+        bytes += grows(c->tacts.cap(), 2 * n_actuated, 1024, 4 * sizeof(double));
+    }
     bytes += grows(c->tiles.cap(), tiles, 64, sizeof(TileDesc));
     bytes += grows(c->halos.cap(), 2 * n_masses, 1024, sizeof(int32_t));
     bytes += grows(c->rdescs.cap(), nb, 64, sizeof(ResidentDesc));
@@ -2393,7 +2433,10 @@ int titan_reserve(titan_ctx *c, int64_t n_masses, int64_t n_springs, int64_t n_a
     check_growth(c, bytes);
     ensure_chunks(c, chunks);
     ensure_incs(c, incs);
-    if (n_actuated > 0) ensure_acts(c, n_actuated);
+    if (n_actuated > 0) {
+        ensure_acts(c, n_actuated);
+        ensure_tacts(c, 2 * n_actuated);  // a spring is in the lists of at most two tiles
+    }
     ensure_tiles(c, tiles);
     ensure_halos(c, 2 * n_masses);
     ensure_rdescs(c, nb);
@@ -2716,9 +2759,11 @@ int titan_debug_body_bytes(titan_ctx *c, int32_t body_id, int32_t what, void *ou
                 t[q].body = 0;
                 t[q].chunk0 -= (int32_t)b.chunk0;
                 t[q].halo_off -= b.halo0;
+                t[q].act_off -= b.tact0;
             }
             break;
         }
+        case TITAN_DEBUG_TACT: dev(c->tact + 4 * b.tact0, 32 * (size_t)b.tact_len); break;
         case TITAN_DEBUG_HALO: {
             dev(c->halo + b.halo0, 4 * (size_t)b.nhalo);
             int32_t *h = reinterpret_cast<int32_t *>(buf.data());

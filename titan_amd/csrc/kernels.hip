@@ -465,17 +465,22 @@ struct Group {
     }
 };
 
-// Actuated rest lengths at time T of actuators [first, first + n), one per
-// lane in turn: rest * (1 + a sin(omega T + phi)) (src/sim.cu:1058-1066, the
-// reference's two kinds being a = -0.2 and 0.2 with phi = 0), evaluated in
-// the reference's order.
+// Actuated rest lengths at time T of the n actuator records at rec into
+// dst[0, n), one per lane in turn: rest * (1 + a sin(omega T + phi))
+// (src/sim.cu:1058-1066, the reference's two kinds being a = -0.2 and 0.2
+// with phi = 0), evaluated in the reference's order.
+__device__ __forceinline__ void actuate_records(const double *rec, int n, double T, double *__restrict__ dst,
+                                                int lane, int stride) {
+    const double2 *__restrict__ A = reinterpret_cast<const double2 *>(rec);
+    for (int q = lane; q < n; q += stride) {
+        const double2 aw = A[2 * q], pr = A[2 * q + 1];  // (a, omega), (phi, rest)
+        dst[q] = pr.y * (1 + aw.x * sin(aw.y * T + pr.x));
+    }
+}
+// The same for actuators [first, first + n) of the arena, into dst[first, first + n).
 __device__ __forceinline__ void actuate_slice(const StepArgs &a, int64_t first, int n, double T,
                                               double *__restrict__ dst, int lane, int stride) {
-    const double2 *__restrict__ A = reinterpret_cast<const double2 *>(a.act);
-    for (int q = lane; q < n; q += stride) {
-        const double2 aw = A[2 * (first + q)], pr = A[2 * (first + q) + 1];  // (a, omega), (phi, rest)
-        dst[first + q] = pr.y * (1 + aw.x * sin(aw.y * T + pr.x));
-    }
+    actuate_records(a.act + 4 * first, n, T, dst + first, lane, stride);
 }
 
 // Add -G of one incident spring (axis nrm, u) to the mass's force.
@@ -763,13 +768,18 @@ __global__ __launch_bounds__(kBlock) void k_step_gather(const StepArgs a) {
 // other endpoints of its springs outside the tile, rows nmass..) are staged
 // in LDS, 48 bytes per row, after which each lane walks its incidences as
 // the gather step does, reading the other endpoints from LDS rows instead
-// of global memory.
+// of global memory.  ACT: the workgroup computes the actuated rest lengths
+// of the tile's actuators at the body's clock into LDS after the halo rows,
+// where the walk reads those of its actuated incidences (a gather from a
+// buffer written by the previous launch made each group's evaluation wait
+// for the next group's loads too: vector loads retire in order).
 // The tiled step after the workgroup's uniform fields are read (see
 // k_step_tiles), for a body whose constant mode is KD.
 template <bool ACT, int KD>
 __device__ __forceinline__ void tiles_body(const StepArgs &a, double2 *__restrict__ S, int t, int nmass, int nhalo,
-                                           int64_t halo_off, int64_t chunk, bool live, int64_t ebase, int ng,
-                                           int b, const BodyKD &bkd, int s, int64_t left, int64_t nst, double T) {
+                                           int64_t halo_off, int nact, int64_t act_off, int64_t chunk, bool live,
+                                           int64_t ebase, int ng, int b, const BodyKD &bkd, int64_t left, int64_t nst,
+                                           double T) {
     const int64_t n6 = a.ncap;
     const int p = (int)(nst & 1);
     const double2 *__restrict__ cur = reinterpret_cast<const double2 *>(a.st + (int64_t)p * 6 * n6);
@@ -793,10 +803,10 @@ __device__ __forceinline__ void tiles_body(const StepArgs &a, double2 *__restric
         d = a.deg[i];
         m = (bkd.kd & kKdUniformM) ? a.b_m[b] : a.m[i];  // uniform over the workgroup
         w.prime(ebase, ng, t & (kChunk - 1), bkd);
-        // the chunk's share of its body's actuation factors for the next step
-        if (ACT && left > 1)
-            actuate_slice(a, a.chunk_act0[chunk], a.chunk_nact[chunk], T + a.hdt, a.ract[s ^ 1], t & (kChunk - 1), kChunk);
     }
+    // the tile's actuated rest lengths, at this step's clock
+    double *Sact = reinterpret_cast<double *>(S + 3 * (nmass + nhalo));
+    if (ACT && nact > 0) actuate_records(a.tact + 4 * act_off, nact, T, Sact, t, kTileMasses);
     // halo rows, two per thread per round: both indices, then both states.
     // Loads past the halo are clamped to its last entry and stored to the
     // spare row a.ldim, so that no load sits behind a branch.
@@ -836,7 +846,7 @@ __device__ __forceinline__ void tiles_body(const StepArgs &a, double2 *__restric
     r.vx = o1.y;
     r.vy = o2.x;
     r.vz = o2.y;
-    w.run(r, d, a.ract[s], [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
+    w.run(r, d, Sact, [&](uint32_t o, double2 &q0, double2 &q1, double2 &q2) {
         q0 = S[3 * o];
         q1 = S[3 * o + 1];
         q2 = S[3 * o + 2];
@@ -859,8 +869,8 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     const TileDesc *__restrict__ td = a.tiles + tile;
     const int b = td->body;
     if (b < 0) return;
-    const int nmass = td->nmass, nhalo = td->nhalo;
-    const int64_t halo_off = td->halo_off;
+    const int nmass = td->nmass, nhalo = td->nhalo, nact = ACT ? td->nact : 0;
+    const int64_t halo_off = td->halo_off, act_off = ACT ? td->act_off : 0;
     const int64_t chunk = (int64_t)td->chunk0 + uniform(t >> 6);
     // a wave that holds tile masses loads for all its lanes (the chunk's
     // padding slots are allocated)
@@ -880,10 +890,22 @@ __global__ __launch_bounds__(kTileMasses) void k_step_tiles(const StepArgs a) {
     if (left <= 0) return;  // uniform over the workgroup
 
     switch (bkd.kd & 3u) {  // uniform over the workgroup
-        case 0: tiles_body<ACT, 0>(a, S, t, nmass, nhalo, halo_off, chunk, live, ebase, ng, b, bkd, s, left, nst, T); break;
-        case 1: tiles_body<ACT, 1>(a, S, t, nmass, nhalo, halo_off, chunk, live, ebase, ng, b, bkd, s, left, nst, T); break;
-        case 2: tiles_body<ACT, 2>(a, S, t, nmass, nhalo, halo_off, chunk, live, ebase, ng, b, bkd, s, left, nst, T); break;
-        default: tiles_body<ACT, 3>(a, S, t, nmass, nhalo, halo_off, chunk, live, ebase, ng, b, bkd, s, left, nst, T); break;
+        case 0:
+            tiles_body<ACT, 0>(a, S, t, nmass, nhalo, halo_off, nact, act_off, chunk, live, ebase, ng, b, bkd,
+                               left, nst, T);
+            break;
+        case 1:
+            tiles_body<ACT, 1>(a, S, t, nmass, nhalo, halo_off, nact, act_off, chunk, live, ebase, ng, b, bkd,
+                               left, nst, T);
+            break;
+        case 2:
+            tiles_body<ACT, 2>(a, S, t, nmass, nhalo, halo_off, nact, act_off, chunk, live, ebase, ng, b, bkd,
+                               left, nst, T);
+            break;
+        default:
+            tiles_body<ACT, 3>(a, S, t, nmass, nhalo, halo_off, nact, act_off, chunk, live, ebase, ng, b, bkd,
+                               left, nst, T);
+            break;
     }
 }
 

@@ -45,9 +45,19 @@ constexpr uint32_t kPad = 16u;      // padding slot of a chunk (never read)
 // 1M-mass lattice, full 256-mass tiles (4 workgroups per CU) beat tiles
 // capped at 682 rows (5 per CU), whose extra halo costs more than the
 // occupancy gains; the default budget is therefore the maximum.
+//
+// A tile of an actuated body also lists its actuators: the actuated springs
+// with an endpoint in the tile, in ascending spring index, each with a copy
+// of its record (a, omega, phi, rest) in the tile actuator arena.  The step
+// computes their actuated rest lengths at the body's clock into LDS after
+// the halo rows (kActsPerRow to a row), and an actuated incidence of the
+// tile names its spring's position in that list, so that the walk reads the
+// value from LDS.  These rows count against the row budget like the others.
 constexpr int kTileMasses = 256;
 constexpr int kMaxTileRows = 2304;  // LDS budget: 2304 x 48 bytes = 108 KiB
 constexpr int kTileRows = kMaxTileRows;
+constexpr int kActsPerRow = 6;      // 8-byte actuated rest lengths per 48-byte LDS row
+constexpr int64_t tile_act_rows(int64_t nact) { return (nact + kActsPerRow - 1) / kActsPerRow; }
 
 struct TileDesc {
     int32_t body;
@@ -56,7 +66,8 @@ struct TileDesc {
     int32_t nhalo;
     int64_t halo_off;   // into the halo arena
     int32_t first;      // 1 for the body's first tile (advances the body clock)
-    int32_t pad;
+    int32_t nact;       // actuators of the tile
+    int64_t act_off;    // its first actuator record in the tile actuator arena
 };
 
 // Resident layout (k_rollout_resident): one workgroup runs every step of a
@@ -160,8 +171,10 @@ constexpr int kGroupTiled = 2;
 
 // An actuated incidence is marked by the top bit of its name (bit 31 of a
 // gather slot, bit 15 of a 16-bit LDS row); its rest-length entry then holds
-// the actuator's arena index (an int64 in the same 8 bytes), whose actuated
-// rest length the step reads from the factor buffer instead.
+// an int64 in the same 8 bytes: for a tiled body, the spring's position in
+// its tile's actuator list (the step reads the actuated rest length from
+// LDS); otherwise the actuator's arena index, whose actuated rest length the
+// step reads from the factor buffer.
 constexpr uint32_t kActBit32 = 0x80000000u;
 // b_kd bit 2: every mass of the body has the same m, which the tiled step
 // reads from the body table instead of per mass
@@ -203,6 +216,7 @@ struct StepArgs {
     const double *inc_k;      // per incidence, for bodies with per-spring k (else null)
     const double *inc_damp;   // per incidence, for bodies with per-spring damping (else null)
     const double *act;        // 4 per actuator: a, omega, phi, rest
+    const double *tact;       // tiled bodies: 4 per tile actuator (the same record), by tile
     // integrator; RK2: the state a first stage leaves for the second (pos,
     // vel after the constraints: 6 per mass, as the state buffers)
     int32_t integ;
@@ -245,7 +259,7 @@ struct StepArgs {
     const TileDesc *tiles;
     int64_t tile_base, ntiles;    // launch range: tiles [tile_base, ntiles)
     const int32_t *halo;      // arena mass slots
-    int32_t ldim;             // LDS rows: largest nmass + nhalo of a tile
+    int32_t ldim;             // LDS rows: largest nmass + nhalo + tile_act_rows(nact) of a tile
     const uint8_t *b_mode;    // per body: kModeGather, kModeTiled or kModeResident
     // resident path
     const ResidentDesc *rdesc;

@@ -96,24 +96,36 @@ void incidence_lists(int64_t n, const SpringsIn &sp, Incidence &inc, bool with_s
 }  // namespace
 
 bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb, int tile_rows) {
+    const bool acts = sp.act != nullptr;
     Incidence inc;
-    incidence_lists(n, sp, inc, false);
+    incidence_lists(n, sp, inc, acts);
     const std::vector<int64_t> &row = inc.row;
     const std::vector<int32_t> &other = inc.other;
     tb.tiles.clear();
     tb.halo.clear();
+    tb.tact.clear();
+    tb.acts = acts;
     tb.hmax = 0;
     tb.rmax = 0;
     std::vector<int32_t> best;
-    std::vector<int64_t> stamp(n, -1);
+    std::vector<int64_t> alist;
+    std::vector<int64_t> stamp(n, -1), sstamp(acts ? sp.n_springs : 0, -1);
     int64_t mark = 0;
+    // an actuated spring of incidence j not yet stamped with mk: stamp it
+    auto new_act = [&](int64_t j, int64_t mk) -> bool {
+        const int64_t s = inc.spring[j];
+        if (sp.act[s] < 0 || sstamp[s] == mk) return false;
+        sstamp[s] = mk;
+        return true;
+    };
     for (int64_t m0 = 0; m0 < n;) {
         // grow the tile a chunk at a time (up to 4, at least one) while its
-        // masses and halo fit tile_rows, counting the halo incrementally: a
-        // joining mass leaves it, the joining masses' outside neighbours
-        // enter it
+        // masses, halo and actuator rows fit tile_rows, counting the halo
+        // incrementally (a joining mass leaves it, the joining masses'
+        // outside neighbours enter it) and the actuators as the distinct
+        // actuated springs of the joining masses
         const int64_t mk = ++mark;
-        int64_t cnt = 0, e_prev = m0, m1 = m0;
+        int64_t cnt = 0, acnt = 0, e_prev = m0, m1 = m0;
         for (int k = 1; k <= kTileMasses / kChunk; ++k) {
             if (k > 1 && m0 + (int64_t)(k - 1) * kChunk >= n) break;
             const int64_t e = std::min(n, m0 + (int64_t)k * kChunk);
@@ -129,25 +141,30 @@ bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb, int tile_rows) {
                         stamp[o] = mk;
                         ++cnt;
                     }
+                    if (acts && new_act(j, mk)) ++acnt;
                 }
-            if (k > 1 && e - m0 + cnt > tile_rows) break;
+            if (k > 1 && e - m0 + cnt + tile_act_rows(acnt) > tile_rows) break;
             m1 = e;
             e_prev = e;
         }
-        // the chosen range's halo, ascending
+        // the chosen range's halo and actuators, ascending
         const int64_t mh = ++mark;
         best.clear();
+        alist.clear();
         for (int64_t j = row[m0]; j < row[m1]; ++j) {
             const int32_t o = other[j];
             if ((o < m0 || o >= m1) && stamp[o] != mh) {
                 stamp[o] = mh;
                 best.push_back(o);
             }
+            if (acts && new_act(j, mh)) alist.push_back(inc.spring[j]);
         }
         std::sort(best.begin(), best.end());
-        const int64_t rows = m1 - m0 + (int64_t)best.size();
+        std::sort(alist.begin(), alist.end());
+        const int64_t rows = m1 - m0 + (int64_t)best.size() + tile_act_rows((int64_t)alist.size());
         if (rows > kMaxTileRows) {
-            tb.why = "a chunk and its halo need more than " + std::to_string(kMaxTileRows) + " LDS rows";
+            tb.why = "a chunk with its halo and actuators needs more than " + std::to_string(kMaxTileRows) +
+                     " LDS rows";
             return false;
         }
         tb.hmax = std::max<int32_t>(tb.hmax, (int32_t)best.size());
@@ -159,7 +176,10 @@ bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb, int tile_rows) {
         d.nhalo = (int32_t)best.size();
         d.halo_off = (int64_t)tb.halo.size();
         d.first = m0 == 0;
+        d.nact = (int32_t)alist.size();
+        d.act_off = (int64_t)tb.tact.size();
         tb.halo.insert(tb.halo.end(), best.begin(), best.end());
+        tb.tact.insert(tb.tact.end(), alist.begin(), alist.end());
         tb.tiles.push_back(d);
         m0 = m1;
     }
@@ -196,8 +216,11 @@ void build_sell(int64_t n, const SpringsIn &sp, bool per_k, bool per_d, const Ti
     if (per_d) out.damp.assign(ne, 0.0);
     if (sp.act) out.act.assign(ne, -1);
     // tiled: the LDS row of every mass the current tile names (its masses,
-    // then its halo), set when the walk enters the tile
+    // then its halo), and the list position of each of its actuators, set
+    // when the walk enters the tile
     std::vector<uint32_t> rowmap(tb ? n : 0);
+    const bool tile_acts = tb && tb->acts && sp.act;
+    std::vector<int32_t> aloc(tile_acts ? sp.n_springs : 0);
     int32_t cur_tile = -1;
     for (int64_t c = 0; c < nch; ++c) {
         const int64_t c0 = out.coff[c];
@@ -212,6 +235,8 @@ void build_sell(int64_t n, const SpringsIn &sp, bool per_k, bool per_d, const Ti
                 for (int32_t r = 0; r < d.nmass; ++r) rowmap[m0 + r] = (uint32_t)r;
                 const int32_t *h = tb->halo.data() + d.halo_off;
                 for (int32_t r = 0; r < d.nhalo; ++r) rowmap[h[r]] = (uint32_t)(d.nmass + r);
+                if (tile_acts)
+                    for (int32_t r = 0; r < d.nact; ++r) aloc[tb->tact[d.act_off + r]] = r;
             }
         }
         for (int l = 0; l < kChunk; ++l) {
@@ -231,7 +256,7 @@ void build_sell(int64_t n, const SpringsIn &sp, bool per_k, bool per_d, const Ti
                     out.rest[e] = sp.rest[s];
                     if (per_k) out.k[e] = sp.k[s];
                     if (per_d) out.damp[e] = sp.damping ? sp.damping[s] : 0.0;
-                    if (sp.act) out.act[e] = sp.act[s];
+                    if (sp.act) out.act[e] = (tile_acts && sp.act[s] >= 0) ? aloc[s] : sp.act[s];
                 }
                 if (tb) {  // the LDS rows of a group's two entries packed at c0 + (e - c0) / 2
                     const int64_t rel = e - c0;

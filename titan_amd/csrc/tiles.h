@@ -21,11 +21,15 @@ struct SpringsIn {
 };
 
 struct TileBuild {
-    std::vector<TileDesc> tiles;   // chunk0 and halo_off relative to this body
+    std::vector<TileDesc> tiles;   // chunk0, halo_off and act_off relative to this body
     std::vector<int32_t> halo;     // body-local mass indices (device order), ascending per tile
+    // the tiles' actuator lists (acts: the body has actuated springs): the
+    // actuated springs with an endpoint in the tile, ascending, at act_off
+    std::vector<int64_t> tact;
+    bool acts = false;
     std::vector<int32_t> tile_of_chunk;
     int32_t hmax = 0;              // largest halo
-    int32_t rmax = 0;              // largest nmass + nhalo (LDS rows)
+    int32_t rmax = 0;              // largest nmass + nhalo + tile_act_rows(nact) (LDS rows)
     std::string why;               // reason when the layout does not apply
 };
 
@@ -43,13 +47,17 @@ struct SellBuild {
     // entries (low half: the even entry); the chunk's other half is unused
     std::vector<uint32_t> nbr;
     std::vector<double> rest, k, damp;  // k, damp: only when built per incidence
-    std::vector<int32_t> act;      // body-local actuator index or -1, only if sp.act
+    // only if sp.act: per entry -1, or the spring's position in its tile's
+    // actuator list (a tile build that lists actuators), else the body-local
+    // actuator index
+    std::vector<int32_t> act;
 };
 
 // Mark the actuated incidences (sb.act[e] >= 0) for the kernels: the name's
 // top bit is set (bit 31 of a gather slot; bit 15 of a packed 16-bit row
-// when `packed`) and the rest-length entry takes the actuator's arena index,
-// act0 + sb.act[e], as an int64 in its 8 bytes.
+// when `packed`) and the rest-length entry takes act0 + sb.act[e] as an
+// int64 in its 8 bytes: the actuator's arena index (act0 = the body's first
+// actuator), or its position in the tile's list (act0 = 0).
 void mark_actuated(SellBuild &sb, bool packed, int64_t act0);
 
 // Order of the masses along a Morton curve of their positions (3 doubles
@@ -57,9 +65,10 @@ void mark_actuated(SellBuild &sb, bool packed, int64_t act0);
 std::vector<int32_t> morton_order(const double *pos, int64_t n);
 
 // Cut a body of n masses (device order) into tiles of whole chunks (see
-// layout.h): each tile takes the most chunks (up to 4) whose masses and
-// halo fit tile_rows LDS rows.  Lists each tile's halo.  Returns false (with
-// tb.why) if one chunk and its halo exceed kMaxTileRows.
+// layout.h): each tile takes the most chunks (up to 4) whose masses, halo
+// and actuator rows fit tile_rows LDS rows.  Lists each tile's halo and,
+// when the body has actuated springs, its actuators.  Returns false (with
+// tb.why) if one chunk with its halo and actuators exceeds kMaxTileRows.
 bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb, int tile_rows = kTileRows);
 
 // Reorder the masses inside each tile (tb; null: the whole body as one tile)
@@ -72,7 +81,9 @@ std::vector<int32_t> degree_sort_tiles(int64_t n, const SpringsIn &sp, TileBuild
 
 // Build the incidence lists.  With tb == nullptr (gather layout) an
 // incidence names the other endpoint's arena slot, g0 + its body index;
-// with a tile build it names the endpoint's LDS row in the mass's tile.
+// with a tile build it names the endpoint's LDS row in the mass's tile (and
+// an actuated one its spring's position in the tile's actuator list, when
+// the tiles list them).
 // per_k / per_d: also list each incidence's spring constant / damping.
 void build_sell(int64_t n, const SpringsIn &sp, bool per_k, bool per_d, const TileBuild *tb, int64_t g0,
                 SellBuild &out);
