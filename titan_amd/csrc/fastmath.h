@@ -70,12 +70,17 @@ __device__ __forceinline__ bool fast_range(double n2, double tx, double ty, doub
     return ((int)(n2 >= 0x1p-767) & (int)(n2 <= 0x1p1000) & num_ok(tx) & num_ok(ty) & num_ok(tz)) != 0;
 }
 
-// A cheaper, stricter form of fast_range for straight-line code: also false
-// when a component is zero (such axes take the general path, which admits
-// zeros).
-__device__ __forceinline__ bool fast_range_nz(double n2, double tx, double ty, double tz) {
+// A cheaper, stricter form of fast_range for straight-line code, as a wave
+// mask: the lanes whose axis has every |t_c| >= 2^-383 and n2 <= 2^1000.  A
+// zero component fails (such axes take the general path, which admits
+// zeros).  Every t_c^2 is then at least 2^-766 (a correctly rounded square
+// is monotone in |t_c|, and so is a sum of non-negative terms), so n2 >=
+// 2^-766 needs no test of its own.  Each condition is one compare whose
+// lane mask the ballot takes as it is, so the guard costs two min and two
+// compares per axis and no VALU work to combine them.
+__device__ __forceinline__ uint64_t fast_lanes_nz(double n2, double tx, double ty, double tz) {
     const double m = fmin(fmin(fabs(tx), fabs(ty)), fabs(tz));
-    return ((int)(n2 >= 0x1p-767) & (int)(n2 <= 0x1p1000) & (int)(m >= 0x1p-500)) != 0;
+    return __builtin_amdgcn_ballot_w64(n2 <= 0x1p1000) & __builtin_amdgcn_ballot_w64(m >= 0x1p-383);
 }
 
 // nrm = sqrt(tx tx + ty ty + tz tz) and u = t / nrm, correctly rounded, as

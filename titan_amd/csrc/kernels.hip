@@ -49,6 +49,10 @@ __device__ __forceinline__ int64_t step_block(const StepArgs &a) {
 }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+// Wave votes over the active lanes, from the lane mask of the predicate (a
+// compare's mask is used as it is, without first making a VGPR of it).
+__device__ __forceinline__ bool wave_all(bool p) { return __builtin_amdgcn_ballot_w64(p) == __builtin_amdgcn_read_exec(); }
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
 
 // Load base[byte_off / sizeof(T)] with a 32-bit byte offset from a
 // wave-uniform base, so the access compiles to a scalar-base global load.
@@ -457,7 +461,7 @@ struct Group {
         for (int q = 0; q < G; ++q) {
             const bool on = n.actuated(q);
             f[q] = rest[q];
-            if (__any(on)) {
+            if (wave_any(on)) {
                 const double v = rc[on ? __double_as_longlong(rest[q]) : 0];
                 f[q] = on ? v : rest[q];
             }
@@ -516,16 +520,16 @@ __device__ __forceinline__ void eval_set(MassRegs &r, const Group<ACT, G, KD> (&
         for (int q = 0; q < G; ++q) fetch(oq[q], nb[j * G + q][0], nb[j * G + q][1], nb[j * G + q][2]);
     }
     double tx[N], ty[N], tz[N], n2[N];
-    bool fast = G * g0 + N <= d;
+    uint64_t fast = __builtin_amdgcn_ballot_w64(G * g0 + N <= d);  // lanes that may take the fast path
 #pragma unroll
     for (int e = 0; e < N; ++e) {
         tx[e] = nb[e][0].x - r.px;
         ty[e] = nb[e][0].y - r.py;
         tz[e] = nb[e][1].x - r.pz;
         n2[e] = tx[e] * tx[e] + ty[e] * ty[e] + tz[e] * tz[e];
-        fast = fast & fast_range_nz(n2[e], tx[e], ty[e], tz[e]);
+        fast &= fast_lanes_nz(n2[e], tx[e], ty[e], tz[e]);
     }
-    if (__all(fast)) {
+    if (fast == __builtin_amdgcn_read_exec()) {
 #pragma unroll
         for (int e = 0; e < N; ++e) {
             const double nrm = fast_sqrt(n2[e]);
@@ -1124,7 +1128,7 @@ __device__ __forceinline__ void split_terms(int x, int nsp, const uint32_t *__re
         n2[u] = tx[u] * tx[u] + ty[u] * ty[u] + tz[u] * tz[u];
         fast = fast & fast_range(n2[u], tx[u], ty[u], tz[u]);
     }
-    if (__all(fast)) {  // the same operations as unit_axis's fast branch
+    if (wave_all(fast)) {  // the same operations as unit_axis's fast branch
         double y[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -1175,7 +1179,7 @@ __device__ __forceinline__ void terms_pass(int t, int nsp, const uint32_t *__res
                                            const double2 *__restrict__ cur, const double *__restrict__ rc,
                                            double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz) {
     int x = t;
-    if (__any(x + BLOCK < nsp)) {
+    if (wave_any(x + BLOCK < nsp)) {
         split_terms<ACT, 2, BLOCK, KD>(x, nsp, csw, cr, kp, dp, bkd, cur, rc, fx, fy, fz);
         x += 2 * BLOCK;
     }
