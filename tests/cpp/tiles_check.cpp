@@ -141,7 +141,7 @@ static void check(int64_t n, const std::vector<int32_t> &L, const std::vector<in
         CHECK(d.nhalo == (int32_t)halo.size());
         CHECK(tb.acts == (sp.act != nullptr));
         CHECK(d.nact == (int32_t)acts.size());
-        if (d.nact == (int32_t)acts.size())
+        if (!tb.tact.empty() && d.nact == (int32_t)acts.size())  // listed (after build_sell)
             for (int32_t q = 0; q < d.nact; ++q) CHECK(tb.tact[d.act_off + q] == acts[q]);
         const int64_t r = d.nmass + d.nhalo + tile_act_rows(d.nact);
         // within the row budget, unless the tile is a single chunk
@@ -154,7 +154,8 @@ static void check(int64_t n, const std::vector<int32_t> &L, const std::vector<in
         for (int64_t q = m0; q < m1; ++q) CHECK(tb.tile_of_chunk[q / kChunk] == (int32_t)t);
     }
     CHECK(next == n);
-    CHECK((int64_t)tb.tact.size() == (tb.tiles.empty() ? 0 : tb.tiles.back().act_off + tb.tiles.back().nact));
+    if (!tb.tact.empty())
+        CHECK((int64_t)tb.tact.size() == tb.tiles.back().act_off + tb.tiles.back().nact);
     CHECK(tb.rmax == rmax);
     CHECK(tb.hmax == hmax);
     };
@@ -164,6 +165,12 @@ static void check(int64_t n, const std::vector<int32_t> &L, const std::vector<in
     verify_tiles(L, R, tb, rows);
     SellBuild st;
     build_sell(n, sp, kd, kd, &tb, 0, st);
+    verify_tiles(L, R, tb, rows);  // now with the actuator lists
+    if (kd) {
+        int64_t want = 0;
+        for (const TileDesc &d : tb.tiles) want += d.nact;
+        CHECK((int64_t)tb.tact.size() == want);
+    }
     check_sell(n, L, R, k, rest, damp, kd ? act : noact, kd, &tb, 0, st);
     // reorder each tile's masses by degree: a permutation inside each tile,
     // degrees non-increasing there, halos rewritten in the new indices
@@ -192,6 +199,7 @@ static void check(int64_t n, const std::vector<int32_t> &L, const std::vector<in
     sp2.right = R2.data();
     SellBuild s2;
     build_sell(n, sp2, kd, kd, &tb, 0, s2);
+    verify_tiles(L2, R2, tb, rows);
     check_sell(n, L2, R2, k, rest, damp, kd ? act : noact, kd, &tb, 0, s2);
     }
 }

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <cmath>
 #include <numeric>
+#include <stdexcept>
 
 namespace titan_mi355x {
 
@@ -61,13 +62,16 @@ namespace {
 // Each mass's incident springs in ascending spring index, as CSR: for every
 // live spring s (ascending), its right end lists (L, s), then its left end
 // lists (R, s), so that a self-spring appears twice, right end first.
+// mark_act: the other endpoint's top bit marks an actuated spring (kMarkAct),
+// which costs the fill pass no array of its own.
 struct Incidence {
     std::vector<int64_t> row;    // n + 1 offsets
     std::vector<int32_t> other;  // the other endpoint
     std::vector<int64_t> spring;
 };
+constexpr uint32_t kMarkAct = 0x80000000u;
 
-void incidence_lists(int64_t n, const SpringsIn &sp, Incidence &inc, bool with_springs) {
+void incidence_lists(int64_t n, const SpringsIn &sp, Incidence &inc, bool with_springs, bool mark_act = false) {
     const int64_t S = sp.n_springs;
     inc.row.assign(n + 1, 0);
     for (int64_t s = 0; s < S; ++s) {
@@ -79,13 +83,15 @@ void incidence_lists(int64_t n, const SpringsIn &sp, Incidence &inc, bool with_s
     inc.other.resize(inc.row[n]);
     if (with_springs) inc.spring.resize(inc.row[n]);
     std::vector<int64_t> fill(inc.row.begin(), inc.row.end() - 1);
+    const bool marks = mark_act && sp.act;
     for (int64_t s = 0; s < S; ++s) {
         const int32_t L = sp.left[s], R = sp.right[s];
         if (L < 0 || R < 0) continue;
+        const uint32_t mk = marks && sp.act[s] >= 0 ? kMarkAct : 0u;
         const int64_t a = fill[R]++;
-        inc.other[a] = L;
+        inc.other[a] = (int32_t)((uint32_t)L | mk);
         const int64_t b = fill[L]++;
-        inc.other[b] = R;
+        inc.other[b] = (int32_t)((uint32_t)R | mk);
         if (with_springs) {
             inc.spring[a] = s;
             inc.spring[b] = s;
@@ -98,7 +104,7 @@ void incidence_lists(int64_t n, const SpringsIn &sp, Incidence &inc, bool with_s
 bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb, int tile_rows) {
     const bool acts = sp.act != nullptr;
     Incidence inc;
-    incidence_lists(n, sp, inc, acts);
+    incidence_lists(n, sp, inc, false, acts);
     const std::vector<int64_t> &row = inc.row;
     const std::vector<int32_t> &other = inc.other;
     tb.tiles.clear();
@@ -108,24 +114,19 @@ bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb, int tile_rows) {
     tb.hmax = 0;
     tb.rmax = 0;
     std::vector<int32_t> best;
-    std::vector<int64_t> alist;
-    std::vector<int64_t> stamp(n, -1), sstamp(acts ? sp.n_springs : 0, -1);
-    int64_t mark = 0;
-    // an actuated spring of incidence j not yet stamped with mk: stamp it
-    auto new_act = [&](int64_t j, int64_t mk) -> bool {
-        const int64_t s = inc.spring[j];
-        if (sp.act[s] < 0 || sstamp[s] == mk) return false;
-        sstamp[s] = mk;
-        return true;
-    };
+    std::vector<int64_t> stamp(n, -1);
+    int64_t mark = 0, nact_all = 0;
     for (int64_t m0 = 0; m0 < n;) {
         // grow the tile a chunk at a time (up to 4, at least one) while its
         // masses, halo and actuator rows fit tile_rows, counting the halo
         // incrementally (a joining mass leaves it, the joining masses'
-        // outside neighbours enter it) and the actuators as the distinct
-        // actuated springs of the joining masses
+        // outside neighbours enter it) and the actuators too: a joining
+        // mass's actuated spring is new to the tile unless its other end
+        // joined earlier, and one between two joining masses counts at its
+        // lower end (a self-spring's two listings count half each; acnt2
+        // counts halves)
         const int64_t mk = ++mark;
-        int64_t cnt = 0, acnt = 0, e_prev = m0, m1 = m0;
+        int64_t cnt = 0, acnt2 = 0, e_prev = m0, m1 = m0;
         for (int k = 1; k <= kTileMasses / kChunk; ++k) {
             if (k > 1 && m0 + (int64_t)(k - 1) * kChunk >= n) break;
             const int64_t e = std::min(n, m0 + (int64_t)k * kChunk);
@@ -136,32 +137,40 @@ bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb, int tile_rows) {
                 }
             for (int64_t m = e_prev; m < e; ++m)
                 for (int64_t j = row[m]; j < row[m + 1]; ++j) {
-                    const int32_t o = other[j];
-                    if ((o < m0 || o >= e) && stamp[o] != mk) {
+                    const int32_t o = other[j] & (int32_t)~kMarkAct;
+                    const bool out = o < m0 || o >= e;
+                    if (out && stamp[o] != mk) {
                         stamp[o] = mk;
                         ++cnt;
                     }
-                    if (acts && new_act(j, mk)) ++acnt;
+                    if (other[j] < 0) {  // actuated
+                        if (out) acnt2 += 2;
+                        else if (o >= e_prev) acnt2 += o > m ? 2 : o == m ? 1 : 0;
+                        // else its other end joined earlier and counted it
+                    }
                 }
-            if (k > 1 && e - m0 + cnt + tile_act_rows(acnt) > tile_rows) break;
+            if (k > 1 && e - m0 + cnt + tile_act_rows(acnt2 / 2) > tile_rows) break;
             m1 = e;
             e_prev = e;
         }
-        // the chosen range's halo and actuators, ascending
+        // the chosen range's halo, ascending, and its actuators (listed
+        // below), counted once each: from outside, or at the lower end
         const int64_t mh = ++mark;
         best.clear();
-        alist.clear();
-        for (int64_t j = row[m0]; j < row[m1]; ++j) {
-            const int32_t o = other[j];
-            if ((o < m0 || o >= m1) && stamp[o] != mh) {
-                stamp[o] = mh;
-                best.push_back(o);
+        int64_t nact2 = 0;
+        for (int64_t m = m0; m < m1; ++m)
+            for (int64_t j = row[m]; j < row[m + 1]; ++j) {
+                const int32_t o = other[j] & (int32_t)~kMarkAct;
+                const bool out = o < m0 || o >= m1;
+                if (out && stamp[o] != mh) {
+                    stamp[o] = mh;
+                    best.push_back(o);
+                }
+                if (other[j] < 0) nact2 += (out || o > m) ? 2 : o == m ? 1 : 0;
             }
-            if (acts && new_act(j, mh)) alist.push_back(inc.spring[j]);
-        }
         std::sort(best.begin(), best.end());
-        std::sort(alist.begin(), alist.end());
-        const int64_t rows = m1 - m0 + (int64_t)best.size() + tile_act_rows((int64_t)alist.size());
+        const int64_t nact = nact2 / 2;
+        const int64_t rows = m1 - m0 + (int64_t)best.size() + tile_act_rows(nact);
         if (rows > kMaxTileRows) {
             tb.why = "a chunk with its halo and actuators needs more than " + std::to_string(kMaxTileRows) +
                      " LDS rows";
@@ -176,10 +185,10 @@ bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb, int tile_rows) {
         d.nhalo = (int32_t)best.size();
         d.halo_off = (int64_t)tb.halo.size();
         d.first = m0 == 0;
-        d.nact = (int32_t)alist.size();
-        d.act_off = (int64_t)tb.tact.size();
+        d.nact = (int32_t)nact;
+        d.act_off = nact_all;
+        nact_all += nact;
         tb.halo.insert(tb.halo.end(), best.begin(), best.end());
-        tb.tact.insert(tb.tact.end(), alist.begin(), alist.end());
         tb.tiles.push_back(d);
         m0 = m1;
     }
@@ -188,6 +197,22 @@ bool build_tiles(int64_t n, const SpringsIn &sp, TileBuild &tb, int tile_rows) {
     for (size_t q = 0; q < tb.tiles.size(); ++q) {
         const TileDesc &d = tb.tiles[q];
         for (int64_t c = 0; c < (d.nmass + kChunk - 1) / kChunk; ++c) tb.tile_of_chunk[d.chunk0 + c] = (int32_t)q;
+    }
+    // the actuator lists: the springs in ascending order, each appended to
+    // the lists of its endpoints' tiles (so every list comes out ascending)
+    if (acts) {
+        tb.tact.assign(nact_all, -1);
+        std::vector<int64_t> at(tb.tiles.size());
+        for (size_t q = 0; q < tb.tiles.size(); ++q) at[q] = tb.tiles[q].act_off;
+        for (int64_t s = 0; s < sp.n_springs; ++s) {
+            const int32_t L = sp.left[s], R = sp.right[s];
+            if (sp.act[s] < 0 || L < 0 || R < 0) continue;
+            const int32_t a = tb.tile_of_chunk[L / kChunk], b = tb.tile_of_chunk[R / kChunk];
+            tb.tact[at[a]++] = s;
+            if (b != a) tb.tact[at[b]++] = s;
+        }
+        for (size_t q = 0; q < tb.tiles.size(); ++q)
+            if (at[q] != tb.tiles[q].act_off + tb.tiles[q].nact) throw std::logic_error("build_tiles: tile actuator count");
     }
     return true;
 }
