@@ -649,3 +649,47 @@ def test_split_form_lean_and_general_mass_pass():
             got = ctx.download(bid)
         np.testing.assert_array_equal(got["pos"], o.pos)
         np.testing.assert_array_equal(got["vel"], o.vel)
+
+
+TILE_DTYPE = np.dtype([("body", "<i4"), ("chunk0", "<i4"), ("nmass", "<i4"), ("nhalo", "<i4"),
+                       ("halo_off", "<i8"), ("first", "<i4"), ("nact", "<i4"), ("act_off", "<i8")])
+
+
+def test_tile_actuator_rows_bound_tiles_and_match_gather():
+    """A tile of an actuated body computes its actuators into LDS rows after
+    its halo (six to a row), and those rows count against the tile row
+    budget: under a 300-row budget a 60%-actuated 12^3 lattice is cut into
+    more, smaller tiles than its passive copy.  Its steps equal those of the
+    gather layout (which reads the actuated rest lengths that the previous
+    launch wrote to a buffer) bit for bit, and the oracle's to 1e-10."""
+    rng = np.random.default_rng(7)
+    b = scenes.actuate(scenes.lattice_body((0.0, 0.0, 0.45), (1.0, 1.0, 1.0), 12, damping=0.1), 0.6, rng)
+    sc = scenes.Scene(bodies=[b], planes=[(0.0, 0.0, 1.0, 0.0, 0.8, 1.0)], dt=1e-4)
+    passive = dict(b, type=None, omega=None, act_a=None, act_phi=None)
+    _say(f"case tile actuator rows: 12^3 lattice, {sc.n_actuated} of {sc.n_springs} springs actuated, "
+         "300-row tiles, 200 steps")
+    with Context() as ctx:
+        ctx.set_schedule(tile_rows=300)
+        [bid] = sc.load(ctx, layout=LAYOUT_TILED)
+        pid = ctx.add_body(**passive, layout=LAYOUT_TILED)
+        t = np.frombuffer(ctx.layout_bytes(bid, "tiles"), TILE_DTYPE)
+        tp = np.frombuffer(ctx.layout_bytes(pid, "tiles"), TILE_DTYPE)
+        rec = np.frombuffer(ctx.layout_bytes(bid, "tact"), "<f8").reshape(-1, 4)
+        assert np.all(tp["nact"] == 0) and np.all(t["nact"] > 0)
+        rows = t["nmass"] + t["nhalo"] + (t["nact"] + 5) // 6
+        assert np.all((rows <= 300) | (t["nmass"] <= 64))
+        assert len(t) > len(tp)  # the actuator rows took room from masses and halo
+        assert np.array_equal(t["act_off"], np.concatenate([[0], np.cumsum(t["nact"])[:-1]]))
+        assert len(rec) == t["nact"].sum()
+        # each record is an actuated spring's (a, omega, phi, rest)
+        act = np.asarray(b["type"]) == 4
+        want = {tuple(r) for r in np.stack([b["act_a"][act], b["omega"][act], b["act_phi"][act],
+                                            np.asarray(b["rest"])[act]], axis=1)}
+        assert {tuple(r) for r in rec} == want
+    eng, _ = run_engine(sc, 200, layout=LAYOUT_TILED)
+    ref, _ = run_engine(sc, 200, layout=LAYOUT_GATHER)
+    np.testing.assert_array_equal(eng["pos"], ref["pos"])
+    np.testing.assert_array_equal(eng["vel"], ref["vel"])
+    orc = run_oracle(sc, 200)
+    assert normwise_rel(eng["pos"], orc.pos) <= TOL_ACT
+    assert normwise_rel(eng["vel"], orc.vel) <= TOL_ACT
