@@ -112,8 +112,10 @@ const char *titan_last_error(const titan_ctx *ctx);
  * reference, whose every mass and spring is an allocation of its own: this
  * is for servers, which reserve their HBM at startup rather than grow it
  * under load (a growth waits for the steps in flight).  A capacity hint:
- * bodies beyond it still grow the arenas on demand.  Counts toward the HBM
- * budget (TITAN_E_NOMEM if it would exceed it; nothing is grown then). */
+ * bodies beyond it still grow the arenas on demand (the tiled layout keeps
+ * a copy of an actuator's record in each tile it touches, reserved for
+ * two tiles per actuated spring).  Counts toward the HBM budget
+ * (TITAN_E_NOMEM if it would exceed it; nothing is grown then). */
 int titan_reserve(titan_ctx *ctx, int64_t n_masses, int64_t n_springs, int64_t n_actuated, int32_t n_bodies);
 
 /* ---- globals (src/sim.cu:797-807,2081-2087,1998-2040) ---- */
