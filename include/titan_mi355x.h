@@ -260,10 +260,11 @@ int titan_autotune(titan_ctx *ctx, const titan_body_desc *samples, int32_t n_sam
  *      thread, src/sim.cu:1389,1503-1617) ----
  * A request is a body and a horizon (steps).  titan_serve_submit copies the
  * body and queues it.  titan_serve_pump, called by the host loop (the
- * library starts no thread), admits queued requests in arrival order while
- * the running set stays within max_running requests and the serving HBM
- * budget (estimated bytes per body; a body that does not fit waits until
- * others finish), steps every running request together, up to `quantum`
+ * library starts worker threads for the layouts only), admits queued
+ * requests in arrival order while the running set stays within max_running
+ * requests and the serving HBM budget (estimated bytes per body; a body that
+ * does not fit waits until others finish, and no later request overtakes
+ * it), steps every running request together, up to `quantum`
  * steps each and never past its horizon, in one launch per step, and
  * retires the finished ones: their final positions and velocities go to the
  * request's out buffers and their device memory is released.  A request's
@@ -271,8 +272,16 @@ int titan_autotune(titan_ctx *ctx, const titan_body_desc *samples, int32_t n_sam
  * submit to completion on the host's steady clock: for a request with out
  * buffers, when they are written; for one without, when a pump sees that
  * the device has run its last step (until then it counts as running).
- * Admission within a pump stops, between batches, once the device has run
- * the steps that pump enqueued. */
+ * Layouts are built by worker threads from submission; while the device has
+ * work, a pump admits only requests whose layouts are built (in arrival
+ * order among those; one still being built is passed over and a later pump
+ * admits it), and with nothing running it waits for the oldest request's.
+ * Admission within a pump stops, between placements, once the device has
+ * run the quantum enqueued before that pump's.  With TITAN_SERVE_ADMIT=wait
+ * in the environment when the context first serves, admission waits for
+ * every layout it picks and places every request that fits instead, so that
+ * what a pump admits depends on neither the workers' nor the device's
+ * timing. */
 #define TITAN_REQ_QUEUED 0
 #define TITAN_REQ_RUNNING 1
 #define TITAN_REQ_DONE 2

@@ -115,7 +115,8 @@ def _replay(bodies, horizons, budget, max_running, quantum):
     return peak
 
 
-def test_admission_respects_request_limit_and_budget():
+def test_admission_respects_request_limit_and_budget(monkeypatch):
+    monkeypatch.setenv("TITAN_SERVE_ADMIT", "wait")  # admission by pump, as _replay models it
     bodies, horizons, planes = _requests()
     _, peak, stats = _serve(bodies, horizons, planes, max_running=2, quantum=32)
     assert peak == 2 and stats["completed"] == len(bodies)
@@ -127,7 +128,8 @@ def test_admission_respects_request_limit_and_budget():
     assert _replay(bodies, horizons, big, 4096, 32) < len(bodies)
 
 
-def test_cancel_and_stats():
+def test_cancel_and_stats(monkeypatch):
+    monkeypatch.setenv("TITAN_SERVE_ADMIT", "wait")  # the first pump admits the first two
     bodies, horizons, planes = _requests()
     with Context() as ctx:
         ctx.set_dt(1e-4)
@@ -176,7 +178,7 @@ def test_requests_without_results_complete_after_their_steps():
 
 
 @pytest.mark.parametrize("site", [INJECT_DOWNLOAD, INJECT_DESTROY])
-def test_failed_retirement_loses_no_request(site):
+def test_failed_retirement_loses_no_request(site, monkeypatch):
     # The pump that retires request 1 (37 steps, quantum 16: its third pump)
     # also admits the four requests that arrived meanwhile; its retirement
     # fails once (the result's download, or the body's release).  The pump
@@ -186,6 +188,7 @@ def test_failed_retirement_loses_no_request(site):
     bodies, horizons, planes = _requests()
     want, _, _ = _serve(bodies, horizons, planes, quantum=16)
     print(f"case failed retirement at site {site}", flush=True)
+    monkeypatch.setenv("TITAN_SERVE_ADMIT", "wait")  # the failing pump admits the new arrivals
     with Context() as ctx:
         ctx.set_dt(1e-4)
         for p in planes:

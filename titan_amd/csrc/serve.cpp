@@ -7,9 +7,10 @@
 // at its own horizon), and retires the finished ones.  Layouts are built
 // on the host by background worker threads from the moment a request is
 // submitted (in submission order, under the schedule current at its
-// submission), so that admission mostly places bodies already laid out,
-// while the device runs the quantum just enqueued; placing a body on the
-// device does not wait for the launches in flight.  The engine keeps every
+// submission), so that admission places bodies already laid out while the
+// device runs the quantum just enqueued (it waits for a layout only when
+// nothing runs, and passes over requests whose layouts are not built yet);
+// placing a body on the device does not wait for the launches in flight.  The engine keeps every
 // body's arithmetic independent of the others, so a request's result is the
 // same alone and in any mix.
 #include <algorithm>
@@ -17,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -159,6 +161,13 @@ struct Server {
     int32_t max_running = 4096;
     uint64_t hbm_bytes = 0;
     int64_t quantum = 256;
+    // TITAN_SERVE_ADMIT=wait (read when the server is made): admission waits
+    // for the layout of every request it picks and places every request that
+    // fits, so that which requests a pump admits depends neither on the
+    // workers' timing nor on the device's (tests that count pumps use it);
+    // by default it admits only layouts already built, and only until the
+    // device has run the older of its two quanta
+    bool admit_waits = false;
     int64_t next_ticket = 1;
     std::map<int64_t, std::unique_ptr<Request>> reqs;
     std::deque<Request *> queue;
@@ -238,6 +247,13 @@ struct Server {
         prep_cv.wait(lk, [&] { return r->prep_state == 2; });
     }
 
+    // Whether r's layout is built (with no worker thread, admission lays it
+    // out itself, so it counts as ready).
+    bool layout_ready(Request *r) {
+        std::lock_guard<std::mutex> lk(prep_mu);
+        return r->prep_state == 2 || workers.empty();
+    }
+
     // Wait for r's layout (the workers take the to-do list in order, and r
     // is among its oldest); with no worker thread, lay it out here.
     void wait_layout(Request *r) {
@@ -311,7 +327,11 @@ std::map<const titan_ctx *, std::unique_ptr<Server>> g_servers;
 Server &server_of(titan_ctx *ctx) {
     std::lock_guard<std::mutex> lk(g_mu);
     auto &p = g_servers[ctx];
-    if (!p) p.reset(new Server);
+    if (!p) {
+        p.reset(new Server);
+        const char *e = std::getenv("TITAN_SERVE_ADMIT");
+        p->admit_waits = e && std::strcmp(e, "wait") == 0;
+    }
     return *p;
 }
 
@@ -469,16 +489,20 @@ static int serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
         }
     } reconcile{sv, fresh};
     // admit, in arrival order, while the running set has room: up to `cap`
-    // requests, in batches of the queue's head requests that fit, each laid
-    // out by the workers (waited for) and then placed one by one; `until`:
-    // stop (between batches) once the step stream has passed this event, so
-    // that the device does not wait for admission
+    // requests, in batches of queued requests whose layouts the workers have
+    // built, placed one by one.  A request whose layout is still being built
+    // is passed over (a later pump admits it), so that the pump never waits
+    // for the host while the device has work and a large layout holds up no
+    // other request; with nothing running or admitted the pump waits for the
+    // oldest request's.  A request that does not fit the limits stops the
+    // admission, so that none overtakes it there.  `until`: stop (between
+    // placements, after the first) once the step stream has passed this
+    // event, so that the device does not wait for admission (placing a body
+    // copies its arrays to the device: milliseconds for a large one)
     auto admit = [&](size_t cap, hipEvent_t until) -> int {
         const unsigned nt = host_threads();
-        bool first = true;
+        size_t placed = 0;
         while (!sv.queue.empty() && cap > 0) {
-            if (until && !first && hipEventQuery(until) == hipSuccess) break;
-            first = false;
             std::vector<Request *> pick;
             uint64_t bytes = sv.running_bytes;
             size_t n_run = sv.running.size() + fresh.size();
@@ -487,17 +511,20 @@ static int serve_pump(titan_ctx *ctx, int32_t *n_running, int32_t *n_queued) {
                 if ((int64_t)(n_run + pick.size()) >= sv.max_running) break;
                 const bool busy = n_run + pick.size() > 0;
                 if (sv.hbm_bytes && busy && bytes + r->est_bytes > sv.hbm_bytes) break;
+                if (busy && !sv.admit_waits && !sv.layout_ready(r)) continue;
                 pick.push_back(r);
                 bytes += r->est_bytes;
             }
             if (pick.empty()) break;
             for (Request *r : pick) sv.wait_layout(r);
             for (Request *r : pick) {
+                if (until && !sv.admit_waits && placed > 0 && hipEventQuery(until) == hipSuccess) return TITAN_OK;
+                ++placed;
                 const bool busy = !sv.running.empty() || !fresh.empty();
                 int32_t id = -1;
                 int rc = r->prep ? titan_mi355x::commit_body_device(ctx, r->prep, &id) : r->prep_rc;
                 if (rc == TITAN_E_NOMEM && busy) return TITAN_OK;  // wait for memory to free up
-                sv.queue.pop_front();  // r is the queue's head
+                sv.queue.erase(std::find(sv.queue.begin(), sv.queue.end(), r));
                 --cap;
                 if (rc != TITAN_OK) {  // the body is invalid or can never fit: it fails alone
                     finish(sv, r, TITAN_REQ_FAILED, now_s());
