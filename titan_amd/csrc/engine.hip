@@ -190,7 +190,8 @@ struct titan_ctx {
     int64_t planes_cap = 0, balls_cap = 0;
     // ball grid (rebuilt before the next launch after the balls change)
     bool ball_grid_dirty = false;
-    int32_t *bg_start = nullptr, *bg_list = nullptr;
+    int32_t *bg_start = nullptr;
+    double *bg_rec = nullptr;  // 4 per listed ball, plus one spare record
     double bg_x0 = 0, bg_y0 = 0, bg_z0 = 0, bg_inv_h = 0;
     int32_t bg_n[3] = {0, 0, 0};
 
@@ -711,7 +712,7 @@ StepArgs make_args(titan_ctx *c, int slot, const Range &rg) {
     // with the same result
     const bool grid = !c->ball_grid_dirty;
     a.bg_start = grid ? c->bg_start : nullptr;
-    a.bg_list = grid ? c->bg_list : nullptr;
+    a.bg_rec = grid ? c->bg_rec : nullptr;
     a.bg_x0 = c->bg_x0;
     a.bg_y0 = c->bg_y0;
     a.bg_z0 = c->bg_z0;
@@ -798,8 +799,9 @@ int64_t live_body_hw(titan_ctx *c) { return c->bids.high_water(); }
 void build_ball_grid(titan_ctx *c) {
     c->ball_grid_dirty = false;
     dfree(c, c->bg_start);
-    dfree(c, c->bg_list);
-    c->bg_start = c->bg_list = nullptr;
+    dfree(c, c->bg_rec);
+    c->bg_start = nullptr;
+    c->bg_rec = nullptr;
     const int64_t nb = (int64_t)c->balls_h.size() / 4;
     if (nb < kBallGridMin) return;
     const double *B = c->balls_h.data();
@@ -861,9 +863,14 @@ void build_ball_grid(titan_ctx *c) {
         list.insert(list.end(), cells[q].begin(), cells[q].end());
     }
     c->bg_start = dalloc<int32_t>(c, ncell + 1);
-    c->bg_list = dalloc<int32_t>(c, std::max<int64_t>(1, (int64_t)list.size()));
+    // the listed balls' records, and a spare one (zeros) that the walk over
+    // a list may load one past its end
+    std::vector<double> rec(4 * (list.size() + 1), 0.0);
+    for (size_t q = 0; q < list.size(); ++q)
+        for (int k = 0; k < 4; ++k) rec[4 * q + k] = B[4 * (size_t)list[q] + k];
+    c->bg_rec = dalloc<double>(c, (int64_t)rec.size());
     h2d(c, c->bg_start, start.data(), ncell + 1);
-    h2d(c, c->bg_list, list.data(), (int64_t)list.size());
+    h2d(c, c->bg_rec, rec.data(), (int64_t)rec.size());
     HIPCHK(hipStreamSynchronize(c->stream));
     c->bg_x0 = lo[0];
     c->bg_y0 = lo[1];

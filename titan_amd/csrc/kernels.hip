@@ -234,9 +234,17 @@ __device__ __forceinline__ void mass_pass(const StepArgs &a, MassRegs &r, int64_
         if (fx >= 0.0 && fx < (double)a.bg_nx && fy >= 0.0 && fy < (double)a.bg_ny && fz >= 0.0 &&
             fz < (double)a.bg_nz) {
             const int64_t cell = ((int64_t)(int)fz * a.bg_ny + (int)fy) * a.bg_nx + (int)fx;
-            for (int32_t q = a.bg_start[cell]; q < a.bg_start[cell + 1]; ++q) {
-                const double *B = a.balls + 4 * (int64_t)a.bg_list[q];
-                ball(r, B[0], B[1], B[2], B[3]);
+            // the next record is loaded before the current one is tested (the
+            // list ends in a spare record), so that a wave's iterations, as
+            // many as its longest list, do not each wait for a load
+            const double2 *__restrict__ R = reinterpret_cast<const double2 *>(a.bg_rec);
+            const int32_t q0 = a.bg_start[cell], q1 = a.bg_start[cell + 1];
+            double2 c0 = R[2 * q0], c1 = R[2 * q0 + 1];  // (cx, cy), (cz, r)
+            for (int32_t q = q0; q < q1; ++q) {
+                const double2 n0 = R[2 * q + 2], n1 = R[2 * q + 3];
+                ball(r, c0.x, c0.y, c1.x, c1.y);
+                c0 = n0;
+                c1 = n1;
             }
         }
     } else {
