@@ -234,17 +234,26 @@ __device__ __forceinline__ void mass_pass(const StepArgs &a, MassRegs &r, int64_
         if (fx >= 0.0 && fx < (double)a.bg_nx && fy >= 0.0 && fy < (double)a.bg_ny && fz >= 0.0 &&
             fz < (double)a.bg_nz) {
             const int64_t cell = ((int64_t)(int)fz * a.bg_ny + (int)fy) * a.bg_nx + (int)fx;
-            // the next record is loaded before the current one is tested (the
-            // list ends in a spare record), so that a wave's iterations, as
-            // many as its longest list, do not each wait for a load
+            // records are loaded four at a time, then tested in order, so that
+            // a typical list (3.9 balls per mass on the 2M kNN cloud with 1000
+            // balls, 4.9 iterations per wave) costs one round trip to the
+            // cache rather than one per ball; a load past the list's end reads
+            // the next list's first record or the spare one that ends them.
+            // Eight at a time took the step to 132 VGPRs and was 7% slower.
             const double2 *__restrict__ R = reinterpret_cast<const double2 *>(a.bg_rec);
             const int32_t q0 = a.bg_start[cell], q1 = a.bg_start[cell + 1];
-            double2 c0 = R[2 * q0], c1 = R[2 * q0 + 1];  // (cx, cy), (cz, r)
-            for (int32_t q = q0; q < q1; ++q) {
-                const double2 n0 = R[2 * q + 2], n1 = R[2 * q + 3];
-                ball(r, c0.x, c0.y, c1.x, c1.y);
-                c0 = n0;
-                c1 = n1;
+            constexpr int PB = 4;
+            for (int32_t qb = q0; qb < q1; qb += PB) {
+                double2 c[PB][2];  // (cx, cy), (cz, r)
+#pragma unroll
+                for (int j = 0; j < PB; ++j) {
+                    const int32_t q = min(qb + j, q1);
+                    c[j][0] = R[2 * q];
+                    c[j][1] = R[2 * q + 1];
+                }
+#pragma unroll
+                for (int j = 0; j < PB; ++j)
+                    if (qb + j < q1) ball(r, c[j][0].x, c[j][0].y, c[j][1].x, c[j][1].y);
             }
         }
     } else {

@@ -250,7 +250,7 @@ struct StepArgs {
     // uniform grid over the balls (null below kBallGridMin balls): cell
     // (ix, iy, iz) lists, in creation order, the balls that can touch it,
     // each as a copy of its record (cx cy cz r), so that a mass's test of a
-    // listed ball loads it directly; one spare record ends the list
+    // listed ball loads it directly; one spare record ends the lists
     const int32_t *bg_start;  // nx*ny*nz + 1 offsets into bg_rec (in records)
     const double *bg_rec;
     double bg_x0, bg_y0, bg_z0, bg_inv_h;
