@@ -39,13 +39,39 @@ double now_s() {
     return std::chrono::duration<double>(Clock::now().time_since_epoch()).count();
 }
 
-template <class T>
-std::vector<T> copy_of(const T *p, int64_t n) {
-    return p && n > 0 ? std::vector<T>(p, p + n) : std::vector<T>();
-}
-template <class T>
-const T *ptr_or_null(const std::vector<T> &v) {
-    return v.empty() ? nullptr : v.data();
+// One array of a body to copy into a request's block.
+struct CopyJob {
+    void *dst;
+    const void *src;
+    size_t bytes;
+};
+
+// Copy the jobs' bytes, split into `parts` equal spans of their
+// concatenation: the caller's thread copies the first, one thread each the
+// others (a span runs on the caller's thread when no thread can be made).
+// The destination pages are first touched by these copies, so that page
+// faults are taken by several threads at once.
+void copy_spans(const std::vector<CopyJob> &jobs, size_t total, unsigned parts) {
+    auto span = [&jobs](size_t lo, size_t hi) {
+        size_t base = 0;
+        for (const CopyJob &j : jobs) {
+            const size_t a = std::max(lo, base), b = std::min(hi, base + j.bytes);
+            if (a < b) std::memcpy(static_cast<char *>(j.dst) + (a - base), static_cast<const char *>(j.src) + (a - base), b - a);
+            base += j.bytes;
+            if (base >= hi) break;
+        }
+    };
+    std::vector<std::thread> th;
+    for (unsigned p = 1; p < parts; ++p) {
+        const size_t lo = total * p / parts, hi = total * (p + 1) / parts;
+        try {
+            th.emplace_back(span, lo, hi);
+        } catch (const std::exception &) {
+            span(lo, hi);
+        }
+    }
+    span(0, total / parts);
+    for (auto &t : th) t.join();
 }
 
 // A request with its own copy of the body's arrays.
@@ -67,66 +93,73 @@ struct Request {
     titan_schedule sched{};  // the context's schedule when submitted
 
     ~Request() { titan_mi355x::free_prepared(prep); }
-    std::vector<double> pos, vel, m, T, ext, k, rest, damping, omega, act_a, act_phi, lc_vec, lc_num;
-    std::vector<uint8_t> fixed;
-    std::vector<int32_t> drag, left, right, type, lc_type;
-    std::vector<int64_t> lc_ptr;
+    // the body's arrays, copied at submission into one block (64-byte
+    // aligned pieces) that desc points into
+    std::unique_ptr<unsigned char[]> block;
 
+    // Point `field` at the next piece of the block (with `jobs`, after the
+    // block is allocated) for `cnt` elements of src; a null or empty array
+    // stays null.
+    template <class T>
+    void piece(const T *src, int64_t cnt, const T *&field, size_t &off, std::vector<CopyJob> *jobs) {
+        field = nullptr;
+        if (!src || cnt <= 0) return;
+        const size_t bytes = (size_t)cnt * sizeof(T);
+        if (jobs) {
+            T *dst = reinterpret_cast<T *>(block.get() + off);
+            jobs->push_back({dst, src, bytes});
+            field = dst;
+        }
+        off += (bytes + 63) & ~size_t(63);
+    }
+    size_t pieces(const titan_body_desc &d, std::vector<CopyJob> *jobs) {
+        const int64_t nl = d.lc_ptr ? d.lc_ptr[n] : 0;
+        size_t off = 0;
+        piece(d.pos, 3 * n, desc.pos, off, jobs);
+        piece(d.vel, 3 * n, desc.vel, off, jobs);
+        piece(d.m, n, desc.m, off, jobs);
+        piece(d.T, n, desc.T, off, jobs);
+        piece(d.ext, 3 * n, desc.ext, off, jobs);
+        piece(d.fixed, n, desc.fixed, off, jobs);
+        piece(d.drag, n, desc.drag, off, jobs);
+        piece(d.left, s, desc.left, off, jobs);
+        piece(d.right, s, desc.right, off, jobs);
+        piece(d.k, s, desc.k, off, jobs);
+        piece(d.rest, s, desc.rest, off, jobs);
+        piece(d.damping, s, desc.damping, off, jobs);
+        piece(d.type, s, desc.type, off, jobs);
+        piece(d.omega, s, desc.omega, off, jobs);
+        piece(d.act_a, s, desc.act_a, off, jobs);
+        piece(d.act_phi, s, desc.act_phi, off, jobs);
+        piece(d.lc_ptr, d.lc_ptr ? n + 1 : 0, desc.lc_ptr, off, jobs);
+        piece(d.lc_type, nl, desc.lc_type, off, jobs);
+        piece(d.lc_vec, 3 * nl, desc.lc_vec, off, jobs);
+        piece(d.lc_num, nl, desc.lc_num, off, jobs);
+        return off;
+    }
+
+    // Copy the body's arrays; a body of 4 MB or more is copied by one
+    // thread per 2 MB, up to four: copying into fresh pages runs at the rate
+    // the kernel maps them, about 10 GB/s on one thread, and the caller's
+    // thread is the one that also pumps the server.
     void copy_from(const titan_body_desc &d) {
         n = d.n_masses;
         s = d.n_springs;
-        pos = copy_of(d.pos, 3 * n);
-        vel = copy_of(d.vel, 3 * n);
-        m = copy_of(d.m, n);
-        T = copy_of(d.T, n);
-        ext = copy_of(d.ext, 3 * n);
-        fixed = copy_of(d.fixed, n);
-        drag = copy_of(d.drag, n);
-        left = copy_of(d.left, s);
-        right = copy_of(d.right, s);
-        k = copy_of(d.k, s);
-        rest = copy_of(d.rest, s);
-        damping = copy_of(d.damping, s);
-        type = copy_of(d.type, s);
-        omega = copy_of(d.omega, s);
-        act_a = copy_of(d.act_a, s);
-        act_phi = copy_of(d.act_phi, s);
-        lc_ptr = copy_of(d.lc_ptr, d.lc_ptr ? n + 1 : 0);
-        const int64_t nl = lc_ptr.empty() ? 0 : lc_ptr[n];
-        lc_type = copy_of(d.lc_type, nl);
-        lc_vec = copy_of(d.lc_vec, 3 * nl);
-        lc_num = copy_of(d.lc_num, nl);
         desc = d;
-        desc.pos = ptr_or_null(pos);
-        desc.vel = ptr_or_null(vel);
-        desc.m = ptr_or_null(m);
-        desc.T = ptr_or_null(T);
-        desc.ext = ptr_or_null(ext);
-        desc.fixed = ptr_or_null(fixed);
-        desc.drag = ptr_or_null(drag);
-        desc.left = ptr_or_null(left);
-        desc.right = ptr_or_null(right);
-        desc.k = ptr_or_null(k);
-        desc.rest = ptr_or_null(rest);
-        desc.damping = ptr_or_null(damping);
-        desc.type = ptr_or_null(type);
-        desc.omega = ptr_or_null(omega);
-        desc.act_a = ptr_or_null(act_a);
-        desc.act_phi = ptr_or_null(act_phi);
-        desc.lc_ptr = ptr_or_null(lc_ptr);
-        desc.lc_type = ptr_or_null(lc_type);
-        desc.lc_vec = ptr_or_null(lc_vec);
-        desc.lc_num = ptr_or_null(lc_num);
+        const size_t size = pieces(d, nullptr);
+        block.reset(new unsigned char[std::max<size_t>(size, 1)]);
+        std::vector<CopyJob> jobs;
+        pieces(d, &jobs);
+        size_t total = 0;
+        for (const CopyJob &j : jobs) total += j.bytes;
+        const size_t part = size_t(2) << 20;
+        copy_spans(jobs, total, (unsigned)std::min<size_t>(4, std::max<size_t>(1, total / part)));
     }
 
     void release_arrays() {
         titan_mi355x::free_prepared(prep);
         prep = nullptr;
-        for (auto *v : {&pos, &vel, &m, &T, &ext, &k, &rest, &damping, &omega, &act_a, &act_phi, &lc_vec, &lc_num})
-            std::vector<double>().swap(*v);
-        std::vector<uint8_t>().swap(fixed);
-        for (auto *v : {&drag, &left, &right, &type, &lc_type}) std::vector<int32_t>().swap(*v);
-        std::vector<int64_t>().swap(lc_ptr);
+        block.reset();
     }
 };
 
@@ -168,6 +201,9 @@ struct Server {
     // by default it admits only layouts already built, and only until the
     // device has run the older of its two quanta
     bool admit_waits = false;
+    // layout worker threads (TITAN_SERVE_WORKERS, read when the server is
+    // made; default: one per host thread but the caller's)
+    unsigned n_workers = 1;
     int64_t next_ticket = 1;
     std::map<int64_t, std::unique_ptr<Request>> reqs;
     std::deque<Request *> queue;
@@ -331,6 +367,9 @@ Server &server_of(titan_ctx *ctx) {
         p.reset(new Server);
         const char *e = std::getenv("TITAN_SERVE_ADMIT");
         p->admit_waits = e && std::strcmp(e, "wait") == 0;
+        const char *w = std::getenv("TITAN_SERVE_WORKERS");
+        const long nw = w ? std::strtol(w, nullptr, 10) : 0;
+        p->n_workers = nw > 0 ? (unsigned)std::min(nw, 64L) : std::max(1u, host_threads() - 1);
     }
     return *p;
 }
@@ -461,7 +500,7 @@ static int serve_submit(titan_ctx *ctx, const titan_request *req, int64_t *ticke
         std::lock_guard<std::mutex> lk(sv.prep_mu);
         sv.todo.push_back(rp);
     }
-    if (sv.workers.empty()) sv.start_workers(std::max(1u, host_threads() - 1));
+    if (sv.workers.empty()) sv.start_workers(sv.n_workers);
     sv.prep_cv.notify_one();
     return TITAN_OK;
 }
