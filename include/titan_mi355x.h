@@ -282,7 +282,7 @@ int titan_autotune(titan_ctx *ctx, const titan_body_desc *samples, int32_t n_sam
  * every layout it picks and places every request that fits instead, so that
  * what a pump admits depends on neither the workers' nor the device's
  * timing.  TITAN_SERVE_WORKERS=<n> sets the number of layout threads
- * (default: one per host thread but the caller's, at most 15). */
+ * (default: one per host thread but four, at most 12). */
 #define TITAN_REQ_QUEUED 0
 #define TITAN_REQ_RUNNING 1
 #define TITAN_REQ_DONE 2

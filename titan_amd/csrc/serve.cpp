@@ -202,7 +202,8 @@ struct Server {
     // device has run the older of its two quanta
     bool admit_waits = false;
     // layout worker threads (TITAN_SERVE_WORKERS, read when the server is
-    // made; default: one per host thread but the caller's)
+    // made; default: one per host thread but four, which are left to the
+    // caller's thread and the three that help it copy a submitted body)
     unsigned n_workers = 1;
     int64_t next_ticket = 1;
     std::map<int64_t, std::unique_ptr<Request>> reqs;
@@ -369,7 +370,7 @@ Server &server_of(titan_ctx *ctx) {
         p->admit_waits = e && std::strcmp(e, "wait") == 0;
         const char *w = std::getenv("TITAN_SERVE_WORKERS");
         const long nw = w ? std::strtol(w, nullptr, 10) : 0;
-        p->n_workers = nw > 0 ? (unsigned)std::min(nw, 64L) : std::max(1u, host_threads() - 1);
+        p->n_workers = nw > 0 ? (unsigned)std::min(nw, 64L) : (host_threads() > 4 ? host_threads() - 4 : 1u);
     }
     return *p;
 }
