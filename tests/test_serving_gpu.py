@@ -249,3 +249,43 @@ def test_knn_cloud_requests_alone_and_mixed():
         o.run(horizons[q])
         assert normwise_rel(alone[q][0], o.pos) <= 1e-10
         assert normwise_rel(alone[q][1], o.vel) <= 1e-10
+
+
+def _body_with_every_array(side, seed):
+    """A passive lattice with every optional array of titan_body_desc set:
+    velocities, local times, extern forces, fixed masses, drag, and local
+    constraints (constraint planes, directions and contact planes)."""
+    rng = np.random.default_rng(seed)
+    b = scenes.lattice_body((0.0, 0.0, 0.58), (1.0, 1.0, 1.0), side, damping=0.1)
+    n = len(b["pos"])
+    cnt = rng.integers(0, 3, n)
+    b["lc_ptr"] = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
+    tot = int(cnt.sum())
+    b["lc_type"] = rng.choice([0, 3, 1], size=tot).astype(np.int32)
+    v = rng.normal(size=(tot, 3))
+    b["lc_vec"] = v / np.sqrt((v * v).sum(1))[:, None]
+    b["lc_num"] = rng.uniform(0.0, 0.4, tot)
+    b["vel"] = rng.normal(0, 0.3, (n, 3))
+    b["T"] = rng.uniform(0.0, 1e-2, n)
+    b["fixed"] = (np.arange(n) % 19 == 0).astype(np.uint8)
+    b["drag"] = (np.arange(n) % 4).astype(np.int32)
+    b["ext"] = rng.normal(0, 0.2, (n, 3)) * (np.arange(n) % 2 == 0)[:, None]
+    return b
+
+
+def test_every_array_of_a_request_reaches_the_device():
+    # submission copies a request's arrays into one block of its own, a large
+    # body (4 MB or more) by several threads: a small body and a 40^3 lattice
+    # (about 30 MB of arrays), served together, match the oracle bit for bit
+    planes = [(0.0, 0.0, 1.0, 0.0, 0.8, 1.0)]
+    bodies = [_body_with_every_array(6, 1), _body_with_every_array(40, 2)]
+    horizons = [90, 40]
+    print(f"case serving every array: masses {[len(b['pos']) for b in bodies]}, "
+          f"springs {[len(b['left']) for b in bodies]}, horizons {horizons}", flush=True)
+    res, _, stats = _serve(bodies, horizons, planes)
+    assert stats["completed"] == 2
+    for q in range(2):
+        o = to_oracle(scenes.Scene(bodies=[bodies[q]], planes=planes, dt=1e-4))
+        o.run(horizons[q])
+        np.testing.assert_array_equal(res[q][0], o.pos)
+        np.testing.assert_array_equal(res[q][1], o.vel)
